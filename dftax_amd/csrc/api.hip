@@ -1,0 +1,814 @@
+// C ABI of the DF-KS library: handle state, build pipeline, energy + Fock, SCF.
+//
+// Data layout in HBM (all FP64, row-major unless noted):
+//   B      [nao][nao][nfit]   fitted three-centre tensor, B = (mn|P) L, V+ = L L^T
+//   ao     [4][ng][nao]       AO values and x/y/z gradients on the grid
+//   zw     [ng][nao]          Z = Phi P, overwritten in place by W (XC kernel)
+//   T      [nao][nfit][nf]    exchange intermediate T[m,Q,i] = sum_l B[l,m,Q] Cf[l,i]
+//   S, H, P, F, J, K, V       nao x nao (symmetric)
+//   X, C   column-major       nao x nmo orthonormaliser / MO coefficients
+// Plain GEMM/SYRK/SYR2K shapes go to rocBLAS and the symmetric eigensolves to
+// rocSOLVER; every other step is a kernel of this library.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dfks.h"
+#include "dfks_common.h"
+
+namespace dfks {
+void upload_c2s_tables();
+hipError_t launch_eri_wave(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,
+                           const ShellSet& A, const ShellSet& C, int b_unit, const double* boys, double* out,
+                           int64_t sa, int64_t sb, int tr, hipStream_t stream);
+hipError_t launch_int1e(const ShellSet& A, int natm, const double* atm, const double* boys, double* S, double* T,
+                        double* V, int64_t nao, hipStream_t stream);
+hipError_t launch_becke(int64_t n, const double* pts, const double* raw, const int32_t* owner, int natm,
+                        const double* atm, const double* invR, const double* aAB, double* w, hipStream_t st);
+hipError_t launch_ao(int64_t ng, const double* pts, const ShellSet& S, int64_t nao, double* ao, hipStream_t st);
+hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w, double* zw, int xc_id, double thresh,
+                     double* exc, double* rho_out, hipStream_t st);
+hipError_t launch_xc_pointwise(int64_t n, int xc_id, const double* rho, const double* sigma, double* out,
+                               hipStream_t st);
+hipError_t launch_dot(int64_t n, const double* x, const double* y, double* part, double scale, double* out,
+                      int accumulate, hipStream_t st);
+hipError_t launch_gamma(int nao, int nq, int nf, const double* T, const double* Cf, const double* sgn, double* gamma,
+                        int accumulate, hipStream_t st);
+hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, double* J, hipStream_t st);
+hipError_t launch_fock(int n, const double* H, const double* J, const double* Ku, double cK, const double* Vu,
+                       double* F, hipStream_t st);
+hipError_t launch_symfill(int n, double* A, hipStream_t st);
+hipError_t launch_metric_factor(int nq, int k0, const double* U, const double* w, double* Lt, hipStream_t st);
+hipError_t launch_factor_cols(int n, int nf, const double* V, const double* lam, const int32_t* sel, double* Cf,
+                              double* sgn, hipStream_t st);
+hipError_t launch_axpy(int64_t n, double a, const double* x, double* y, int acc, hipStream_t st);
+hipError_t launch_antisym(int n, const double* A, double* E, hipStream_t st);
+hipError_t launch_diis_solve(int m, int count, const double* gram, double* coef, hipStream_t st);
+hipError_t launch_diis_extrap(int m, int64_t n, const double* coef, const double* Fs, double* out, hipStream_t st);
+hipError_t launch_gram_update(int m, int slot, int count, const double* dots, double* gram, hipStream_t st);
+hipError_t launch_scaled_cols(int n, int k0, const double* U, const double* w, double* X, hipStream_t st);
+}  // namespace dfks
+
+using namespace dfks;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail {
+  int code;
+};
+
+void fail(int code, const std::string& msg) {
+  g_err = msg;
+  throw Fail{code};
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) fail(2, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+#define RBCHK(expr)                                                                           \
+  do {                                                                                        \
+    rocblas_status s_ = (expr);                                                               \
+    if (s_ != rocblas_status_success) fail(3, std::string(#expr) + ": rocblas status " +      \
+                                              std::to_string((int)s_));                       \
+  } while (0)
+
+template <class T>
+T* dalloc(size_t n) {
+  if (n == 0) n = 1;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) fail(4, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes: " + hipGetErrorString(e));
+  return static_cast<T*>(p);
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+struct Basis {
+  std::vector<int32_t> l, nprim, prim0, ao0;
+  std::vector<double> centre, exps, coefs;
+  int32_t spherical = 1, nfunc = 0;
+  int32_t *d_l = nullptr, *d_nprim = nullptr, *d_prim0 = nullptr, *d_ao0 = nullptr;
+  double *d_centre = nullptr, *d_exps = nullptr, *d_coefs = nullptr;
+  int n() const { return (int)l.size(); }
+  int maxl() const { return l.empty() ? 0 : *std::max_element(l.begin(), l.end()); }
+  ShellSet view() const {
+    return ShellSet{(int32_t)l.size(), d_l, d_nprim, d_prim0, d_ao0, d_centre, d_exps, d_coefs, spherical};
+  }
+  void load(const dfks_basis_desc& d) {
+    if (d.nshell <= 0) fail(1, "basis with no shells");
+    l.assign(d.l, d.l + d.nshell);
+    nprim.assign(d.nprim, d.nprim + d.nshell);
+    prim0.assign(d.prim0, d.prim0 + d.nshell);
+    ao0.assign(d.ao0, d.ao0 + d.nshell);
+    centre.assign(d.centre, d.centre + 3 * d.nshell);
+    exps.assign(d.exps, d.exps + d.nprim_total);
+    coefs.assign(d.coefs, d.coefs + d.nprim_total);
+    spherical = d.spherical;
+    nfunc = d.nfunc;
+    int expect = 0;
+    for (int i = 0; i < d.nshell; ++i) {
+      if (l[i] < 0 || l[i] > kLMax) fail(1, "shell angular momentum outside 0..4");
+      if (ao0[i] != expect) fail(1, "shell ao0 offsets are not contiguous");
+      if (prim0[i] < 0 || nprim[i] <= 0 || prim0[i] + nprim[i] > d.nprim_total) fail(1, "bad primitive range");
+      expect += nfun(l[i], spherical);
+    }
+    if (expect != nfunc) fail(1, "nfunc does not match the shells");
+    auto up = [](auto& v) {
+      using T = typename std::decay_t<decltype(v)>::value_type;
+      T* p = dalloc<T>(v.size());
+      HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+      return p;
+    };
+    d_l = up(l); d_nprim = up(nprim); d_prim0 = up(prim0); d_ao0 = up(ao0);
+    d_centre = up(centre); d_exps = up(exps); d_coefs = up(coefs);
+  }
+  void release() {
+    dfree(d_l); dfree(d_nprim); dfree(d_prim0); dfree(d_ao0); dfree(d_centre); dfree(d_exps); dfree(d_coefs);
+    d_l = d_nprim = d_prim0 = d_ao0 = nullptr;
+    d_centre = d_exps = d_coefs = nullptr;
+  }
+};
+
+std::vector<double> boys_table() {
+  std::vector<double> tbl(kBoysNg * kBoysN);
+  const int n_top = std::max(kBoysN - 1, (int)kBoysTmax) + 20;
+  for (int j = 0; j < kBoysNg; ++j) {
+    double t = j * kBoysDt, two_t = 2.0 * t, emt = std::exp(-t);
+    double term = 1.0 / (2 * n_top + 1), total = term;
+    for (int k = 1; k < 500; ++k) {
+      term = term * two_t / (2 * n_top + 2 * k + 1);
+      total += term;
+      if (term <= 1e-18 * total) break;
+    }
+    double f = emt * total;
+    for (int n = n_top; n > 0; --n) {
+      if (n <= kBoysN - 1) tbl[j * kBoysN + n] = f;
+      f = (two_t * f + emt) / (2 * n - 1);
+    }
+    tbl[j * kBoysN + 0] = f;
+  }
+  return tbl;
+}
+
+double hf_coeff(int xc) {
+  switch (xc) {
+    case 1: return 0.0;
+    case 2: return 0.25;
+    case 3: return 0.20;
+    default: fail(1, "unknown xc_id " + std::to_string(xc));
+  }
+  return 0.0;
+}
+
+}  // namespace
+
+struct dfks_ctx {
+  int nao = 0, naux = 0, nfit = 0, nocc = 0, natom = 0, xc_id = 0;
+  int64_t ng = 0;
+  double hf = 0.0, e_nn = 0.0;
+  dfks_options opt{};
+  Basis orb, aux;
+  std::vector<double> atm_host;  // x, y, z, Z
+  std::vector<double> h_sgn;     // factor signs of the last Fock build
+  rocblas_handle rb = nullptr;
+  bool built = false;
+
+  double* d_boys = nullptr;
+  double* d_atm = nullptr;
+  double *d_S = nullptr, *d_T = nullptr, *d_V = nullptr, *d_H = nullptr;
+  double *d_int2c = nullptr, *d_int3c = nullptr, *d_B = nullptr;
+  double *d_gxyz = nullptr, *d_gw = nullptr, *d_ao = nullptr, *d_zw = nullptr, *d_exc = nullptr;
+  // Fock-build scratch
+  int nf_chunk = 0;
+  double *d_Tx = nullptr, *d_gamma = nullptr, *d_J = nullptr, *d_K = nullptr, *d_Vxc = nullptr;
+  double *d_part = nullptr, *d_Edev = nullptr, *d_Cf = nullptr, *d_sgn = nullptr;
+  double *d_W1 = nullptr, *d_W2 = nullptr, *d_evals = nullptr, *d_offd = nullptr;
+  int32_t *d_info = nullptr, *d_sel = nullptr;
+  // SCF state
+  int nmo = 0, m = 8, it = 0;
+  double level_shift = 0.0, e_prev = 0.0;
+  double *d_X = nullptr, *d_P = nullptr, *d_F = nullptr, *d_Fs = nullptr, *d_Es = nullptr, *d_gram = nullptr;
+  double *d_coef = nullptr, *d_dots = nullptr, *d_Fx = nullptr, *d_C = nullptr, *d_eps = nullptr, *d_Ws = nullptr;
+
+  ~dfks_ctx() {
+    orb.release();
+    aux.release();
+    void* ptrs[] = {d_boys, d_atm, d_S, d_T, d_V, d_H, d_int2c, d_int3c, d_B, d_gxyz, d_gw, d_ao, d_zw, d_exc,
+                    d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
+                    d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws};
+    for (void* p : ptrs) dfree(p);
+    if (rb) rocblas_destroy_handle(rb);
+  }
+};
+
+namespace {
+
+std::mutex g_mu;
+
+hipStream_t S_(dfks_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 9;
+  }
+}
+
+void sync(hipStream_t st) { HIPCHK(hipStreamSynchronize(st)); }
+
+// ---------------------------------------------------------------------------
+// build
+// ---------------------------------------------------------------------------
+void build_all(dfks_ctx* h, hipStream_t st) {
+  RBCHK(rocblas_set_stream(h->rb, st));
+  const int nao = h->nao, naux = h->naux;
+  const ShellSet O = h->orb.view(), A = h->aux.view();
+  // one-electron
+  h->d_S = dalloc<double>((size_t)nao * nao);
+  h->d_T = dalloc<double>((size_t)nao * nao);
+  h->d_V = dalloc<double>((size_t)nao * nao);
+  h->d_H = dalloc<double>((size_t)nao * nao);
+  HIPCHK(launch_int1e(O, h->natom, h->d_atm, h->d_boys, h->d_S, h->d_T, h->d_V, nao, st));
+  HIPCHK(hipMemcpyAsync(h->d_H, h->d_T, sizeof(double) * nao * nao, hipMemcpyDeviceToDevice, st));
+  HIPCHK(launch_axpy((int64_t)nao * nao, 1.0, h->d_V, h->d_H, 1, st));
+
+  // class lists
+  const int lo = h->orb.maxl(), lx = h->aux.maxl();
+  std::vector<std::vector<int2>> pairs((lo + 1) * (lo + 1));
+  for (int a = 0; a < h->orb.n(); ++a)
+    for (int b = 0; b < h->orb.n(); ++b) {
+      int la = h->orb.l[a], lb = h->orb.l[b];
+      if (la < lb || (la == lb && b > a)) continue;
+      pairs[la * (lo + 1) + lb].push_back(make_int2(a, b));
+    }
+  std::vector<std::vector<int32_t>> auxl(lx + 1);
+  for (int c = 0; c < h->aux.n(); ++c) auxl[h->aux.l[c]].push_back(c);
+  std::vector<std::vector<int2>> apairs(lx + 1);
+  for (int c = 0; c < h->aux.n(); ++c) apairs[h->aux.l[c]].push_back(make_int2(c, c));
+  // flatten to device
+  std::vector<int2> all_pairs;
+  std::vector<size_t> pair_off;
+  for (auto& v : pairs) { pair_off.push_back(all_pairs.size()); all_pairs.insert(all_pairs.end(), v.begin(), v.end()); }
+  std::vector<size_t> apair_off;
+  for (auto& v : apairs) { apair_off.push_back(all_pairs.size()); all_pairs.insert(all_pairs.end(), v.begin(), v.end()); }
+  std::vector<int32_t> all_aux;
+  std::vector<size_t> aux_off;
+  for (auto& v : auxl) { aux_off.push_back(all_aux.size()); all_aux.insert(all_aux.end(), v.begin(), v.end()); }
+  int2* d_pairs = dalloc<int2>(all_pairs.size());
+  int32_t* d_auxl = dalloc<int32_t>(all_aux.size());
+  HIPCHK(hipMemcpy(d_pairs, all_pairs.data(), all_pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_auxl, all_aux.data(), all_aux.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+
+  // two-centre metric
+  h->d_int2c = dalloc<double>((size_t)naux * naux);
+  for (int la = 0; la <= lx; ++la)
+    for (int lc = 0; lc <= lx; ++lc)
+      HIPCHK(launch_eri_wave(la, 0, lc, d_pairs + apair_off[la], (int)apairs[la].size(), d_auxl + aux_off[lc],
+                             (int)auxl[lc].size(), A, A, 1, h->d_boys, h->d_int2c, naux, 0, 0, st));
+  // three-centre
+  const size_t n3 = (size_t)nao * nao * naux;
+  double* d3 = dalloc<double>(n3);
+  for (int la = 0; la <= lo; ++la)
+    for (int lb = 0; lb <= la; ++lb)
+      for (int lc = 0; lc <= lx; ++lc) {
+        auto& pv = pairs[la * (lo + 1) + lb];
+        HIPCHK(launch_eri_wave(la, lb, lc, d_pairs + pair_off[la * (lo + 1) + lb], (int)pv.size(),
+                               d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A, 0, h->d_boys, d3,
+                               (int64_t)nao * naux, naux, 1, st));
+      }
+  sync(st);
+  dfree(d_pairs);
+  dfree(d_auxl);
+
+  // metric pseudo-inverse factor L = U_keep w_keep^{-1/2}
+  double* d_U = dalloc<double>((size_t)naux * naux);
+  double* d_w = dalloc<double>(naux);
+  double* d_e = dalloc<double>(naux);
+  HIPCHK(hipMemcpyAsync(d_U, h->d_int2c, sizeof(double) * naux * naux, hipMemcpyDeviceToDevice, st));
+  RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, naux, d_U, naux, d_w, d_e, h->d_info));
+  std::vector<double> w(naux);
+  HIPCHK(hipMemcpyAsync(w.data(), d_w, sizeof(double) * naux, hipMemcpyDeviceToHost, st));
+  int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+  sync(st);
+  if (info != 0) fail(5, "metric eigensolver failed, info=" + std::to_string(info));
+  const double wmax = w[naux - 1];
+  int k0 = 0;
+  while (k0 < naux && !(w[k0] > h->opt.metric_cutoff * wmax)) ++k0;
+  h->nfit = naux - k0;
+  double* d_Lt = dalloc<double>((size_t)h->nfit * naux);  // column-major nfit x naux
+  HIPCHK(launch_metric_factor(naux, k0, d_U, d_w, d_Lt, st));
+  // B_cm (nfit x nao^2) = Lt (nfit x naux) * int3c_cm (naux x nao^2)
+  h->d_B = dalloc<double>((size_t)nao * nao * h->nfit);
+  const double one = 1.0, zero = 0.0;
+  RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, h->nfit, (int64_t)nao * nao, naux,
+                         &one, d_Lt, h->nfit, d3, naux, &zero, h->d_B, h->nfit));
+  sync(st);
+  dfree(d_U); dfree(d_w); dfree(d_e); dfree(d_Lt);
+  if (h->opt.keep_int3c) h->d_int3c = d3;
+  else dfree(d3);
+
+  // AO values and gradients on the grid
+  if (h->ng > 0) {
+    h->d_ao = dalloc<double>((size_t)4 * h->ng * nao);
+    h->d_zw = dalloc<double>((size_t)h->ng * nao);
+    h->d_exc = dalloc<double>(h->ng);
+    HIPCHK(launch_ao(h->ng, h->d_gxyz, O, nao, h->d_ao, st));
+  }
+  // Fock-build scratch: exchange intermediate for up to nf_chunk factor columns
+  h->nf_chunk = std::max(1, std::min(nao, std::max(h->nocc, 1)));
+  h->d_Tx = dalloc<double>((size_t)nao * h->nfit * h->nf_chunk);
+  h->d_gamma = dalloc<double>(h->nfit);
+  h->d_J = dalloc<double>((size_t)nao * nao);
+  h->d_K = dalloc<double>((size_t)nao * nao);
+  h->d_Vxc = dalloc<double>((size_t)nao * nao);
+  h->d_Cf = dalloc<double>((size_t)nao * nao);
+  h->d_sgn = dalloc<double>(nao);
+  h->d_W1 = dalloc<double>((size_t)nao * nao);
+  h->d_W2 = dalloc<double>((size_t)nao * nao);
+  h->d_evals = dalloc<double>(nao);
+  h->d_offd = dalloc<double>(nao);
+  h->d_sel = dalloc<int32_t>(nao);
+  sync(st);
+  h->built = true;
+}
+
+// ---------------------------------------------------------------------------
+// energy + Fock for P = sum_i s_i c_i c_i^T (Cf column-major nao x nf)
+// ---------------------------------------------------------------------------
+void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const std::vector<double>& sgn, int nf,
+                          double* E, double* F, hipStream_t st) {
+  if (!h->built) fail(6, "dfks_build has not been called");
+  RBCHK(rocblas_set_stream(h->rb, st));
+  const int nao = h->nao, nfit = h->nfit;
+  const int64_t nn = (int64_t)nao * nao;
+  const double one = 1.0, zero = 0.0;
+  HIPCHK(hipMemsetAsync(E, 0, 8 * sizeof(double), st));
+  // one-electron energy
+  HIPCHK(launch_dot(nn, P, h->d_H, h->d_part, 1.0, E + DFKS_E_ONE, 0, st));
+  // signs to the device (host copy kept in the handle until the next call)
+  if (nf > nao) fail(1, "more factor columns than basis functions");
+  h->h_sgn.assign(sgn.begin(), sgn.end());
+  HIPCHK(hipMemcpyAsync(h->d_sgn, h->h_sgn.data(), sizeof(double) * nf, hipMemcpyHostToDevice, st));
+  // Coulomb + exchange over runs of equal sign, chunked to nf_chunk columns
+  bool first = true;
+  HIPCHK(hipMemsetAsync(h->d_gamma, 0, sizeof(double) * nfit, st));
+  HIPCHK(hipMemsetAsync(h->d_K, 0, sizeof(double) * nn, st));
+  int i0 = 0;
+  while (i0 < nf) {
+    int i1 = i0 + 1;
+    while (i1 < nf && i1 - i0 < h->nf_chunk && sgn[i1] == sgn[i0]) ++i1;
+    const int nc = i1 - i0;
+    const double sg = sgn[i0];
+    // T[m,Q,i] = sum_l B[l,m,Q] Cf[l,i]
+    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc,
+                           (int64_t)nao * nfit, nao, &one, Cf + (int64_t)i0 * nao, nao, h->d_B,
+                           (int64_t)nao * nfit, &zero, h->d_Tx, nc));
+    HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, 1, st));
+    if (h->hf != 0.0) {
+      const double beta = first ? 0.0 : 1.0;
+      RBCHK(rocblas_dsyrk_64(h->rb, rocblas_fill_upper, rocblas_operation_transpose, nao, (int64_t)nfit * nc, &sg,
+                             h->d_Tx, (int64_t)nfit * nc, &beta, h->d_K, nao));
+    }
+    first = false;
+    i0 = i1;
+  }
+  // E_J = 1/2 |gamma|^2, J = B gamma
+  HIPCHK(launch_dot(nfit, h->d_gamma, h->d_gamma, h->d_part, 0.5, E + DFKS_E_J, 0, st));
+  HIPCHK(launch_jvec(nn, nfit, h->d_B, h->d_gamma, h->d_J, st));
+  // E_K = -1/4 a_x sum P o K
+  if (h->hf != 0.0) {
+    HIPCHK(launch_symfill(nao, h->d_K, st));
+    HIPCHK(launch_dot(nn, P, h->d_K, h->d_part, -0.25 * h->hf, E + DFKS_E_K, 0, st));
+  }
+  // XC
+  if (h->ng > 0) {
+    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, nao, h->ng, nao, &one, P, nao,
+                           h->d_ao, nao, &zero, h->d_zw, nao));
+    HIPCHK(launch_xc(h->ng, nao, h->d_ao, h->d_gw, h->d_zw, h->xc_id, h->opt.rho_thresh, h->d_exc, nullptr, st));
+    HIPCHK(launch_dot(h->ng, h->d_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
+    RBCHK(rocblas_dsyr2k_64(h->rb, rocblas_fill_upper, rocblas_operation_none, nao, h->ng, &one, h->d_ao, nao,
+                            h->d_zw, nao, &zero, h->d_Vxc, nao));
+  }
+  HIPCHK(launch_fock(nao, h->d_H, h->d_J, h->hf != 0.0 ? h->d_K : nullptr, -0.5 * h->hf,
+                     h->ng > 0 ? h->d_Vxc : nullptr, F, st));
+  // totals: E_tot = e1 + eJ + eK + exc + enn
+  HIPCHK(hipMemcpyAsync(E + DFKS_E_NN, &h->e_nn, sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(launch_dot(5, E + 1, nullptr, h->d_part, 1.0, E + DFKS_E_TOT, 0, st));
+}
+
+// signed eigen-factorisation of a symmetric P (positive columns first)
+int factor_density(dfks_ctx* h, const double* P, hipStream_t st) {
+  const int nao = h->nao;
+  RBCHK(rocblas_set_stream(h->rb, st));
+  HIPCHK(hipMemcpyAsync(h->d_W1, P, sizeof(double) * nao * nao, hipMemcpyDeviceToDevice, st));
+  RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, nao, h->d_W1, nao, h->d_evals, h->d_offd,
+                         h->d_info));
+  std::vector<double> lam(nao);
+  HIPCHK(hipMemcpyAsync(lam.data(), h->d_evals, sizeof(double) * nao, hipMemcpyDeviceToHost, st));
+  sync(st);
+  double amax = 0.0;
+  for (double v : lam) amax = std::max(amax, std::fabs(v));
+  std::vector<int32_t> sel;
+  std::vector<double> sg;
+  for (int i = nao - 1; i >= 0; --i)
+    if (lam[i] > 1e-14 * amax) { sel.push_back(i); sg.push_back(1.0); }
+  for (int i = 0; i < nao; ++i)
+    if (lam[i] < -1e-14 * amax) { sel.push_back(i); sg.push_back(-1.0); }
+  const int nf = (int)sel.size();
+  HIPCHK(hipMemcpyAsync(h->d_sel, sel.data(), sizeof(int32_t) * std::max(nf, 1), hipMemcpyHostToDevice, st));
+  HIPCHK(launch_factor_cols(nao, nf, h->d_W1, h->d_evals, h->d_sel, h->d_Cf, h->d_sgn, st));
+  sync(st);
+  return nf;
+}
+
+void energy_fock_general(dfks_ctx* h, const double* P, double* E, double* F, hipStream_t st) {
+  int nf = factor_density(h, P, st);
+  std::vector<double> sg(nf);
+  std::vector<double> dev(nf);
+  HIPCHK(hipMemcpy(dev.data(), h->d_sgn, sizeof(double) * std::max(nf, 1), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nf; ++i) sg[i] = dev[i];
+  if (nf == 0) {
+    sg.push_back(1.0);
+    HIPCHK(hipMemsetAsync(h->d_Cf, 0, sizeof(double) * h->nao, st));
+    nf = 1;
+  }
+  energy_fock_factored(h, P, h->d_Cf, sg, nf, E, F, st);
+}
+
+// ---------------------------------------------------------------------------
+// SCF
+// ---------------------------------------------------------------------------
+void gemm(dfks_ctx* h, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+          const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  RBCHK(rocblas_dgemm_64(h->rb, ta ? rocblas_operation_transpose : rocblas_operation_none,
+                         tb ? rocblas_operation_transpose : rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb,
+                         &beta, C, ldc));
+}
+
+void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level_shift, hipStream_t st) {
+  if (!h->built) fail(6, "dfks_build has not been called");
+  if (m < 1 || m > 16) fail(1, "diis_space must be in 1..16");
+  RBCHK(rocblas_set_stream(h->rb, st));
+  const int nao = h->nao;
+  const int64_t nn = (int64_t)nao * nao;
+  // canonical orthonormaliser X = U_keep s_keep^{-1/2} (dftax/ks/scf.py:72-86)
+  HIPCHK(hipMemcpyAsync(h->d_W1, h->d_S, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+  RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, nao, h->d_W1, nao, h->d_evals, h->d_offd,
+                         h->d_info));
+  std::vector<double> s(nao);
+  HIPCHK(hipMemcpyAsync(s.data(), h->d_evals, sizeof(double) * nao, hipMemcpyDeviceToHost, st));
+  sync(st);
+  int k0 = 0;
+  while (k0 < nao && !(s[k0] > lindep)) ++k0;
+  const int nmo = nao - k0;
+  if (nmo < h->nocc) fail(7, "orthonormaliser keeps fewer orbitals than occupied");
+  h->nmo = nmo;
+  h->m = m;
+  h->it = 0;
+  h->level_shift = level_shift;
+  auto realloc = [](double*& p, size_t n) { dfree(p); p = dalloc<double>(n); };
+  realloc(h->d_X, (size_t)nao * nmo);
+  HIPCHK(launch_scaled_cols(nao, k0, h->d_W1, h->d_evals, h->d_X, st));
+  realloc(h->d_P, nn);
+  realloc(h->d_F, nn);
+  realloc(h->d_Fx, nn);
+  realloc(h->d_Fs, (size_t)m * nn);
+  realloc(h->d_Es, (size_t)m * nmo * nmo);
+  realloc(h->d_gram, (size_t)m * m);
+  realloc(h->d_coef, m);
+  realloc(h->d_dots, m);
+  realloc(h->d_C, (size_t)nao * nmo);
+  realloc(h->d_eps, nmo);
+  realloc(h->d_Ws, (size_t)nao * nao);
+  HIPCHK(hipMemsetAsync(h->d_gram, 0, sizeof(double) * m * m, st));
+  HIPCHK(hipMemsetAsync(h->d_Fs, 0, sizeof(double) * m * nn, st));
+  HIPCHK(hipMemsetAsync(h->d_Es, 0, sizeof(double) * m * nmo * nmo, st));
+  HIPCHK(hipMemsetAsync(h->d_coef, 0, sizeof(double) * m, st));
+  HIPCHK(hipMemcpyAsync(h->d_P, P0, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+  energy_fock_general(h, h->d_P, h->d_Edev, h->d_F, st);
+  double e[8];
+  HIPCHK(hipMemcpyAsync(e, h->d_Edev, sizeof(e), hipMemcpyDeviceToHost, st));
+  sync(st);
+  h->e_prev = e[0];
+}
+
+// One iteration of dftax/ks/scf.py:343-379 on the device.
+void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
+  if (!h->d_X) fail(6, "dfks_scf_begin has not been called");
+  RBCHK(rocblas_set_stream(h->rb, st));
+  const int nao = h->nao, nmo = h->nmo, m = h->m;
+  const int64_t nn = (int64_t)nao * nao, mm = (int64_t)nmo * nmo;
+  const int slot = h->it % m;
+  const int count = std::min(h->it + 1, m);
+  double* err = h->d_Es + slot * mm;
+  // err = X^T (F P S - S P F) X ; S P F = (F P S)^T for symmetric F, P, S
+  gemm(h, false, false, nao, nao, nao, 1.0, h->d_F, nao, h->d_P, nao, 0.0, h->d_W1, nao);
+  gemm(h, false, false, nao, nao, nao, 1.0, h->d_W1, nao, h->d_S, nao, 0.0, h->d_W2, nao);
+  HIPCHK(launch_antisym(nao, h->d_W2, h->d_W1, st));
+  gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_W1, nao, 0.0, h->d_W2, nmo);
+  gemm(h, false, false, nmo, nmo, nao, 1.0, h->d_W2, nmo, h->d_X, nao, 0.0, err, nmo);
+  // DIIS history
+  HIPCHK(hipMemcpyAsync(h->d_Fs + slot * nn, h->d_F, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+  for (int i = 0; i < count; ++i)
+    HIPCHK(launch_dot(mm, err, h->d_Es + i * mm, h->d_part, 1.0, h->d_dots + i, 0, st));
+  HIPCHK(launch_gram_update(m, slot, count, h->d_dots, h->d_gram, st));
+  HIPCHK(launch_diis_solve(m, count, h->d_gram, h->d_coef, st));
+  HIPCHK(launch_diis_extrap(m, nn, h->d_coef, h->d_Fs, h->d_Fx, st));
+  if (h->level_shift != 0.0) {
+    // F += b (S - 1/2 S P S)
+    gemm(h, false, false, nao, nao, nao, 1.0, h->d_S, nao, h->d_P, nao, 0.0, h->d_W1, nao);
+    gemm(h, false, false, nao, nao, nao, -0.5 * h->level_shift, h->d_W1, nao, h->d_S, nao, 1.0, h->d_Fx, nao);
+    HIPCHK(launch_axpy(nn, h->level_shift, h->d_S, h->d_Fx, 1, st));
+  }
+  // eigh(X^T F X), C = X C', P = 2 C_occ C_occ^T
+  gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_Fx, nao, 0.0, h->d_W2, nmo);
+  gemm(h, false, false, nmo, nmo, nao, 1.0, h->d_W2, nmo, h->d_X, nao, 0.0, h->d_Ws, nmo);
+  RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, nmo, h->d_Ws, nmo, h->d_eps, h->d_offd,
+                         h->d_info));
+  gemm(h, false, false, nao, nmo, nmo, 1.0, h->d_X, nao, h->d_Ws, nmo, 0.0, h->d_C, nao);
+  const int nocc = h->nocc;
+  HIPCHK(launch_axpy((int64_t)nao * nocc, std::sqrt(2.0), h->d_C, h->d_Cf, 0, st));
+  gemm(h, false, true, nao, nao, nocc, 1.0, h->d_Cf, nao, h->d_Cf, nao, 0.0, h->d_P, nao);
+  std::vector<double> sg(nocc, 1.0);
+  energy_fock_factored(h, h->d_P, h->d_Cf, sg, nocc, h->d_Edev, h->d_F, st);
+  // derr = |err|_F
+  HIPCHK(launch_dot(mm, err, err, h->d_part, 1.0, h->d_Edev + 7, 0, st));
+  double e[8];
+  HIPCHK(hipMemcpyAsync(e, h->d_Edev, sizeof(e), hipMemcpyDeviceToHost, st));
+  int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+  sync(st);
+  if (info != 0) fail(5, "Fock eigensolver failed, info=" + std::to_string(info));
+  out3[0] = e[0];
+  out3[1] = e[0] - h->e_prev;
+  out3[2] = std::sqrt(e[7]);
+  h->e_prev = e[0];
+  h->it += 1;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* dfks_last_error(void) { return g_err.c_str(); }
+
+int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handle* out) {
+  return guard([&] {
+    if (!sys || !out) fail(1, "null argument");
+    std::lock_guard<std::mutex> lk(g_mu);
+    static bool tables = false;
+    if (!tables) {
+      upload_c2s_tables();
+      tables = true;
+    }
+    auto* h = new dfks_ctx();
+    try {
+      if (opt) h->opt = *opt;
+      else h->opt = dfks_options{0, 1e-7, 1e-10, 0};
+      if (h->opt.precision != 0) fail(1, "only precision=0 (FP64) is implemented in this build");
+      h->xc_id = sys->xc_id;
+      h->hf = hf_coeff(sys->xc_id);
+      if (sys->nelec <= 0 || sys->nelec % 2) fail(1, "closed shell needs an even, positive electron count");
+      h->nocc = sys->nelec / 2;
+      h->orb.load(sys->orbital);
+      h->aux.load(sys->aux);
+      if (h->orb.maxl() > 3) fail(1, "orbital shells above f (l > 3) are not supported");
+      if (h->aux.maxl() > 4) fail(1, "auxiliary shells above g (l > 4) are not supported");
+      h->nao = h->orb.nfunc;
+      h->naux = h->aux.nfunc;
+      if (h->nocc > h->nao) fail(1, "more occupied orbitals than basis functions");
+      h->natom = sys->natom;
+      h->atm_host.resize(4 * std::max(1, sys->natom));
+      double enn = 0.0;
+      for (int a = 0; a < sys->natom; ++a) {
+        for (int x = 0; x < 3; ++x) h->atm_host[4 * a + x] = sys->atom_xyz[3 * a + x];
+        h->atm_host[4 * a + 3] = sys->atom_z[a];
+        for (int b = 0; b < a; ++b) {
+          double d2 = 0;
+          for (int x = 0; x < 3; ++x) {
+            double d = sys->atom_xyz[3 * a + x] - sys->atom_xyz[3 * b + x];
+            d2 += d * d;
+          }
+          enn += sys->atom_z[a] * sys->atom_z[b] / std::sqrt(d2);
+        }
+      }
+      h->e_nn = enn;
+      h->d_atm = dalloc<double>(h->atm_host.size());
+      HIPCHK(hipMemcpy(h->d_atm, h->atm_host.data(), h->atm_host.size() * sizeof(double), hipMemcpyHostToDevice));
+      std::vector<double> tbl = boys_table();
+      h->d_boys = dalloc<double>(tbl.size());
+      HIPCHK(hipMemcpy(h->d_boys, tbl.data(), tbl.size() * sizeof(double), hipMemcpyHostToDevice));
+      h->ng = sys->ngrid;
+      if (h->ng > 0) {
+        h->d_gxyz = dalloc<double>(3 * h->ng);
+        h->d_gw = dalloc<double>(h->ng);
+        HIPCHK(hipMemcpy(h->d_gxyz, sys->grid_xyz, 3 * h->ng * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_gw, sys->grid_w, h->ng * sizeof(double), hipMemcpyHostToDevice));
+      }
+      h->d_part = dalloc<double>(1024);
+      h->d_Edev = dalloc<double>(8);
+      h->d_info = dalloc<int32_t>(1);
+      if (rocblas_create_handle(&h->rb) != rocblas_status_success) fail(3, "rocblas_create_handle failed");
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int dfks_destroy(dfks_handle h) {
+  return guard([&] { delete h; });
+}
+
+int dfks_build(dfks_handle h, dfks_stream stream) {
+  return guard([&] {
+    if (!h) fail(1, "null handle");
+    if (h->built) fail(1, "handle already built");
+    build_all(h, S_(stream));
+  });
+}
+
+int dfks_sizes(dfks_handle h, int64_t* out5) {
+  return guard([&] {
+    if (!h || !out5) fail(1, "null argument");
+    out5[0] = h->nao;
+    out5[1] = h->naux;
+    out5[2] = h->nfit;
+    out5[3] = h->nocc;
+    out5[4] = h->ng;
+  });
+}
+
+int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream stream) {
+  return guard([&] {
+    if (!h || !dev_out) fail(1, "null argument");
+    if (!h->built) fail(6, "dfks_build has not been called");
+    const size_t nn = (size_t)h->nao * h->nao;
+    const double* src = nullptr;
+    size_t n = 0;
+    switch (which) {
+      case DFKS_S: src = h->d_S; n = nn; break;
+      case DFKS_HCORE: src = h->d_H; n = nn; break;
+      case DFKS_KINETIC: src = h->d_T; n = nn; break;
+      case DFKS_NUCLEAR: src = h->d_V; n = nn; break;
+      case DFKS_INT2C: src = h->d_int2c; n = (size_t)h->naux * h->naux; break;
+      case DFKS_INT3C:
+        if (!h->d_int3c) fail(1, "int3c was not kept (options.keep_int3c = 0)");
+        src = h->d_int3c; n = nn * h->naux; break;
+      case DFKS_BFIT: src = h->d_B; n = nn * h->nfit; break;
+      case DFKS_AO: src = h->d_ao; n = 4 * (size_t)h->ng * h->nao; break;
+      case DFKS_GRID_W: src = h->d_gw; n = (size_t)h->ng; break;
+      default: fail(1, "unknown matrix selector");
+    }
+    HIPCHK(hipMemcpyAsync(dev_out, src, n * sizeof(double), hipMemcpyDeviceToDevice, S_(stream)));
+  });
+}
+
+int dfks_energy_fock(dfks_handle h, const double* P, double* E, double* F, dfks_stream stream) {
+  return guard([&] {
+    if (!h || !P || !E || !F) fail(1, "null argument");
+    energy_fock_general(h, P, E, F, S_(stream));
+  });
+}
+
+int dfks_energy_fock_factored(dfks_handle h, const double* P, const double* Cf, const double* signs_host, int32_t nf,
+                              double* E, double* F, dfks_stream stream) {
+  return guard([&] {
+    if (!h || !P || !Cf || !signs_host || !E || !F) fail(1, "null argument");
+    if (nf <= 0) fail(1, "nf must be positive");
+    std::vector<double> sg(signs_host, signs_host + nf);
+    for (double s : sg)
+      if (s != 1.0 && s != -1.0) fail(1, "signs must be +1 or -1");
+    energy_fock_factored(h, P, Cf, sg, nf, E, F, S_(stream));
+  });
+}
+
+int dfks_xc_eval(int32_t xc_id, int64_t n, const double* rho, const double* sigma, double* out3, dfks_stream stream) {
+  return guard([&] {
+    hf_coeff(xc_id);
+    HIPCHK(launch_xc_pointwise(n, xc_id, rho, sigma, out3, S_(stream)));
+  });
+}
+
+int dfks_scf_begin(dfks_handle h, const double* P0, double lindep, int32_t diis_space, double level_shift,
+                   dfks_stream stream) {
+  return guard([&] {
+    if (!h || !P0) fail(1, "null argument");
+    scf_begin(h, P0, lindep, diis_space, level_shift, S_(stream));
+  });
+}
+
+int dfks_scf_step(dfks_handle h, double* out3, dfks_stream stream) {
+  return guard([&] {
+    if (!h || !out3) fail(1, "null argument");
+    scf_step(h, out3, S_(stream));
+  });
+}
+
+int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* nmo, dfks_stream stream) {
+  return guard([&] {
+    if (!h) fail(1, "null handle");
+    if (!h->d_X) fail(6, "dfks_scf_begin has not been called");
+    hipStream_t st = S_(stream);
+    if (P) HIPCHK(hipMemcpyAsync(P, h->d_P, sizeof(double) * h->nao * h->nao, hipMemcpyDeviceToDevice, st));
+    if (C) HIPCHK(hipMemcpyAsync(C, h->d_C, sizeof(double) * h->nao * h->nmo, hipMemcpyDeviceToDevice, st));
+    if (eps) HIPCHK(hipMemcpyAsync(eps, h->d_eps, sizeof(double) * h->nmo, hipMemcpyDeviceToDevice, st));
+    if (nmo) *nmo = h->nmo;
+  });
+}
+
+int dfks_overlap(const dfks_basis_desc* basis, double* host_out) {
+  return guard([&] {
+    if (!basis || !host_out) fail(1, "null argument");
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      static bool tables = false;
+      if (!tables) {
+        upload_c2s_tables();
+        tables = true;
+      }
+    }
+    Basis b;
+    b.load(*basis);
+    struct Rel {
+      Basis* b;
+      ~Rel() { b->release(); }
+    } rel{&b};
+    const int64_t n = b.nfunc;
+    double *dS = dalloc<double>(n * n), *dT = dalloc<double>(n * n), *dV = dalloc<double>(n * n);
+    double* dummy = dalloc<double>(kBoysNg * kBoysN);
+    struct Free {
+      std::vector<void*> p;
+      ~Free() { for (void* q : p) dfree(q); }
+    } fr{{dS, dT, dV, dummy}};
+    HIPCHK(launch_int1e(b.view(), 0, dummy, dummy, dS, dT, dV, n, nullptr));
+    HIPCHK(hipMemcpy(host_out, dS, n * n * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int dfks_becke_weights(int64_t n, const double* pts, const double* raw_w, const int32_t* owner, int32_t natom,
+                       const double* atom_xyz, const double* bragg, double* w_out) {
+  return guard([&] {
+    if (n < 0 || natom <= 0) fail(1, "bad sizes");
+    if (n == 0) return;
+    for (int64_t i = 0; i < n; ++i)
+      if (owner[i] < 0 || owner[i] >= natom) fail(1, "owner atom index out of range");
+    std::vector<double> invR((size_t)natom * natom, 0.0), aAB((size_t)natom * natom, 0.0);
+    for (int a = 0; a < natom; ++a)
+      for (int b = 0; b < natom; ++b) {
+        double d2 = 0;
+        for (int x = 0; x < 3; ++x) {
+          double d = atom_xyz[3 * a + x] - atom_xyz[3 * b + x];
+          d2 += d * d;
+        }
+        double R = std::sqrt(d2 + (a == b ? 1.0 : 0.0));
+        invR[(size_t)a * natom + b] = 1.0 / R;
+        double chi = bragg[a] / bragg[b];
+        double u = (chi - 1.0) / (chi + 1.0);
+        double av = u / (u * u - 1.0);
+        aAB[(size_t)a * natom + b] = std::min(0.45, std::max(-0.45, av));
+      }
+    double *d_pts = dalloc<double>(3 * n), *d_raw = dalloc<double>(n), *d_w = dalloc<double>(n);
+    int32_t* d_own = dalloc<int32_t>(n);
+    double *d_atm = dalloc<double>(3 * natom), *d_invR = dalloc<double>(invR.size()), *d_a = dalloc<double>(aAB.size());
+    struct Free {
+      std::vector<void*> p;
+      ~Free() { for (void* q : p) dfree(q); }
+    } fr{{d_pts, d_raw, d_w, d_own, d_atm, d_invR, d_a}};
+    HIPCHK(hipMemcpy(d_pts, pts, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_raw, raw_w, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_own, owner, n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_atm, atom_xyz, 3 * natom * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_invR, invR.data(), invR.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_a, aAB.data(), aAB.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(launch_becke(n, d_pts, d_raw, d_own, natom, d_atm, d_invR, d_a, d_w, nullptr));
+    HIPCHK(hipMemcpy(w_out, d_w, n * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

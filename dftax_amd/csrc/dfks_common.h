@@ -1,0 +1,110 @@
+// Shared device-side definitions for the DF-KS library (gfx950, FP64 path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "c2s_tables.h"
+
+namespace dfks {
+
+constexpr int kLMax = 4;              // largest l of an orbital shell (def2-TZVP f; aux g)
+constexpr int kBoysNg = 401;          // table nodes t = 0, 0.1, ..., 40
+constexpr int kBoysN = 25;            // F_0 .. F_24 tabulated
+constexpr int kBoysOrder = 6;         // local Taylor degree
+constexpr double kBoysDt = 0.1;
+constexpr double kBoysTmax = 40.0;
+constexpr double kPi = 3.14159265358979323846;
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// Shell-major basis on the device (structure of arrays).
+struct ShellSet {
+  int32_t n;              // shells
+  const int32_t* l;
+  const int32_t* nprim;
+  const int32_t* prim0;
+  const int32_t* ao0;     // first function in the final (spherical or cartesian) span
+  const double* centre;   // [3n]
+  const double* exps;
+  const double* coefs;    // primitive norm (and the l<=1 contracted norm) folded in
+  int32_t spherical;
+};
+
+// Final-span size of a shell.
+__host__ __device__ inline int nfun(int l, int spherical) {
+  return (spherical && l >= 2) ? nsph(l) : ncart(l);
+}
+
+// Compile-time cartesian component table.
+template <int L>
+__device__ __forceinline__ constexpr int cart_exp(int c, int x) {
+  if constexpr (L == 0) return kCart0[c][x];
+  else if constexpr (L == 1) return kCart1[c][x];
+  else if constexpr (L == 2) return kCart2[c][x];
+  else if constexpr (L == 3) return kCart3[c][x];
+  else if constexpr (L == 4) return kCart4[c][x];
+  else if constexpr (L == 5) return kCart5[c][x];
+  else return kCart6[c][x];
+}
+
+template <int L>
+__device__ __forceinline__ constexpr double c2s_coef(int c, int s) {
+  if constexpr (L == 0) return kC2S0[c][s];
+  else if constexpr (L == 1) return kC2S1[c][s];
+  else if constexpr (L == 2) return kC2S2[c][s];
+  else if constexpr (L == 3) return kC2S3[c][s];
+  else if constexpr (L == 4) return kC2S4[c][s];
+  else if constexpr (L == 5) return kC2S5[c][s];
+  else return kC2S6[c][s];
+}
+
+// F_m(T) for m = 0..NMAX. Top order by the tabulated degree-6 Taylor expansion
+// (nearest node of a 0.1-spaced grid), lower orders by the stable downward
+// recursion F_{m-1} = (2T F_m + e^{-T}) / (2m - 1); beyond T = 40 the
+// asymptote Gamma(m+1/2) / (2 T^{m+1/2}) with its upward recursion.
+template <int NMAX>
+__device__ __forceinline__ void boys_array(double T, double* F, const double* __restrict__ tbl) {
+  if (T < kBoysTmax) {
+    int idx = __double2int_rn(T * 10.0);
+    double dx = idx * kBoysDt - T;
+    const double* row = tbl + idx * kBoysN + NMAX;
+    double acc = row[kBoysOrder];
+#pragma unroll
+    for (int k = kBoysOrder - 1; k >= 0; --k) acc = row[k] + acc * dx * (1.0 / (k + 1));
+    F[NMAX] = acc;
+    if constexpr (NMAX > 0) {
+      double e = exp(-T);
+      double t2 = 2.0 * T;
+#pragma unroll
+      for (int m = NMAX; m > 0; --m) F[m - 1] = (t2 * F[m] + e) * (1.0 / (2 * m - 1));
+    }
+  } else {
+    double it = 1.0 / T;
+    F[0] = 0.88622692545275801365 * sqrt(it);  // sqrt(pi)/2 / sqrt(T)
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m) F[m + 1] = F[m] * (2 * m + 1) * 0.5 * it;
+  }
+}
+
+// Runtime-order variant (same algorithm) for kernels without a fixed order.
+__device__ __forceinline__ void boys_array_rt(int nmax, double T, double* F, const double* __restrict__ tbl) {
+  if (T < kBoysTmax) {
+    int idx = __double2int_rn(T * 10.0);
+    double dx = idx * kBoysDt - T;
+    const double* row = tbl + idx * kBoysN + nmax;
+    double acc = row[kBoysOrder];
+    for (int k = kBoysOrder - 1; k >= 0; --k) acc = row[k] + acc * dx / (k + 1);
+    F[nmax] = acc;
+    double e = exp(-T);
+    for (int m = nmax; m > 0; --m) F[m - 1] = (2.0 * T * F[m] + e) / (2 * m - 1);
+  } else {
+    double it = 1.0 / T;
+    F[0] = 0.88622692545275801365 * sqrt(it);
+    for (int m = 0; m < nmax; ++m) F[m + 1] = F[m] * (2 * m + 1) * 0.5 * it;
+  }
+}
+
+}  // namespace dfks

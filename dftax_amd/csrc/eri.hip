@@ -1,0 +1,435 @@
+// McMurchie-Davidson 1-, 2- and 3-centre integrals on gfx950 (FP64).
+//
+// Algorithm (dftax/integrals/eri3c_bucketed.py:290-426, 697-734):
+//   (ab|c) = K_AB 2 pi^2.5 / (p q sqrt(p+q)) *
+//            sum_{tuv} E^{ab}_t E^{ab}_u E^{ab}_v
+//            sum_{tau nu phi} (-1)^{tau+nu+phi} E^c_tau E^c_nu E^c_phi R_{t+tau,u+nu,v+phi}
+// with the Hermite E tables of the bra pair, the single-centre E table of the
+// auxiliary Gaussian, and R built by the descending m-ladder from
+// (-2 rho)^m F_m(rho |PC|^2).
+//
+// Work decomposition: one 64-lane wavefront per task (bra shell pair x aux
+// shell). The R ladder is built cooperatively in LDS in compact (t+u+v <= L)
+// indexing, each lane accumulates a strided subset of the cartesian outputs
+// in an LDS block, and the cart->sph transform reads that block with sparse
+// coefficient lists before the result is written (both bra triangles for 3c).
+// Limits: orbital l <= 3, auxiliary l <= 4 (LDS sized for (ff|g)).
+#include <cstring>
+
+#include "dfks_common.h"
+
+namespace dfks {
+
+constexpr int kWave = 64;
+constexpr int kMaxL3 = 3 + 3 + 4;                       // (ff|g)
+constexpr int kNR = (kMaxL3 + 1) * (kMaxL3 + 2) * (kMaxL3 + 3) / 6;  // 286
+constexpr int kMaxCartBlk = 10 * 10 * 15;
+constexpr int kMaxOutPerLane = (kMaxCartBlk + kWave - 1) / kWave;     // 24
+
+__device__ __forceinline__ int ridx(int t, int u, int v) {
+  int s = t + u + v;
+  int r = s - t;
+  return s * (s + 1) * (s + 2) / 6 + r * (r + 1) / 2 + v;
+}
+
+// Sparse cart->sph lists: for l and spherical index s, up to 12 (cart, coef).
+struct C2SList {
+  int n[7][13];
+  int cart[7][13][12];
+  double coef[7][13][12];
+};
+__constant__ C2SList g_c2s;
+
+// cartesian exponent table for runtime l (l <= 6)
+__constant__ int g_cart[7][28][3];
+
+__device__ __forceinline__ int cexp(int l, int c, int x) { return g_cart[l][c][x]; }
+
+struct Task3 {
+  const int2* pairs;     // (ia, ib) per bra pair of this class
+  const int32_t* aux;    // aux shell ids of this lc
+  int npair, naux_l;
+};
+
+// E^{ij}_t for one axis into a [(la+1)][(lb+1)][mt] LDS block (lanes 0..2 call it).
+__device__ void e_table_1d(int la, int lb, double al, double be, double xab, int mt, double* E) {
+  double p = al + be, i2g = 0.5 / p, xpa = -be * xab / p, xpb = al * xab / p;
+  int nb = lb + 1;
+  for (int i = 0; i <= la; ++i)
+    for (int j = 0; j <= lb; ++j)
+      for (int t = 0; t < mt; ++t) E[(i * nb + j) * mt + t] = 0.0;
+  E[0] = 1.0;
+  for (int i = 0; i < la; ++i)
+    for (int t = 0; t <= i + 1 && t < mt; ++t) {
+      double v = xpa * E[(i * nb) * mt + t];
+      if (t > 0) v += i2g * E[(i * nb) * mt + t - 1];
+      if (t + 1 < mt) v += (t + 1) * E[(i * nb) * mt + t + 1];
+      E[((i + 1) * nb) * mt + t] = v;
+    }
+  for (int j = 0; j < lb; ++j)
+    for (int i = 0; i <= la; ++i)
+      for (int t = 0; t <= i + j + 1 && t < mt; ++t) {
+        double v = xpb * E[(i * nb + j) * mt + t];
+        if (t > 0) v += i2g * E[(i * nb + j) * mt + t - 1];
+        if (t + 1 < mt) v += (t + 1) * E[(i * nb + j) * mt + t + 1];
+        E[(i * nb + j + 1) * mt + t] = v;
+      }
+}
+
+// Build R^{(0)}_{tuv}, t+u+v <= L, in LDS (compact). buf0/buf1 ping-pong.
+// Returns a pointer to the buffer that holds level 0.
+__device__ double* r_ladder(int L, double rho, double pcx, double pcy, double pcz, const double* F,
+                            double* buf0, double* buf1) {
+  const int lane = threadIdx.x;
+  double* prev = buf0;
+  double* cur = buf1;
+  double n2r = -2.0 * rho;
+  // level L: only R_000
+  if (lane == 0) {
+    double pw = 1.0;
+    for (int k = 0; k < L; ++k) pw *= n2r;
+    prev[0] = pw * F[L];
+  }
+  __syncthreads();
+  for (int m = L - 1; m >= 0; --m) {
+    int top = L - m;                           // highest t+u+v at this level
+    int count = (top + 1) * (top + 2) * (top + 3) / 6;
+    for (int e = lane; e < count; e += kWave) {
+      // decode compact index e -> (t,u,v)
+      int s = 0;
+      while ((s + 1) * (s + 2) * (s + 3) / 6 <= e) ++s;
+      int rem = e - s * (s + 1) * (s + 2) / 6;
+      int r = 0;
+      while ((r + 1) * (r + 2) / 2 <= rem) ++r;
+      int v = rem - r * (r + 1) / 2;
+      int t = s - r, u = r - v;
+      double val;
+      if (s == 0) {
+        double pw = 1.0;
+        for (int k = 0; k < m; ++k) pw *= n2r;
+        val = pw * F[m];
+      } else if (t > 0) {
+        val = pcx * prev[ridx(t - 1, u, v)] + (t > 1 ? (t - 1) * prev[ridx(t - 2, u, v)] : 0.0);
+      } else if (u > 0) {
+        val = pcy * prev[ridx(t, u - 1, v)] + (u > 1 ? (u - 1) * prev[ridx(t, u - 2, v)] : 0.0);
+      } else {
+        val = pcz * prev[ridx(t, u, v - 1)] + (v > 1 ? (v - 1) * prev[ridx(t, u, v - 2)] : 0.0);
+      }
+      cur[e] = val;
+    }
+    __syncthreads();
+    double* tmp = prev;
+    prev = cur;
+    cur = tmp;
+  }
+  return prev;
+}
+
+// Generic 2-/3-centre kernel. For the 2-centre metric the bra partner b is a
+// unit s function of zero exponent on A (b_unit != 0), which makes (a b|c)
+// the two-centre (a|c).
+// Output index: ao_a*sa + ao_b*sb + ao_c (+ transposed bra write if tr).
+__global__ __launch_bounds__(64) void k_eri_wave(
+    int la, int lb, int lc, Task3 task, ShellSet A, ShellSet C, int b_unit,
+    const double* __restrict__ boys, double* __restrict__ out, int64_t sa, int64_t sb, int tr, int64_t blk0) {
+  __shared__ double s_E[3][4 * 4 * 8];
+  __shared__ double s_Ec[5 * 11];
+  __shared__ double s_F[12];
+  __shared__ double s_R[2][kNR];
+  __shared__ double s_cart[kMaxCartBlk];
+
+  const int lane = threadIdx.x;
+  const int64_t tid = blk0 + blockIdx.x;
+  const int k = (int)(tid / task.naux_l);
+  const int j = (int)(tid % task.naux_l);
+  if (k >= task.npair) return;
+  int2 pr = task.pairs[k];
+  const int ia = pr.x, ib = pr.y, ic = task.aux[j];
+  const int L = la + lb + lc;
+  const int mt = L + 1;
+  const int na = ncart(la), nb = ncart(lb), nc = ncart(lc);
+  const int nout = na * nb * nc;
+
+  double Ax = A.centre[3 * ia], Ay = A.centre[3 * ia + 1], Az = A.centre[3 * ia + 2];
+  double Bx = Ax, By = Ay, Bz = Az;
+  int npa = A.nprim[ia], npb = 1, p0a = A.prim0[ia], p0b = 0;
+  if (!b_unit) {
+    Bx = A.centre[3 * ib]; By = A.centre[3 * ib + 1]; Bz = A.centre[3 * ib + 2];
+    npb = A.nprim[ib]; p0b = A.prim0[ib];
+  }
+  double Cx = C.centre[3 * ic], Cy = C.centre[3 * ic + 1], Cz = C.centre[3 * ic + 2];
+  double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;
+  double ab2 = ABx * ABx + ABy * ABy + ABz * ABz;
+
+  for (int o = lane; o < nout; o += kWave) s_cart[o] = 0.0;
+
+  const int lab = la + lb;
+  for (int pa = 0; pa < npa; ++pa)
+    for (int pb = 0; pb < npb; ++pb) {
+      double al = A.exps[p0a + pa];
+      double be = b_unit ? 0.0 : A.exps[p0b + pb];
+      double cab = A.coefs[p0a + pa] * (b_unit ? 1.0 : A.coefs[p0b + pb]);
+      double p = al + be;
+      double Px = (al * Ax + be * Bx) / p, Py = (al * Ay + be * By) / p, Pz = (al * Az + be * Bz) / p;
+      double Kab = exp(-al * be / p * ab2);
+      __syncthreads();
+      if (lane < 3) {
+        double xab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
+        e_table_1d(la, lb, al, be, xab, lab + 1, s_E[lane]);
+      }
+      for (int pc = 0; pc < C.nprim[ic]; ++pc) {
+        double ga = C.exps[C.prim0[ic] + pc];
+        double cc = C.coefs[C.prim0[ic] + pc];
+        double rho = p * ga / (p + ga);
+        double pref = cab * cc * Kab * 2.0 * 17.493418327624862 / (p * ga * sqrt(p + ga));  // 2 pi^2.5
+        double pcx = Px - Cx, pcy = Py - Cy, pcz = Pz - Cz;
+        double T = rho * (pcx * pcx + pcy * pcy + pcz * pcz);
+        __syncthreads();
+        if (lane == 0) {
+          boys_array_rt(L, T, s_F, boys);
+          // single-centre E^k_tau for the aux Gaussian
+          double i2g = 0.5 / ga;
+          for (int q = 0; q < (lc + 1) * mt; ++q) s_Ec[q] = 0.0;
+          s_Ec[0] = 1.0;
+          for (int kk = 0; kk < lc; ++kk)
+            for (int t = 0; t < mt; ++t) {
+              double v = 0.0;
+              if (t > 0) v += i2g * s_Ec[kk * mt + t - 1];
+              if (t + 1 < mt) v += (t + 1) * s_Ec[kk * mt + t + 1];
+              s_Ec[(kk + 1) * mt + t] = v;
+            }
+        }
+        __syncthreads();
+        const double* R = r_ladder(L, rho, pcx, pcy, pcz, s_F, s_R[0], s_R[1]);
+        // contraction
+        const int nbb = lb + 1;
+        const int mtab = lab + 1;
+        for (int o = lane; o < nout; o += kWave) {
+          int ci = o % nc, bi = (o / nc) % nb, ai = o / (nc * nb);
+          int a0 = cexp(la, ai, 0), a1 = cexp(la, ai, 1), a2 = cexp(la, ai, 2);
+          int b0 = cexp(lb, bi, 0), b1 = cexp(lb, bi, 1), b2 = cexp(lb, bi, 2);
+          int c0 = cexp(lc, ci, 0), c1 = cexp(lc, ci, 1), c2 = cexp(lc, ci, 2);
+          const double* Ex = s_E[0] + (a0 * nbb + b0) * mtab;
+          const double* Ey = s_E[1] + (a1 * nbb + b1) * mtab;
+          const double* Ez = s_E[2] + (a2 * nbb + b2) * mtab;
+          const double* Ecx = s_Ec + c0 * mt;
+          const double* Ecy = s_Ec + c1 * mt;
+          const double* Ecz = s_Ec + c2 * mt;
+          double sum = 0.0;
+          for (int t = 0; t <= a0 + b0; ++t) {
+            double et = Ex[t];
+            if (et == 0.0) continue;
+            for (int u = 0; u <= a1 + b1; ++u) {
+              double eu = et * Ey[u];
+              if (eu == 0.0) continue;
+              for (int v = 0; v <= a2 + b2; ++v) {
+                double ev = eu * Ez[v];
+                if (ev == 0.0) continue;
+                double inner = 0.0;
+                for (int ta = c0 & 1; ta <= c0; ta += 2) {
+                  double gx = Ecx[ta];
+                  for (int nu = c1 & 1; nu <= c1; nu += 2) {
+                    double gy = gx * Ecy[nu];
+                    for (int ph = c2 & 1; ph <= c2; ph += 2) {
+                      double g = gy * Ecz[ph];
+                      inner += g * R[ridx(t + ta, u + nu, v + ph)];
+                    }
+                  }
+                }
+                // (-1)^{tau+nu+phi} = (-1)^{c0+c1+c2} since tau == c0 mod 2 etc.
+                sum += ev * inner;
+              }
+            }
+          }
+          s_cart[o] += ((lc & 1) ? -pref : pref) * sum;
+        }
+      }
+    }
+
+  // cart -> final span via LDS
+  __syncthreads();
+  const int sphA = A.spherical && la >= 2, sphB = A.spherical && lb >= 2 && !b_unit, sphC = C.spherical && lc >= 2;
+  const int fa = sphA ? nsph(la) : na, fb = sphB ? nsph(lb) : nb, fc = sphC ? nsph(lc) : nc;
+  const int nf = fa * fb * fc;
+  const int64_t ao_a = A.ao0[ia], ao_b = b_unit ? 0 : A.ao0[ib], ao_c = C.ao0[ic];
+  for (int o = lane; o < nf; o += kWave) {
+    int Ci = o % fc, Bi = (o / fc) % fb, Ai = o / (fc * fb);
+    int nA = sphA ? g_c2s.n[la][Ai] : 1;
+    int nB = sphB ? g_c2s.n[lb][Bi] : 1;
+    int nC = sphC ? g_c2s.n[lc][Ci] : 1;
+    double v = 0.0;
+    for (int x = 0; x < nA; ++x) {
+      int ai = sphA ? g_c2s.cart[la][Ai][x] : Ai;
+      double wa = sphA ? g_c2s.coef[la][Ai][x] : 1.0;
+      for (int y = 0; y < nB; ++y) {
+        int bi = sphB ? g_c2s.cart[lb][Bi][y] : Bi;
+        double wb = wa * (sphB ? g_c2s.coef[lb][Bi][y] : 1.0);
+        for (int z = 0; z < nC; ++z) {
+          int ci = sphC ? g_c2s.cart[lc][Ci][z] : Ci;
+          double wc = wb * (sphC ? g_c2s.coef[lc][Ci][z] : 1.0);
+          v += wc * s_cart[(ai * nb + bi) * nc + ci];
+        }
+      }
+    }
+    out[(ao_a + Ai) * sa + (ao_b + Bi) * sb + ao_c + Ci] = v;
+    if (tr && ia != ib) out[(ao_b + Bi) * sa + (ao_a + Ai) * sb + ao_c + Ci] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One-electron S, T, V: one wave per ordered shell pair, lanes over the
+// cartesian outputs; the nuclear R ladder per (primitive pair, atom) in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_int1e(ShellSet A, int natm, const double* __restrict__ atm,
+                                              const double* __restrict__ boys, double* __restrict__ S,
+                                              double* __restrict__ Tm, double* __restrict__ V, int64_t nao) {
+  __shared__ double s_E[3][4 * 6 * 10];
+  __shared__ double s_F[12];
+  __shared__ double s_R[2][84];
+  __shared__ double s_cart[3][100];
+  const int lane = threadIdx.x;
+  const int ia = blockIdx.x / A.n, ib = blockIdx.x % A.n;
+  const int la = A.l[ia], lb = A.l[ib];
+  const int na = ncart(la), nb = ncart(lb);
+  const int L = la + lb;
+  const int mtS = L + 3;   // kinetic needs j + 2
+  const int nbb = lb + 3;
+  double Ax = A.centre[3 * ia], Ay = A.centre[3 * ia + 1], Az = A.centre[3 * ia + 2];
+  double Bx = A.centre[3 * ib], By = A.centre[3 * ib + 1], Bz = A.centre[3 * ib + 2];
+  double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;
+  double ab2 = ABx * ABx + ABy * ABy + ABz * ABz;
+  const int nout = na * nb;
+  for (int o = lane; o < nout; o += kWave) { s_cart[0][o] = 0.0; s_cart[1][o] = 0.0; s_cart[2][o] = 0.0; }
+  for (int pa = 0; pa < A.nprim[ia]; ++pa)
+    for (int pb = 0; pb < A.nprim[ib]; ++pb) {
+      double al = A.exps[A.prim0[ia] + pa], be = A.exps[A.prim0[ib] + pb];
+      double cab = A.coefs[A.prim0[ia] + pa] * A.coefs[A.prim0[ib] + pb];
+      double p = al + be;
+      double Px = (al * Ax + be * Bx) / p, Py = (al * Ay + be * By) / p, Pz = (al * Az + be * Bz) / p;
+      double Kab = exp(-al * be / p * ab2);
+      __syncthreads();
+      if (lane < 3) {
+        double xab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
+        e_table_1d(la, lb + 2, al, be, xab, mtS, s_E[lane]);
+      }
+      __syncthreads();
+      double s0 = pow(kPi / p, 1.5) * Kab * cab;
+      for (int o = lane; o < nout; o += kWave) {
+        int ai = o / nb, bi = o % nb;
+        double s1[3], t1[3];
+        for (int x = 0; x < 3; ++x) {
+          int i = cexp(la, ai, x), jj = cexp(lb, bi, x);
+          const double* E = s_E[x];
+          double sij = E[(i * nbb + jj) * mtS];
+          double tt = -2.0 * be * be * E[(i * nbb + jj + 2) * mtS] + be * (2 * jj + 1) * sij;
+          if (jj >= 2) tt -= 0.5 * jj * (jj - 1) * E[(i * nbb + jj - 2) * mtS];
+          s1[x] = sij;
+          t1[x] = tt;
+        }
+        s_cart[0][o] += s0 * s1[0] * s1[1] * s1[2];
+        s_cart[1][o] += s0 * (t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2]);
+      }
+      for (int c = 0; c < natm; ++c) {
+        double pcx = Px - atm[4 * c], pcy = Py - atm[4 * c + 1], pcz = Pz - atm[4 * c + 2];
+        double Z = atm[4 * c + 3];
+        __syncthreads();
+        if (lane == 0) boys_array_rt(L, p * (pcx * pcx + pcy * pcy + pcz * pcz), s_F, boys);
+        __syncthreads();
+        const double* R = r_ladder(L, p, pcx, pcy, pcz, s_F, s_R[0], s_R[1]);
+        double pref = -Z * cab * Kab * 2.0 * kPi / p;
+        for (int o = lane; o < nout; o += kWave) {
+          int ai = o / nb, bi = o % nb;
+          int a0 = cexp(la, ai, 0), a1 = cexp(la, ai, 1), a2 = cexp(la, ai, 2);
+          int b0 = cexp(lb, bi, 0), b1 = cexp(lb, bi, 1), b2 = cexp(lb, bi, 2);
+          const double* Ex = s_E[0] + (a0 * nbb + b0) * mtS;
+          const double* Ey = s_E[1] + (a1 * nbb + b1) * mtS;
+          const double* Ez = s_E[2] + (a2 * nbb + b2) * mtS;
+          double sum = 0.0;
+          for (int t = 0; t <= a0 + b0; ++t)
+            for (int u = 0; u <= a1 + b1; ++u)
+              for (int v = 0; v <= a2 + b2; ++v) sum += Ex[t] * Ey[u] * Ez[v] * R[ridx(t, u, v)];
+          s_cart[2][o] += pref * sum;
+        }
+        __syncthreads();
+      }
+    }
+  __syncthreads();
+  const int sphA = A.spherical && la >= 2, sphB = A.spherical && lb >= 2;
+  const int fa = sphA ? nsph(la) : na, fb = sphB ? nsph(lb) : nb;
+  for (int o = lane; o < fa * fb; o += kWave) {
+    int Ai = o / fb, Bi = o % fb;
+    int nA = sphA ? g_c2s.n[la][Ai] : 1, nB = sphB ? g_c2s.n[lb][Bi] : 1;
+    double v[3] = {0, 0, 0};
+    for (int x = 0; x < nA; ++x) {
+      int ai = sphA ? g_c2s.cart[la][Ai][x] : Ai;
+      double wa = sphA ? g_c2s.coef[la][Ai][x] : 1.0;
+      for (int y = 0; y < nB; ++y) {
+        int bi = sphB ? g_c2s.cart[lb][Bi][y] : Bi;
+        double w = wa * (sphB ? g_c2s.coef[lb][Bi][y] : 1.0);
+        for (int z = 0; z < 3; ++z) v[z] += w * s_cart[z][ai * nb + bi];
+      }
+    }
+    int64_t idx = (int64_t)(A.ao0[ia] + Ai) * nao + A.ao0[ib] + Bi;
+    S[idx] = v[0];
+    Tm[idx] = v[1];
+    V[idx] = v[2];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+void upload_c2s_tables() {
+  static C2SList h;
+  static int cart[7][28][3];
+  memset(&h, 0, sizeof(h));
+  auto fill = [&](int l, auto blk) {
+    int nc = ncart(l), ns = nsph(l);
+    for (int s = 0; s < ns; ++s) {
+      int k = 0;
+      for (int c = 0; c < nc; ++c) {
+        double v = blk[c][s];
+        if (v != 0.0 && k < 12) {
+          h.cart[l][s][k] = c;
+          h.coef[l][s][k] = v;
+          ++k;
+        }
+      }
+      h.n[l][s] = k;
+    }
+  };
+  fill(0, kC2S0); fill(1, kC2S1); fill(2, kC2S2); fill(3, kC2S3); fill(4, kC2S4); fill(5, kC2S5); fill(6, kC2S6);
+  auto fc = [&](int l, auto tab) {
+    for (int c = 0; c < ncart(l); ++c)
+      for (int x = 0; x < 3; ++x) cart[l][c][x] = tab[c][x];
+  };
+  fc(0, kCart0); fc(1, kCart1); fc(2, kCart2); fc(3, kCart3); fc(4, kCart4); fc(5, kCart5); fc(6, kCart6);
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(g_c2s), &h, sizeof(h));
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(g_cart), cart, sizeof(cart));
+}
+
+hipError_t launch_eri_wave(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,
+                           const ShellSet& A, const ShellSet& C, int b_unit, const double* boys, double* out,
+                           int64_t sa, int64_t sb, int tr, hipStream_t stream) {
+  if (npair == 0 || naux_l == 0) return hipSuccess;
+  Task3 t{pairs, aux, npair, naux_l};
+  const int64_t nblk = (int64_t)npair * naux_l;
+  // keep gridDim.x * blockDim.x below 2^32 threads per launch
+  const int64_t chunk = 1LL << 24;
+  for (int64_t b0 = 0; b0 < nblk; b0 += chunk) {
+    int64_t nb = nblk - b0 < chunk ? nblk - b0 : chunk;
+    hipLaunchKernelGGL(k_eri_wave, dim3((unsigned)nb), dim3(kWave), 0, stream, la, lb, lc, t, A, C, b_unit, boys,
+                       out, sa, sb, tr, b0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_int1e(const ShellSet& A, int natm, const double* atm, const double* boys, double* S, double* T,
+                        double* V, int64_t nao, hipStream_t stream) {
+  hipLaunchKernelGGL(k_int1e, dim3((unsigned)(A.n * A.n)), dim3(kWave), 0, stream, A, natm, atm, boys, S, T, V, nao);
+  return hipGetLastError();
+}
+
+}  // namespace dfks

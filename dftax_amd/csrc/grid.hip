@@ -1,0 +1,157 @@
+// Grid kernels: Becke fuzzy-cell partition and AO values/gradients (FP64).
+//
+// Becke partition (dftax/grid/becke.py:68-108): for a point p and atoms A, B
+//   mu_AB = (|p-A| - |p-B|) / R_AB,  nu = mu + a_AB (1 - mu^2),
+//   f = three iterations of f -> 1.5 f - 0.5 f^3,  s = (1 - f)/2,
+//   P_A = prod_{B != A} s(nu_AB),   w = w_raw * P_owner / sum_A P_A.
+// AO values (dftax/energy/gto.py:371-412): phi = x^lx y^ly z^lz sum_k c_k e^{-a_k r^2},
+// followed by the per-shell cart->sph block; gradients are analytic.
+// Layout of the AO block: ao[4][ng][nao] (value, d/dx, d/dy, d/dz), row-major
+// per point so a wavefront reading one point's row is contiguous.
+#include "dfks_common.h"
+
+namespace dfks {
+
+__global__ __launch_bounds__(256) void k_becke(int64_t n, const double* __restrict__ pts, const double* __restrict__ raw,
+                                                const int32_t* __restrict__ owner, int natm,
+                                                const double* __restrict__ atm /*[natm*3]*/,
+                                                const double* __restrict__ invR /*[natm*natm]*/,
+                                                const double* __restrict__ aAB /*[natm*natm]*/, double* __restrict__ w) {
+  extern __shared__ double s_atm[];  // 3*natm
+  for (int i = threadIdx.x; i < 3 * natm; i += blockDim.x) s_atm[i] = atm[i];
+  __syncthreads();
+  int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  double px = pts[3 * g], py = pts[3 * g + 1], pz = pts[3 * g + 2];
+  const int own = owner[g];
+  double total = 0.0, mine = 0.0;
+  for (int a = 0; a < natm; ++a) {
+    double dx = px - s_atm[3 * a], dy = py - s_atm[3 * a + 1], dz = pz - s_atm[3 * a + 2];
+    double ra = sqrt(dx * dx + dy * dy + dz * dz + 1e-30);
+    double prod = 1.0;
+    for (int b = 0; b < natm; ++b) {
+      if (b == a) continue;
+      double ex = px - s_atm[3 * b], ey = py - s_atm[3 * b + 1], ez = pz - s_atm[3 * b + 2];
+      double rb = sqrt(ex * ex + ey * ey + ez * ez + 1e-30);
+      double mu = (ra - rb) * invR[a * natm + b];
+      double f = mu + aAB[a * natm + b] * (1.0 - mu * mu);
+      f = 1.5 * f - 0.5 * f * f * f;
+      f = 1.5 * f - 0.5 * f * f * f;
+      f = 1.5 * f - 0.5 * f * f * f;
+      prod *= 0.5 * (1.0 - f);
+    }
+    total += prod;
+    if (a == own) mine = prod;
+  }
+  w[g] = raw[g] * mine / total;
+}
+
+template <int L>
+__device__ __forceinline__ double ipow(double x) {
+  if constexpr (L == 0) return 1.0;
+  else return x * ipow<L - 1>(x);
+}
+
+// One thread per (point, shell). Shell index fastest so a wavefront's writes
+// land in one point's AO row.
+template <int L>
+__device__ __forceinline__ void ao_shell(double dx, double dy, double dz, double R, double dR,
+                                          int sph, double* __restrict__ row0, double* __restrict__ row1,
+                                          double* __restrict__ row2, double* __restrict__ row3) {
+  constexpr int NC = ncart(L);
+  double v[NC], gx[NC], gy[NC], gz[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    constexpr int dummy = 0;
+    (void)dummy;
+    const int lx = cart_exp<L>(c, 0), ly = cart_exp<L>(c, 1), lz = cart_exp<L>(c, 2);
+    double px = 1.0, py = 1.0, pz = 1.0, qx = 0.0, qy = 0.0, qz = 0.0;
+    // x^l and l x^{l-1}
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (k < lx) { qx = qx * dx + px; px *= dx; }
+      if (k < ly) { qy = qy * dy + py; py *= dy; }
+      if (k < lz) { qz = qz * dz + pz; pz *= dz; }
+    }
+    // qx accumulated sum_{k} x^k * x^{lx-1-k} = lx x^{lx-1}
+    double ang = px * py * pz;
+    v[c] = ang * R;
+    gx[c] = qx * py * pz * R + ang * dR * dx;
+    gy[c] = px * qy * pz * R + ang * dR * dy;
+    gz[c] = px * py * qz * R + ang * dR * dz;
+  }
+  if (sph && L >= 2) {
+    constexpr int NS = nsph(L);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const double w = c2s_coef<L>(c, s);
+        if (w != 0.0) {
+          a0 += w * v[c]; a1 += w * gx[c]; a2 += w * gy[c]; a3 += w * gz[c];
+        }
+      }
+      row0[s] = a0; row1[s] = a1; row2[s] = a2; row3[s] = a3;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) { row0[c] = v[c]; row1[c] = gx[c]; row2[c] = gy[c]; row3[c] = gz[c]; }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ao(int64_t npts, int64_t g0, int64_t ng, const double* __restrict__ pts,
+                                             ShellSet S, int64_t nao, double* __restrict__ ao) {
+  int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (gid >= npts * S.n) return;
+  const int64_t g = g0 + gid / S.n;
+  const int ish = (int)(gid % S.n);
+  double dx = pts[3 * g] - S.centre[3 * ish];
+  double dy = pts[3 * g + 1] - S.centre[3 * ish + 1];
+  double dz = pts[3 * g + 2] - S.centre[3 * ish + 2];
+  double r2 = dx * dx + dy * dy + dz * dz;
+  double R = 0.0, dR = 0.0;
+  const int p0 = S.prim0[ish], np = S.nprim[ish];
+  for (int k = 0; k < np; ++k) {
+    double a = S.exps[p0 + k];
+    double e = S.coefs[p0 + k] * exp(-a * r2);
+    R += e;
+    dR += a * e;
+  }
+  dR *= -2.0;
+  const int64_t plane = ng * nao;
+  double* r0 = ao + g * nao + S.ao0[ish];
+  switch (S.l[ish]) {
+    case 0: ao_shell<0>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+    case 1: ao_shell<1>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+    case 2: ao_shell<2>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+    case 3: ao_shell<3>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+    default: ao_shell<4>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+  }
+}
+
+hipError_t launch_becke(int64_t n, const double* pts, const double* raw, const int32_t* owner, int natm,
+                        const double* atm, const double* invR, const double* aAB, double* w, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const int bs = 256;
+  hipLaunchKernelGGL(k_becke, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 3 * natm * sizeof(double), st, n, pts,
+                     raw, owner, natm, atm, invR, aAB, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_ao(int64_t ng, const double* pts, const ShellSet& S, int64_t nao, double* ao, hipStream_t st) {
+  const int bs = 256;
+  const int64_t total = ng * S.n;
+  const int64_t chunk_pts = ((1LL << 31) / S.n / bs) * bs;  // keep each launch under 2^31 threads
+  for (int64_t g0 = 0; g0 < ng; g0 += chunk_pts) {
+    int64_t n = ng - g0 < chunk_pts ? ng - g0 : chunk_pts;
+    int64_t th = n * S.n;
+    hipLaunchKernelGGL(k_ao, dim3((unsigned)((th + bs - 1) / bs)), dim3(bs), 0, st, n, g0, ng, pts, S, nao, ao);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  (void)total;
+  return hipSuccess;
+}
+
+}  // namespace dfks

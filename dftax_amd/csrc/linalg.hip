@@ -1,0 +1,308 @@
+// Bandwidth-bound helpers of the Fock build and the SCF step (FP64).
+//
+//  * k_gamma    gamma_Q = sum_{m,i} s_i Cf[m,i] T[m,Q,i]   (fitted density, from the
+//               exchange intermediate T = B^T Cf, so B is not re-read for it)
+//  * k_jvec     J[mn] = sum_Q B[mn,Q] gamma_Q              (one read of B, HBM bound)
+//  * k_fock     F = hcore + J + cK sym(K) + sym(Vxc)        (upper-triangle inputs)
+//  * deterministic two-pass reductions for the energy terms
+//  * DIIS: Gram row, bordered solve (one lane, FP64, partial pivoting),
+//    extrapolation (dftax/ks/scf.py:252-276)
+#include "dfks_common.h"
+
+namespace dfks {
+
+constexpr int kRedBlocks = 512;
+
+// partial[b] = sum over a grid-stride slice of x[i]*y[i] (y may be null -> x only)
+__global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const double* __restrict__ x,
+                                                     const double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += y ? x[i] * y[i] : x[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// out[0] = scale * sum(part[0..np))  (+= if accumulate)
+__global__ __launch_bounds__(256) void k_sum_final(int np, const double* __restrict__ part, double scale,
+                                                   double* __restrict__ out, int accumulate) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = accumulate ? out[0] + scale * s[0] : scale * s[0];
+}
+
+hipError_t launch_dot(int64_t n, const double* x, const double* y, double* part, double scale, double* out,
+                      int accumulate, hipStream_t st) {
+  int nb = (int)((n + 255) / 256);
+  if (nb > kRedBlocks) nb = kRedBlocks;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(256), 0, st, n, x, y, part);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, nb, part, scale, out, accumulate);
+  return hipGetLastError();
+}
+
+// gamma for one block of 8 aux indices; threads over i (factor columns), loop over m.
+// T layout: T[(m*nq + q)*nf + i]; Cf column-major: Cf[i*nao + m].
+__global__ __launch_bounds__(256) void k_gamma(int nao, int nq, int nf, const double* __restrict__ T,
+                                               const double* __restrict__ Cf, const double* __restrict__ sgn,
+                                               double* __restrict__ gamma, int accumulate) {
+  __shared__ double s[256];
+  const int q = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+    double a = 0.0;
+    const double* c = Cf + (int64_t)i * nao;
+    for (int m = 0; m < nao; ++m) a += c[m] * T[((int64_t)m * nq + q) * nf + i];
+    acc += sgn[i] * a;
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gamma[q] = accumulate ? gamma[q] + s[0] : s[0];
+}
+
+hipError_t launch_gamma(int nao, int nq, int nf, const double* T, const double* Cf, const double* sgn, double* gamma,
+                        int accumulate, hipStream_t st) {
+  hipLaunchKernelGGL(k_gamma, dim3(nq), dim3(256), 0, st, nao, nq, nf, T, Cf, sgn, gamma, accumulate);
+  return hipGetLastError();
+}
+
+// J[r] = sum_q B[r*nq + q] g[q]; one wavefront per row, 4 rows per block per step.
+__global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const double* __restrict__ B,
+                                              const double* __restrict__ g, double* __restrict__ J) {
+  extern __shared__ double s_g[];
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) s_g[i] = g[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (int64_t r = blockIdx.x * 4 + (threadIdx.x >> 6); r < nrows; r += (int64_t)gridDim.x * 4) {
+    const double* row = B + r * nq;
+    double acc = 0.0;
+    for (int q = lane; q < nq; q += 64) acc += row[q] * s_g[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) J[r] = acc;
+  }
+}
+
+hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, double* J, hipStream_t st) {
+  int64_t nb = (nrows + 3) / 4;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_jvec, dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g, J);
+  return hipGetLastError();
+}
+
+// F = H + J + cK * sym(Ku) + sym(Vu) where Ku, Vu hold valid upper triangles
+// (column-major upper == row-major lower; both symmetric by construction).
+// Any of Ku / Vu may be null.
+__global__ void k_fock(int n, const double* __restrict__ H, const double* __restrict__ J, const double* __restrict__ Ku,
+                       double cK, const double* __restrict__ Vu, double* __restrict__ F) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  int r = (int)(idx / n), c = (int)(idx % n);
+  // column-major upper stores (row<=col) at [col*n + row]; pick the stored triangle
+  int64_t up = r <= c ? (int64_t)c * n + r : (int64_t)r * n + c;
+  double f = H[idx] + J[idx];
+  if (Ku) f += cK * Ku[up];
+  if (Vu) f += Vu[up];
+  F[idx] = f;
+}
+
+hipError_t launch_fock(int n, const double* H, const double* J, const double* Ku, double cK, const double* Vu,
+                       double* F, hipStream_t st) {
+  int64_t tot = (int64_t)n * n;
+  hipLaunchKernelGGL(k_fock, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, H, J, Ku, cK, Vu, F);
+  return hipGetLastError();
+}
+
+// Fill the lower part of a column-major upper-triangular symmetric matrix.
+__global__ void k_symfill(int n, double* __restrict__ A) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  int col = (int)(idx / n), row = (int)(idx % n);
+  if (row > col) A[idx] = A[(int64_t)row * n + col];
+}
+
+hipError_t launch_symfill(int n, double* A, hipStream_t st) {
+  int64_t tot = (int64_t)n * n;
+  hipLaunchKernelGGL(k_symfill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, A);
+  return hipGetLastError();
+}
+
+// Metric factor: L_cm[k + q*nk] = U[q + (k0+k)*nq] / sqrt(w[k0+k]) for the kept
+// (largest) eigenpairs; the B GEMM then consumes L^T directly.
+__global__ void k_metric_factor(int nq, int k0, const double* __restrict__ U, const double* __restrict__ w,
+                                double* __restrict__ Lt) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int nk = nq - k0;
+  if (idx >= (int64_t)nk * nq) return;
+  int q = (int)(idx / nk), k = (int)(idx % nk);
+  Lt[idx] = U[q + (int64_t)(k0 + k) * nq] / sqrt(w[k0 + k]);
+}
+
+hipError_t launch_metric_factor(int nq, int k0, const double* U, const double* w, double* Lt, hipStream_t st) {
+  int64_t tot = (int64_t)(nq - k0) * nq;
+  hipLaunchKernelGGL(k_metric_factor, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nq, k0, U, w, Lt);
+  return hipGetLastError();
+}
+
+// Cf[:, i] = sqrt(|lam_i|) V[:, sel_i]; sgn_i = sign(lam_i). sel lists kept columns.
+__global__ void k_factor_cols(int n, int nf, const double* __restrict__ V, const double* __restrict__ lam,
+                              const int32_t* __restrict__ sel, double* __restrict__ Cf, double* __restrict__ sgn) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * nf) return;
+  int i = (int)(idx / n), m = (int)(idx % n);
+  int c = sel[i];
+  double l = lam[c];
+  Cf[idx] = sqrt(fabs(l)) * V[(int64_t)c * n + m];
+  if (m == 0) sgn[i] = l >= 0 ? 1.0 : -1.0;
+}
+
+hipError_t launch_factor_cols(int n, int nf, const double* V, const double* lam, const int32_t* sel, double* Cf,
+                              double* sgn, hipStream_t st) {
+  int64_t tot = (int64_t)n * nf;
+  if (tot == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_factor_cols, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, nf, V, lam, sel, Cf,
+                     sgn);
+  return hipGetLastError();
+}
+
+// y = a*x (+ y if acc)
+__global__ void k_axpy(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y, int acc) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = acc ? y[i] + a * x[i] : a * x[i];
+}
+
+hipError_t launch_axpy(int64_t n, double a, const double* x, double* y, int acc, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, x, y, acc);
+  return hipGetLastError();
+}
+
+// E = A - A^T  (n x n)
+__global__ void k_antisym(int n, const double* __restrict__ A, double* __restrict__ E) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  int r = (int)(idx / n), c = (int)(idx % n);
+  E[idx] = A[idx] - A[(int64_t)c * n + r];
+}
+
+hipError_t launch_antisym(int n, const double* A, double* E, hipStream_t st) {
+  int64_t tot = (int64_t)n * n;
+  hipLaunchKernelGGL(k_antisym, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, A, E);
+  return hipGetLastError();
+}
+
+// DIIS bordered system: gram holds err_i . err_j for filled slots (m x m, row-major).
+// Builds A, solves A c = (0,..,0,-1) by Gaussian elimination with partial pivoting
+// on one lane and writes c[0..m). Mirrors dftax/ks/scf.py:252-276 (identity on
+// empty slots, 1e-10 ridge on filled ones).
+__global__ void k_diis_solve(int m, int count, const double* __restrict__ gram, double* __restrict__ coef) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double A[17][18];
+  const int n = m + 1;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= n; ++j) A[i][j] = 0.0;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      bool vi = i < count, vj = j < count;
+      A[i][j] = (vi && vj) ? gram[i * m + j] : (i == j ? 1.0 : 0.0);
+      if (i == j && vi) A[i][j] += 1e-10;
+    }
+  for (int i = 0; i < m; ++i) {
+    double b = i < count ? -1.0 : 0.0;
+    A[i][m] = b;
+    A[m][i] = b;
+  }
+  A[m][n] = -1.0;  // rhs column
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    double best = fabs(A[k][k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(A[i][k]) > best) { best = fabs(A[i][k]); piv = i; }
+    if (piv != k)
+      for (int j = 0; j <= n; ++j) { double t = A[k][j]; A[k][j] = A[piv][j]; A[piv][j] = t; }
+    double d = A[k][k];
+    for (int i = k + 1; i < n; ++i) {
+      double f = A[i][k] / d;
+      if (f == 0.0) continue;
+      for (int j = k; j <= n; ++j) A[i][j] -= f * A[k][j];
+    }
+  }
+  double x[17];
+  for (int i = n - 1; i >= 0; --i) {
+    double s = A[i][n];
+    for (int j = i + 1; j < n; ++j) s -= A[i][j] * x[j];
+    x[i] = s / A[i][i];
+  }
+  for (int i = 0; i < m; ++i) coef[i] = x[i];
+}
+
+hipError_t launch_diis_solve(int m, int count, const double* gram, double* coef, hipStream_t st) {
+  hipLaunchKernelGGL(k_diis_solve, dim3(1), dim3(64), 0, st, m, count, gram, coef);
+  return hipGetLastError();
+}
+
+// Fx = sum_i coef[i] F_i  (F slots contiguous, n elements each)
+__global__ void k_diis_extrap(int m, int64_t n, const double* __restrict__ coef, const double* __restrict__ Fs,
+                              double* __restrict__ out) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  double s = 0.0;
+  for (int i = 0; i < m; ++i) s += coef[i] * Fs[(int64_t)i * n + idx];
+  out[idx] = s;
+}
+
+hipError_t launch_diis_extrap(int m, int64_t n, const double* coef, const double* Fs, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_diis_extrap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, m, n, coef, Fs, out);
+  return hipGetLastError();
+}
+
+// gram row/column update for the new slot: gram[slot*m+i] = gram[i*m+slot] = dots[i]
+__global__ void k_gram_update(int m, int slot, int count, const double* __restrict__ dots, double* __restrict__ gram) {
+  int i = threadIdx.x;
+  if (i >= count) return;
+  gram[slot * m + i] = dots[i];
+  gram[i * m + slot] = dots[i];
+}
+
+hipError_t launch_gram_update(int m, int slot, int count, const double* dots, double* gram, hipStream_t st) {
+  hipLaunchKernelGGL(k_gram_update, dim3(1), dim3(64), 0, st, m, slot, count, dots, gram);
+  return hipGetLastError();
+}
+
+}  // namespace dfks
+
+namespace dfks {
+// X[r + k*n] = U[r + (k0+k)*n] / sqrt(w[k0+k])  (column-major n x (n-k0))
+__global__ void k_scaled_cols(int n, int k0, const double* __restrict__ U, const double* __restrict__ w,
+                              double* __restrict__ X) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int nk = n - k0;
+  if (idx >= (int64_t)n * nk) return;
+  int k = (int)(idx / n), r = (int)(idx % n);
+  X[idx] = U[r + (int64_t)(k0 + k) * n] / sqrt(w[k0 + k]);
+}
+
+hipError_t launch_scaled_cols(int n, int k0, const double* U, const double* w, double* X, hipStream_t st) {
+  int64_t tot = (int64_t)n * (n - k0);
+  if (tot == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scaled_cols, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, k0, U, w, X);
+  return hipGetLastError();
+}
+}  // namespace dfks

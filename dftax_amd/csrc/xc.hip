@@ -1,0 +1,229 @@
+// Exchange-correlation on the grid (FP64), closed shell.
+//
+// Per point g:  rho = sum_n Z_gn phi_gn,  grad rho = 2 sum_n Z_gn grad phi_gn
+// with Z = Phi P (a GEMM done before this kernel), then the pointwise
+// functional f(rho, sigma) = eps_xc * rho with sigma = |grad rho|^2, masked to
+// zero where rho <= rho_thresh (dftax/energy/grid.py:10-44). The Vxc matrix is
+// V = Phi^T W + W^T Phi with W_gn = w_g (1/2 f_rho phi_gn + 2 f_sigma grad rho . grad phi_gn),
+// i.e. sym(dE_xc/dP) of the quadrature (dftax/ks/terms.py:791-828 + jax.grad).
+//
+// The functionals are transcribed from dftax/energy/xc.py (PBEExchange 87-138,
+// PBECorrelation 141-239, B88Exchange 255-294, VWN_RPACorrelation 376-419,
+// LYPCorrelation 427-519, LDAExchange 64-84; PBE0 584-601, B3LYP 527-567),
+// including the same density/gradient floors, and evaluated on a forward-mode
+// dual number carrying d/drho and d/dsigma, so f_rho and f_sigma are exact to
+// rounding.
+#include "dfks_common.h"
+
+namespace dfks {
+
+enum XcId { kXcPBE = 1, kXcPBE0 = 2, kXcB3LYP = 3 };
+
+struct D {
+  double v, r, s;  // value, d/drho, d/dsigma
+};
+__device__ __forceinline__ D dc(double c) { return {c, 0.0, 0.0}; }
+__device__ __forceinline__ D operator+(D a, D b) { return {a.v + b.v, a.r + b.r, a.s + b.s}; }
+__device__ __forceinline__ D operator-(D a, D b) { return {a.v - b.v, a.r - b.r, a.s - b.s}; }
+__device__ __forceinline__ D operator-(D a) { return {-a.v, -a.r, -a.s}; }
+__device__ __forceinline__ D operator*(D a, D b) { return {a.v * b.v, a.r * b.v + a.v * b.r, a.s * b.v + a.v * b.s}; }
+__device__ __forceinline__ D operator/(D a, D b) {
+  double iv = 1.0 / b.v, q = a.v * iv;
+  return {q, (a.r - q * b.r) * iv, (a.s - q * b.s) * iv};
+}
+__device__ __forceinline__ D operator+(D a, double c) { return {a.v + c, a.r, a.s}; }
+__device__ __forceinline__ D operator+(double c, D a) { return {a.v + c, a.r, a.s}; }
+__device__ __forceinline__ D operator-(D a, double c) { return {a.v - c, a.r, a.s}; }
+__device__ __forceinline__ D operator-(double c, D a) { return {c - a.v, -a.r, -a.s}; }
+__device__ __forceinline__ D operator*(D a, double c) { return {a.v * c, a.r * c, a.s * c}; }
+__device__ __forceinline__ D operator*(double c, D a) { return {a.v * c, a.r * c, a.s * c}; }
+__device__ __forceinline__ D operator/(D a, double c) { return {a.v / c, a.r / c, a.s / c}; }
+__device__ __forceinline__ D operator/(double c, D a) {
+  double q = c / a.v;
+  double dq = -q / a.v;
+  return {q, dq * a.r, dq * a.s};
+}
+__device__ __forceinline__ D chain(D a, double f, double df) { return {f, df * a.r, df * a.s}; }
+__device__ __forceinline__ D dsqrt(D a) { double f = sqrt(a.v); return chain(a, f, 0.5 / f); }
+__device__ __forceinline__ D dexp(D a) { double f = exp(a.v); return chain(a, f, f); }
+__device__ __forceinline__ D dlog(D a) { return chain(a, log(a.v), 1.0 / a.v); }
+__device__ __forceinline__ D dlog1p(D a) { return chain(a, log1p(a.v), 1.0 / (1.0 + a.v)); }
+__device__ __forceinline__ D dexpm1(D a) { return chain(a, expm1(a.v), exp(a.v)); }
+__device__ __forceinline__ D datan(D a) { return chain(a, atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+__device__ __forceinline__ D dasinh(D a) { return chain(a, asinh(a.v), 1.0 / sqrt(1.0 + a.v * a.v)); }
+__device__ __forceinline__ D dpow(D a, double p) {
+  double f = pow(a.v, p);
+  return chain(a, f, p * f / a.v);
+}
+// jnp.clip(x, lo): derivative passes where x >= lo
+__device__ __forceinline__ D dclip(D a, double lo) { return a.v >= lo ? a : dc(lo); }
+
+constexpr double kPi2 = kPi * kPi;
+
+__device__ D lda_x(D rho) {
+  D kF = dpow(3.0 * kPi2 * rho + 1e-30, 1.0 / 3.0);
+  return -3.0 * kF / (4.0 * kPi);
+}
+
+// |grad + 1e-30| of dftax's PBE/B88 exchange, written on sigma
+__device__ __forceinline__ D grad_norm_x(D sig) { return dsqrt(sig + 3e-60); }
+
+__device__ D pbe_x(D rho, D sig) {
+  const double kappa = 0.804, mu = 0.2195149727645171;
+  D gn = grad_norm_x(sig);
+  D kF = dpow(3.0 * kPi2 * rho + 1e-30, 1.0 / 3.0);
+  D s = gn / dclip(2.0 * kF * rho, 1e-30);
+  D eps_unif = -3.0 * kF / (4.0 * kPi);
+  D f = 1.0 + kappa - kappa / (1.0 + mu * s * s / kappa);
+  return eps_unif * f;
+}
+
+__device__ D pbe_c(D rho, D sig) {
+  const double gamma = (1.0 - 0.69314718055994530942) / kPi2;
+  const double beta = 0.06672455060314922;
+  const double A = 0.0310907, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+  D rt = dclip(rho, 1e-20);
+  D rs = dpow(3.0 / (4.0 * kPi * rt) + 1e-20, 1.0 / 3.0);
+  D kf = dpow(3.0 * kPi2 * rt + 1e-20, 1.0 / 3.0);
+  D ks = dsqrt(4.0 * kf / kPi);
+  D gn = dsqrt(sig + 1e-30);
+  D t = gn / dclip(2.0 * ks * rt, 1e-20);
+  D rs_sqrt = dsqrt(rs);
+  D denom = 2.0 * A * (b1 * rs_sqrt + b2 * rs + b3 * rs * rs_sqrt + b4 * rs * rs);
+  // zeta = 0: the PW92 spin interpolation reduces to the paramagnetic branch
+  D eps_pw = -2.0 * A * (1.0 + a1 * rs) * dlog1p(1.0 / dclip(denom, 1e-20));
+  D denom_A = gamma * dexpm1(-eps_pw / gamma);
+  D Ac = beta / dclip(denom_A, 1e-20);
+  D t2 = t * t;
+  D f1 = t2 + Ac * t2 * t2;
+  D frac = (beta * f1) / dclip(gamma * (1.0 + Ac * f1), 1e-20);
+  return eps_pw + gamma * dlog1p(frac);
+}
+
+__device__ D b88_x(D rho, D sig) {
+  const double beta = 0.0042;
+  D gn = grad_norm_x(sig);
+  D r = dclip(rho * 0.5, 1e-20);
+  D r13 = dpow(r, 1.0 / 3.0);
+  const double C = 0.75 * 1.2407009817988000;  // (6/pi)^(1/3)
+  D x = (gn * 0.5) / dclip(dpow(r, 4.0 / 3.0), 1e-20);
+  D den = 1.0 + 6.0 * beta * x * dasinh(x);
+  return -C * r13 - beta * r13 * x * x / dclip(den, 1e-20);
+}
+
+__device__ D vwn_rpa_c(D rho) {
+  const double A = 0.0310907, b = 13.0720, c = 42.7198, x0 = -0.409286;
+  D r = dclip(rho, 1e-20);
+  D rs = dpow(3.0 / (4.0 * kPi * r), 1.0 / 3.0);
+  D x = dsqrt(rs);
+  D Xx = x * x + b * x + c;
+  double Xx0 = x0 * x0 + b * x0 + c;
+  double Q = sqrt(4.0 * c - b * b);
+  D at = datan(Q / dclip(2.0 * x + b, 1e-20));
+  D term1 = dlog(x * x / dclip(Xx, 1e-20));
+  D term2 = (2.0 * b / Q) * at;
+  D xm = x - x0;
+  D term3 = -(b * x0 / Xx0) * (dlog(xm * xm / dclip(Xx, 1e-20)) + (2.0 * (b + 2.0 * x0) / Q) * at);
+  return A * (term1 + term2 + term3);
+}
+
+__device__ D lyp_c(D rho, D sig) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double C_F = 0.3 * 9.5707800006273;  // (3 pi^2)^(2/3)
+  D r = dclip(rho, 1e-20);
+  D ri3 = dpow(r, -1.0 / 3.0);
+  D g = 1.0 / (1.0 + d * ri3);
+  D om = dexp(-c * ri3) * g * dpow(r, -11.0 / 3.0);
+  D delta = c * ri3 + d * ri3 * g;
+  D term_ff = C_F * dpow(r, 8.0 / 3.0) - sig * (3.0 + 7.0 * delta) / 72.0;
+  return -a * g - a * b * om * r * term_ff;
+}
+
+__device__ D eps_xc(int id, D rho, D sig) {
+  switch (id) {
+    case kXcPBE: return pbe_x(rho, sig) + pbe_c(rho, sig);
+    case kXcPBE0: return 0.75 * pbe_x(rho, sig) + pbe_c(rho, sig);
+    default:
+      return 0.08 * lda_x(rho) + 0.72 * b88_x(rho, sig) + 0.19 * vwn_rpa_c(rho) + 0.81 * lyp_c(rho, sig);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One wavefront per grid point, 4 points per 256-thread block.
+// zw: on entry Z = Phi P (ng x nao); on exit W (ng x nao), in place.
+__global__ __launch_bounds__(256) void k_xc(int64_t ng, int64_t nao, const double* __restrict__ ao,
+                                            const double* __restrict__ wts, double* __restrict__ zw, int xc_id,
+                                            double thresh, double* __restrict__ exc, double* __restrict__ rho_out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ng) return;
+  const int64_t plane = ng * nao;
+  const double* p0 = ao + g * nao;
+  const double* p1 = p0 + plane;
+  const double* p2 = p0 + 2 * plane;
+  const double* p3 = p0 + 3 * plane;
+  double* zrow = zw + g * nao;
+  double r = 0, gx = 0, gy = 0, gz = 0;
+  for (int64_t n = lane; n < nao; n += 64) {
+    double z = zrow[n];
+    r += z * p0[n];
+    gx += z * p1[n];
+    gy += z * p2[n];
+    gz += z * p3[n];
+  }
+  r = wave_sum(r);
+  gx = 2.0 * wave_sum(gx);
+  gy = 2.0 * wave_sum(gy);
+  gz = 2.0 * wave_sum(gz);
+  if (rho_out && lane == 0) {
+    rho_out[4 * g] = r; rho_out[4 * g + 1] = gx; rho_out[4 * g + 2] = gy; rho_out[4 * g + 3] = gz;
+  }
+  double a = 0.0, bx = 0.0, by = 0.0, bz = 0.0, e = 0.0;
+  if (r > thresh) {
+    const double w = wts[g];
+    D rho{r, 1.0, 0.0};
+    D sig{gx * gx + gy * gy + gz * gz, 0.0, 1.0};
+    D f = eps_xc(xc_id, rho, sig) * rho;
+    e = w * f.v;
+    a = 0.5 * w * f.r;
+    const double t = 2.0 * w * f.s;
+    bx = t * gx; by = t * gy; bz = t * gz;
+  }
+  if (lane == 0) exc[g] = e;
+  for (int64_t n = lane; n < nao; n += 64) zrow[n] = a * p0[n] + bx * p1[n] + by * p2[n] + bz * p3[n];
+}
+
+hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w, double* zw, int xc_id, double thresh,
+                     double* exc, double* rho_out, hipStream_t st) {
+  if (ng == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xc, dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w, zw, xc_id, thresh, exc,
+                     rho_out);
+  return hipGetLastError();
+}
+
+// Pointwise functional only (for parity tests of f, f_rho, f_sigma).
+__global__ void k_xc_pointwise(int64_t n, int xc_id, const double* __restrict__ rho, const double* __restrict__ sigma,
+                               double* __restrict__ out /*[n][3]*/) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  D r{rho[i], 1.0, 0.0};
+  D s{sigma[i], 0.0, 1.0};
+  D f = eps_xc(xc_id, r, s) * r;
+  out[3 * i] = f.v;
+  out[3 * i + 1] = f.r;
+  out[3 * i + 2] = f.s;
+}
+
+hipError_t launch_xc_pointwise(int64_t n, int xc_id, const double* rho, const double* sigma, double* out,
+                               hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xc_pointwise, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, xc_id, rho, sigma, out);
+  return hipGetLastError();
+}
+
+}  // namespace dfks

@@ -1,0 +1,215 @@
+"""The KS energy functional with the MI355X density-fitting engine.
+
+Mirrors dftax's builder surface (dftax/ks/energy.py:439-723,
+dftax/ks/terms.py:63-148): ``KS(mol, xc, grid=becke(...), coulomb=df(aux, ...))``
+owns ``S``, ``hcore``, ``e_nn`` and the Coulomb + XC terms, and exposes
+``electronic(P)`` / ``total(P)``. The one addition is the engine value on the
+DF spec, ``df(aux, engine=hip(...))`` (the default here), which resolves the
+Coulomb term to :class:`HipDFKS`: one native handle that evaluates E(P) and
+F = sym(dE/dP) (J, -a_x/2 K and Vxc together) on the GPU, with the same
+``energy(P, S, nocc)`` contract as dftax's ``CoulombTerm`` (terms.py:512-527).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from dftax_amd import _native
+from dftax_amd.basis import BasisSet, build_basis
+from dftax_amd.grid import Becke, atom_bragg, atom_centred_points, becke
+from dftax_amd.system import Molecule
+from dftax_amd.xc import XCFunctional, resolve
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclass(frozen=True)
+class HipSpec:
+    """Engine value: run the DF-KS term through libdfks on the current GPU.
+
+    ``precision="fp64"`` computes everything in FP64. ``metric_cutoff`` is the
+    relative eigenvalue cutoff of the metric pseudo-inverse (terms.py:190-192).
+    """
+
+    precision: str = "fp64"
+    metric_cutoff: float = 1e-7
+    rho_thresh: float = 1e-10
+    keep_int3c: bool = False
+
+
+def hip(*, precision: str = "fp64", metric_cutoff: float = 1e-7, rho_thresh: float = 1e-10,
+        keep_int3c: bool = False) -> HipSpec:
+    if precision not in ("fp64",):
+        raise ValueError(f"precision {precision!r} is not available in this build (use 'fp64')")
+    return HipSpec(precision=precision, metric_cutoff=metric_cutoff, rho_thresh=rho_thresh, keep_int3c=keep_int3c)
+
+
+@dataclass(frozen=True)
+class DFSpec:
+    """Density-fitting spec (dftax/ks/terms.py:63-76) plus the engine value."""
+
+    auxbasis: str = "even-tempered-jk"
+    chunk: object = "auto"
+    screen: float | None = None
+    spherical: bool | None = None
+    engine: HipSpec = field(default_factory=HipSpec)
+
+
+def df(auxbasis: str = "even-tempered-jk", *, chunk="auto", screen: float | None = None,
+       spherical: bool | None = None, engine: HipSpec | None = None) -> DFSpec:
+    """Density-fitted Coulomb + exchange (terms.py:96-148).
+
+    ``chunk`` and ``screen`` are accepted for interface parity; this engine
+    always materialises B in HBM and computes every bra pair. ``spherical``
+    False is rejected: the auxiliary span is spherical, as on dftax's
+    materialised path (energy.py:308-314).
+    """
+    if spherical is False:
+        raise NotImplementedError("the HIP engine fits in the spherical auxiliary span")
+    if screen is not None:
+        raise NotImplementedError("df(screen=...) is not implemented by the HIP engine yet")
+    if isinstance(chunk, int):
+        raise NotImplementedError("streamed df(chunk=<int>) is not implemented by the HIP engine yet")
+    return DFSpec(auxbasis=auxbasis, chunk=chunk, screen=screen, spherical=spherical,
+                  engine=engine if engine is not None else HipSpec())
+
+
+def build_grid(symbols, coords, spec: Becke):
+    """Atom-centred points, GPU Becke weights, then the weight cutoff."""
+    pts, raw, owner = atom_centred_points(symbols, coords, spec)
+    w = _native.becke_weights(pts, raw, owner, coords, atom_bragg(symbols))
+    if spec.cutoff is not None:
+        keep = np.abs(w) > spec.cutoff
+        pts, w = pts[keep], w[keep]
+    return np.ascontiguousarray(pts), np.ascontiguousarray(w)
+
+
+class HipDFKS:
+    """Native DF Coulomb/exchange + grid XC term (one libdfks handle)."""
+
+    def __init__(self, basis: BasisSet, aux: BasisSet, mol: Molecule, grid_xyz, grid_w, xc: XCFunctional,
+                 nelec: int, spec: HipSpec, device=None):
+        torch = _torch()
+        self.L = _native.lib()
+        self.device = torch.device("cuda") if device is None else torch.device(device)
+        self._arrays = (_native.BasisArrays(basis), _native.BasisArrays(aux))
+        self._xyz = np.ascontiguousarray(mol.coords, np.float64)
+        self._z = np.ascontiguousarray(mol.atom_charges(), np.float64)
+        self._gxyz = np.ascontiguousarray(grid_xyz, np.float64)
+        self._gw = np.ascontiguousarray(grid_w, np.float64)
+        desc = _native.SystemDesc(self._arrays[0].desc, self._arrays[1].desc, len(self._z), _native.f64p(self._xyz),
+                                  _native.f64p(self._z), self._gw.shape[0], _native.f64p(self._gxyz),
+                                  _native.f64p(self._gw), xc.xc_id, int(nelec))
+        opts = _native.Options(0, float(spec.metric_cutoff), float(spec.rho_thresh), int(spec.keep_int3c))
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _native.check(self.L.dfks_create(ctypes.byref(desc), ctypes.byref(opts), ctypes.byref(h)))
+            self.handle = h
+            _native.check(self.L.dfks_build(self.handle, self._stream()))
+        sizes = (ctypes.c_int64 * 5)()
+        _native.check(self.L.dfks_sizes(self.handle, sizes))
+        self.nao, self.naux, self.nfit, self.nocc, self.ngrid = (int(v) for v in sizes)
+        self.hf_coeff = xc.hf_coeff
+        self._E = torch.zeros(8, dtype=torch.float64, device=self.device)
+
+    def _stream(self):
+        return ctypes.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None:
+            self.L.dfks_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def matrix(self, which: int, shape):
+        torch = _torch()
+        out = torch.empty(shape, dtype=torch.float64, device=self.device)
+        _native.check(self.L.dfks_get_matrix(self.handle, which, ctypes.c_void_p(out.data_ptr()), self._stream()))
+        return out
+
+    def energy_fock(self, P):
+        """(E components [8], F) for a device density P (nao x nao)."""
+        torch = _torch()
+        P = P.to(device=self.device, dtype=torch.float64).contiguous()
+        F = torch.empty_like(P)
+        _native.check(self.L.dfks_energy_fock(self.handle, ctypes.c_void_p(P.data_ptr()),
+                                              ctypes.c_void_p(self._E.data_ptr()), ctypes.c_void_p(F.data_ptr()),
+                                              self._stream()))
+        return self._E.clone(), F
+
+    def energy(self, P, S=None, nocc=None):
+        """dftax's CoulombTerm contract: the J + K + XC energy of a spin-stacked P."""
+        E, _ = self.energy_fock(P[0] if P.dim() == 3 else P)
+        return float(E[2] + E[3] + E[4])
+
+
+class KS:
+    """Closed-shell KS energy functional on the MI355X engine (energy.py:439-723)."""
+
+    def __init__(self, system: Molecule, xc, *, grid: Becke | None = None, coulomb: DFSpec | None = None,
+                 spin: int | None = None, device=None):
+        torch = _torch()
+        if spin not in (None, 0) or system.spin != 0:
+            raise NotImplementedError("the HIP engine implements the closed-shell (nspin=1) case")
+        nelec = system.nelectron
+        if nelec % 2:
+            raise ValueError(f"nelec={nelec} is odd but the system has spin=0")
+        self.xc = resolve(xc)
+        spec = coulomb if coulomb is not None else df()
+        if not isinstance(spec, DFSpec):
+            raise TypeError("the HIP engine is a df(...) Coulomb backend")
+        self.mol = system
+        self.symbols = tuple(system.symbols)
+        self.atom_coords = np.asarray(system.coords)
+        self.nelec = nelec
+        self.nocc = (nelec // 2,)
+        self.basis = build_basis(system.symbols, system.coords, system.basis, spherical=system.spherical)
+        self.aux_basis = build_basis(system.symbols, system.coords, spec.auxbasis, spherical=True)
+        gspec = grid if grid is not None else becke()
+        self.grid_xyz, self.grid_w = build_grid(system.symbols, system.coords, gspec)
+        self.coulomb = HipDFKS(self.basis, self.aux_basis, system, self.grid_xyz, self.grid_w, self.xc, nelec,
+                               spec.engine, device=device)
+        self.xc_term = None  # XC is evaluated inside the native term
+        n = self.coulomb.nao
+        self.S = self.coulomb.matrix(_native.DFKS_S, (n, n))
+        self.hcore = self.coulomb.matrix(_native.DFKS_HCORE, (n, n))
+        self.e_nn = float(_nuclear_repulsion(system.coords, system.atom_charges()))
+        self.e_disp = 0.0
+        self.device = self.coulomb.device
+
+    @property
+    def nao(self) -> int:
+        return self.coulomb.nao
+
+    def energy_fock(self, P):
+        """(E_total, F) at a density P (device tensor nao x nao or 1 x nao x nao)."""
+        P2 = P[0] if P.dim() == 3 else P
+        E, F = self.coulomb.energy_fock(P2)
+        return float(E[0]), F
+
+    def electronic(self, P) -> float:
+        P2 = P[0] if P.dim() == 3 else P
+        E, _ = self.coulomb.energy_fock(P2)
+        return float(E[1] + E[2] + E[3] + E[4])
+
+    def total(self, P) -> float:
+        return self.electronic(P) + self.e_nn + self.e_disp
+
+
+def _nuclear_repulsion(xyz, z) -> float:
+    xyz = np.asarray(xyz)
+    z = np.asarray(z)
+    d = np.linalg.norm(xyz[:, None] - xyz[None], axis=-1)
+    iu = np.triu_indices(len(z), 1)
+    return float(np.sum(z[iu[0]] * z[iu[1]] / d[iu]))

@@ -1,0 +1,111 @@
+"""SCF driver on the device (dftax/ks/scf.py:394-495 surface).
+
+Each iteration runs entirely on the GPU (libdfks ``dfks_scf_step``): the
+commutator error X^T(FPS - SPF)X, Pulay DIIS (ridge 1e-10, identity-masked
+empty slots), the optional level shift, eigh(X^T F X), the aufbau density and
+the fused energy + Fock build at the new density. The host reads back three
+scalars per iteration for the convergence test |dE| < e_tol and |err| < d_tol.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+import warnings
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from dftax_amd import _native
+from dftax_amd.guess import CoreSpec, MinAOSpec, minao_density
+
+
+@dataclass
+class KSResult:
+    """Outcome of a solve (fields of dftax's KSResult, scf.py:49-69, plus timings)."""
+
+    e_tot: float
+    e_elec: float
+    converged: bool
+    n_iter: int
+    nocc: tuple
+    mo_energy: object
+    mo_coeff: object
+    P: object
+    ts: float = 0.0
+    iter_times: list = field(default_factory=list)
+    history: list = field(default_factory=list)
+
+
+def initial_density(ks, guess):
+    """Device P0 for a guess spec or an explicit array (guess.py:602-683)."""
+    torch = __import__("torch")
+    n = ks.nao
+    if guess is None or isinstance(guess, MinAOSpec):
+        name = guess.basis if isinstance(guess, MinAOSpec) else "sto-3g"
+        P = minao_density(ks.symbols, ks.atom_coords, ks.basis, name)
+        Pt = torch.tensor(P, dtype=torch.float64, device=ks.device)
+        Pt = Pt * (ks.nelec / float(torch.sum(Pt * ks.S)))
+        return Pt
+    if isinstance(guess, CoreSpec):
+        return _core_density(ks)
+    P0 = torch.as_tensor(guess, dtype=torch.float64, device=ks.device)
+    if P0.dim() == 3:
+        P0 = P0[0]
+    if tuple(P0.shape) != (n, n):
+        raise ValueError(f"guess density has shape {tuple(P0.shape)}, expected {(n, n)}")
+    return P0.contiguous()
+
+
+def _core_density(ks):
+    """Aufbau fill of the hcore eigenvectors in the orthonormal basis."""
+    torch = __import__("torch")
+    s, U = torch.linalg.eigh(ks.S)
+    keep = s > 1e-7
+    X = U[:, keep] / torch.sqrt(s[keep])[None, :]
+    _, Cp = torch.linalg.eigh(X.T @ ks.hcore @ X)
+    C = X @ Cp
+    Co = C[:, : ks.nocc[0]]
+    return (2.0 * Co @ Co.T).contiguous()
+
+
+def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, diis_space: int = 8,
+        lindep_thresh: float = 1e-7, level_shift: float = 0.0, guess=None, verbose: bool = False) -> KSResult:
+    """Run the KS SCF to self-consistency (closed shell)."""
+    torch = __import__("torch")
+    L = ks.coulomb.L
+    h = ks.coulomb.handle
+    stream = ks.coulomb._stream()
+    P0 = initial_density(ks, guess)
+    _native.check(L.dfks_scf_begin(h, ctypes.c_void_p(P0.data_ptr()), float(lindep_thresh), int(diis_space),
+                                   float(level_shift), stream))
+    out = (ctypes.c_double * 3)()
+    converged = False
+    it = 0
+    e = float("nan")
+    times, hist = [], []
+    while it < max_iter and not converged:
+        t0 = time.perf_counter()
+        _native.check(L.dfks_scf_step(h, out, stream))
+        times.append(time.perf_counter() - t0)
+        e, de, derr = out[0], out[1], out[2]
+        hist.append((e, de, derr))
+        converged = abs(de) < e_tol and derr < d_tol
+        if verbose:
+            print(f"  scf {it}: E={e:.10f} dE={de:+.2e} |[F,P]|={derr:.2e}")
+        it += 1
+    n = ks.nao
+    P = torch.empty((n, n), dtype=torch.float64, device=ks.device)
+    nmo = ctypes.c_int32()
+    _native.check(L.dfks_scf_result(h, None, None, None, ctypes.byref(nmo), stream))
+    C = torch.empty((nmo.value, n), dtype=torch.float64, device=ks.device)   # column-major nao x nmo
+    eps = torch.empty(nmo.value, dtype=torch.float64, device=ks.device)
+    _native.check(L.dfks_scf_result(h, ctypes.c_void_p(P.data_ptr()), ctypes.c_void_p(C.data_ptr()),
+                                    ctypes.c_void_p(eps.data_ptr()), ctypes.byref(nmo), stream))
+    torch.cuda.synchronize(ks.device)
+    res = KSResult(e_tot=float(e), e_elec=float(e) - ks.e_nn, converged=converged, n_iter=it, nocc=ks.nocc,
+                   mo_energy=eps[None], mo_coeff=C.T[None], P=P[None], iter_times=times, history=hist)
+    if not converged:
+        warnings.warn(f"SCF did NOT converge in {it} iterations (e_tol={e_tol}, d_tol={d_tol}); the returned "
+                      f"energy is unreliable. Increase max_iter or try level_shift>0.", stacklevel=2)
+    return res

@@ -1,0 +1,139 @@
+/*
+ * dfks.h -- C ABI of the MI355X density-fitted Kohn-Sham library (libdfks.so).
+ *
+ * One handle holds one molecule's device state: shells, grid, S/hcore, the
+ * fitted three-centre tensor B = (mn|P) L (V+ = L L^T), the AO values and
+ * gradients on the grid, and the SCF working set (DIIS history, orbitals).
+ * All matrices are row-major nao x nao FP64 device buffers owned by the
+ * caller unless stated otherwise; calls enqueue on the given HIP stream and
+ * do not synchronise it unless documented. Every call returns 0 on success
+ * and a non-zero status otherwise; dfks_last_error() gives the message
+ * (thread-local).
+ *
+ * Reference interfaces replaced (dftax, Python/JAX):
+ *   dfks_create + dfks_build     KS.__init__ / _build_integrals (dftax/ks/energy.py:439-676, 368-436)
+ *                                with coulomb=df(...) resolved by _make_coulomb (dftax/ks/terms.py:951-988)
+ *   dfks_energy_fock             KS.electronic(P) + jax.grad -> sym(dE/dP)
+ *                                (dftax/ks/energy.py:704-712, dftax/ks/scf.py:42-46);
+ *                                its terms DFCoulomb.energy (terms.py:592-613) and
+ *                                GridXC.energy (terms.py:810-828)
+ *   dfks_scf_begin/step/result   the body of _scf_solve (dftax/ks/scf.py:279-391)
+ *   dfks_becke_weights           becke_partition + weight assembly (dftax/grid/becke.py:68-108,
+ *                                dftax/grid/grid.py:240-248)
+ *   dfks_get_matrix              KS.S / KS.hcore and the integral tensors (energy.py:369-436)
+ *   dfks_overlap                 overlap_matrix / cross_overlap_matrix as used by the minao
+ *                                guess (dftax/ks/guess.py:319-337)
+ */
+#ifndef DFKS_H
+#define DFKS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dfks_ctx* dfks_handle;
+typedef void* dfks_stream; /* a hipStream_t; NULL = the default stream */
+
+/* Shell-major contracted Gaussian basis (host arrays, copied at create). */
+typedef struct {
+  int32_t nshell;
+  const int32_t* l;        /* [nshell] angular momentum, <= 4 */
+  const int32_t* nprim;    /* [nshell] */
+  const int32_t* prim0;    /* [nshell] offset into exps/coefs */
+  const int32_t* ao0;      /* [nshell] first function in the final span */
+  const double* centre;    /* [3*nshell] Bohr */
+  const double* exps;      /* [nprim_total] */
+  const double* coefs;     /* [nprim_total] normalised contraction coefficients */
+  int32_t nprim_total;
+  int32_t spherical;       /* 1: 2l+1 functions for l >= 2 */
+  int32_t nfunc;           /* functions in the final span */
+} dfks_basis_desc;
+
+typedef struct {
+  dfks_basis_desc orbital;
+  dfks_basis_desc aux;     /* density-fitting set */
+  int32_t natom;
+  const double* atom_xyz;  /* [3*natom] Bohr */
+  const double* atom_z;    /* [natom] nuclear charges */
+  int64_t ngrid;
+  const double* grid_xyz;  /* [3*ngrid] Bohr */
+  const double* grid_w;    /* [ngrid] quadrature weights */
+  int32_t xc_id;           /* 1 PBE, 2 PBE0, 3 B3LYP */
+  int32_t nelec;           /* closed shell: nocc = nelec / 2 */
+} dfks_system_desc;
+
+typedef struct {
+  int32_t precision;       /* 0: FP64 throughout */
+  double metric_cutoff;    /* relative eigenvalue cutoff of the metric pseudo-inverse (1e-7) */
+  double rho_thresh;       /* density mask of the XC quadrature (1e-10) */
+  int32_t keep_int3c;      /* 1: keep the unfitted (mn|P) tensor for inspection */
+} dfks_options;
+
+/* Matrix selectors for dfks_get_matrix. */
+enum {
+  DFKS_S = 0,        /* overlap, nao x nao */
+  DFKS_HCORE = 1,    /* T + V */
+  DFKS_KINETIC = 2,
+  DFKS_NUCLEAR = 3,
+  DFKS_INT2C = 4,    /* (P|Q), naux x naux */
+  DFKS_INT3C = 5,    /* (mn|P), nao x nao x naux (needs keep_int3c) */
+  DFKS_BFIT = 6,     /* B = (mn|P) L, nao x nao x nfit */
+  DFKS_AO = 7,       /* AO values and gradients, 4 x ngrid x nao */
+  DFKS_GRID_W = 8    /* grid weights, ngrid */
+};
+
+/* Energy components written by dfks_energy_fock (device double[8]). */
+enum {
+  DFKS_E_TOT = 0, DFKS_E_ONE = 1, DFKS_E_J = 2, DFKS_E_K = 3, DFKS_E_XC = 4, DFKS_E_NN = 5
+};
+
+const char* dfks_last_error(void);
+
+int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handle* out);
+int dfks_destroy(dfks_handle h);
+
+/* Integrals, metric factor, fitted B and AO grid. Synchronises the stream. */
+int dfks_build(dfks_handle h, dfks_stream stream);
+
+/* Sizes: [nao, naux, nfit, nocc] and ngrid. */
+int dfks_sizes(dfks_handle h, int64_t* out5);
+
+int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream stream);
+
+/* E(P) and F = sym(dE/dP) = hcore + J - a_x/2 K + Vxc for a symmetric density P
+ * (any P: K uses a signed eigen-factorisation of P). E: device double[8]. */
+int dfks_energy_fock(dfks_handle h, const double* P, double* E, double* F, dfks_stream stream);
+
+/* Same for P = sum_i s_i c_i c_i^T given its factor: Cf column-major
+ * [nf][nao] (column i contiguous), signs s_i in {+1,-1} (host array). */
+int dfks_energy_fock_factored(dfks_handle h, const double* P, const double* Cf, const double* signs_host,
+                              int32_t nf, double* E, double* F, dfks_stream stream);
+
+/* Pointwise functional f = eps_xc * rho and its derivatives f_rho, f_sigma. */
+int dfks_xc_eval(int32_t xc_id, int64_t n, const double* rho, const double* sigma, double* out3,
+                 dfks_stream stream);
+
+/* Device SCF (dftax/ks/scf.py:279-391): canonical orthonormaliser of S with
+ * eigenvalue cutoff lindep, Pulay DIIS of depth m, level shift. P0: device
+ * nao x nao initial density. Synchronises the stream. */
+int dfks_scf_begin(dfks_handle h, const double* P0, double lindep, int32_t diis_space, double level_shift,
+                   dfks_stream stream);
+/* One iteration; host out3 = {E_new, E_new - E_prev, |err|}. Synchronises. */
+int dfks_scf_step(dfks_handle h, double* out3, dfks_stream stream);
+/* Current density, orbitals (nao x nmo, column-major), orbital energies. Any may be NULL. */
+int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* nmo, dfks_stream stream);
+
+/* Overlap matrix of one basis (host out, nfunc x nfunc); used by the minao guess. */
+int dfks_overlap(const dfks_basis_desc* basis, double* host_out);
+
+/* Becke partition weights for an atom-centred grid (host arrays in and out). */
+int dfks_becke_weights(int64_t n, const double* pts, const double* raw_w, const int32_t* owner, int32_t natom,
+                       const double* atom_xyz, const double* bragg, double* w_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFKS_H */

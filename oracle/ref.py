@@ -1,0 +1,589 @@
+"""ORACLE -- test infrastructure only.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker. The product (dftax_amd) never imports it.
+
+CPU FP64 restatement of dftax's density-fitted Kohn-Sham SCF iteration:
+
+* integrals: ``md_oracle.c`` (McMurchie-Davidson, Boys), loaded with ctypes;
+* metric pseudo-inverse ``V+ = U diag(w > 1e-7 w_max ? 1/w : 0) U^T``
+  (dftax/ks/terms.py:155-192);
+* DF Coulomb/exchange energy (terms.py:500-509, 592-613):
+  ``gamma = (mn|P) P_mn``, ``E_J = 1/2 gamma V+ gamma``,
+  ``K(P)_mn = (ml|P) V+_PQ (ns|Q) P_ls``, ``E_K = -1/4 a_x sum P o K(P)``;
+* AO values on the grid (dftax/energy/gto.py:371-412) with analytic
+  gradients;
+* Becke partition (dftax/grid/becke.py:68-108);
+* functionals PBE0 and B3LYP and their parts, transcribed formula by formula
+  from dftax/energy/xc.py:64-601 in torch float64, with the rho > 1e-10
+  double-where mask of dftax/energy/grid.py:10-44;
+* the Fock matrix ``F = sym(dE/dP)`` by torch autograd, the same definition
+  as dftax/ks/scf.py:42-46 (plus an analytic J/K/Vxc form checked against it);
+* the SCF loop of dftax/ks/scf.py:252-391 (Pulay DIIS m=8, ridge 1e-10,
+  canonical orthonormaliser, aufbau fill) and the minao guess of
+  dftax/ks/guess.py:296-337, 529-545, 602-616.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "_build" / "libdfks_oracle.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(_LIB_PATH))
+        _lib.oracle_boys.restype = ctypes.c_double
+        _lib.oracle_boys.argtypes = [ctypes.c_int, ctypes.c_double]
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# Integrals
+# ---------------------------------------------------------------------------
+
+def _c2s_pack():
+    from dftax_amd.basis import cart2sph_block
+    blocks = [np.ascontiguousarray(cart2sph_block(l), dtype=np.float64) for l in range(7)]
+    off = np.zeros(7, dtype=np.int32)
+    o = 0
+    for l, b in enumerate(blocks):
+        off[l] = o
+        o += b.size
+    data = np.concatenate([b.ravel() for b in blocks])
+    return data, off
+
+
+def _basis_args(b):
+    arrs = dict(l=np.ascontiguousarray(b.l, np.int32), prim0=np.ascontiguousarray(b.prim0, np.int32),
+                nprim=np.ascontiguousarray(b.nprim, np.int32), ao0=np.ascontiguousarray(b.ao0, np.int32),
+                centre=np.ascontiguousarray(b.centre, np.float64), exps=np.ascontiguousarray(b.exps, np.float64),
+                coefs=np.ascontiguousarray(b.coefs, np.float64))
+    args = [ctypes.c_int(b.nshell), _p(arrs["l"]), _p(arrs["prim0"]), _p(arrs["nprim"]), _p(arrs["ao0"]),
+            _p(arrs["centre"]), _p(arrs["exps"]), _p(arrs["coefs"]), ctypes.c_int(int(b.spherical)),
+            ctypes.c_int(b.nao)]
+    return args, arrs
+
+
+def boys(n: int, t: float) -> float:
+    return lib().oracle_boys(int(n), float(t))
+
+
+def int1e(basis, atom_xyz, atom_z):
+    args, keep = _basis_args(basis)
+    c2s, off = _c2s_pack()
+    xyz = np.ascontiguousarray(atom_xyz, np.float64)
+    z = np.ascontiguousarray(atom_z, np.float64)
+    n = basis.nao
+    S = np.zeros((n, n)); T = np.zeros((n, n)); V = np.zeros((n, n))
+    lib().oracle_int1e(*args, ctypes.c_int(len(z)), _p(xyz), _p(z), _p(c2s), _p(off), _p(S), _p(T), _p(V))
+    return S, T, V
+
+
+def int2c(aux):
+    args, keep = _basis_args(aux)
+    c2s, off = _c2s_pack()
+    out = np.zeros((aux.nao, aux.nao))
+    lib().oracle_int2c(*args, _p(c2s), _p(off), _p(out))
+    return out
+
+
+def int3c(basis, aux):
+    a1, k1 = _basis_args(basis)
+    a2, k2 = _basis_args(aux)
+    c2s, off = _c2s_pack()
+    out = np.zeros((basis.nao, basis.nao, aux.nao))
+    lib().oracle_int3c(*a1, *a2, _p(c2s), _p(off), _p(out))
+    return out
+
+
+def nuclear_repulsion(xyz, z):
+    e = 0.0
+    for i in range(len(z)):
+        for j in range(i):
+            e += z[i] * z[j] / np.linalg.norm(xyz[i] - xyz[j])
+    return e
+
+
+def metric_pinv(V, cutoff=1e-7):
+    """terms.py:187-192: eigen-cutoff pseudo-inverse and its factor L (V+ = L L^T)."""
+    w, U = np.linalg.eigh(V)
+    keep = w > cutoff * w[-1]
+    inv_w = np.where(keep, 1.0 / np.where(keep, w, 1.0), 0.0)
+    Vp = (U * inv_w) @ U.T
+    L = U[:, keep] / np.sqrt(w[keep])[None, :]
+    return Vp, L
+
+
+# ---------------------------------------------------------------------------
+# Grid
+# ---------------------------------------------------------------------------
+
+def becke_partition(points, coords, bragg, chunk=4096):
+    """Unnormalised Becke cell functions P_A(point), (ng, natom) (becke.py:68-108)."""
+    coords = np.asarray(coords, np.float64)
+    n = coords.shape[0]
+    dAB = coords[:, None, :] - coords[None, :, :]
+    RAB = np.sqrt(np.sum(dAB * dAB, axis=2) + np.eye(n))
+    chi = bragg[:, None] / bragg[None, :]
+    u = (chi - 1.0) / (chi + 1.0)
+    a = np.clip(u / (u * u - 1.0), -0.45, 0.45)
+    eye = np.eye(n, dtype=bool)
+    out = np.empty((points.shape[0], n))
+    for s in range(0, points.shape[0], chunk):
+        p = points[s:s + chunk]
+        d = p[:, None, :] - coords[None, :, :]
+        rA = np.sqrt(np.sum(d * d, axis=2) + 1e-30)
+        mu = (rA[:, :, None] - rA[:, None, :]) / RAB[None]
+        f = mu + a[None] * (1.0 - mu * mu)
+        for _ in range(3):
+            f = 1.5 * f - 0.5 * f ** 3
+        sfun = 0.5 * (1.0 - f)
+        sfun = np.where(eye[None], 1.0, sfun)
+        out[s:s + chunk] = np.prod(sfun, axis=2)
+    return out
+
+
+def becke_grid(symbols, coords, spec):
+    """Full molecular grid (grid.py:206-255 + cutoff of energy.py:247-254)."""
+    from dftax_amd.grid import atom_centred_points, atom_bragg
+    pts, raw, owner = atom_centred_points(symbols, coords, spec)
+    bragg = atom_bragg(symbols)
+    P = becke_partition(pts, coords, bragg)
+    w = raw * P[np.arange(len(owner)), owner] / P.sum(axis=1)
+    if spec.cutoff is not None:
+        keep = np.abs(w) > spec.cutoff
+        pts, w = pts[keep], w[keep]
+    return pts, w
+
+
+# ---------------------------------------------------------------------------
+# AO values
+# ---------------------------------------------------------------------------
+
+def eval_ao(basis, points, deriv=True):
+    """AO values (ng, nao) and gradients (ng, nao, 3) in the final span."""
+    from dftax_amd.basis import cart_components, cart2sph_block
+    ng = points.shape[0]
+    ao = np.zeros((ng, basis.nao))
+    dao = np.zeros((ng, basis.nao, 3)) if deriv else None
+    for i in range(basis.nshell):
+        l = int(basis.l[i])
+        p0, npr = int(basis.prim0[i]), int(basis.nprim[i])
+        e = basis.exps[p0:p0 + npr]
+        c = basis.coefs[p0:p0 + npr]
+        d = points - basis.centre[i][None, :]
+        r2 = np.sum(d * d, axis=1)
+        ex = np.exp(-r2[:, None] * e[None, :])
+        R = ex @ c
+        dR = -2.0 * (ex @ (c * e))                   # d R / d(r^2) * 2 -> multiply by d
+        comps = cart_components(l)
+        cart = np.zeros((ng, len(comps)))
+        dcart = np.zeros((ng, len(comps), 3))
+        for k, (lx, ly, lz) in enumerate(comps):
+            px, py, pz = d[:, 0] ** lx, d[:, 1] ** ly, d[:, 2] ** lz
+            ang = px * py * pz
+            cart[:, k] = ang * R
+            if deriv:
+                gx = (lx * d[:, 0] ** (lx - 1) if lx > 0 else 0.0) * py * pz
+                gy = px * (ly * d[:, 1] ** (ly - 1) if ly > 0 else 0.0) * pz
+                gz = px * py * (lz * d[:, 2] ** (lz - 1) if lz > 0 else 0.0)
+                dcart[:, k, 0] = gx * R + ang * dR * d[:, 0]
+                dcart[:, k, 1] = gy * R + ang * dR * d[:, 1]
+                dcart[:, k, 2] = gz * R + ang * dR * d[:, 2]
+        T = cart2sph_block(l) if (basis.spherical and l >= 2) else np.eye(len(comps))
+        a0 = int(basis.ao0[i])
+        ao[:, a0:a0 + T.shape[1]] = cart @ T
+        if deriv:
+            dao[:, a0:a0 + T.shape[1], :] = np.einsum("gkx,kf->gfx", dcart, T)
+    return ao, dao
+
+
+# ---------------------------------------------------------------------------
+# Functionals (torch float64), transcribed from dftax/energy/xc.py
+# ---------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def lda_x(rho):
+    t = _torch()
+    kF = (3 * math.pi ** 2 * rho + 1e-30) ** (1 / 3)
+    return -3 * kF / (4 * math.pi)
+
+
+def pbe_x(rho, grad):
+    t = _torch()
+    kappa, mu = 0.804, 0.2195149727645171
+    gn = t.linalg.norm(grad + 1e-30, dim=-1)
+    kF = (3 * math.pi ** 2 * rho + 1e-30) ** (1 / 3)
+    s = gn / t.clamp(2 * kF * rho, min=1e-30)
+    eps_unif = -3 * kF / (4 * math.pi)
+    f = 1 + kappa - kappa / (1 + mu * s ** 2 / kappa)
+    return eps_unif * f
+
+
+_PW = np.array([
+    [0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294],
+    [0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517],
+    [0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671],
+])
+
+
+def pbe_c(rho, grad):
+    t = _torch()
+    gamma = (1.0 - math.log(2.0)) / (math.pi ** 2)
+    beta = 0.06672455060314922
+    fz20 = 1.709920934161365617563962776245
+    rho_tot = t.clamp(rho, min=1e-20)
+    rs = (3 / (4 * math.pi * rho_tot) + 1e-20) ** (1 / 3)
+    zeta, phi = 0.0, 1.0
+    kf = (3 * math.pi ** 2 * rho_tot + 1e-20) ** (1 / 3)
+    ks = t.sqrt(4 * kf / math.pi)
+    gnorm = t.sqrt(t.sum(grad ** 2, dim=-1) + 1e-30)
+    tt = gnorm / t.clamp(2 * phi * ks * rho_tot, min=1e-20)
+    g = []
+    rs_sqrt = t.sqrt(rs)
+    for row in range(3):
+        A, a1, b1, b2, b3, b4 = _PW[row]
+        denom = 2.0 * A * (b1 * rs_sqrt + b2 * rs + b3 * rs * rs_sqrt + b4 * rs ** 2)
+        g.append(-2.0 * A * (1.0 + a1 * rs) * t.log1p(1 / t.clamp(denom, min=1e-20)))
+    f_z = ((1.0 + zeta) ** (4 / 3) + (1.0 - zeta) ** (4 / 3) - 2.0) / (2.0 ** (4 / 3) - 2.0)
+    eps_pw = g[0] + (zeta ** 4) * f_z * (g[1] - g[0] + g[2] / fz20) - f_z * g[2] / fz20
+    gphi3 = gamma * phi ** 3
+    denom_A = gamma * t.expm1(-eps_pw / gphi3)
+    Acoef = beta / t.clamp(denom_A, min=1e-20)
+    f1 = tt ** 2 + Acoef * tt ** 4
+    frac = (beta * f1) / t.clamp(gamma * (1.0 + Acoef * f1), min=1e-20)
+    return eps_pw + gphi3 * t.log1p(frac)
+
+
+def b88_x(rho, grad):
+    t = _torch()
+    beta = 0.0042
+    gn = t.linalg.norm(grad + 1e-30, dim=-1)
+    r = t.clamp(rho / 2, min=1e-20)
+    r13 = r ** (1 / 3)
+    C = 0.75 * (6 / math.pi) ** (1 / 3)
+    x = (gn / 2) / t.clamp(r ** (4 / 3), min=1e-20)
+    denom = 1.0 + 6 * beta * x * t.asinh(x)
+    return -C * r13 - beta * r13 * x ** 2 / t.clamp(denom, min=1e-20)
+
+
+def _vwn_branch(x, A, b, c, x0):
+    t = _torch()
+    Xx = x * x + b * x + c
+    Xx0 = x0 * x0 + b * x0 + c
+    Q = math.sqrt(4.0 * c - b * b)
+    at = t.atan(Q / t.clamp(2.0 * x + b, min=1e-20))
+    term1 = t.log(x * x / t.clamp(Xx, min=1e-20))
+    term2 = (2.0 * b / Q) * at
+    term3 = -(b * x0 / Xx0) * (t.log((x - x0) ** 2 / t.clamp(Xx, min=1e-20)) + (2.0 * (b + 2.0 * x0) / Q) * at)
+    return A * (term1 + term2 + term3)
+
+
+def vwn_rpa_c(rho):
+    t = _torch()
+    r = t.clamp(rho, min=1e-20)
+    rs = (3.0 / (4.0 * math.pi * r)) ** (1 / 3)
+    return _vwn_branch(t.sqrt(rs), 0.0310907, 13.0720, 42.7198, -0.409286)
+
+
+def lyp_c(rho, grad):
+    t = _torch()
+    a, b, c, d = 0.04918, 0.132, 0.2533, 0.349
+    C_F = 0.3 * (3.0 * math.pi ** 2) ** (2 / 3)
+    gam = t.sum(grad ** 2, dim=-1)
+    r = t.clamp(rho, min=1e-20)
+    ri3 = r ** (-1 / 3)
+    g = 1.0 / (1.0 + d * ri3)
+    omega = t.exp(-c * ri3) * g * r ** (-11 / 3)
+    delta = c * ri3 + d * ri3 * g
+    term_ff = C_F * r ** (8 / 3) - gam * (3.0 + 7.0 * delta) / 72.0
+    return -a * g - a * b * omega * r * term_ff
+
+
+def eps_xc(name, rho, grad):
+    if name == "PBE0":
+        return 0.75 * pbe_x(rho, grad) + pbe_c(rho, grad)
+    if name == "PBE":
+        return pbe_x(rho, grad) + pbe_c(rho, grad)
+    if name == "B3LYP":
+        return 0.08 * lda_x(rho) + 0.72 * b88_x(rho, grad) + 0.19 * vwn_rpa_c(rho) + 0.81 * lyp_c(rho, grad)
+    raise ValueError(name)
+
+
+HF_COEFF = {"PBE0": 0.25, "B3LYP": 0.20, "PBE": 0.0}
+
+
+def xc_energy_t(name, rho, grad, w, thresh=1e-10):
+    """energy/grid.py:10-44: nan-safe masked quadrature."""
+    t = _torch()
+    mask = rho > thresh
+    safe_rho = t.where(mask, rho, t.ones_like(rho))
+    safe_grad = t.where(mask[:, None], grad, t.zeros_like(grad))
+    eps = eps_xc(name, safe_rho, safe_grad)
+    return t.sum(t.where(mask, w * eps * rho, t.zeros_like(rho)))
+
+
+# ---------------------------------------------------------------------------
+# The DF-KS energy functional E(P) and its Fock matrix
+# ---------------------------------------------------------------------------
+
+class OracleKS:
+    """All FP64 inputs of one molecule, built by the oracle's own routines."""
+
+    def __init__(self, mol, xc, auxbasis="even-tempered-jk", grid=None, int3c_tensor=True):
+        from dftax_amd.basis import build_basis
+        from dftax_amd.grid import becke as becke_spec
+        self.mol = mol
+        self.xc = xc
+        self.hf = HF_COEFF[xc]
+        self.basis = build_basis(mol.symbols, mol.coords, mol.basis, spherical=mol.spherical)
+        self.aux = build_basis(mol.symbols, mol.coords, auxbasis, spherical=True)
+        z = mol.atom_charges()
+        self.S, T, Vn = int1e(self.basis, mol.coords, z)
+        self.hcore = T + Vn
+        self.e_nn = nuclear_repulsion(mol.coords, z)
+        self.int2c = int2c(self.aux)
+        self.Vp, self.L = metric_pinv(self.int2c)
+        self.int3c = int3c(self.basis, self.aux) if int3c_tensor else None
+        spec = grid if grid is not None else becke_spec()
+        self.grid_xyz, self.grid_w = becke_grid(mol.symbols, mol.coords, spec)
+        self.ao, self.dao = eval_ao(self.basis, self.grid_xyz)
+        self.nocc = mol.nelectron // 2
+
+    # -- energy pieces --------------------------------------------------------
+    def energy_t(self, P):
+        """Total energy as a torch function of P (nao, nao): terms.py:592-613 + 791-828."""
+        t = _torch()
+        I3 = t.from_numpy(self.int3c)
+        Vp = t.from_numpy(self.Vp)
+        ao = t.from_numpy(self.ao)
+        dao = t.from_numpy(self.dao)
+        w = t.from_numpy(self.grid_w)
+        e1 = t.sum(P * t.from_numpy(self.hcore))
+        gamma = t.einsum("mnP,mn->P", I3, P)
+        eJ = 0.5 * t.dot(gamma, Vp @ gamma)
+        eK = t.zeros((), dtype=t.float64)
+        if self.hf != 0.0:
+            Kmat = t.einsum("mlP,PQ,nsQ,ls->mn", I3, Vp, I3, P)
+            eK = -0.25 * self.hf * t.sum(P * Kmat)
+        rho = t.einsum("gm,mn,gn->g", ao, P, ao)
+        grad = 2.0 * t.einsum("gm,mn,gnx->gx", ao, P, dao)
+        exc = xc_energy_t(self.xc, rho, grad, w)
+        return e1 + eJ + eK + exc + self.e_nn, dict(e1=e1, eJ=eJ, eK=eK, exc=exc)
+
+    def energy_fock_autograd(self, P):
+        t = _torch()
+        Pt = t.tensor(P, dtype=t.float64, requires_grad=True)
+        e, parts = self.energy_t(Pt)
+        (g,) = t.autograd.grad(e, Pt)
+        g = g.numpy()
+        return float(e), 0.5 * (g + g.T), {k: float(v) for k, v in parts.items()}
+
+    def energy_fock(self, P):
+        """Analytic form: F = hcore + J - 1/2 a_x K + Vxc (checked against autograd)."""
+        t = _torch()
+        B = np.einsum("mnP,PQ->mnQ", self.int3c, self.L, optimize=True)
+        nao = P.shape[0]
+        Bm = B.reshape(nao * nao, -1)
+        g = Bm.T @ P.reshape(-1)
+        eJ = 0.5 * float(g @ g)
+        J = (Bm @ g).reshape(nao, nao)
+        eK = 0.0
+        K = np.zeros_like(P)
+        if self.hf != 0.0:
+            for q in range(B.shape[2]):
+                Bq = B[:, :, q]
+                K += Bq @ P @ Bq
+            eK = -0.25 * self.hf * float(np.sum(P * K))
+        # XC via pointwise autograd of the functional
+        rho = np.einsum("gm,mn,gn->g", self.ao, P, self.ao, optimize=True)
+        grad = 2.0 * np.einsum("gm,mn,gnx->gx", self.ao, P, self.dao, optimize=True)
+        rt = t.tensor(rho, requires_grad=True)
+        gt = t.tensor(grad, requires_grad=True)
+        wt = t.from_numpy(self.grid_w)
+        exc_t = xc_energy_t(self.xc, rt, gt, wt)
+        vr, vg = t.autograd.grad(exc_t, (rt, gt))
+        vr, vg = vr.numpy(), vg.numpy()
+        W = 0.5 * vr[:, None] * self.ao + np.einsum("gx,gnx->gn", vg, self.dao)
+        Vxc = self.ao.T @ W
+        Vxc = Vxc + Vxc.T
+        e1 = float(np.sum(P * self.hcore))
+        e = e1 + eJ + eK + float(exc_t) + self.e_nn
+        F = self.hcore + J - 0.5 * self.hf * K + Vxc
+        return e, F, dict(e1=e1, eJ=eJ, eK=eK, exc=float(exc_t), J=J, K=K, Vxc=Vxc)
+
+
+# ---------------------------------------------------------------------------
+# Guess and SCF (dftax/ks/guess.py, dftax/ks/scf.py)
+# ---------------------------------------------------------------------------
+
+_MADELUNG = ((1, 0), (2, 0), (2, 1), (3, 0), (3, 1))
+
+
+def _ground_state_config(Z):
+    left = Z
+    out = []
+    for n, l in _MADELUNG:
+        c = min(left, 2 * (2 * l + 1))
+        if c > 0:
+            out.append((n, l, c))
+        left -= c
+        if left == 0:
+            break
+    return out
+
+
+def minao_density(mol, basis):
+    """Projected sto-3g atomic densities (guess.py:296-337, 529-545)."""
+    from dftax_amd.basis import build_basis
+    from dftax_amd.system import symbol_to_Z
+    nao = basis.nao
+    P = np.zeros((nao, nao))
+    slices = basis.atom_ao_slices(mol.natm)
+    cache = {}
+    for a, sym in enumerate(mol.symbols):
+        sl = slices[a]
+        if sym not in cache:
+            idx = np.where(basis.atom == a)[0]
+            sub = _sub_basis(basis, idx)
+            minb = build_basis([sym], mol.coords[a:a + 1], "sto-3g", spherical=True)
+            per_l = {}
+            for _, l, c in _ground_state_config(symbol_to_Z(sym)):
+                per_l.setdefault(l, []).append(float(c))
+            f = []
+            for ish in range(minb.nshell):
+                l = int(minb.l[ish])
+                lst = per_l.get(l, [])
+                c = lst.pop(0) if lst else 0.0
+                f.extend([c / (2 * l + 1)] * (2 * l + 1))
+            f = np.array(f)
+            S_aa = _overlap_pair(sub, sub)
+            S_am = _overlap_pair(sub, minb)
+            C = np.linalg.solve(S_aa, S_am)
+            cache[sym] = (C * f[None, :]) @ C.T
+        P[sl, sl] = cache[sym]
+    return P
+
+
+def _sub_basis(basis, idx):
+    from dftax_amd.basis import BasisSet
+    ls, prim0, nprim, ao0, cart0, exps, coefs = [], [], [], [], [], [], []
+    p = ao = cart = 0
+    from dftax_amd.basis import ncart, nsph
+    for i in idx:
+        l = int(basis.l[i])
+        a, n = int(basis.prim0[i]), int(basis.nprim[i])
+        ls.append(l); prim0.append(p); nprim.append(n); ao0.append(ao); cart0.append(cart)
+        exps.extend(basis.exps[a:a + n]); coefs.extend(basis.coefs[a:a + n])
+        p += n
+        ao += nsph(l) if (basis.spherical and l >= 2) else ncart(l)
+        cart += ncart(l)
+    return BasisSet(l=np.array(ls, np.int32), atom=np.zeros(len(ls), np.int32),
+                    centre=basis.centre[idx].copy(), prim0=np.array(prim0, np.int32),
+                    nprim=np.array(nprim, np.int32), ao0=np.array(ao0, np.int32),
+                    cart0=np.array(cart0, np.int32), exps=np.array(exps), coefs=np.array(coefs),
+                    spherical=basis.spherical, nao=ao, nao_cart=cart)
+
+
+def _overlap_pair(b1, b2):
+    """Cross overlap <b1|b2> by the oracle's 1e routine on the union basis."""
+    from dftax_amd.basis import BasisSet
+    n1 = b1.nao
+    off = b1.exps.shape[0]
+    u = BasisSet(l=np.concatenate([b1.l, b2.l]), atom=np.zeros(b1.nshell + b2.nshell, np.int32),
+                 centre=np.concatenate([b1.centre, b2.centre]),
+                 prim0=np.concatenate([b1.prim0, b2.prim0 + off]), nprim=np.concatenate([b1.nprim, b2.nprim]),
+                 ao0=np.concatenate([b1.ao0, b2.ao0 + n1]), cart0=np.concatenate([b1.cart0, b2.cart0 + b1.nao_cart]),
+                 exps=np.concatenate([b1.exps, b2.exps]), coefs=np.concatenate([b1.coefs, b2.coefs]),
+                 spherical=b1.spherical, nao=n1 + b2.nao, nao_cart=b1.nao_cart + b2.nao_cart)
+    S, _, _ = int1e(u, np.zeros((1, 3)), np.zeros(1))
+    return S[:n1, n1:] if b2 is not b1 else S[:n1, :n1]
+
+
+def canonical_orthonormalizer(S, thresh=1e-7):
+    s, U = np.linalg.eigh(S)
+    keep = s > thresh
+    return U[:, keep] / np.sqrt(s[keep])[None, :]
+
+
+def diis_extrapolate(dF, dErr, count, m):
+    errs = dErr.reshape(m, -1)
+    B = errs @ errs.T
+    valid = np.arange(m) < count
+    vv = valid[:, None] & valid[None, :]
+    B = np.where(vv, B, np.eye(m))
+    B = B + 1e-10 * np.diag(valid.astype(float))
+    A = np.zeros((m + 1, m + 1))
+    A[:m, :m] = B
+    border = np.where(valid, -1.0, 0.0)
+    A[:m, m] = border
+    A[m, :m] = border
+    rhs = np.zeros(m + 1)
+    rhs[m] = -1.0
+    c = np.linalg.solve(A, rhs)[:m]
+    return np.einsum("i,iab->ab", c, dF)
+
+
+def scf(oks, max_iter=128, e_tol=1e-8, d_tol=1e-6, m=8, P0=None, fock=None, verbose=False):
+    """The reference loop (scf.py:279-391), restated on the analytic Fock."""
+    fock = fock or oks.energy_fock
+    S = oks.S
+    X = canonical_orthonormalizer(S)
+    nao, nmo = X.shape
+    nocc = oks.nocc
+    if P0 is None:
+        P0 = minao_density(oks.mol, oks.basis)
+        P0 = P0 * (oks.mol.nelectron / np.sum(P0 * S))
+    P = P0
+    e_prev, F, _ = fock(P)
+    dF = np.zeros((m, nao, nao))
+    dE = np.zeros((m, nmo, nmo))
+    conv = False
+    it = 0
+    hist = []
+    while it < max_iter and not conv:
+        err = X.T @ (F @ P @ S - S @ P @ F) @ X
+        derr = float(np.linalg.norm(err))
+        slot = it % m
+        dF[slot] = F
+        dE[slot] = err
+        Fx = diis_extrapolate(dF, dE, min(it + 1, m), m)
+        eps, Cp = np.linalg.eigh(X.T @ Fx @ X)
+        C = X @ Cp
+        Co = C[:, :nocc]
+        P = 2.0 * Co @ Co.T
+        e, F, _ = fock(P)
+        de = e - e_prev
+        conv = abs(de) < e_tol and derr < d_tol
+        hist.append((e, de, derr))
+        if verbose:
+            print(f"  scf {it}: E={e:.10f} dE={de:+.2e} |[F,P]|={derr:.2e}")
+        e_prev = e
+        it += 1
+    return dict(e_tot=e_prev, P=P, C=C, eps=eps, converged=conv, n_iter=it, hist=hist)

@@ -1,0 +1,182 @@
+"""GPU parity: the HIP path through the C ABI against the CPU FP64 oracle.
+
+Tolerances: 1e-10 max-abs on (ij|P) and on the
+Fock matrix, 1e-8 Ha on the total energy.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_INT = 1e-10
+TOL_FOCK = 1e-10
+TOL_E = 1e-8
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _oracle(mol, xc, grid=None):
+    from oracle import ref
+    return ref.OracleKS(mol, xc, grid=grid)
+
+
+def _hip(mol, xc, grid=None, keep=True):
+    from dftax_amd import KS, df, hip
+    return KS(mol, xc, grid=grid, coulomb=df(engine=hip(keep_int3c=keep)))
+
+
+@pytest.fixture(scope="module")
+def water_pair(torch_cuda, water_svp):
+    _say("case: water def2-SVP PBE0, default grid")
+    return _oracle(water_svp, "PBE0"), _hip(water_svp, "PBE0")
+
+
+@pytest.fixture(scope="module")
+def dimer_pair(torch_cuda, water_dimer_tzvp):
+    from dftax_amd import becke
+    _say("case: water dimer def2-TZVP B3LYP, becke(35, 110)")
+    g = becke(35, 110)
+    return _oracle(water_dimer_tzvp, "B3LYP", grid=g), _hip(water_dimer_tzvp, "B3LYP", grid=g)
+
+
+def _mat(ks, which, shape):
+    return ks.coulomb.matrix(which, shape).cpu().numpy()
+
+
+@pytest.mark.parametrize("case", ["water_pair", "dimer_pair"])
+def test_one_electron(case, request):
+    from dftax_amd import _native as N
+    o, k = request.getfixturevalue(case)
+    n = k.nao
+    assert np.abs(_mat(k, N.DFKS_S, (n, n)) - o.S).max() < TOL_INT
+    assert np.abs(_mat(k, N.DFKS_HCORE, (n, n)) - o.hcore).max() < TOL_INT
+
+
+@pytest.mark.parametrize("case", ["water_pair", "dimer_pair"])
+def test_int2c(case, request):
+    from dftax_amd import _native as N
+    o, k = request.getfixturevalue(case)
+    m = k.coulomb.naux
+    assert np.abs(_mat(k, N.DFKS_INT2C, (m, m)) - o.int2c).max() < TOL_INT
+
+
+@pytest.mark.parametrize("case", ["water_pair", "dimer_pair"])
+def test_int3c(case, request):
+    from dftax_amd import _native as N
+    o, k = request.getfixturevalue(case)
+    n, m = k.nao, k.coulomb.naux
+    g = _mat(k, N.DFKS_INT3C, (n, n, m))
+    assert np.abs(g - o.int3c).max() < TOL_INT
+    assert np.abs(g - g.transpose(1, 0, 2)).max() < 1e-13
+
+
+@pytest.mark.parametrize("case", ["water_pair", "dimer_pair"])
+def test_fitted_tensor(case, request):
+    """B B^T reproduces the fitted 4-index (mn|V+|ls) of the oracle (basis-independent of L)."""
+    from dftax_amd import _native as N
+    o, k = request.getfixturevalue(case)
+    n = k.nao
+    B = _mat(k, N.DFKS_BFIT, (n * n, k.coulomb.nfit))
+    I = o.int3c.reshape(n * n, -1)
+    ref = I @ o.Vp @ I.T
+    got = B @ B.T
+    assert k.coulomb.nfit == o.L.shape[1]
+    assert np.abs(got - ref).max() < 1e-9 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("case", ["water_pair", "dimer_pair"])
+def test_grid_and_ao(case, request):
+    from dftax_amd import _native as N
+    o, k = request.getfixturevalue(case)
+    assert k.grid_w.shape == o.grid_w.shape
+    assert np.abs(k.grid_xyz - o.grid_xyz).max() == 0.0
+    assert np.abs(k.grid_w - o.grid_w).max() < 1e-13 * max(1.0, np.abs(o.grid_w).max())
+    ng, n = k.grid_w.shape[0], k.nao
+    ao = _mat(k, N.DFKS_AO, (4, ng, n))
+    assert np.abs(ao[0] - o.ao).max() < 1e-12
+    assert np.abs(np.moveaxis(ao[1:], 0, -1) - o.dao).max() < 1e-11
+
+
+@pytest.mark.parametrize("xc", ["PBE", "PBE0", "B3LYP"])
+def test_xc_pointwise(torch_cuda, xc):
+    import ctypes
+    from dftax_amd import _native as N
+    from dftax_amd.xc import resolve
+    from oracle import ref
+    torch = torch_cuda
+    rng = np.random.default_rng(7)
+    rho = 10.0 ** rng.uniform(-9.5, 3, size=4000)
+    sigma = (rho ** (8 / 3)) * 10.0 ** rng.uniform(-4, 2, size=4000)
+    sigma[:10] = 0.0
+    r = torch.tensor(rho, device="cuda")
+    s = torch.tensor(sigma, device="cuda")
+    out = torch.empty((4000, 3), dtype=torch.float64, device="cuda")
+    N.check(N.lib().dfks_xc_eval(resolve(xc).xc_id, 4000, ctypes.c_void_p(r.data_ptr()),
+                                 ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(out.data_ptr()), None))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    # oracle: the torch transcription with a gradient vector along x
+    rt = torch.tensor(rho, requires_grad=True)
+    gt = torch.zeros((4000, 3), dtype=torch.float64)
+    gt[:, 0] = torch.tensor(np.sqrt(sigma))
+    gt.requires_grad_(True)
+    f = ref.eps_xc(xc, rt, gt) * rt
+    fr, fg = torch.autograd.grad(f.sum(), (rt, gt))
+    f = f.detach().numpy()
+    fr = fr.numpy()
+    # d f / d grad_x = 2 f_sigma grad_x
+    gx = np.sqrt(sigma)
+    mask = gx > 0
+    fs = np.where(mask, fg.numpy()[:, 0] / np.where(mask, 2 * gx, 1.0), got[:, 2])
+    # value to 1e-12 relative; derivatives to 1e-10 (autograd through |grad| vs dual numbers)
+    assert np.all(np.abs(got[:, 0] - f) <= 1e-12 * np.maximum(1.0, np.abs(f)))
+    for a, b in ((got[:, 1], fr), (got[:, 2], fs)):
+        assert np.all(np.abs(a - b) <= 1e-10 * np.maximum(1.0, np.abs(b)))
+
+
+@pytest.mark.parametrize("case", ["water_pair", "dimer_pair"])
+def test_energy_fock_guess_density(case, request, torch_cuda):
+    """Non-idempotent minao P0: general signed-factor path, all energy terms and F."""
+    from oracle import ref
+    o, k = request.getfixturevalue(case)
+    torch = torch_cuda
+    P0 = ref.minao_density(o.mol, o.basis)
+    P0 *= o.mol.nelectron / np.sum(P0 * o.S)
+    e_ref, F_ref, parts = o.energy_fock(P0)
+    E, F = k.coulomb.energy_fock(torch.tensor(P0, device="cuda"))
+    E = E.cpu().numpy()
+    F = F.cpu().numpy()
+    _say(f"E gpu {E[0]:.12f} oracle {e_ref:.12f}")
+    assert abs(E[1] - parts["e1"]) < 1e-9
+    assert abs(E[2] - parts["eJ"]) < 1e-9
+    assert abs(E[3] - parts["eK"]) < 1e-9
+    assert abs(E[4] - parts["exc"]) < 1e-9
+    assert abs(E[0] - e_ref) < TOL_E
+    assert np.abs(F - F_ref).max() < TOL_FOCK
+
+
+def test_scf_water_pbe0(water_pair):
+    """Full SCF: device DIIS + eigensolve against the oracle's restated loop."""
+    from dftax_amd import scf
+    from oracle import ref
+    o, k = water_pair
+    r_ref = ref.scf(o, e_tol=1e-9)
+    r = scf(k, e_tol=1e-9)
+    _say(f"SCF gpu {r.e_tot:.12f} ({r.n_iter} it) oracle {r_ref['e_tot']:.12f} ({r_ref['n_iter']} it)")
+    assert r.converged and r_ref["converged"]
+    assert abs(r.e_tot - r_ref["e_tot"]) < TOL_E
+    P = r.P[0].cpu().numpy()
+    # idempotency P S P = 2 P and Tr(P S) = N
+    S = o.S
+    assert np.abs(P @ S @ P - 2 * P).max() < 1e-8
+    assert abs(np.sum(P * S) - o.mol.nelectron) < 1e-10
