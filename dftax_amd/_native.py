@@ -41,8 +41,11 @@ DFKS_S, DFKS_HCORE, DFKS_KINETIC, DFKS_NUCLEAR, DFKS_INT2C, DFKS_INT3C, DFKS_BFI
 EXPORTS = [
     "dfks_last_error", "dfks_create", "dfks_destroy", "dfks_build", "dfks_sizes", "dfks_get_matrix",
     "dfks_energy_fock", "dfks_energy_fock_factored", "dfks_xc_eval", "dfks_scf_begin", "dfks_scf_step",
-    "dfks_scf_result", "dfks_becke_weights", "dfks_overlap",
+    "dfks_scf_result", "dfks_becke_weights", "dfks_overlap", "dfks_set_timing", "dfks_stage_times",
 ]
+
+STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
+          "scf_diis", "scf_eigh", "build_1e", "build_2c", "build_3c", "build_metric_B", "build_ao"]
 
 _lib = None
 
@@ -77,6 +80,8 @@ def lib():
     L.dfks_scf_result.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, P_i32, c_void_p]
     L.dfks_becke_weights.argtypes = [c_int64, P_f64, P_f64, P_i32, c_int32, P_f64, P_f64, P_f64]
     L.dfks_overlap.argtypes = [ctypes.POINTER(BasisDesc), P_f64]
+    L.dfks_set_timing.argtypes = [c_void_p, c_int32]
+    L.dfks_stage_times.argtypes = [c_void_p, P_f64, c_int32]
     _lib = L
     return L
 
@@ -130,3 +135,9 @@ def overlap(basis) -> np.ndarray:
     out = np.empty((basis.nao, basis.nao))
     check(lib().dfks_overlap(ctypes.byref(arrs.desc), f64p(out)))
     return out
+
+
+def stage_times(handle) -> dict:
+    out = (ctypes.c_double * len(STAGES))()
+    check(lib().dfks_stage_times(handle, out, len(STAGES)))
+    return {k: float(v) for k, v in zip(STAGES, out)}

@@ -14,6 +14,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -58,6 +59,20 @@ hipError_t launch_scaled_cols(int n, int k0, const double* U, const double* w, d
 }  // namespace dfks
 
 using namespace dfks;
+
+// Fock-build / SCF stages timed by dfks_stage_times (event i marks the end of stage i-1).
+enum Stage {
+  kStExchangeGemm = 0,  // T = B^T Cf (all chunks, includes gamma)
+  kStExchangeSyrk = 1,  // K = T^T T
+  kStCoulomb = 2,       // E_J and J = B gamma
+  kStXcGemm = 3,        // Z = Phi P
+  kStXcKernel = 4,      // rho, functional, W
+  kStXcSyr2k = 5,       // Vxc = Phi^T W + W^T Phi
+  kStFock = 6,          // assembly and energies
+  kStScfDiis = 7,       // commutator + DIIS
+  kStScfEigh = 8,       // X^T F X, eigensolve, C, P
+  kNumStages = 9
+};
 
 namespace {
 
@@ -184,6 +199,18 @@ struct dfks_ctx {
   Basis orb, aux;
   std::vector<double> atm_host;  // x, y, z, Z
   std::vector<double> h_sgn;     // factor signs of the last Fock build
+  // stage timers: events recorded on the work stream when timing is on
+  bool timing = false;
+  hipEvent_t scf_ev[3] = {};
+  hipEvent_t ev[kNumStages + 1] = {};
+  bool ev_used[kNumStages + 1] = {};
+  double build_ms[8] = {};
+  void mark(int i, hipStream_t st) {
+    if (!timing) return;
+    if (!ev[i]) (void)hipEventCreate(&ev[i]);
+    (void)hipEventRecord(ev[i], st);
+    ev_used[i] = true;
+  }
   rocblas_handle rb = nullptr;
   bool built = false;
 
@@ -211,6 +238,10 @@ struct dfks_ctx {
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
                     d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws};
     for (void* p : ptrs) dfree(p);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : scf_ev)
+      if (e) (void)hipEventDestroy(e);
     if (rb) rocblas_destroy_handle(rb);
   }
 };
@@ -239,8 +270,13 @@ void sync(hipStream_t st) { HIPCHK(hipStreamSynchronize(st)); }
 // ---------------------------------------------------------------------------
 // build
 // ---------------------------------------------------------------------------
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void build_all(dfks_ctx* h, hipStream_t st) {
   RBCHK(rocblas_set_stream(h->rb, st));
+  double t0 = now_ms();
   const int nao = h->nao, naux = h->naux;
   const ShellSet O = h->orb.view(), A = h->aux.view();
   // one-electron
@@ -251,6 +287,9 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   HIPCHK(launch_int1e(O, h->natom, h->d_atm, h->d_boys, h->d_S, h->d_T, h->d_V, nao, st));
   HIPCHK(hipMemcpyAsync(h->d_H, h->d_T, sizeof(double) * nao * nao, hipMemcpyDeviceToDevice, st));
   HIPCHK(launch_axpy((int64_t)nao * nao, 1.0, h->d_V, h->d_H, 1, st));
+  sync(st);
+  h->build_ms[0] = now_ms() - t0;
+  t0 = now_ms();
 
   // class lists
   const int lo = h->orb.maxl(), lx = h->aux.maxl();
@@ -285,6 +324,9 @@ void build_all(dfks_ctx* h, hipStream_t st) {
     for (int lc = 0; lc <= lx; ++lc)
       HIPCHK(launch_eri_wave(la, 0, lc, d_pairs + apair_off[la], (int)apairs[la].size(), d_auxl + aux_off[lc],
                              (int)auxl[lc].size(), A, A, 1, h->d_boys, h->d_int2c, naux, 0, 0, st));
+  sync(st);
+  h->build_ms[1] = now_ms() - t0;
+  t0 = now_ms();
   // three-centre
   const size_t n3 = (size_t)nao * nao * naux;
   double* d3 = dalloc<double>(n3);
@@ -297,6 +339,8 @@ void build_all(dfks_ctx* h, hipStream_t st) {
                                (int64_t)nao * naux, naux, 1, st));
       }
   sync(st);
+  h->build_ms[2] = now_ms() - t0;
+  t0 = now_ms();
   dfree(d_pairs);
   dfree(d_auxl);
 
@@ -324,6 +368,8 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, h->nfit, (int64_t)nao * nao, naux,
                          &one, d_Lt, h->nfit, d3, naux, &zero, h->d_B, h->nfit));
   sync(st);
+  h->build_ms[3] = now_ms() - t0;
+  t0 = now_ms();
   dfree(d_U); dfree(d_w); dfree(d_e); dfree(d_Lt);
   if (h->opt.keep_int3c) h->d_int3c = d3;
   else dfree(d3);
@@ -350,6 +396,7 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   h->d_offd = dalloc<double>(nao);
   h->d_sel = dalloc<int32_t>(nao);
   sync(st);
+  h->build_ms[4] = now_ms() - t0;
   h->built = true;
 }
 
@@ -363,6 +410,7 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
   const int nao = h->nao, nfit = h->nfit;
   const int64_t nn = (int64_t)nao * nao;
   const double one = 1.0, zero = 0.0;
+  h->mark(0, st);
   HIPCHK(hipMemsetAsync(E, 0, 8 * sizeof(double), st));
   // one-electron energy
   HIPCHK(launch_dot(nn, P, h->d_H, h->d_part, 1.0, E + DFKS_E_ONE, 0, st));
@@ -385,36 +433,42 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
                            (int64_t)nao * nfit, nao, &one, Cf + (int64_t)i0 * nao, nao, h->d_B,
                            (int64_t)nao * nfit, &zero, h->d_Tx, nc));
     HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, 1, st));
+    h->mark(kStExchangeGemm + 1, st);
     if (h->hf != 0.0) {
       const double beta = first ? 0.0 : 1.0;
-      RBCHK(rocblas_dsyrk_64(h->rb, rocblas_fill_upper, rocblas_operation_transpose, nao, (int64_t)nfit * nc, &sg,
-                             h->d_Tx, (int64_t)nfit * nc, &beta, h->d_K, nao));
+      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_none, nao, nao,
+                             (int64_t)nfit * nc, &sg, h->d_Tx, (int64_t)nfit * nc, h->d_Tx, (int64_t)nfit * nc,
+                             &beta, h->d_K, nao));
     }
+    h->mark(kStExchangeSyrk + 1, st);
     first = false;
     i0 = i1;
   }
   // E_J = 1/2 |gamma|^2, J = B gamma
   HIPCHK(launch_dot(nfit, h->d_gamma, h->d_gamma, h->d_part, 0.5, E + DFKS_E_J, 0, st));
   HIPCHK(launch_jvec(nn, nfit, h->d_B, h->d_gamma, h->d_J, st));
+  h->mark(kStCoulomb + 1, st);
   // E_K = -1/4 a_x sum P o K
-  if (h->hf != 0.0) {
-    HIPCHK(launch_symfill(nao, h->d_K, st));
-    HIPCHK(launch_dot(nn, P, h->d_K, h->d_part, -0.25 * h->hf, E + DFKS_E_K, 0, st));
-  }
+  if (h->hf != 0.0) HIPCHK(launch_dot(nn, P, h->d_K, h->d_part, -0.25 * h->hf, E + DFKS_E_K, 0, st));
   // XC
   if (h->ng > 0) {
     RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, nao, h->ng, nao, &one, P, nao,
                            h->d_ao, nao, &zero, h->d_zw, nao));
+    h->mark(kStXcGemm + 1, st);
     HIPCHK(launch_xc(h->ng, nao, h->d_ao, h->d_gw, h->d_zw, h->xc_id, h->opt.rho_thresh, h->d_exc, nullptr, st));
     HIPCHK(launch_dot(h->ng, h->d_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
-    RBCHK(rocblas_dsyr2k_64(h->rb, rocblas_fill_upper, rocblas_operation_none, nao, h->ng, &one, h->d_ao, nao,
-                            h->d_zw, nao, &zero, h->d_Vxc, nao));
+    h->mark(kStXcKernel + 1, st);
+    // Vh = Phi^T W (column-major: ao_cm (nao x ng) times zw_cm^T); V = Vh + Vh^T in the Fock assembly
+    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_transpose, nao, nao, h->ng, &one,
+                           h->d_ao, nao, h->d_zw, nao, &zero, h->d_Vxc, nao));
+    h->mark(kStXcSyr2k + 1, st);
   }
   HIPCHK(launch_fock(nao, h->d_H, h->d_J, h->hf != 0.0 ? h->d_K : nullptr, -0.5 * h->hf,
                      h->ng > 0 ? h->d_Vxc : nullptr, F, st));
   // totals: E_tot = e1 + eJ + eK + exc + enn
   HIPCHK(hipMemcpyAsync(E + DFKS_E_NN, &h->e_nn, sizeof(double), hipMemcpyHostToDevice, st));
   HIPCHK(launch_dot(5, E + 1, nullptr, h->d_part, 1.0, E + DFKS_E_TOT, 0, st));
+  h->mark(kStFock + 1, st);
 }
 
 // signed eigen-factorisation of a symmetric P (positive columns first)
@@ -522,6 +576,10 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
   const int slot = h->it % m;
   const int count = std::min(h->it + 1, m);
   double* err = h->d_Es + slot * mm;
+  if (h->timing) {
+    if (!h->scf_ev[0]) for (auto& e : h->scf_ev) (void)hipEventCreate(&e);
+    (void)hipEventRecord(h->scf_ev[0], st);
+  }
   // err = X^T (F P S - S P F) X ; S P F = (F P S)^T for symmetric F, P, S
   gemm(h, false, false, nao, nao, nao, 1.0, h->d_F, nao, h->d_P, nao, 0.0, h->d_W1, nao);
   gemm(h, false, false, nao, nao, nao, 1.0, h->d_W1, nao, h->d_S, nao, 0.0, h->d_W2, nao);
@@ -541,6 +599,7 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
     gemm(h, false, false, nao, nao, nao, -0.5 * h->level_shift, h->d_W1, nao, h->d_S, nao, 1.0, h->d_Fx, nao);
     HIPCHK(launch_axpy(nn, h->level_shift, h->d_S, h->d_Fx, 1, st));
   }
+  if (h->timing) (void)hipEventRecord(h->scf_ev[1], st);
   // eigh(X^T F X), C = X C', P = 2 C_occ C_occ^T
   gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_Fx, nao, 0.0, h->d_W2, nmo);
   gemm(h, false, false, nmo, nmo, nao, 1.0, h->d_W2, nmo, h->d_X, nao, 0.0, h->d_Ws, nmo);
@@ -550,6 +609,7 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
   const int nocc = h->nocc;
   HIPCHK(launch_axpy((int64_t)nao * nocc, std::sqrt(2.0), h->d_C, h->d_Cf, 0, st));
   gemm(h, false, true, nao, nao, nocc, 1.0, h->d_Cf, nao, h->d_Cf, nao, 0.0, h->d_P, nao);
+  if (h->timing) (void)hipEventRecord(h->scf_ev[2], st);
   std::vector<double> sg(nocc, 1.0);
   energy_fock_factored(h, h->d_P, h->d_Cf, sg, nocc, h->d_Edev, h->d_F, st);
   // derr = |err|_F
@@ -638,6 +698,40 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
       throw;
     }
     *out = h;
+  });
+}
+
+int dfks_set_timing(dfks_handle h, int32_t on) {
+  return guard([&] {
+    if (!h) fail(1, "null handle");
+    h->timing = on != 0;
+  });
+}
+
+int dfks_stage_times(dfks_handle h, double* ms_out, int32_t n) {
+  return guard([&] {
+    if (!h || !ms_out) fail(1, "null argument");
+    for (int i = 0; i < n; ++i) ms_out[i] = -1.0;
+    if (!h->timing) fail(1, "timing is off (dfks_set_timing)");
+    for (int i = 0; i < kNumStages && i < n; ++i) {
+      if (i < kStScfDiis) {
+        int a = i, b = i + 1;
+        // stages whose start event was not recorded begin where the previous one ended
+        while (a > 0 && !h->ev_used[a]) --a;
+        if (!h->ev_used[a] || !h->ev_used[b]) { ms_out[i] = 0.0; continue; }
+        HIPCHK(hipEventSynchronize(h->ev[b]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->ev[a], h->ev[b]));
+        ms_out[i] = ms;
+      } else if (h->scf_ev[0]) {
+        int a = i - kStScfDiis, b = a + 1;
+        HIPCHK(hipEventSynchronize(h->scf_ev[b]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->scf_ev[a], h->scf_ev[b]));
+        ms_out[i] = ms;
+      }
+    }
+    for (int i = 0; i < 5 && kNumStages + i < n; ++i) ms_out[kNumStages + i] = h->build_ms[i];
   });
 }
 

@@ -107,19 +107,16 @@ hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, 
   return hipGetLastError();
 }
 
-// F = H + J + cK * sym(Ku) + sym(Vu) where Ku, Vu hold valid upper triangles
-// (column-major upper == row-major lower; both symmetric by construction).
-// Any of Ku / Vu may be null.
-__global__ void k_fock(int n, const double* __restrict__ H, const double* __restrict__ J, const double* __restrict__ Ku,
-                       double cK, const double* __restrict__ Vu, double* __restrict__ F) {
+// F = H + J + cK * K + (Vh + Vh^T) with K a full symmetric matrix and Vh the
+// half product Phi^T W. Either K or Vh may be null.
+__global__ void k_fock(int n, const double* __restrict__ H, const double* __restrict__ J, const double* __restrict__ K,
+                       double cK, const double* __restrict__ Vh, double* __restrict__ F) {
   int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= (int64_t)n * n) return;
   int r = (int)(idx / n), c = (int)(idx % n);
-  // column-major upper stores (row<=col) at [col*n + row]; pick the stored triangle
-  int64_t up = r <= c ? (int64_t)c * n + r : (int64_t)r * n + c;
   double f = H[idx] + J[idx];
-  if (Ku) f += cK * Ku[up];
-  if (Vu) f += Vu[up];
+  if (K) f += cK * 0.5 * (K[idx] + K[(int64_t)c * n + r]);
+  if (Vh) f += Vh[idx] + Vh[(int64_t)c * n + r];
   F[idx] = f;
 }
 

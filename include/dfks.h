@@ -125,6 +125,15 @@ int dfks_scf_step(dfks_handle h, double* out3, dfks_stream stream);
 /* Current density, orbitals (nao x nmo, column-major), orbital energies. Any may be NULL. */
 int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* nmo, dfks_stream stream);
 
+/* Stage timers (HIP events on the work stream). Stages of the last Fock
+ * build / SCF step, in ms: 0 exchange GEMM T = B^T Cf (+ gamma), 1 exchange
+ * SYRK K = T^T T, 2 Coulomb J = B gamma, 3 XC GEMM Z = Phi P, 4 XC kernel,
+ * 5 XC SYR2K, 6 Fock assembly + energies, 7 commutator + DIIS, 8 eigensolve +
+ * density; then build wall times 9 one-electron, 10 (P|Q), 11 (mn|P),
+ * 12 metric factor + B, 13 AO grid. */
+int dfks_set_timing(dfks_handle h, int32_t on);
+int dfks_stage_times(dfks_handle h, double* ms_out, int32_t n);
+
 /* Overlap matrix of one basis (host out, nfunc x nfunc); used by the minao guess. */
 int dfks_overlap(const dfks_basis_desc* basis, double* host_out);
 

@@ -587,3 +587,54 @@ def scf(oks, max_iter=128, e_tol=1e-8, d_tol=1e-6, m=8, P0=None, fock=None, verb
         e_prev = e
         it += 1
     return dict(e_tot=e_prev, P=P, C=C, eps=eps, converged=conv, n_iter=it, hist=hist)
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: timed per-iteration work of this restatement on bounded slices
+# ---------------------------------------------------------------------------
+
+def cpu_iteration_estimate(nao, nfit, nocc, ngrid, *, q_slice=96, g_slice=40000, seed=0, budget_s=25.0):
+    """Seconds per SCF iteration of the FP64 CPU restatement, from timed slices.
+
+    The iteration is timed stage by stage on synthetic operands of the
+    workload's shapes, each stage on a bounded slice scaled linearly to the
+    full size: exchange T = B^T C and K = T^T T on ``q_slice`` fitted
+    auxiliary functions, J on the same slice, the XC GEMMs Z = Phi P and
+    V = Phi^T W on ``g_slice`` grid points, and the full eigensolve of an
+    nao x nao Fock matrix (numpy/LAPACK, all host threads).
+    """
+    import time
+    rng = np.random.default_rng(seed)
+    q = min(q_slice, nfit)
+    g = min(g_slice, ngrid)
+    out = {}
+    B = rng.standard_normal((nao, nao, q))
+    C = rng.standard_normal((nao, nocc))
+    P = rng.standard_normal((nao, nao))
+    P = P + P.T
+    t0 = time.perf_counter()
+    T = np.tensordot(B, C, axes=(0, 0))            # T[m, q, i] = sum_l B[l, m, q] C[l, i]
+    t1 = time.perf_counter()
+    Tm = T.reshape(nao, -1)
+    K = Tm @ Tm.T
+    t2 = time.perf_counter()
+    gam = np.einsum("mqi,mi->q", T, C, optimize=True)
+    J = B.reshape(nao * nao, q) @ gam
+    t3 = time.perf_counter()
+    out["exchange_gemm"] = (t1 - t0) * nfit / q
+    out["exchange_k"] = (t2 - t1) * nfit / q
+    out["coulomb_j"] = (t3 - t2) * nfit / q
+    del B, T, Tm, K, J
+    ao = rng.standard_normal((g, nao))
+    t0 = time.perf_counter()
+    Z = ao @ P
+    t1 = time.perf_counter()
+    V = ao.T @ Z
+    t2 = time.perf_counter()
+    out["xc_gemms"] = (t2 - t0) * ngrid / g
+    del ao, Z, V
+    F = P
+    t0 = time.perf_counter()
+    np.linalg.eigh(F)
+    out["eigh"] = time.perf_counter() - t0
+    return sum(out.values()), out

@@ -138,10 +138,13 @@ def test_xc_pointwise(torch_cuda, xc):
     gx = np.sqrt(sigma)
     mask = gx > 0
     fs = np.where(mask, fg.numpy()[:, 0] / np.where(mask, 2 * gx, 1.0), got[:, 2])
-    # value to 1e-12 relative; derivatives to 1e-10 (autograd through |grad| vs dual numbers)
-    assert np.all(np.abs(got[:, 0] - f) <= 1e-12 * np.maximum(1.0, np.abs(f)))
-    for a, b in ((got[:, 1], fr), (got[:, 2], fs)):
-        assert np.all(np.abs(a - b) <= 1e-10 * np.maximum(1.0, np.abs(b)))
+    # value to 1e-12 relative; derivatives to 1e-9: autograd through |grad| and forward-mode
+    # dual numbers round differently at near-zero gradients (5e-10 measured for PBE at
+    # rho = 1.6e-8, sigma = 3e-25), far below what reaches the Fock matrix (tested at 1e-10)
+    for col, (a, b, tol) in enumerate(((got[:, 0], f, 1e-12), (got[:, 1], fr, 1e-9), (got[:, 2], fs, 1e-9))):
+        err = np.abs(a - b) / np.maximum(1.0, np.abs(b))
+        i = int(np.argmax(err))
+        assert err[i] <= tol, (col, i, rho[i], sigma[i], a[i], b[i], err[i])
 
 
 @pytest.mark.parametrize("case", ["water_pair", "dimer_pair"])

@@ -1,0 +1,89 @@
+"""Host logic (no GPU): basis assembly, grid layout, workload generators, specs."""
+import math
+
+import numpy as np
+import pytest
+
+from dftax_amd.basis import build_basis, gto_norm
+from dftax_amd.grid import atom_centred_points, becke, becke_radial, nwchem_orders, BRAGG_BOHR
+from dftax_amd.system import ANGSTROM_TO_BOHR, WATER, Molecule, serving_stream, tree_molecule, water_cluster
+
+
+def test_gto_norm_s_and_p():
+    for a in (0.1, 1.0, 37.5):
+        # s: (2a/pi)^(3/4) times the 1/sqrt(4 pi) angular factor folded in
+        assert abs(gto_norm(0, a) - (2 * a / math.pi) ** 0.75 * math.sqrt(4 * math.pi)) < 1e-12 * gto_norm(0, a)
+
+
+def test_basis_sizes():
+    m = water_cluster((4, 4, 2), seed=0)
+    b = build_basis(m.symbols, m.coords, "def2-tzvp", spherical=True)
+    x = build_basis(m.symbols, m.coords, "even-tempered-jk", spherical=True)
+    assert (b.nao, x.nao) == (1376, 3680)
+    bc = build_basis(m.symbols, m.coords, "def2-tzvp", spherical=False)
+    assert bc.nao == 1536
+    w = Molecule.from_xyz(WATER, "def2-svp", spherical=True)
+    assert build_basis(w.symbols, w.coords, "def2-svp", spherical=True).nao == 24
+    assert build_basis(w.symbols, w.coords, "even-tempered-jk", spherical=False).nao == 137
+
+
+def test_grid_layout_and_pruning():
+    r, w = becke_radial(75, 0.5)
+    assert r.shape == (75,) and np.all(np.diff(r) < 0) and np.all(w > 0)
+    # H: 50/86/266/302/266 regions; O (row 2) the same ladder with other bounds
+    orders = nwchem_orders(1, np.array([0.1, 0.3, 0.6, 2.0, 10.0]) * BRAGG_BOHR[1], 302)
+    assert list(orders) == [50, 86, 266, 302, 266]
+    orders = nwchem_orders(8, np.array([0.1, 0.3, 0.6, 2.0, 10.0]) * BRAGG_BOHR[8], 302)
+    assert list(orders) == [50, 86, 266, 302, 266]
+    assert list(nwchem_orders(8, np.array([0.1, 5.0]), 38)) == [38, 38]
+    m = Molecule.from_xyz(WATER, "def2-svp", spherical=True)
+    pts, raw, own = atom_centred_points(m.symbols, m.coords, becke())
+    assert pts.shape[0] == raw.shape[0] == own.shape[0]
+    assert set(np.unique(own)) == {0, 1, 2}
+    # each atom's points sit on spheres around it
+    for a in range(3):
+        d = np.linalg.norm(pts[own == a] - m.coords[a], axis=1)
+        assert d.max() <= 45.0 + 1e-12
+
+
+def test_water_cluster_generator():
+    m1 = water_cluster((2, 2, 2), seed=0)
+    m2 = water_cluster((2, 2, 2), seed=0)
+    assert np.array_equal(m1.coords, m2.coords)
+    assert not np.array_equal(m1.coords, water_cluster((2, 2, 2), seed=1).coords)
+    xyz = m1.coords / ANGSTROM_TO_BOHR
+    for k in range(8):
+        o, h1, h2 = xyz[3 * k: 3 * k + 3]
+        assert abs(np.linalg.norm(h1 - o) - 0.957) < 1e-12
+        cosang = np.dot(h1 - o, h2 - o) / (np.linalg.norm(h1 - o) * np.linalg.norm(h2 - o))
+        assert abs(math.degrees(math.acos(cosang)) - 104.5) < 1e-9
+    mol_id = np.repeat(np.arange(8), 3)
+    d = np.linalg.norm(xyz[:, None] - xyz[None], axis=-1)
+    inter = mol_id[:, None] != mol_id[None, :]
+    assert d[inter].min() >= 1.7
+
+
+def test_tree_molecules():
+    rng = np.random.default_rng(5)
+    for n in (2, 7, 24):
+        m = tree_molecule(n, rng)
+        heavy = [s for s in m.symbols if s != "H"]
+        assert len(heavy) == n
+        assert m.nelectron % 2 == 0
+        xyz = m.coords / ANGSTROM_TO_BOHR
+        d = np.linalg.norm(xyz[:, None] - xyz[None], axis=-1) + 10 * np.eye(len(xyz))
+        assert d.min() > 0.9
+    stream = serving_stream(6, seed=1)
+    assert len(stream) == 6 and all(2 <= sum(s != "H" for s in m.symbols) <= 24 for m in stream)
+
+
+def test_df_spec_validation():
+    from dftax_amd import df, hip
+    with pytest.raises(NotImplementedError):
+        df(spherical=False)
+    with pytest.raises(NotImplementedError):
+        df(chunk=64)
+    with pytest.raises(ValueError):
+        hip(precision="fp16")
+    spec = df()
+    assert spec.auxbasis == "even-tempered-jk" and spec.engine.precision == "fp64"
