@@ -33,7 +33,7 @@ class SystemDesc(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", c_int32), ("metric_cutoff", c_double), ("rho_thresh", c_double),
-                ("keep_int3c", c_int32)]
+                ("keep_int3c", c_int32), ("gemm", c_int32)]
 
 
 DFKS_S, DFKS_HCORE, DFKS_KINETIC, DFKS_NUCLEAR, DFKS_INT2C, DFKS_INT3C, DFKS_BFIT, DFKS_AO, DFKS_GRID_W = range(9)
@@ -42,6 +42,7 @@ EXPORTS = [
     "dfks_last_error", "dfks_create", "dfks_destroy", "dfks_build", "dfks_sizes", "dfks_get_matrix",
     "dfks_energy_fock", "dfks_energy_fock_factored", "dfks_xc_eval", "dfks_scf_begin", "dfks_scf_step",
     "dfks_scf_result", "dfks_becke_weights", "dfks_overlap", "dfks_set_timing", "dfks_stage_times",
+    "dfks_set_precision",
 ]
 
 STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
@@ -81,6 +82,7 @@ def lib():
     L.dfks_becke_weights.argtypes = [c_int64, P_f64, P_f64, P_i32, c_int32, P_f64, P_f64, P_f64]
     L.dfks_overlap.argtypes = [ctypes.POINTER(BasisDesc), P_f64]
     L.dfks_set_timing.argtypes = [c_void_p, c_int32]
+    L.dfks_set_precision.argtypes = [c_void_p, c_int32]
     L.dfks_stage_times.argtypes = [c_void_p, P_f64, c_int32]
     _lib = L
     return L

@@ -42,7 +42,7 @@ hipError_t launch_xc_pointwise(int64_t n, int xc_id, const double* rho, const do
 hipError_t launch_dot(int64_t n, const double* x, const double* y, double* part, double scale, double* out,
                       int accumulate, hipStream_t st);
 hipError_t launch_gamma(int nao, int nq, int nf, const double* T, const double* Cf, const double* sgn, double* gamma,
-                        int accumulate, hipStream_t st);
+                        double* part, int accumulate, hipStream_t st);
 hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, double* J, hipStream_t st);
 hipError_t launch_fock(int n, const double* H, const double* J, const double* Ku, double cK, const double* Vu,
                        double* F, hipStream_t st);
@@ -56,6 +56,27 @@ hipError_t launch_diis_solve(int m, int count, const double* gram, double* coef,
 hipError_t launch_diis_extrap(int m, int64_t n, const double* coef, const double* Fs, double* out, hipStream_t st);
 hipError_t launch_gram_update(int m, int slot, int count, const double* dots, double* gram, hipStream_t st);
 hipError_t launch_scaled_cols(int n, int k0, const double* U, const double* w, double* X, hipStream_t st);
+hipError_t launch_xc_f32(int64_t ng, int64_t nao, const float* ao, const double* w, float* zw, int xc_id,
+                         double thresh, double* exc, hipStream_t st);
+hipError_t launch_gamma_f32(int nao, int nq, int nf, const float* T, const double* Cf, const double* sgn,
+                            double* gamma, double* part, int accumulate, hipStream_t st);
+hipError_t launch_jvec_f32(int64_t nrows, int nq, const float* B, const double* g, double* J, hipStream_t st);
+hipError_t launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st);
+hipError_t launch_to_f64(int64_t n, const float* x, double* y, hipStream_t st);
+enum GemmPrec { kF64 = 0, kBF16x3 = 1 };
+struct GemmArgs {
+  int M, N;
+  int64_t K;
+  const void* A;
+  int64_t lda;
+  bool a_kcontig;
+  const void* B;
+  int64_t ldb;
+  bool b_kcontig;
+};
+hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, double beta, int nsplit, int sym,
+                       void* ws, hipStream_t st);
+hipError_t launch_gemm_f32out(const GemmArgs& g, float* C, hipStream_t st);
 }  // namespace dfks
 
 using namespace dfks;
@@ -224,6 +245,15 @@ struct dfks_ctx {
   double *d_Tx = nullptr, *d_gamma = nullptr, *d_J = nullptr, *d_K = nullptr, *d_Vxc = nullptr;
   double *d_part = nullptr, *d_Edev = nullptr, *d_Cf = nullptr, *d_sgn = nullptr;
   double *d_W1 = nullptr, *d_W2 = nullptr, *d_evals = nullptr, *d_offd = nullptr;
+  double* d_gws = nullptr;    // split-K workspace (64 x nao^2 doubles)
+  double* d_gpart = nullptr;  // per-row partial sums of gamma (nao x nfit)
+  // mixed-precision operands (precision = 1)
+  bool mixed_now = false;
+  bool refresh_fock = false;  // rebuild F in FP64 at the current P before the next step
+  int diis_base = 0;          // iteration index where the DIIS history restarted
+  int last_nf = 0;            // factor columns of the last Fock build (signs in h_sgn)
+  float *d_B32 = nullptr, *d_ao32 = nullptr, *d_zw32 = nullptr, *d_T32 = nullptr, *d_Cf32 = nullptr, *d_P32 = nullptr;
+  float *d_ws32 = nullptr, *d_eps32 = nullptr, *d_offd32 = nullptr;
   int32_t *d_info = nullptr, *d_sel = nullptr;
   // SCF state
   int nmo = 0, m = 8, it = 0;
@@ -238,6 +268,8 @@ struct dfks_ctx {
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
                     d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws};
     for (void* p : ptrs) dfree(p);
+    void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, d_T32, d_Cf32, d_P32, d_ws32, d_eps32, d_offd32};
+    for (void* p : ptrs32) dfree(p);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : scf_ev)
@@ -382,7 +414,7 @@ void build_all(dfks_ctx* h, hipStream_t st) {
     HIPCHK(launch_ao(h->ng, h->d_gxyz, O, nao, h->d_ao, st));
   }
   // Fock-build scratch: exchange intermediate for up to nf_chunk factor columns
-  h->nf_chunk = std::max(1, std::min(nao, std::max(h->nocc, 1)));
+  h->nf_chunk = std::max(1, std::min(std::min(nao, 512), std::max(h->nocc, 1)));
   h->d_Tx = dalloc<double>((size_t)nao * h->nfit * h->nf_chunk);
   h->d_gamma = dalloc<double>(h->nfit);
   h->d_J = dalloc<double>((size_t)nao * nao);
@@ -395,6 +427,24 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   h->d_evals = dalloc<double>(nao);
   h->d_offd = dalloc<double>(nao);
   h->d_sel = dalloc<int32_t>(nao);
+  h->d_gws = dalloc<double>((size_t)64 * nao * nao);
+  h->d_gpart = dalloc<double>((size_t)nao * h->nfit);
+  if (h->opt.precision == 1) {
+    const size_t nB = (size_t)nao * nao * h->nfit;
+    h->d_B32 = dalloc<float>(nB);
+    HIPCHK(launch_to_f32((int64_t)nB, h->d_B, h->d_B32, st));
+    if (h->ng > 0) {
+      h->d_ao32 = dalloc<float>((size_t)4 * h->ng * nao);
+      HIPCHK(launch_to_f32((int64_t)4 * h->ng * nao, h->d_ao, h->d_ao32, st));
+      h->d_zw32 = dalloc<float>((size_t)h->ng * nao);
+    }
+    h->d_T32 = dalloc<float>((size_t)nao * h->nfit * h->nf_chunk);
+    h->d_Cf32 = dalloc<float>((size_t)nao * nao);
+    h->d_P32 = dalloc<float>((size_t)nao * nao);
+    h->d_ws32 = dalloc<float>((size_t)nao * nao);
+    h->d_eps32 = dalloc<float>(nao);
+    h->d_offd32 = dalloc<float>(nao);
+  }
   sync(st);
   h->build_ms[4] = now_ms() - t0;
   h->built = true;
@@ -403,13 +453,28 @@ void build_all(dfks_ctx* h, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // energy + Fock for P = sum_i s_i c_i c_i^T (Cf column-major nao x nf)
 // ---------------------------------------------------------------------------
+// split count so a K-long product fills the chip (~2048 workgroups), k-chunks >= 512
+int split_for(int64_t tiles, int64_t K) {
+  int64_t s = 2048 / std::max<int64_t>(tiles, 1);
+  s = std::min<int64_t>(s, std::max<int64_t>(1, K / 512));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(s, 64));
+}
+
 void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const std::vector<double>& sgn, int nf,
                           double* E, double* F, hipStream_t st) {
   if (!h->built) fail(6, "dfks_build has not been called");
   RBCHK(rocblas_set_stream(h->rb, st));
   const int nao = h->nao, nfit = h->nfit;
   const int64_t nn = (int64_t)nao * nao;
+  const int64_t ng = h->ng;
   const double one = 1.0, zero = 0.0;
+  const bool mixed = h->mixed_now;
+  // FP64 products: gemm 0 (default) runs the symmetric exchange product on the
+  // library kernel and the plain T, Z, V products on rocBLAS; 1 runs all four on
+  // rocBLAS; 2 runs all four on the library kernel.
+  const bool lib = h->opt.gemm == 2;
+  const bool lib_k = h->opt.gemm != 1;
+  if (mixed && !h->d_B32) fail(1, "mixed precision needs options.precision = 1 at create");
   h->mark(0, st);
   HIPCHK(hipMemsetAsync(E, 0, 8 * sizeof(double), st));
   // one-electron energy
@@ -417,7 +482,9 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
   // signs to the device (host copy kept in the handle until the next call)
   if (nf > nao) fail(1, "more factor columns than basis functions");
   h->h_sgn.assign(sgn.begin(), sgn.end());
+  h->last_nf = nf;
   HIPCHK(hipMemcpyAsync(h->d_sgn, h->h_sgn.data(), sizeof(double) * nf, hipMemcpyHostToDevice, st));
+  if (mixed) HIPCHK(launch_to_f32((int64_t)nao * nf, Cf, h->d_Cf32, st));
   // Coulomb + exchange over runs of equal sign, chunked to nf_chunk columns
   bool first = true;
   HIPCHK(hipMemsetAsync(h->d_gamma, 0, sizeof(double) * nfit, st));
@@ -428,17 +495,40 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
     while (i1 < nf && i1 - i0 < h->nf_chunk && sgn[i1] == sgn[i0]) ++i1;
     const int nc = i1 - i0;
     const double sg = sgn[i0];
+    const double beta = first ? 0.0 : 1.0;
     // T[m,Q,i] = sum_l B[l,m,Q] Cf[l,i]
-    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc,
-                           (int64_t)nao * nfit, nao, &one, Cf + (int64_t)i0 * nao, nao, h->d_B,
-                           (int64_t)nao * nfit, &zero, h->d_Tx, nc));
-    HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, 1, st));
+    if (mixed) {
+      GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
+                 h->d_Cf32 + (int64_t)i0 * nao, nao, true};
+      HIPCHK(launch_gemm_f32out(g, h->d_T32, st));
+      HIPCHK(launch_gamma_f32(nao, nfit, nc, h->d_T32, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1, st));
+    } else if (lib) {
+      GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B, (int64_t)nao * nfit, false, Cf + (int64_t)i0 * nao,
+                 nao, true};
+      HIPCHK(launch_gemm(kF64, g, h->d_Tx, 1.0, 0.0, 1, 0, nullptr, st));
+      HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1, st));
+    } else {
+      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc,
+                             (int64_t)nao * nfit, nao, &one, Cf + (int64_t)i0 * nao, nao, h->d_B,
+                             (int64_t)nao * nfit, &zero, h->d_Tx, nc));
+      HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1, st));
+    }
     h->mark(kStExchangeGemm + 1, st);
     if (h->hf != 0.0) {
-      const double beta = first ? 0.0 : 1.0;
-      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_none, nao, nao,
-                             (int64_t)nfit * nc, &sg, h->d_Tx, (int64_t)nfit * nc, h->d_Tx, (int64_t)nfit * nc,
-                             &beta, h->d_K, nao));
+      // K += s T T^T (symmetric: upper tiles, mirrored by the split reduction)
+      const int64_t kd = (int64_t)nfit * nc;
+      const int ntn = (nao + 127) / 128;
+      const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, kd);
+      if (mixed) {
+        GemmArgs g{nao, nao, kd, h->d_T32, kd, true, h->d_T32, kd, true};
+        HIPCHK(launch_gemm(kBF16x3, g, h->d_K, sg, beta, nsplit, 1, h->d_gws, st));
+      } else if (lib_k) {
+        GemmArgs g{nao, nao, kd, h->d_Tx, kd, true, h->d_Tx, kd, true};
+        HIPCHK(launch_gemm(kF64, g, h->d_K, sg, beta, nsplit, 1, h->d_gws, st));
+      } else {
+        RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_none, nao, nao, kd, &sg,
+                               h->d_Tx, kd, h->d_Tx, kd, &beta, h->d_K, nao));
+      }
     }
     h->mark(kStExchangeSyrk + 1, st);
     first = false;
@@ -446,25 +536,48 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
   }
   // E_J = 1/2 |gamma|^2, J = B gamma
   HIPCHK(launch_dot(nfit, h->d_gamma, h->d_gamma, h->d_part, 0.5, E + DFKS_E_J, 0, st));
-  HIPCHK(launch_jvec(nn, nfit, h->d_B, h->d_gamma, h->d_J, st));
+  if (mixed) HIPCHK(launch_jvec_f32(nn, nfit, h->d_B32, h->d_gamma, h->d_J, st));
+  else HIPCHK(launch_jvec(nn, nfit, h->d_B, h->d_gamma, h->d_J, st));
   h->mark(kStCoulomb + 1, st);
   // E_K = -1/4 a_x sum P o K
   if (h->hf != 0.0) HIPCHK(launch_dot(nn, P, h->d_K, h->d_part, -0.25 * h->hf, E + DFKS_E_K, 0, st));
-  // XC
-  if (h->ng > 0) {
-    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, nao, h->ng, nao, &one, P, nao,
-                           h->d_ao, nao, &zero, h->d_zw, nao));
-    h->mark(kStXcGemm + 1, st);
-    HIPCHK(launch_xc(h->ng, nao, h->d_ao, h->d_gw, h->d_zw, h->xc_id, h->opt.rho_thresh, h->d_exc, nullptr, st));
-    HIPCHK(launch_dot(h->ng, h->d_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
-    h->mark(kStXcKernel + 1, st);
-    // Vh = Phi^T W (column-major: ao_cm (nao x ng) times zw_cm^T); V = Vh + Vh^T in the Fock assembly
-    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_transpose, nao, nao, h->ng, &one,
-                           h->d_ao, nao, h->d_zw, nao, &zero, h->d_Vxc, nao));
+  // XC: Z = Phi P, W (in place), Vh = Phi^T W; V = Vh + Vh^T in the Fock assembly
+  if (ng > 0) {
+    const int ntm = (nao + 127) / 128;
+    const int nsplit = split_for((int64_t)ntm * ntm, ng);
+    if (mixed) {
+      HIPCHK(launch_to_f32(nn, P, h->d_P32, st));
+      GemmArgs gz{(int)ng, nao, nao, h->d_ao32, nao, true, h->d_P32, nao, true};
+      HIPCHK(launch_gemm_f32out(gz, h->d_zw32, st));
+      h->mark(kStXcGemm + 1, st);
+      HIPCHK(launch_xc_f32(ng, nao, h->d_ao32, h->d_gw, h->d_zw32, h->xc_id, h->opt.rho_thresh, h->d_exc, st));
+      HIPCHK(launch_dot(ng, h->d_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
+      h->mark(kStXcKernel + 1, st);
+      GemmArgs gv{nao, nao, ng, h->d_ao32, nao, false, h->d_zw32, nao, false};
+      HIPCHK(launch_gemm(kBF16x3, gv, h->d_Vxc, 1.0, 0.0, nsplit, 0, h->d_gws, st));
+    } else if (lib) {
+      GemmArgs gz{(int)ng, nao, nao, h->d_ao, nao, true, P, nao, true};
+      HIPCHK(launch_gemm(kF64, gz, h->d_zw, 1.0, 0.0, 1, 0, nullptr, st));
+      h->mark(kStXcGemm + 1, st);
+      HIPCHK(launch_xc(ng, nao, h->d_ao, h->d_gw, h->d_zw, h->xc_id, h->opt.rho_thresh, h->d_exc, nullptr, st));
+      HIPCHK(launch_dot(ng, h->d_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
+      h->mark(kStXcKernel + 1, st);
+      GemmArgs gv{nao, nao, ng, h->d_ao, nao, false, h->d_zw, nao, false};
+      HIPCHK(launch_gemm(kF64, gv, h->d_Vxc, 1.0, 0.0, nsplit, 0, h->d_gws, st));
+    } else {
+      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, nao, ng, nao, &one, P, nao,
+                             h->d_ao, nao, &zero, h->d_zw, nao));
+      h->mark(kStXcGemm + 1, st);
+      HIPCHK(launch_xc(ng, nao, h->d_ao, h->d_gw, h->d_zw, h->xc_id, h->opt.rho_thresh, h->d_exc, nullptr, st));
+      HIPCHK(launch_dot(ng, h->d_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
+      h->mark(kStXcKernel + 1, st);
+      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_transpose, nao, nao, ng, &one,
+                             h->d_ao, nao, h->d_zw, nao, &zero, h->d_Vxc, nao));
+    }
     h->mark(kStXcSyr2k + 1, st);
   }
   HIPCHK(launch_fock(nao, h->d_H, h->d_J, h->hf != 0.0 ? h->d_K : nullptr, -0.5 * h->hf,
-                     h->ng > 0 ? h->d_Vxc : nullptr, F, st));
+                     ng > 0 ? h->d_Vxc : nullptr, F, st));
   // totals: E_tot = e1 + eJ + eK + exc + enn
   HIPCHK(hipMemcpyAsync(E + DFKS_E_NN, &h->e_nn, sizeof(double), hipMemcpyHostToDevice, st));
   HIPCHK(launch_dot(5, E + 1, nullptr, h->d_part, 1.0, E + DFKS_E_TOT, 0, st));
@@ -540,6 +653,8 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   h->nmo = nmo;
   h->m = m;
   h->it = 0;
+  h->diis_base = 0;
+  h->refresh_fock = false;
   h->level_shift = level_shift;
   auto realloc = [](double*& p, size_t n) { dfree(p); p = dalloc<double>(n); };
   realloc(h->d_X, (size_t)nao * nmo);
@@ -573,8 +688,18 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
   RBCHK(rocblas_set_stream(h->rb, st));
   const int nao = h->nao, nmo = h->nmo, m = h->m;
   const int64_t nn = (int64_t)nao * nao, mm = (int64_t)nmo * nmo;
-  const int slot = h->it % m;
-  const int count = std::min(h->it + 1, m);
+  if (h->refresh_fock) {
+    // the stored F came from a mixed-precision build: rebuild it in FP64 at the current P
+    std::vector<double> sg(h->h_sgn.begin(), h->h_sgn.begin() + h->last_nf);
+    energy_fock_factored(h, h->d_P, h->d_Cf, sg, h->last_nf, h->d_Edev, h->d_F, st);
+    double e[8];
+    HIPCHK(hipMemcpyAsync(e, h->d_Edev, sizeof(e), hipMemcpyDeviceToHost, st));
+    sync(st);
+    h->e_prev = e[0];
+    h->refresh_fock = false;
+  }
+  const int slot = (h->it - h->diis_base) % m;
+  const int count = std::min(h->it - h->diis_base + 1, m);
   double* err = h->d_Es + slot * mm;
   if (h->timing) {
     if (!h->scf_ev[0]) for (auto& e : h->scf_ev) (void)hipEventCreate(&e);
@@ -603,6 +728,8 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
   // eigh(X^T F X), C = X C', P = 2 C_occ C_occ^T
   gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_Fx, nao, 0.0, h->d_W2, nmo);
   gemm(h, false, false, nmo, nmo, nao, 1.0, h->d_W2, nmo, h->d_X, nao, 0.0, h->d_Ws, nmo);
+  // FP64 eigensolve in every step: an FP32 solve's eigenvector error (~eps_32 |F| / gap)
+  // stalls the orbital gradient of the mixed phase above the switch threshold.
   RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, nmo, h->d_Ws, nmo, h->d_eps, h->d_offd,
                          h->d_info));
   gemm(h, false, false, nao, nmo, nmo, 1.0, h->d_X, nao, h->d_Ws, nmo, 0.0, h->d_C, nao);
@@ -648,8 +775,9 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
     auto* h = new dfks_ctx();
     try {
       if (opt) h->opt = *opt;
-      else h->opt = dfks_options{0, 1e-7, 1e-10, 0};
-      if (h->opt.precision != 0) fail(1, "only precision=0 (FP64) is implemented in this build");
+      else h->opt = dfks_options{0, 1e-7, 1e-10, 0, 0};
+      if (h->opt.precision != 0 && h->opt.precision != 1) fail(1, "precision must be 0 (FP64) or 1 (mixed)");
+      if (h->opt.gemm < 0 || h->opt.gemm > 2) fail(1, "gemm must be 0 (auto), 1 (rocBLAS) or 2 (library MFMA)");
       h->xc_id = sys->xc_id;
       h->hf = hf_coeff(sys->xc_id);
       if (sys->nelec <= 0 || sys->nelec % 2) fail(1, "closed shell needs an even, positive electron count");
@@ -698,6 +826,20 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
       throw;
     }
     *out = h;
+  });
+}
+
+int dfks_set_precision(dfks_handle h, int32_t mixed) {
+  return guard([&] {
+    if (!h) fail(1, "null handle");
+    if (mixed && h->opt.precision != 1) fail(1, "mixed steps need options.precision = 1 at create");
+    const bool was = h->mixed_now;
+    h->mixed_now = mixed != 0;
+    if (was && !h->mixed_now && h->d_X) {
+      // leaving mixed steps inside an SCF: FP64 Fock at the current density, fresh DIIS history
+      h->refresh_fock = true;
+      h->diis_base = h->it;
+    }
   });
 }
 

@@ -1,0 +1,363 @@
+// MFMA GEMMs for the Fock build (gfx950).
+//
+//   C(m, n) = sum_k A(m, k) B(n, k)
+//
+// A(m, k) lives at A[m*lda + k] (K-contiguous, "AK") or A[k*lda + m]
+// (M-contiguous); B(n, k) likewise. C is written row-major, C[m*ldc + n].
+// The same kernel covers the four products of an iteration:
+//   K  = T T^T   (A = B = T, K-contiguous, symmetric: upper tiles only)
+//   T  = B^T C   (A M-contiguous over the fitted tensor, B K-contiguous)
+//   Z  = Phi P   (both K-contiguous)
+//   V  = Phi^T W (both M-contiguous, K = grid points)
+//
+// Arithmetic:
+//   kF64       FP64 operands, v_mfma_f64_16x16x4f64, FP64 accumulate
+//   kBF16x3    FP32 operands split on the fly into bf16 hi + lo while staging
+//              to LDS; per k-step three v_mfma_f32_16x16x32_bf16 products
+//              (hi*hi + hi*lo + lo*hi), FP32 accumulate
+//
+// Tiling: 256 threads = 4 wavefronts in a 2x2 arrangement over a 128x128
+// output tile, each wavefront 64x64 = 4x4 MFMA 16x16 blocks. K is split over
+// gridDim.z into a workspace of per-split partial tiles, then a deterministic
+// reduction kernel sums the splits in order (and mirrors the symmetric case).
+#include <cstdint>
+
+#include "dfks_common.h"
+
+namespace dfks {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+enum GemmPrec { kF64 = 0, kBF16x3 = 1 };
+
+constexpr int kBM = 128;
+constexpr int kBN = 128;
+
+template <int PREC>
+struct GemmTraits;
+
+template <>
+struct GemmTraits<kF64> {
+  using In = double;
+  static constexpr int BK = 16;
+};
+
+template <>
+struct GemmTraits<kBF16x3> {
+  using In = float;
+  static constexpr int BK = 32;
+};
+
+// Map a linear tile id to (tm, tn); symmetric mode enumerates tm <= tn only.
+__device__ __forceinline__ void tile_of(int tid, int ntn, bool sym, int& tm, int& tn) {
+  if (!sym) {
+    tm = tid / ntn;
+    tn = tid % ntn;
+    return;
+  }
+  // row r holds tiles (r, r..ntn-1): count ntn - r
+  int r = 0, base = 0;
+  while (base + (ntn - r) <= tid) {
+    base += ntn - r;
+    ++r;
+  }
+  tm = r;
+  tn = r + (tid - base);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8). Give each XCD a contiguous range of tile ids so tiles that share an
+// operand panel (consecutive ids) share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
+  const int per = (ntiles + 7) / 8;
+  return (bid % 8) * per + bid / 8;
+}
+
+// Per-thread slice of a 128-row operand tile: row r, k offset kq, VEC values.
+template <bool KC, int VEC>
+__device__ __forceinline__ void slice_of(int tid, int& r, int& kq) {
+  if (KC) {  // rows take 256/128 = 2 threads each, VEC consecutive k per thread
+    r = tid >> 1;
+    kq = (tid & 1) * VEC;
+  } else {   // 128 consecutive rows per k-group: coalesced along the row index
+    r = tid & 127;
+    kq = (tid >> 7) * VEC;
+  }
+}
+
+// Load VEC values X(row, k0+kq .. k0+kq+VEC-1) into regs (zeros outside).
+template <bool KC, int VEC, typename T>
+__device__ __forceinline__ void load_slice(const T* __restrict__ X, int64_t ld, int rows, int row0, int r, int64_t k0,
+                                           int kq, int64_t k_end, bool vec_ok, T* regs) {
+  const int row = row0 + r;
+  const int64_t kb = k0 + kq;
+  if (row >= rows) {
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) regs[j] = (T)0;
+    return;
+  }
+  if (KC) {
+    const T* src = X + (int64_t)row * ld + kb;
+    if (vec_ok && kb + VEC <= k_end) {
+      constexpr int PER = 16 / sizeof(T);
+      typedef T vecT __attribute__((ext_vector_type(PER)));
+#pragma unroll
+      for (int v = 0; v < VEC / PER; ++v) {
+        vecT x = *reinterpret_cast<const vecT*>(src + v * PER);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) regs[v * PER + j] = x[j];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) regs[j] = (kb + j < k_end) ? src[j] : (T)0;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) regs[j] = (kb + j < k_end) ? X[(kb + j) * ld + row] : (T)0;
+  }
+}
+
+template <int PREC, bool AK, bool BKC, typename Out>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typename GemmTraits<PREC>::In* __restrict__ A,
+                                              int64_t lda, const typename GemmTraits<PREC>::In* __restrict__ B,
+                                              int64_t ldb, Out* __restrict__ C, int64_t ldc, int64_t split_stride,
+                                              int64_t kchunk, int sym, int ntiles, int vec_a, int vec_b) {
+  using In = typename GemmTraits<PREC>::In;
+  constexpr int BK = GemmTraits<PREC>::BK;
+  constexpr int VEC = (128 * BK) / 256;  // values per thread per operand tile
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  const int ntn = (N + kBN - 1) / kBN;
+  int tm, tn;
+  tile_of(tile, ntn, sym != 0, tm, tn);
+  const int m0 = tm * kBM, n0 = tn * kBN;
+  const int64_t k_begin = (int64_t)blockIdx.z * kchunk;
+  const int64_t k_end = k_begin + kchunk < K ? k_begin + kchunk : K;
+  Out* Cz = C + (int64_t)blockIdx.z * split_stride;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
+  int ra_r, ra_k, rb_r, rb_k;
+  slice_of<AK, VEC>(tid, ra_r, ra_k);
+  slice_of<BKC, VEC>(tid, rb_r, rb_k);
+  In ra[VEC], rb[VEC];
+  if (k_begin < k_end) {
+    load_slice<AK, VEC>(A, lda, M, m0, ra_r, k_begin, ra_k, k_end, vec_a != 0, ra);
+    load_slice<BKC, VEC>(B, ldb, N, n0, rb_r, k_begin, rb_k, k_end, vec_b != 0, rb);
+  }
+
+  if constexpr (PREC == kF64) {
+    __shared__ double sA[kBM][BK + 1];
+    __shared__ double sB[kBN][BK + 1];
+    dbl4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
+    for (int64_t k0 = k_begin; k0 < k_end; k0 += BK) {
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) sA[ra_r][ra_k + j] = ra[j];
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) sB[rb_r][rb_k + j] = rb[j];
+      __syncthreads();
+      if (k0 + BK < k_end) {  // prefetch the next tile while this one computes
+        load_slice<AK, VEC>(A, lda, M, m0, ra_r, k0 + BK, ra_k, k_end, vec_a != 0, ra);
+        load_slice<BKC, VEC>(B, ldb, N, n0, rb_r, k0 + BK, rb_k, k_end, vec_b != 0, rb);
+      }
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = sA[wm + 16 * i + li][kk + lk];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = sB[wn + 16 * j + li][kk + lk];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+    }
+    // v_mfma_f64_16x16x4f64 result: lane l, register r holds row (l/16) + 4r, column l%16
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int gm = m0 + wm + 16 * i + lk + 4 * r;
+          int gn = n0 + wn + 16 * j + li;
+          if (gm < M && gn < N) Cz[(int64_t)gm * ldc + gn] = (Out)acc[i][j][r];
+        }
+  } else {
+    constexpr int PADK = BK + 8;  // bf16 per LDS row; 80-byte rows keep 16-byte reads conflict-free
+    __shared__ __attribute__((aligned(16))) __bf16 sAh[kBM][PADK];
+    __shared__ __attribute__((aligned(16))) __bf16 sAl[kBM][PADK];
+    __shared__ __attribute__((aligned(16))) __bf16 sBh[kBN][PADK];
+    __shared__ __attribute__((aligned(16))) __bf16 sBl[kBN][PADK];
+    flt4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+    for (int64_t k0 = k_begin; k0 < k_end; k0 += BK) {
+      __syncthreads();
+      // split into bf16 hi + lo and store 16-byte vectors
+#pragma unroll
+      for (int v = 0; v < VEC / 8; ++v) {
+        bf16x8 h, l, hb, lb;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          float x = ra[8 * v + j];
+          __bf16 xh = (__bf16)x;
+          h[j] = xh;
+          l[j] = (__bf16)(x - (float)xh);
+          float y = rb[8 * v + j];
+          __bf16 yh = (__bf16)y;
+          hb[j] = yh;
+          lb[j] = (__bf16)(y - (float)yh);
+        }
+        *reinterpret_cast<bf16x8*>(&sAh[ra_r][ra_k + 8 * v]) = h;
+        *reinterpret_cast<bf16x8*>(&sAl[ra_r][ra_k + 8 * v]) = l;
+        *reinterpret_cast<bf16x8*>(&sBh[rb_r][rb_k + 8 * v]) = hb;
+        *reinterpret_cast<bf16x8*>(&sBl[rb_r][rb_k + 8 * v]) = lb;
+      }
+      __syncthreads();
+      if (k0 + BK < k_end) {
+        load_slice<AK, VEC>(A, lda, M, m0, ra_r, k0 + BK, ra_k, k_end, vec_a != 0, ra);
+        load_slice<BKC, VEC>(B, ldb, N, n0, rb_r, k0 + BK, rb_k, k_end, vec_b != 0, rb);
+      }
+      bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        ah[i] = *reinterpret_cast<const bf16x8*>(&sAh[wm + 16 * i + li][8 * lk]);
+        al[i] = *reinterpret_cast<const bf16x8*>(&sAl[wm + 16 * i + li][8 * lk]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        bh[j] = *reinterpret_cast<const bf16x8*>(&sBh[wn + 16 * j + li][8 * lk]);
+        bl[j] = *reinterpret_cast<const bf16x8*>(&sBl[wn + 16 * j + li][8 * lk]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    // v_mfma_f32_16x16x32_bf16 result: lane l, register r holds row 4(l/16) + r, column l%16
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int gm = m0 + wm + 16 * i + 4 * lk + r;
+          int gn = n0 + wn + 16 * j + li;
+          if (gm < M && gn < N) Cz[(int64_t)gm * ldc + gn] = (Out)acc[i][j][r];
+        }
+  }
+}
+
+// out[m*ldo + n] = alpha * sum_z ws[z][m][n] (+ beta * out); symmetric mode reads
+// the upper triangle (m <= n) and mirrors it. Fixed summation order.
+template <typename W, typename Out>
+__global__ void k_gemm_reduce(int M, int N, int nsplit, const W* __restrict__ ws, int64_t split_stride, int64_t ldw,
+                              Out* __restrict__ out, int64_t ldo, double alpha, double beta, int sym) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)M * N) return;
+  int m = (int)(idx / N), n = (int)(idx % N);
+  int sm = m, sn = n;
+  if (sym && m > n) { sm = n; sn = m; }
+  double s = 0.0;
+  for (int z = 0; z < nsplit; ++z) s += (double)ws[z * split_stride + (int64_t)sm * ldw + sn];
+  double v = alpha * s;
+  if (beta != 0.0) v += beta * (double)out[(int64_t)m * ldo + n];
+  out[(int64_t)m * ldo + n] = (Out)v;
+}
+
+struct GemmArgs {
+  int M, N;
+  int64_t K;
+  const void* A;
+  int64_t lda;
+  bool a_kcontig;
+  const void* B;
+  int64_t ldb;
+  bool b_kcontig;
+};
+
+template <int PREC, typename Out>
+static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nsplit, int sym, hipStream_t st) {
+  using In = typename GemmTraits<PREC>::In;
+  constexpr int BK = GemmTraits<PREC>::BK;
+  const int ntm = (g.M + kBM - 1) / kBM, ntn = (g.N + kBN - 1) / kBN;
+  const int ntiles = sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
+  int64_t kchunk = (g.K + nsplit - 1) / nsplit;
+  kchunk = (kchunk + BK - 1) / BK * BK;
+  const int64_t split_stride = (int64_t)g.M * ldc;
+  const int nblk = (ntiles + 7) / 8 * 8;
+  dim3 grid(nblk, 1, nsplit);
+  const In* A = static_cast<const In*>(g.A);
+  const In* B = static_cast<const In*>(g.B);
+  // 16-byte vector loads need a 16-byte aligned base and row pitch (K-contiguous operands)
+  constexpr int PER = 16 / sizeof(In);
+  const int vec_a = ((uintptr_t)A % 16 == 0) && (g.lda % PER == 0);
+  const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % PER == 0);
+  if (g.a_kcontig && g.b_kcontig)
+    hipLaunchKernelGGL((k_gemm<PREC, true, true, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb, C,
+                       ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b);
+  else if (g.a_kcontig && !g.b_kcontig)
+    hipLaunchKernelGGL((k_gemm<PREC, true, false, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
+                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b);
+  else if (!g.a_kcontig && g.b_kcontig)
+    hipLaunchKernelGGL((k_gemm<PREC, false, true, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
+                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b);
+  else
+    hipLaunchKernelGGL((k_gemm<PREC, false, false, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
+                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b);
+  return hipGetLastError();
+}
+
+// C = alpha * A B^T (+ beta C), C row-major M x N (ldc = N), FP64 output.
+// ws: workspace of nsplit * M * N doubles (FP64) or floats (BF16x3), used when
+// nsplit > 1 or sym; may be null otherwise.
+hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, double beta, int nsplit, int sym,
+                       void* ws, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (nsplit < 1) nsplit = 1;
+  const bool direct = nsplit == 1 && !sym && alpha == 1.0 && beta == 0.0 && prec == kF64;
+  hipError_t e;
+  if (direct) return launch_gemm_t<kF64, double>(g, C, g.N, 1, 0, st);
+  if (!ws) return hipErrorInvalidValue;
+  const int64_t stride = (int64_t)g.M * g.N;
+  const int64_t tot = (int64_t)g.M * g.N;
+  const unsigned nb = (unsigned)((tot + 255) / 256);
+  if (prec == kF64) {
+    e = launch_gemm_t<kF64, double>(g, static_cast<double*>(ws), g.N, nsplit, sym, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_gemm_reduce<double, double>), dim3(nb), dim3(256), 0, st, g.M, g.N, nsplit,
+                       static_cast<const double*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
+  } else {
+    e = launch_gemm_t<kBF16x3, float>(g, static_cast<float*>(ws), g.N, nsplit, sym, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_gemm_reduce<float, double>), dim3(nb), dim3(256), 0, st, g.M, g.N, nsplit,
+                       static_cast<const float*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
+  }
+  return hipGetLastError();
+}
+
+// FP32 output variant (mixed path intermediates): C = A B^T, no split, no sym.
+hipError_t launch_gemm_f32out(const GemmArgs& g, float* C, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  return launch_gemm_t<kBF16x3, float>(g, C, g.N, 1, 0, st);
+}
+
+}  // namespace dfks

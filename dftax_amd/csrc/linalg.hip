@@ -1,12 +1,15 @@
 // Bandwidth-bound helpers of the Fock build and the SCF step (FP64).
 //
-//  * k_gamma    gamma_Q = sum_{m,i} s_i Cf[m,i] T[m,Q,i]   (fitted density, from the
+//  * k_gamma_part + k_colsum
+//               gamma_Q = sum_{m,i} s_i Cf[m,i] T[m,Q,i]   (fitted density, from the
 //               exchange intermediate T = B^T Cf, so B is not re-read for it)
 //  * k_jvec     J[mn] = sum_Q B[mn,Q] gamma_Q              (one read of B, HBM bound)
 //  * k_fock     F = hcore + J + cK sym(K) + sym(Vxc)        (upper-triangle inputs)
 //  * deterministic two-pass reductions for the energy terms
 //  * DIIS: Gram row, bordered solve (one lane, FP64, partial pivoting),
 //    extrapolation (dftax/ks/scf.py:252-276)
+#include <cstdint>
+
 #include "dfks_common.h"
 
 namespace dfks {
@@ -54,46 +57,89 @@ hipError_t launch_dot(int64_t n, const double* x, const double* y, double* part,
   return hipGetLastError();
 }
 
-// gamma for one block of 8 aux indices; threads over i (factor columns), loop over m.
+// gamma_q = sum_{m,i} s_i Cf[m,i] T[m,q,i] in two deterministic passes.
+// Pass 1: one wavefront per (m, block of 8 q): lanes over i, wave reduction;
+// partial[m*nq + q] written. Pass 2: column sums over m, fixed order.
 // T layout: T[(m*nq + q)*nf + i]; Cf column-major: Cf[i*nao + m].
-__global__ __launch_bounds__(256) void k_gamma(int nao, int nq, int nf, const double* __restrict__ T,
-                                               const double* __restrict__ Cf, const double* __restrict__ sgn,
-                                               double* __restrict__ gamma, int accumulate) {
-  __shared__ double s[256];
-  const int q = blockIdx.x;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+template <typename TT>
+__global__ __launch_bounds__(256) void k_gamma_part(int nao, int nq, int nf, const TT* __restrict__ T,
+                                                    const double* __restrict__ Cf, const double* __restrict__ sgn,
+                                                    double* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int qblocks = (nq + 7) / 8;
+  if (w >= (int64_t)nao * qblocks) return;
+  const int m = (int)(w / qblocks);
+  const int q0 = (int)(w % qblocks) * 8;
+  // this lane's coefficients s_i Cf[m, i] for i = lane, lane + 64, ...
+  double c[8];
+  int ni = 0;
+  for (int i = lane; i < nf && ni < 8; i += 64) c[ni++] = sgn[i] * Cf[(int64_t)i * nao + m];
+  for (int q = q0; q < q0 + 8 && q < nq; ++q) {
+    const TT* row = T + ((int64_t)m * nq + q) * nf;
     double a = 0.0;
-    const double* c = Cf + (int64_t)i * nao;
-    for (int m = 0; m < nao; ++m) a += c[m] * T[((int64_t)m * nq + q) * nf + i];
-    acc += sgn[i] * a;
+    int k = 0;
+    for (int i = lane; i < nf && k < 8; i += 64, ++k) a += c[k] * (double)row[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0) part[(int64_t)m * nq + q] = a;
   }
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) gamma[q] = accumulate ? gamma[q] + s[0] : s[0];
 }
 
-hipError_t launch_gamma(int nao, int nq, int nf, const double* T, const double* Cf, const double* sgn, double* gamma,
-                        int accumulate, hipStream_t st) {
-  hipLaunchKernelGGL(k_gamma, dim3(nq), dim3(256), 0, st, nao, nq, nf, T, Cf, sgn, gamma, accumulate);
+__global__ __launch_bounds__(256) void k_colsum(int nrows, int ncols, const double* __restrict__ part,
+                                                double* __restrict__ out, int accumulate) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncols) return;
+  double s = 0.0;
+  for (int m = 0; m < nrows; ++m) s += part[(int64_t)m * ncols + q];
+  out[q] = accumulate ? out[q] + s : s;
+}
+
+template <typename TT>
+static hipError_t gamma_impl(int nao, int nq, int nf, const TT* T, const double* Cf, const double* sgn, double* gamma,
+                             double* part, int accumulate, hipStream_t st) {
+  if (nf > 512) return hipErrorInvalidValue;  // eight coefficients per lane
+  const int64_t waves = (int64_t)nao * ((nq + 7) / 8);
+  hipLaunchKernelGGL((k_gamma_part<TT>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, nao, nq, nf, T, Cf,
+                     sgn, part);
+  hipLaunchKernelGGL(k_colsum, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, nao, nq, part, gamma,
+                     accumulate);
   return hipGetLastError();
 }
 
+hipError_t launch_gamma(int nao, int nq, int nf, const double* T, const double* Cf, const double* sgn, double* gamma,
+                        double* part, int accumulate, hipStream_t st) {
+  return gamma_impl<double>(nao, nq, nf, T, Cf, sgn, gamma, part, accumulate, st);
+}
+
+hipError_t launch_gamma_f32(int nao, int nq, int nf, const float* T, const double* Cf, const double* sgn,
+                            double* gamma, double* part, int accumulate, hipStream_t st) {
+  return gamma_impl<float>(nao, nq, nf, T, Cf, sgn, gamma, part, accumulate, st);
+}
+
 // J[r] = sum_q B[r*nq + q] g[q]; one wavefront per row, 4 rows per block per step.
-__global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const double* __restrict__ B,
+template <typename TB, bool VEC>
+__global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const TB* __restrict__ B,
                                               const double* __restrict__ g, double* __restrict__ J) {
   extern __shared__ double s_g[];
   for (int i = threadIdx.x; i < nq; i += blockDim.x) s_g[i] = g[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
+  constexpr int PER = 16 / sizeof(TB);
+  typedef TB vecT __attribute__((ext_vector_type(PER)));
   for (int64_t r = blockIdx.x * 4 + (threadIdx.x >> 6); r < nrows; r += (int64_t)gridDim.x * 4) {
-    const double* row = B + r * nq;
+    const TB* row = B + r * nq;
     double acc = 0.0;
-    for (int q = lane; q < nq; q += 64) acc += row[q] * s_g[q];
+    if (VEC) {
+      const vecT* rv = reinterpret_cast<const vecT*>(row);
+      for (int q = lane; q < nq / PER; q += 64) {
+        vecT x = rv[q];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) acc += (double)x[j] * s_g[q * PER + j];
+      }
+    } else {
+      for (int q = lane; q < nq; q += 64) acc += (double)row[q] * s_g[q];
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if (lane == 0) J[r] = acc;
@@ -103,7 +149,52 @@ __global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const doubl
 hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, double* J, hipStream_t st) {
   int64_t nb = (nrows + 3) / 4;
   if (nb > 65536) nb = 65536;
-  hipLaunchKernelGGL(k_jvec, dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g, J);
+  const bool vec = (nq % 2 == 0) && ((uintptr_t)B % 16 == 0);
+  if (vec)
+    hipLaunchKernelGGL((k_jvec<double, true>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g,
+                       J);
+  else
+    hipLaunchKernelGGL((k_jvec<double, false>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B,
+                       g, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_jvec_f32(int64_t nrows, int nq, const float* B, const double* g, double* J, hipStream_t st) {
+  int64_t nb = (nrows + 3) / 4;
+  if (nb > 65536) nb = 65536;
+  const bool vec = (nq % 4 == 0) && ((uintptr_t)B % 16 == 0);
+  if (vec)
+    hipLaunchKernelGGL((k_jvec<float, true>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g,
+                       J);
+  else
+    hipLaunchKernelGGL((k_jvec<float, false>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B,
+                       g, J);
+  return hipGetLastError();
+}
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
+__global__ void k_to_f64(int64_t n, const float* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (double)x[i];
+}
+
+hipError_t launch_to_f64(int64_t n, const float* x, double* y, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  int64_t nb = (n + 255) / 256;
+  if (nb > 131072) nb = 131072;
+  hipLaunchKernelGGL(k_to_f64, dim3((unsigned)nb), dim3(256), 0, st, n, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  int64_t nb = (n + 255) / 256;
+  if (nb > 131072) nb = 131072;
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, st, n, x, y);
   return hipGetLastError();
 }
 

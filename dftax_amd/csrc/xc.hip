@@ -13,6 +13,8 @@
 // including the same density/gradient floors, and evaluated on a forward-mode
 // dual number carrying d/drho and d/dsigma, so f_rho and f_sigma are exact to
 // rounding.
+#include <cstdint>
+
 #include "dfks_common.h"
 
 namespace dfks {
@@ -156,25 +158,47 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // One wavefront per grid point, 4 points per 256-thread block.
 // zw: on entry Z = Phi P (ng x nao); on exit W (ng x nao), in place.
-__global__ __launch_bounds__(256) void k_xc(int64_t ng, int64_t nao, const double* __restrict__ ao,
-                                            const double* __restrict__ wts, double* __restrict__ zw, int xc_id,
+// TA / TZ: storage types of the AO planes and of Z/W (double, or float on the
+// mixed-precision path); the reduction and the functional stay FP64.
+template <typename TA, typename TZ, bool VEC>
+__global__ __launch_bounds__(256) void k_xc(int64_t ng, int64_t nao, const TA* __restrict__ ao,
+                                            const double* __restrict__ wts, TZ* __restrict__ zw, int xc_id,
                                             double thresh, double* __restrict__ exc, double* __restrict__ rho_out) {
   const int lane = threadIdx.x & 63;
   const int64_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (g >= ng) return;
   const int64_t plane = ng * nao;
-  const double* p0 = ao + g * nao;
-  const double* p1 = p0 + plane;
-  const double* p2 = p0 + 2 * plane;
-  const double* p3 = p0 + 3 * plane;
-  double* zrow = zw + g * nao;
+  const TA* p0 = ao + g * nao;
+  const TA* p1 = p0 + plane;
+  const TA* p2 = p0 + 2 * plane;
+  const TA* p3 = p0 + 3 * plane;
+  TZ* zrow = zw + g * nao;
   double r = 0, gx = 0, gy = 0, gz = 0;
-  for (int64_t n = lane; n < nao; n += 64) {
-    double z = zrow[n];
-    r += z * p0[n];
-    gx += z * p1[n];
-    gy += z * p2[n];
-    gz += z * p3[n];
+  constexpr int PER = 16 / sizeof(TA);
+  typedef TA vecA __attribute__((ext_vector_type(PER)));
+  typedef TZ vecZ __attribute__((ext_vector_type(PER)));
+  if (VEC) {
+    for (int64_t n = lane; n < nao / PER; n += 64) {
+      vecZ z = reinterpret_cast<const vecZ*>(zrow)[n];
+      vecA a0 = reinterpret_cast<const vecA*>(p0)[n], a1 = reinterpret_cast<const vecA*>(p1)[n];
+      vecA a2 = reinterpret_cast<const vecA*>(p2)[n], a3 = reinterpret_cast<const vecA*>(p3)[n];
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        double zz = (double)z[j];
+        r += zz * (double)a0[j];
+        gx += zz * (double)a1[j];
+        gy += zz * (double)a2[j];
+        gz += zz * (double)a3[j];
+      }
+    }
+  } else {
+    for (int64_t n = lane; n < nao; n += 64) {
+      double z = (double)zrow[n];
+      r += z * (double)p0[n];
+      gx += z * (double)p1[n];
+      gy += z * (double)p2[n];
+      gz += z * (double)p3[n];
+    }
   }
   r = wave_sum(r);
   gx = 2.0 * wave_sum(gx);
@@ -195,14 +219,45 @@ __global__ __launch_bounds__(256) void k_xc(int64_t ng, int64_t nao, const doubl
     bx = t * gx; by = t * gy; bz = t * gz;
   }
   if (lane == 0) exc[g] = e;
-  for (int64_t n = lane; n < nao; n += 64) zrow[n] = a * p0[n] + bx * p1[n] + by * p2[n] + bz * p3[n];
+  if (VEC) {
+    for (int64_t n = lane; n < nao / PER; n += 64) {
+      vecA a0 = reinterpret_cast<const vecA*>(p0)[n], a1 = reinterpret_cast<const vecA*>(p1)[n];
+      vecA a2 = reinterpret_cast<const vecA*>(p2)[n], a3 = reinterpret_cast<const vecA*>(p3)[n];
+      vecZ o;
+#pragma unroll
+      for (int j = 0; j < PER; ++j)
+        o[j] = (TZ)(a * (double)a0[j] + bx * (double)a1[j] + by * (double)a2[j] + bz * (double)a3[j]);
+      reinterpret_cast<vecZ*>(zrow)[n] = o;
+    }
+  } else {
+    for (int64_t n = lane; n < nao; n += 64)
+      zrow[n] = (TZ)(a * (double)p0[n] + bx * (double)p1[n] + by * (double)p2[n] + bz * (double)p3[n]);
+  }
 }
 
 hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w, double* zw, int xc_id, double thresh,
                      double* exc, double* rho_out, hipStream_t st) {
   if (ng == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_xc, dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w, zw, xc_id, thresh, exc,
-                     rho_out);
+  const bool vec = nao % 2 == 0 && (uintptr_t)ao % 16 == 0 && (uintptr_t)zw % 16 == 0;
+  if (vec)
+    hipLaunchKernelGGL((k_xc<double, double, true>), dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w,
+                       zw, xc_id, thresh, exc, rho_out);
+  else
+    hipLaunchKernelGGL((k_xc<double, double, false>), dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao,
+                       w, zw, xc_id, thresh, exc, rho_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_xc_f32(int64_t ng, int64_t nao, const float* ao, const double* w, float* zw, int xc_id,
+                         double thresh, double* exc, hipStream_t st) {
+  if (ng == 0) return hipSuccess;
+  const bool vec = nao % 4 == 0 && (uintptr_t)ao % 16 == 0 && (uintptr_t)zw % 16 == 0;
+  if (vec)
+    hipLaunchKernelGGL((k_xc<float, float, true>), dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w,
+                       zw, xc_id, thresh, exc, (double*)nullptr);
+  else
+    hipLaunchKernelGGL((k_xc<float, float, false>), dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w,
+                       zw, xc_id, thresh, exc, (double*)nullptr);
   return hipGetLastError();
 }
 

@@ -33,21 +33,31 @@ def _torch():
 class HipSpec:
     """Engine value: run the DF-KS term through libdfks on the current GPU.
 
-    ``precision="fp64"`` computes everything in FP64. ``metric_cutoff`` is the
-    relative eigenvalue cutoff of the metric pseudo-inverse (terms.py:190-192).
+    ``precision="fp64"`` computes everything in FP64; ``"mixed"`` also keeps
+    FP32 copies of B and the AO grid so the SCF can run its early iterations
+    with split-BF16 MFMA GEMMs (FP32 accumulate) and refine in FP64.
+    ``gemm`` picks the FP64 GEMM kernels: ``"auto"`` (library symmetric
+    exchange kernel, rocBLAS for the plain products), ``"rocblas"`` or
+    ``"mfma"`` (library kernels for all). Mixed steps always use the
+    library's split-BF16 kernels. ``metric_cutoff`` is the relative eigenvalue cutoff of the
+    metric pseudo-inverse (terms.py:190-192).
     """
 
     precision: str = "fp64"
     metric_cutoff: float = 1e-7
     rho_thresh: float = 1e-10
     keep_int3c: bool = False
+    gemm: str = "auto"
 
 
 def hip(*, precision: str = "fp64", metric_cutoff: float = 1e-7, rho_thresh: float = 1e-10,
-        keep_int3c: bool = False) -> HipSpec:
-    if precision not in ("fp64",):
-        raise ValueError(f"precision {precision!r} is not available in this build (use 'fp64')")
-    return HipSpec(precision=precision, metric_cutoff=metric_cutoff, rho_thresh=rho_thresh, keep_int3c=keep_int3c)
+        keep_int3c: bool = False, gemm: str = "auto") -> HipSpec:
+    if precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+    if gemm not in ("auto", "rocblas", "mfma"):
+        raise ValueError(f"gemm must be 'auto', 'rocblas' or 'mfma', got {gemm!r}")
+    return HipSpec(precision=precision, metric_cutoff=metric_cutoff, rho_thresh=rho_thresh, keep_int3c=keep_int3c,
+                   gemm=gemm)
 
 
 @dataclass(frozen=True)
@@ -106,7 +116,10 @@ class HipDFKS:
         desc = _native.SystemDesc(self._arrays[0].desc, self._arrays[1].desc, len(self._z), _native.f64p(self._xyz),
                                   _native.f64p(self._z), self._gw.shape[0], _native.f64p(self._gxyz),
                                   _native.f64p(self._gw), xc.xc_id, int(nelec))
-        opts = _native.Options(0, float(spec.metric_cutoff), float(spec.rho_thresh), int(spec.keep_int3c))
+        opts = _native.Options(1 if spec.precision == "mixed" else 0, float(spec.metric_cutoff),
+                               float(spec.rho_thresh), int(spec.keep_int3c),
+                               {"auto": 0, "rocblas": 1, "mfma": 2}[spec.gemm])
+        self.precision = spec.precision
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             _native.check(self.L.dfks_create(ctypes.byref(desc), ctypes.byref(opts), ctypes.byref(h)))
@@ -137,6 +150,9 @@ class HipDFKS:
         out = torch.empty(shape, dtype=torch.float64, device=self.device)
         _native.check(self.L.dfks_get_matrix(self.handle, which, ctypes.c_void_p(out.data_ptr()), self._stream()))
         return out
+
+    def set_precision(self, mixed: bool) -> None:
+        _native.check(self.L.dfks_set_precision(self.handle, int(bool(mixed))))
 
     def energy_fock(self, P):
         """(E components [8], F) for a device density P (nao x nao)."""

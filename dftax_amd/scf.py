@@ -70,13 +70,31 @@ def _core_density(ks):
 
 
 def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, diis_space: int = 8,
-        lindep_thresh: float = 1e-7, level_shift: float = 0.0, guess=None, verbose: bool = False) -> KSResult:
-    """Run the KS SCF to self-consistency (closed shell)."""
+        lindep_thresh: float = 1e-7, level_shift: float = 0.0, guess=None, verbose: bool = False,
+        precision: str | None = None, mixed_d_tol: float = 1e-4, mixed_max_iter: int = 30) -> KSResult:
+    """Run the KS SCF to self-consistency (closed shell).
+
+    ``precision="mixed"`` (needs a KS built with ``hip(precision="mixed")``)
+    iterates with split-BF16 Fock builds until the orbital gradient
+    ``|err| < mixed_d_tol`` (or ``mixed_max_iter`` mixed steps), then switches
+    to FP64: the Fock matrix is rebuilt in FP64 at that density, the DIIS
+    history restarts, and the FP64 steps run to the usual ``e_tol``/``d_tol``.
+    The switch keys on the gradient only: the split-BF16 energy carries noise
+    of order 1e-5 relative, so |dE| is not a usable test in that phase. The
+    default is the precision the KS was built for, FP64 unless it was built
+    mixed.
+    """
     torch = __import__("torch")
     L = ks.coulomb.L
     h = ks.coulomb.handle
     stream = ks.coulomb._stream()
+    if precision is None:
+        precision = getattr(ks.coulomb, "precision", "fp64")
+    if precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+    mixed = precision == "mixed"
     P0 = initial_density(ks, guess)
+    ks.coulomb.set_precision(mixed)
     _native.check(L.dfks_scf_begin(h, ctypes.c_void_p(P0.data_ptr()), float(lindep_thresh), int(diis_space),
                                    float(level_shift), stream))
     out = (ctypes.c_double * 3)()
@@ -89,10 +107,16 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
         _native.check(L.dfks_scf_step(h, out, stream))
         times.append(time.perf_counter() - t0)
         e, de, derr = out[0], out[1], out[2]
-        hist.append((e, de, derr))
-        converged = abs(de) < e_tol and derr < d_tol
+        hist.append((e, de, derr, "mixed" if mixed else "fp64"))
         if verbose:
-            print(f"  scf {it}: E={e:.10f} dE={de:+.2e} |[F,P]|={derr:.2e}")
+            print(f"  scf {it} [{'mixed' if mixed else 'fp64'}]: E={e:.10f} dE={de:+.2e} |[F,P]|={derr:.2e}")
+        if mixed:
+            n_mixed = sum(1 for h in hist if h[3] == "mixed")
+            if derr < mixed_d_tol or n_mixed >= mixed_max_iter:
+                ks.coulomb.set_precision(False)
+                mixed = False
+        else:
+            converged = abs(de) < e_tol and derr < d_tol
         it += 1
     n = ks.nao
     P = torch.empty((n, n), dtype=torch.float64, device=ks.device)

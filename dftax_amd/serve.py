@@ -1,0 +1,236 @@
+"""Continuous-batching SCF server on one GPU.
+
+Requests (molecules) arrive over time; each one needs a full DF-KS solve.
+The server keeps a FIFO of arrived requests and admits the head as soon as
+a worker slot is free and its HBM footprint fits the remaining budget. The
+footprint counts B (nao^2 * nfit words, plus its FP32 copy and the unfitted
+tensor during the build), the AO grid and the SCF workspaces. Each admitted
+request runs on its own HIP stream from its own worker thread: its native
+handle is built, the SCF runs (mixed precision with FP64 refinement by
+default) and the handle is freed, returning its budget. Kernels of
+different molecules therefore overlap on the device, and a request that
+arrives while others run starts as soon as memory and a slot free up.
+
+There is no counterpart in dftax; the nearest is ``scf_batched``
+(dftax/ks/batched.py:82-304), which vmaps one molecule over geometries.
+
+Command line (one JSON line on stdout):
+    python -m dftax_amd.serve --n 64 --rate 1.0 --seed 0
+where ``--rate`` is the arrival rate in multiples of the isolated-solve
+throughput measured first on the same requests.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from dftax_amd.basis import build_basis
+from dftax_amd.grid import Becke, atom_centred_points, becke
+
+
+@dataclass
+class Request:
+    rid: int
+    mol: object
+    arrival: float = 0.0
+    footprint: int = 0
+    start: float = 0.0
+    end: float = 0.0
+    e_tot: float = float("nan")
+    n_iter: int = 0
+    converged: bool = False
+    error: str = ""
+    extra: dict = field(default_factory=dict)
+
+
+def footprint_bytes(mol, auxbasis: str, grid: Becke, precision: str) -> int:
+    """HBM bytes one request holds at its peak (build and SCF)."""
+    b = build_basis(mol.symbols, mol.coords, mol.basis, spherical=mol.spherical)
+    x = build_basis(mol.symbols, mol.coords, auxbasis, spherical=True)
+    nao, naux = b.nao, x.nao
+    ng = atom_centred_points(mol.symbols, mol.coords, grid)[0].shape[0]
+    nocc = mol.nelectron // 2
+    w3 = nao * nao * naux
+    f32 = precision == "mixed"
+    build_peak = 2 * 8 * w3                                   # (mn|P) and B together
+    steady = 8 * w3 + (4 * w3 if f32 else 0)                  # B (+ FP32 copy)
+    steady += ng * nao * (8 * 5 + (4 * 5 if f32 else 0))      # AO planes + Z/W (+ FP32 copies)
+    steady += nao * naux * max(nocc, 1) * (8 + (4 if f32 else 0))
+    steady += 8 * nao * nao * (64 + 40)                       # split-K workspace, DIIS, matrices
+    return int(max(build_peak, steady) * 1.1) + (64 << 20)
+
+
+class Server:
+    """Admission by HBM budget, one worker thread and HIP stream per slot."""
+
+    def __init__(self, xc: str = "B3LYP", *, auxbasis: str = "even-tempered-jk", grid: Becke | None = None,
+                 precision: str = "mixed", slots: int = 8, budget_bytes: int | None = None, e_tol: float = 1e-8,
+                 d_tol: float = 1e-6, device: int = 0):
+        self.xc = xc
+        self.auxbasis = auxbasis
+        self.grid = grid if grid is not None else becke()
+        self.precision = precision
+        self.slots = slots
+        self.e_tol, self.d_tol = e_tol, d_tol
+        self.device_index = device
+        if budget_bytes is None:
+            import torch
+            free, _ = torch.cuda.mem_get_info(torch.device("cuda", device))
+            budget_bytes = int(0.85 * free)
+        self.budget = budget_bytes
+        self._free = budget_bytes
+        self._cv = threading.Condition()
+        self._streams = {}
+
+    @property
+    def device(self):
+        import torch
+        return torch.device("cuda", self.device_index)
+
+    def stream_for(self, slot: int):
+        """The HIP stream of a worker slot (created on first use)."""
+        import torch
+        if slot not in self._streams:
+            self._streams[slot] = torch.cuda.Stream(device=self.device)
+        return self._streams[slot]
+
+    def estimate(self, req: Request) -> int:
+        return footprint_bytes(req.mol, self.auxbasis, self.grid, self.precision)
+
+    # -- one solve -----------------------------------------------------------
+    def solve(self, req: Request, stream=None) -> Request:
+        import torch
+        from dftax_amd.ks import KS, df, hip
+        from dftax_amd.scf import scf
+        stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        with torch.cuda.device(self.device), torch.cuda.stream(stream):
+            ks = None
+            try:
+                ks = KS(req.mol, self.xc, grid=self.grid,
+                        coulomb=df(self.auxbasis, engine=hip(precision=self.precision)), device=self.device)
+                r = scf(ks, e_tol=self.e_tol, d_tol=self.d_tol, precision=self.precision)
+                req.e_tot, req.n_iter, req.converged = r.e_tot, r.n_iter, r.converged
+                req.extra = {"nao": ks.nao, "ngrid": int(ks.grid_w.shape[0]),
+                             "n_mixed": sum(1 for h in r.history if h[3] == "mixed")}
+            except Exception as exc:  # reported per request, the server keeps going
+                req.error = f"{type(exc).__name__}: {exc}"
+            finally:
+                if ks is not None:
+                    ks.coulomb.close()
+                stream.synchronize()
+        return req
+
+    # -- the loop ------------------------------------------------------------
+    def run(self, requests: list[Request]) -> list[Request]:
+        """Serve requests at their ``arrival`` offsets (seconds from now)."""
+        for r in requests:
+            r.footprint = self.estimate(r)
+            if r.footprint > self.budget:
+                r.error = f"footprint {r.footprint} exceeds the HBM budget {self.budget}"
+        pending = sorted([r for r in requests if not r.error], key=lambda r: r.arrival)
+        t0 = time.perf_counter()
+        waiting: list[Request] = []
+        free_slots = list(range(self.slots))
+        threads = []
+        done: list[Request] = []
+        lock = threading.Lock()
+
+        def worker(req: Request, slot: int):
+            req.start = time.perf_counter() - t0
+            self.solve(req, self.stream_for(slot))
+            req.end = time.perf_counter() - t0
+            with self._cv:
+                self._free += req.footprint
+                free_slots.append(slot)
+                with lock:
+                    done.append(req)
+                self._cv.notify_all()
+
+        i = 0
+        n = len(pending)
+        while len(done) < n:
+            now = time.perf_counter() - t0
+            while i < n and pending[i].arrival <= now:
+                waiting.append(pending[i])
+                i += 1
+            with self._cv:
+                # admit FIFO while the head fits
+                while waiting and free_slots and waiting[0].footprint <= self._free:
+                    req = waiting.pop(0)
+                    self._free -= req.footprint
+                    slot = free_slots.pop(0)
+                    th = threading.Thread(target=worker, args=(req, slot), daemon=True)
+                    threads.append(th)
+                    th.start()
+                next_arrival = pending[i].arrival - now if i < n else None
+                timeout = 0.05 if next_arrival is None else max(0.0, min(0.05, next_arrival))
+                self._cv.wait(timeout=timeout)
+        for th in threads:
+            th.join()
+        return requests
+
+
+def latency_stats(reqs: list[Request]) -> dict:
+    ok = [r for r in reqs if not r.error]
+    lat = np.array([r.end - r.arrival for r in ok])
+    span = max(r.end for r in ok) - min(r.arrival for r in ok) if ok else float("nan")
+    return {
+        "n": len(reqs), "ok": len(ok), "converged": sum(r.converged for r in ok),
+        "solves_per_min": 60.0 * len(ok) / span if ok else 0.0,
+        "p50_s": float(np.percentile(lat, 50)) if ok else None,
+        "p99_s": float(np.percentile(lat, 99)) if ok else None,
+        "mean_s": float(lat.mean()) if ok else None,
+        "span_s": span,
+    }
+
+
+def main() -> int:
+    import torch
+    from dftax_amd.system import serving_stream
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--rate", type=float, nargs="+", default=[1.0, 2.0])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--xc", default="B3LYP")
+    ap.add_argument("--precision", default="mixed")
+    ap.add_argument("--isolated", type=int, default=16, help="requests timed alone to set the rate")
+    args = ap.parse_args()
+    mols = serving_stream(args.n, seed=args.seed)
+    srv = Server(args.xc, precision=args.precision, slots=args.slots)
+    # isolated-solve time on the first requests, one at a time
+    iso = []
+    for k in range(min(args.isolated, len(mols))):
+        t = time.perf_counter()
+        r = srv.solve(Request(rid=k, mol=mols[k]))
+        iso.append(time.perf_counter() - t)
+        if r.error:
+            raise RuntimeError(r.error)
+    tau = float(np.mean(iso[1:])) if len(iso) > 1 else iso[0]
+    out = {"isolated_solve_s_mean": tau, "isolated_solve_s_p99": float(np.percentile(iso[1:] or iso, 99))}
+    rng = np.random.default_rng(args.seed + 1)
+    for rate in args.rate:
+        lam = rate / tau
+        gaps = rng.exponential(1.0 / lam, size=len(mols))
+        arr = np.cumsum(gaps) - gaps[0]
+        reqs = [Request(rid=k, mol=m, arrival=float(a)) for k, (m, a) in enumerate(zip(mols, arr))]
+        srv.run(reqs)
+        st = latency_stats(reqs)
+        st["p99_over_isolated"] = st["p99_s"] / tau if st["p99_s"] else None
+        out[f"rate_{rate:g}x"] = st
+    out.update({"n_requests": len(mols), "xc": args.xc, "basis": "def2-svp / even-tempered-jk",
+                "precision": args.precision, "slots": args.slots, "budget_GB": srv.budget / 1e9,
+                "device": torch.cuda.get_device_name(0)})
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

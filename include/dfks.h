@@ -65,10 +65,13 @@ typedef struct {
 } dfks_system_desc;
 
 typedef struct {
-  int32_t precision;       /* 0: FP64 throughout */
+  int32_t precision;       /* 0: FP64 only; 1: also keep FP32 operands for mixed steps */
   double metric_cutoff;    /* relative eigenvalue cutoff of the metric pseudo-inverse (1e-7) */
   double rho_thresh;       /* density mask of the XC quadrature (1e-10) */
   int32_t keep_int3c;      /* 1: keep the unfitted (mn|P) tensor for inspection */
+  int32_t gemm;            /* FP64 products: 0 library symmetric exchange kernel + rocBLAS for the plain
+                              T, Z, V products (default); 1 rocBLAS for all; 2 library kernels for all.
+                              Mixed steps always use the library's split-BF16 kernels. */
 } dfks_options;
 
 /* Matrix selectors for dfks_get_matrix. */
@@ -124,6 +127,14 @@ int dfks_scf_begin(dfks_handle h, const double* P0, double lindep, int32_t diis_
 int dfks_scf_step(dfks_handle h, double* out3, dfks_stream stream);
 /* Current density, orbitals (nao x nmo, column-major), orbital energies. Any may be NULL. */
 int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* nmo, dfks_stream stream);
+
+/* Arithmetic of the following Fock builds: 0 FP64; 1 mixed (FP32 operands,
+ * three-term split-BF16 MFMA products with FP32 accumulation for the
+ * exchange and XC GEMMs; FP64 reductions, functional, J sum, DIIS, eigh).
+ * Needs options.precision = 1 at create. Switching from mixed to FP64 inside
+ * an SCF rebuilds the Fock matrix in FP64 at the current density before the
+ * next step and restarts the DIIS history. */
+int dfks_set_precision(dfks_handle h, int32_t mixed);
 
 /* Stage timers (HIP events on the work stream). Stages of the last Fock
  * build / SCF step, in ms: 0 exchange GEMM T = B^T Cf (+ gamma), 1 exchange

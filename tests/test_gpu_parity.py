@@ -183,3 +183,64 @@ def test_scf_water_pbe0(water_pair):
     S = o.S
     assert np.abs(P @ S @ P - 2 * P).max() < 1e-8
     assert abs(np.sum(P * S) - o.mol.nelectron) < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# MFMA GEMM backends and the mixed-precision path
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def dimer_backends(torch_cuda, water_dimer_tzvp):
+    from dftax_amd import KS, becke, df, hip
+    _say("case: water dimer def2-TZVP PBE0, becke(35,110): rocBLAS vs MFMA vs mixed")
+    g = becke(35, 110)
+    out = {}
+    for name, eng in (("rocblas", hip(gemm="rocblas")), ("mfma", hip(precision="mixed", gemm="mfma"))):
+        out[name] = KS(water_dimer_tzvp, "PBE0", grid=g, coulomb=df(engine=eng))
+    return out
+
+
+def _p0(ks):
+    from oracle import ref
+    P0 = ref.minao_density(ks.mol, ks.basis)
+    S = ks.S.cpu().numpy()
+    return P0 * (ks.mol.nelectron / np.sum(P0 * S))
+
+
+def test_mfma_fp64_matches_rocblas(dimer_backends, torch_cuda):
+    torch = torch_cuda
+    a, b = dimer_backends["rocblas"], dimer_backends["mfma"]
+    P0 = torch.tensor(_p0(a), device="cuda")
+    Ea, Fa = a.coulomb.energy_fock(P0)
+    b.coulomb.set_precision(False)
+    Eb, Fb = b.coulomb.energy_fock(P0)
+    assert np.abs((Fa - Fb).cpu().numpy()).max() < 1e-11
+    assert np.abs((Ea - Eb).cpu().numpy()).max() < 1e-10
+
+
+def test_mixed_fock_error_is_bounded(dimer_backends, torch_cuda):
+    """Split-BF16 products (FP32 accumulate) against FP64: relative Fock error ~1e-5."""
+    torch = torch_cuda
+    k = dimer_backends["mfma"]
+    P0 = torch.tensor(_p0(k), device="cuda")
+    k.coulomb.set_precision(False)
+    E64, F64 = k.coulomb.energy_fock(P0)
+    k.coulomb.set_precision(True)
+    Em, Fm = k.coulomb.energy_fock(P0)
+    k.coulomb.set_precision(False)
+    dF = np.abs((Fm - F64).cpu().numpy()).max()
+    dE = abs(float(Em[0] - E64[0]))
+    _say(f"mixed vs fp64: max|dF| = {dF:.3e}, |dE| = {dE:.3e}")
+    assert dF < 1e-3 and dE < 1e-3
+    assert dF > 0.0
+
+
+def test_mixed_scf_refines_to_fp64(dimer_backends):
+    from dftax_amd import scf
+    k = dimer_backends["mfma"]
+    r64 = scf(k, precision="fp64", e_tol=1e-9)
+    rmx = scf(k, precision="mixed", e_tol=1e-9)
+    nmix = sum(1 for h in rmx.history if h[3] == "mixed")
+    _say(f"fp64 {r64.e_tot:.12f} ({r64.n_iter} it); mixed {rmx.e_tot:.12f} ({rmx.n_iter} it, {nmix} mixed)")
+    assert r64.converged and rmx.converged and nmix >= 1
+    assert abs(rmx.e_tot - r64.e_tot) < 1e-8
