@@ -42,6 +42,9 @@ def main() -> int:
     ap.add_argument("--workload", default="h2o32", choices=sorted(WORKLOADS))
     ap.add_argument("--xc", default="PBE0")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="short run for hardware-counter passes: a plain 1 GiB device copy (the byte-counter "
+                         "calibration), then --steps timed steps per precision, no full SCF solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -54,6 +57,13 @@ def main() -> int:
     from dftax_amd import KS, Molecule, WATER, df, hip, scf, water_cluster
     from dftax_amd import _native as N
     from dftax_amd.scf import initial_density
+
+    if args.counters:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)                    # reads 1 GiB and writes 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
 
     wl = WORKLOADS[args.workload]
     if wl["shape"] is None:
@@ -90,6 +100,11 @@ def main() -> int:
     dt, stage_avg = timed(False)          # FP64: the headline value
     dt_mx, stage_mx = timed(True)         # mixed: split-BF16 MFMA products
     build = {k: v for k, v in N.stage_times(h).items() if k.startswith("build")}
+    if args.counters:
+        print(json.dumps({"counters_run": args.workload, "steps": args.steps, "copy_bytes": 1 << 30,
+                          "fp64_ms": round(1e3 * dt / args.steps, 3), "mixed_ms": round(1e3 * dt_mx / args.steps, 3)}),
+              flush=True)
+        return 0
 
     # both SCFs to convergence (e_tol 1e-8, d_tol 1e-6): final-energy agreement and time to solution
     torch.cuda.synchronize()

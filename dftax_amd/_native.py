@@ -33,7 +33,7 @@ class SystemDesc(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", c_int32), ("metric_cutoff", c_double), ("rho_thresh", c_double),
-                ("keep_int3c", c_int32), ("gemm", c_int32)]
+                ("keep_int3c", c_int32), ("gemm", c_int32), ("density", c_int32)]
 
 
 DFKS_S, DFKS_HCORE, DFKS_KINETIC, DFKS_NUCLEAR, DFKS_INT2C, DFKS_INT3C, DFKS_BFIT, DFKS_AO, DFKS_GRID_W = range(9)
@@ -46,7 +46,7 @@ EXPORTS = [
 ]
 
 STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
-          "scf_diis", "scf_eigh", "build_1e", "build_2c", "build_3c", "build_metric_B", "build_ao"]
+          "scf_diis", "scf_density", "build_1e", "build_2c", "build_3c", "build_metric_B", "build_ao"]
 
 _lib = None
 
@@ -79,8 +79,8 @@ def lib():
     L.dfks_scf_begin.argtypes = [c_void_p, c_void_p, c_double, c_int32, c_double, c_void_p]
     L.dfks_scf_step.argtypes = [c_void_p, P_f64, c_void_p]
     L.dfks_scf_result.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, P_i32, c_void_p]
-    L.dfks_becke_weights.argtypes = [c_int64, P_f64, P_f64, P_i32, c_int32, P_f64, P_f64, P_f64]
-    L.dfks_overlap.argtypes = [ctypes.POINTER(BasisDesc), P_f64]
+    L.dfks_becke_weights.argtypes = [c_int64, P_f64, P_f64, P_i32, c_int32, P_f64, P_f64, P_f64, c_void_p]
+    L.dfks_overlap.argtypes = [ctypes.POINTER(BasisDesc), P_f64, c_void_p]
     L.dfks_set_timing.argtypes = [c_void_p, c_int32]
     L.dfks_set_precision.argtypes = [c_void_p, c_int32]
     L.dfks_stage_times.argtypes = [c_void_p, P_f64, c_int32]
@@ -118,6 +118,12 @@ class BasisArrays:
                               int(basis.spherical), int(basis.nao))
 
 
+def current_stream() -> c_void_p:
+    """The calling thread's current HIP stream (torch's), as a ctypes pointer."""
+    import torch
+    return c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
 def becke_weights(pts: np.ndarray, raw: np.ndarray, owner: np.ndarray, atom_xyz: np.ndarray,
                   bragg: np.ndarray) -> np.ndarray:
     pts = np.ascontiguousarray(pts, np.float64)
@@ -127,7 +133,7 @@ def becke_weights(pts: np.ndarray, raw: np.ndarray, owner: np.ndarray, atom_xyz:
     br = np.ascontiguousarray(bragg, np.float64)
     out = np.empty(raw.shape[0])
     check(lib().dfks_becke_weights(raw.shape[0], f64p(pts), f64p(raw), i32p(owner), xyz.shape[0], f64p(xyz),
-                                   f64p(br), f64p(out)))
+                                   f64p(br), f64p(out), current_stream()))
     return out
 
 
@@ -135,7 +141,7 @@ def overlap(basis) -> np.ndarray:
     """Overlap matrix of a basis, computed on the GPU (used by the minao guess)."""
     arrs = BasisArrays(basis)
     out = np.empty((basis.nao, basis.nao))
-    check(lib().dfks_overlap(ctypes.byref(arrs.desc), f64p(out)))
+    check(lib().dfks_overlap(ctypes.byref(arrs.desc), f64p(out), current_stream()))
     return out
 
 

@@ -77,6 +77,12 @@ struct GemmArgs {
 hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, double beta, int nsplit, int sym,
                        void* ws, hipStream_t st);
 hipError_t launch_gemm_f32out(const GemmArgs& g, float* C, hipStream_t st);
+hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s);
+hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, hipStream_t s);
+hipError_t launch_sp2_trace_update(int n, double* X, const double* X2, int nocc, double* st, hipStream_t s);
+hipError_t launch_diag(int n, const double* R, double* d, hipStream_t s);
+hipError_t launch_pchol_panel(int n, int k0, int b, const double* R, double* d, double* L, int32_t* piv, double* st,
+                              int last, hipStream_t s);
 }  // namespace dfks
 
 using namespace dfks;
@@ -91,7 +97,7 @@ enum Stage {
   kStXcSyr2k = 5,       // Vxc = Phi^T W + W^T Phi
   kStFock = 6,          // assembly and energies
   kStScfDiis = 7,       // commutator + DIIS
-  kStScfEigh = 8,       // X^T F X, eigensolve, C, P
+  kStScfEigh = 8,       // X^T F X, density (purification or eigensolve), factor, P
   kNumStages = 9
 };
 
@@ -120,17 +126,74 @@ void fail(int code, const std::string& msg) {
                                               std::to_string((int)s_));                       \
   } while (0)
 
+// Device memory is stream-ordered: every entry point that allocates or frees
+// runs with its caller's stream as g_st, and allocations come from the
+// device's default memory pool (hipMallocAsync / hipFreeAsync). Unlike
+// hipFree, nothing here waits for the whole device, so handles of different
+// molecules can be built and destroyed while others compute. The pool keeps
+// released memory reserved for the next handle instead of returning it.
+thread_local hipStream_t g_st = nullptr;
+
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(g_st) { g_st = s; }
+  ~StreamScope() { g_st = prev; }
+};
+
+std::mutex g_pool_mu;
+std::vector<int> g_pool_ready;
+
+void pool_setup() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  if (std::find(g_pool_ready.begin(), g_pool_ready.end(), dev) != g_pool_ready.end()) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  g_pool_ready.push_back(dev);
+}
+
 template <class T>
 T* dalloc(size_t n) {
   if (n == 0) n = 1;
+  pool_setup();
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, n * sizeof(T));
-  if (e != hipSuccess) fail(4, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes: " + hipGetErrorString(e));
+  hipError_t e = hipMallocAsync(&p, n * sizeof(T), g_st);
+  if (e != hipSuccess)
+    fail(4, "hipMallocAsync of " + std::to_string(n * sizeof(T)) + " bytes: " + hipGetErrorString(e));
   return static_cast<T*>(p);
 }
 
 void dfree(void* p) {
-  if (p) (void)hipFree(p);
+  if (p) (void)hipFreeAsync(p, g_st);
+}
+
+// rocBLAS handles are pooled for the process: creating and destroying one per
+// molecule would allocate and free its workspace (a device-wide wait) each time.
+std::mutex g_rb_mu;
+std::vector<rocblas_handle> g_rb_free;
+
+rocblas_handle rb_acquire() {
+  {
+    std::lock_guard<std::mutex> lk(g_rb_mu);
+    if (!g_rb_free.empty()) {
+      rocblas_handle r = g_rb_free.back();
+      g_rb_free.pop_back();
+      return r;
+    }
+  }
+  rocblas_handle r = nullptr;
+  if (rocblas_create_handle(&r) != rocblas_status_success) fail(3, "rocblas_create_handle failed");
+  return r;
+}
+
+void rb_release(rocblas_handle r) {
+  if (!r) return;
+  std::lock_guard<std::mutex> lk(g_rb_mu);
+  g_rb_free.push_back(r);
 }
 
 struct Basis {
@@ -163,10 +226,13 @@ struct Basis {
       expect += nfun(l[i], spherical);
     }
     if (expect != nfunc) fail(1, "nfunc does not match the shells");
+  }
+  // device copies on g_st; the host vectors stay alive with the Basis
+  void upload() {
     auto up = [](auto& v) {
       using T = typename std::decay_t<decltype(v)>::value_type;
       T* p = dalloc<T>(v.size());
-      HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, g_st));
       return p;
     };
     d_l = up(l); d_nprim = up(nprim); d_prim0 = up(prim0); d_ao0 = up(ao0);
@@ -219,6 +285,8 @@ struct dfks_ctx {
   dfks_options opt{};
   Basis orb, aux;
   std::vector<double> atm_host;  // x, y, z, Z
+  std::vector<double> boys_host, gxyz_host, gw_host;
+  hipStream_t st = nullptr;      // stream of the last call (the destructor frees on it)
   std::vector<double> h_sgn;     // factor signs of the last Fock build
   // stage timers: events recorded on the work stream when timing is on
   bool timing = false;
@@ -260,13 +328,20 @@ struct dfks_ctx {
   double level_shift = 0.0, e_prev = 0.0;
   double *d_X = nullptr, *d_P = nullptr, *d_F = nullptr, *d_Fs = nullptr, *d_Es = nullptr, *d_gram = nullptr;
   double *d_coef = nullptr, *d_dots = nullptr, *d_Fx = nullptr, *d_C = nullptr, *d_eps = nullptr, *d_Ws = nullptr;
+  // purification: state, Gershgorin bounds, residual diagonal, occupied factor, pivots
+  double *d_sp2 = nullptr, *d_lohi = nullptr, *d_dvec = nullptr, *d_L = nullptr;
+  int32_t* d_piv = nullptr;
+  int sp2_guess = 24;       // iterations the last purification needed (first chunk of the next)
+  bool c_stale = false;     // d_C / d_eps not yet computed for the last step (purified density)
+  int n_eigh_steps = 0;     // steps that fell back to the eigensolver
 
   ~dfks_ctx() {
     orb.release();
     aux.release();
     void* ptrs[] = {d_boys, d_atm, d_S, d_T, d_V, d_H, d_int2c, d_int3c, d_B, d_gxyz, d_gw, d_ao, d_zw, d_exc,
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
-                    d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws};
+                    d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws,
+                    d_sp2, d_lohi, d_dvec, d_L, d_piv};
     for (void* p : ptrs) dfree(p);
     void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, d_T32, d_Cf32, d_P32, d_ws32, d_eps32, d_offd32};
     for (void* p : ptrs32) dfree(p);
@@ -274,7 +349,7 @@ struct dfks_ctx {
       if (e) (void)hipEventDestroy(e);
     for (auto& e : scf_ev)
       if (e) (void)hipEventDestroy(e);
-    if (rb) rocblas_destroy_handle(rb);
+    rb_release(rb);
   }
 };
 
@@ -307,8 +382,26 @@ double now_ms() {
 }
 
 void build_all(dfks_ctx* h, hipStream_t st) {
+  h->rb = rb_acquire();
   RBCHK(rocblas_set_stream(h->rb, st));
   double t0 = now_ms();
+  // inputs to the device
+  h->orb.upload();
+  h->aux.upload();
+  h->d_atm = dalloc<double>(h->atm_host.size());
+  HIPCHK(hipMemcpyAsync(h->d_atm, h->atm_host.data(), h->atm_host.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  h->d_boys = dalloc<double>(h->boys_host.size());
+  HIPCHK(hipMemcpyAsync(h->d_boys, h->boys_host.data(), h->boys_host.size() * sizeof(double), hipMemcpyHostToDevice,
+                        st));
+  if (h->ng > 0) {
+    h->d_gxyz = dalloc<double>(3 * h->ng);
+    h->d_gw = dalloc<double>(h->ng);
+    HIPCHK(hipMemcpyAsync(h->d_gxyz, h->gxyz_host.data(), 3 * h->ng * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(h->d_gw, h->gw_host.data(), h->ng * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  h->d_part = dalloc<double>(1024);
+  h->d_Edev = dalloc<double>(8);
+  h->d_info = dalloc<int32_t>(1);
   const int nao = h->nao, naux = h->naux;
   const ShellSet O = h->orb.view(), A = h->aux.view();
   // one-electron
@@ -347,8 +440,8 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   for (auto& v : auxl) { aux_off.push_back(all_aux.size()); all_aux.insert(all_aux.end(), v.begin(), v.end()); }
   int2* d_pairs = dalloc<int2>(all_pairs.size());
   int32_t* d_auxl = dalloc<int32_t>(all_aux.size());
-  HIPCHK(hipMemcpy(d_pairs, all_pairs.data(), all_pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d_auxl, all_aux.data(), all_aux.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(d_pairs, all_pairs.data(), all_pairs.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_auxl, all_aux.data(), all_aux.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
 
   // two-centre metric
   h->d_int2c = dalloc<double>((size_t)naux * naux);
@@ -584,8 +677,8 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
   h->mark(kStFock + 1, st);
 }
 
-// signed eigen-factorisation of a symmetric P (positive columns first)
-int factor_density(dfks_ctx* h, const double* P, hipStream_t st) {
+// signed eigen-factorisation of a symmetric P (positive columns first); signs to sg
+int factor_density(dfks_ctx* h, const double* P, std::vector<double>& sg, hipStream_t st) {
   const int nao = h->nao;
   RBCHK(rocblas_set_stream(h->rb, st));
   HIPCHK(hipMemcpyAsync(h->d_W1, P, sizeof(double) * nao * nao, hipMemcpyDeviceToDevice, st));
@@ -597,7 +690,7 @@ int factor_density(dfks_ctx* h, const double* P, hipStream_t st) {
   double amax = 0.0;
   for (double v : lam) amax = std::max(amax, std::fabs(v));
   std::vector<int32_t> sel;
-  std::vector<double> sg;
+  sg.clear();
   for (int i = nao - 1; i >= 0; --i)
     if (lam[i] > 1e-14 * amax) { sel.push_back(i); sg.push_back(1.0); }
   for (int i = 0; i < nao; ++i)
@@ -610,11 +703,8 @@ int factor_density(dfks_ctx* h, const double* P, hipStream_t st) {
 }
 
 void energy_fock_general(dfks_ctx* h, const double* P, double* E, double* F, hipStream_t st) {
-  int nf = factor_density(h, P, st);
-  std::vector<double> sg(nf);
-  std::vector<double> dev(nf);
-  HIPCHK(hipMemcpy(dev.data(), h->d_sgn, sizeof(double) * std::max(nf, 1), hipMemcpyDeviceToHost));
-  for (int i = 0; i < nf; ++i) sg[i] = dev[i];
+  std::vector<double> sg;
+  int nf = factor_density(h, P, sg, st);
   if (nf == 0) {
     sg.push_back(1.0);
     HIPCHK(hipMemsetAsync(h->d_Cf, 0, sizeof(double) * h->nao, st));
@@ -670,6 +760,15 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   realloc(h->d_C, (size_t)nao * nmo);
   realloc(h->d_eps, nmo);
   realloc(h->d_Ws, (size_t)nao * nao);
+  realloc(h->d_sp2, kSpHist + kSp2MaxIter);
+  realloc(h->d_lohi, 2 * (size_t)nmo);
+  realloc(h->d_dvec, nmo);
+  realloc(h->d_L, (size_t)nmo * h->nocc);
+  dfree(h->d_piv);
+  h->d_piv = dalloc<int32_t>(h->nocc);
+  h->sp2_guess = 24;
+  h->c_stale = false;
+  h->n_eigh_steps = 0;
   HIPCHK(hipMemsetAsync(h->d_gram, 0, sizeof(double) * m * m, st));
   HIPCHK(hipMemsetAsync(h->d_Fs, 0, sizeof(double) * m * nn, st));
   HIPCHK(hipMemsetAsync(h->d_Es, 0, sizeof(double) * m * nmo * nmo, st));
@@ -680,6 +779,57 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   HIPCHK(hipMemcpyAsync(e, h->d_Edev, sizeof(e), hipMemcpyDeviceToHost, st));
   sync(st);
   h->e_prev = e[0];
+}
+
+// Occupied projector of F' = d_Ws (nmo x nmo) by SP2 purification, then its
+// factor L (pivoted Cholesky) and Cf = sqrt(2) X L. Returns false when the
+// purification does not reach an idempotent projector of trace nocc (no gap
+// at the Fermi level); the caller then takes the eigensolver path.
+bool purify_density(dfks_ctx* h, hipStream_t st) {
+  const int n = h->nmo, nocc = h->nocc, nao = h->nao;
+  double* X = h->d_W1;
+  double* X2 = h->d_W2;
+  HIPCHK(launch_gersh(n, h->d_Ws, h->d_lohi, h->d_sp2, st));
+  HIPCHK(launch_sp2_init(n, h->d_Ws, h->d_sp2, X, st));
+  const int ntn = (n + 127) / 128;
+  const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, n);
+  const GemmArgs g{n, n, n, X, n, true, X, n, true};
+  int done = 0;
+  int chunk = h->sp2_guess + 1;
+  double stv[kSpHist];
+  bool frozen = false;
+  while (done < kSp2MaxIter) {
+    chunk = std::min(chunk, kSp2MaxIter - done);
+    for (int c = 0; c < chunk; ++c) {
+      HIPCHK(launch_gemm(kF64, g, X2, 1.0, 0.0, nsplit, 1, h->d_gws, st));
+      HIPCHK(launch_sp2_trace_update(n, X, X2, nocc, h->d_sp2, st));
+    }
+    done += chunk;
+    HIPCHK(hipMemcpyAsync(stv, h->d_sp2, sizeof(stv), hipMemcpyDeviceToHost, st));
+    sync(st);
+    if (stv[kSpFlag] < 0.0) {
+      frozen = true;
+      break;
+    }
+    chunk = 4;
+  }
+  if (!frozen) return false;
+  h->sp2_guess = std::max(8, (int)stv[kSpIter]);
+  if (!(std::fabs(stv[kSpErr]) < 1e-10 * std::max(1, nocc) && std::fabs(stv[kSpTrace] - nocc) < 1e-8 * std::max(1, nocc)))
+    return false;
+  // X = L L^T by pivoted Cholesky, panel by panel (X is overwritten by the residual)
+  HIPCHK(launch_diag(n, X, h->d_dvec, st));
+  for (int k0 = 0; k0 < nocc; k0 += kPanel) {
+    const int b = std::min(kPanel, nocc - k0);
+    const int last = k0 + b >= nocc;
+    HIPCHK(launch_pchol_panel(n, k0, b, X, h->d_dvec, h->d_L, h->d_piv, h->d_sp2, last, st));
+    if (!last) {
+      const GemmArgs u{n, n, b, h->d_L + (int64_t)k0 * n, n, false, h->d_L + (int64_t)k0 * n, n, false};
+      HIPCHK(launch_gemm(kF64, u, X, -1.0, 1.0, 1, 1, h->d_gws, st));
+    }
+  }
+  gemm(h, false, false, nao, nocc, n, std::sqrt(2.0), h->d_X, nao, h->d_L, n, 0.0, h->d_Cf, nao);
+  return true;
 }
 
 // One iteration of dftax/ks/scf.py:343-379 on the device.
@@ -725,16 +875,23 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
     HIPCHK(launch_axpy(nn, h->level_shift, h->d_S, h->d_Fx, 1, st));
   }
   if (h->timing) (void)hipEventRecord(h->scf_ev[1], st);
-  // eigh(X^T F X), C = X C', P = 2 C_occ C_occ^T
+  // F' = X^T F X; occupied factor Cf (P = Cf Cf^T = 2 C_occ C_occ^T)
   gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_Fx, nao, 0.0, h->d_W2, nmo);
   gemm(h, false, false, nmo, nmo, nao, 1.0, h->d_W2, nmo, h->d_X, nao, 0.0, h->d_Ws, nmo);
-  // FP64 eigensolve in every step: an FP32 solve's eigenvector error (~eps_32 |F| / gap)
-  // stalls the orbital gradient of the mixed phase above the switch threshold.
-  RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, nmo, h->d_Ws, nmo, h->d_eps, h->d_offd,
-                         h->d_info));
-  gemm(h, false, false, nao, nmo, nmo, 1.0, h->d_X, nao, h->d_Ws, nmo, 0.0, h->d_C, nao);
   const int nocc = h->nocc;
-  HIPCHK(launch_axpy((int64_t)nao * nocc, std::sqrt(2.0), h->d_C, h->d_Cf, 0, st));
+  const bool purified = h->opt.density == 0 && purify_density(h, st);
+  if (!purified) {
+    // FP64 eigensolve (also in mixed steps: an FP32 solve's eigenvector error,
+    // ~eps_32 |F| / gap, stalls the orbital gradient above the switch threshold)
+    RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, nmo, h->d_Ws, nmo, h->d_eps,
+                           h->d_offd, h->d_info));
+    gemm(h, false, false, nao, nmo, nmo, 1.0, h->d_X, nao, h->d_Ws, nmo, 0.0, h->d_C, nao);
+    HIPCHK(launch_axpy((int64_t)nao * nocc, std::sqrt(2.0), h->d_C, h->d_Cf, 0, st));
+    h->c_stale = false;
+    h->n_eigh_steps += 1;
+  } else {
+    h->c_stale = true;
+  }
   gemm(h, false, true, nao, nao, nocc, 1.0, h->d_Cf, nao, h->d_Cf, nao, 0.0, h->d_P, nao);
   if (h->timing) (void)hipEventRecord(h->scf_ev[2], st);
   std::vector<double> sg(nocc, 1.0);
@@ -744,9 +901,15 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
   double e[8];
   HIPCHK(hipMemcpyAsync(e, h->d_Edev, sizeof(e), hipMemcpyDeviceToHost, st));
   int info = 0;
-  HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+  double chol[2] = {0.0, 1.0};
+  if (purified) HIPCHK(hipMemcpyAsync(chol, h->d_sp2 + kSpResid, sizeof(chol), hipMemcpyDeviceToHost, st));
+  else HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
   sync(st);
   if (info != 0) fail(5, "Fock eigensolver failed, info=" + std::to_string(info));
+  // a converged projector has residual trace ~1e-13 after nocc pivots, and pivots >= 1/nmo
+  if (purified && !(std::fabs(chol[0]) < 1e-8 && chol[1] > 0.1 / nmo))
+    fail(5, "occupied factor of the purified density failed: residual trace " + std::to_string(chol[0]) +
+                ", smallest pivot " + std::to_string(chol[1]));
   out3[0] = e[0];
   out3[1] = e[0] - h->e_prev;
   out3[2] = std::sqrt(e[7]);
@@ -775,15 +938,17 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
     auto* h = new dfks_ctx();
     try {
       if (opt) h->opt = *opt;
-      else h->opt = dfks_options{0, 1e-7, 1e-10, 0, 0};
+      else h->opt = dfks_options{0, 1e-7, 1e-10, 0, 0, 0};
       if (h->opt.precision != 0 && h->opt.precision != 1) fail(1, "precision must be 0 (FP64) or 1 (mixed)");
       if (h->opt.gemm < 0 || h->opt.gemm > 2) fail(1, "gemm must be 0 (auto), 1 (rocBLAS) or 2 (library MFMA)");
+      if (h->opt.density != 0 && h->opt.density != 1) fail(1, "density must be 0 (purification) or 1 (eigensolver)");
       h->xc_id = sys->xc_id;
       h->hf = hf_coeff(sys->xc_id);
       if (sys->nelec <= 0 || sys->nelec % 2) fail(1, "closed shell needs an even, positive electron count");
       h->nocc = sys->nelec / 2;
       h->orb.load(sys->orbital);
       h->aux.load(sys->aux);
+      if (sys->ngrid < 0) fail(1, "negative grid size");
       if (h->orb.maxl() > 3) fail(1, "orbital shells above f (l > 3) are not supported");
       if (h->aux.maxl() > 4) fail(1, "auxiliary shells above g (l > 4) are not supported");
       h->nao = h->orb.nfunc;
@@ -805,22 +970,13 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
         }
       }
       h->e_nn = enn;
-      h->d_atm = dalloc<double>(h->atm_host.size());
-      HIPCHK(hipMemcpy(h->d_atm, h->atm_host.data(), h->atm_host.size() * sizeof(double), hipMemcpyHostToDevice));
-      std::vector<double> tbl = boys_table();
-      h->d_boys = dalloc<double>(tbl.size());
-      HIPCHK(hipMemcpy(h->d_boys, tbl.data(), tbl.size() * sizeof(double), hipMemcpyHostToDevice));
+      h->boys_host = boys_table();
       h->ng = sys->ngrid;
       if (h->ng > 0) {
-        h->d_gxyz = dalloc<double>(3 * h->ng);
-        h->d_gw = dalloc<double>(h->ng);
-        HIPCHK(hipMemcpy(h->d_gxyz, sys->grid_xyz, 3 * h->ng * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->d_gw, sys->grid_w, h->ng * sizeof(double), hipMemcpyHostToDevice));
+        if (!sys->grid_xyz || !sys->grid_w) fail(1, "null grid arrays");
+        h->gxyz_host.assign(sys->grid_xyz, sys->grid_xyz + 3 * h->ng);
+        h->gw_host.assign(sys->grid_w, sys->grid_w + h->ng);
       }
-      h->d_part = dalloc<double>(1024);
-      h->d_Edev = dalloc<double>(8);
-      h->d_info = dalloc<int32_t>(1);
-      if (rocblas_create_handle(&h->rb) != rocblas_status_success) fail(3, "rocblas_create_handle failed");
     } catch (...) {
       delete h;
       throw;
@@ -878,13 +1034,18 @@ int dfks_stage_times(dfks_handle h, double* ms_out, int32_t n) {
 }
 
 int dfks_destroy(dfks_handle h) {
-  return guard([&] { delete h; });
+  return guard([&] {
+    if (!h) return;
+    StreamScope ss(h->st);
+    delete h;
+  });
 }
 
 int dfks_build(dfks_handle h, dfks_stream stream) {
   return guard([&] {
     if (!h) fail(1, "null handle");
     if (h->built) fail(1, "handle already built");
+    StreamScope ss(h->st = S_(stream));
     build_all(h, S_(stream));
   });
 }
@@ -928,6 +1089,7 @@ int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream s
 int dfks_energy_fock(dfks_handle h, const double* P, double* E, double* F, dfks_stream stream) {
   return guard([&] {
     if (!h || !P || !E || !F) fail(1, "null argument");
+    StreamScope ss(h->st = S_(stream));
     energy_fock_general(h, P, E, F, S_(stream));
   });
 }
@@ -940,6 +1102,7 @@ int dfks_energy_fock_factored(dfks_handle h, const double* P, const double* Cf, 
     std::vector<double> sg(signs_host, signs_host + nf);
     for (double s : sg)
       if (s != 1.0 && s != -1.0) fail(1, "signs must be +1 or -1");
+    StreamScope ss(h->st = S_(stream));
     energy_fock_factored(h, P, Cf, sg, nf, E, F, S_(stream));
   });
 }
@@ -955,6 +1118,7 @@ int dfks_scf_begin(dfks_handle h, const double* P0, double lindep, int32_t diis_
                    dfks_stream stream) {
   return guard([&] {
     if (!h || !P0) fail(1, "null argument");
+    StreamScope ss(h->st = S_(stream));
     scf_begin(h, P0, lindep, diis_space, level_shift, S_(stream));
   });
 }
@@ -962,6 +1126,7 @@ int dfks_scf_begin(dfks_handle h, const double* P0, double lindep, int32_t diis_
 int dfks_scf_step(dfks_handle h, double* out3, dfks_stream stream) {
   return guard([&] {
     if (!h || !out3) fail(1, "null argument");
+    StreamScope ss(h->st = S_(stream));
     scf_step(h, out3, S_(stream));
   });
 }
@@ -971,6 +1136,21 @@ int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* n
     if (!h) fail(1, "null handle");
     if (!h->d_X) fail(6, "dfks_scf_begin has not been called");
     hipStream_t st = S_(stream);
+    StreamScope ss(h->st = st);
+    const int nao = h->nao, nm = h->nmo;
+    if (h->c_stale && (C || eps)) {
+      // orbitals of the last step's F' (its density came from purification)
+      RBCHK(rocblas_set_stream(h->rb, st));
+      HIPCHK(hipMemcpyAsync(h->d_W2, h->d_Ws, sizeof(double) * nm * nm, hipMemcpyDeviceToDevice, st));
+      RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, nm, h->d_W2, nm, h->d_eps,
+                             h->d_offd, h->d_info));
+      gemm(h, false, false, nao, nm, nm, 1.0, h->d_X, nao, h->d_W2, nm, 0.0, h->d_C, nao);
+      int info = 0;
+      HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+      sync(st);
+      if (info != 0) fail(5, "orbital eigensolver failed, info=" + std::to_string(info));
+      h->c_stale = false;
+    }
     if (P) HIPCHK(hipMemcpyAsync(P, h->d_P, sizeof(double) * h->nao * h->nao, hipMemcpyDeviceToDevice, st));
     if (C) HIPCHK(hipMemcpyAsync(C, h->d_C, sizeof(double) * h->nao * h->nmo, hipMemcpyDeviceToDevice, st));
     if (eps) HIPCHK(hipMemcpyAsync(eps, h->d_eps, sizeof(double) * h->nmo, hipMemcpyDeviceToDevice, st));
@@ -978,9 +1158,11 @@ int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* n
   });
 }
 
-int dfks_overlap(const dfks_basis_desc* basis, double* host_out) {
+int dfks_overlap(const dfks_basis_desc* basis, double* host_out, dfks_stream stream) {
   return guard([&] {
     if (!basis || !host_out) fail(1, "null argument");
+    hipStream_t st = S_(stream);
+    StreamScope ss(st);
     {
       std::lock_guard<std::mutex> lk(g_mu);
       static bool tables = false;
@@ -991,6 +1173,7 @@ int dfks_overlap(const dfks_basis_desc* basis, double* host_out) {
     }
     Basis b;
     b.load(*basis);
+    b.upload();
     struct Rel {
       Basis* b;
       ~Rel() { b->release(); }
@@ -1002,14 +1185,17 @@ int dfks_overlap(const dfks_basis_desc* basis, double* host_out) {
       std::vector<void*> p;
       ~Free() { for (void* q : p) dfree(q); }
     } fr{{dS, dT, dV, dummy}};
-    HIPCHK(launch_int1e(b.view(), 0, dummy, dummy, dS, dT, dV, n, nullptr));
-    HIPCHK(hipMemcpy(host_out, dS, n * n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(launch_int1e(b.view(), 0, dummy, dummy, dS, dT, dV, n, st));
+    HIPCHK(hipMemcpyAsync(host_out, dS, n * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync(st);
   });
 }
 
 int dfks_becke_weights(int64_t n, const double* pts, const double* raw_w, const int32_t* owner, int32_t natom,
-                       const double* atom_xyz, const double* bragg, double* w_out) {
+                       const double* atom_xyz, const double* bragg, double* w_out, dfks_stream stream) {
   return guard([&] {
+    hipStream_t st = S_(stream);
+    StreamScope ss(st);
     if (n < 0 || natom <= 0) fail(1, "bad sizes");
     if (n == 0) return;
     for (int64_t i = 0; i < n; ++i)
@@ -1036,14 +1222,15 @@ int dfks_becke_weights(int64_t n, const double* pts, const double* raw_w, const 
       std::vector<void*> p;
       ~Free() { for (void* q : p) dfree(q); }
     } fr{{d_pts, d_raw, d_w, d_own, d_atm, d_invR, d_a}};
-    HIPCHK(hipMemcpy(d_pts, pts, 3 * n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_raw, raw_w, n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_own, owner, n * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_atm, atom_xyz, 3 * natom * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_invR, invR.data(), invR.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_a, aAB.data(), aAB.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(launch_becke(n, d_pts, d_raw, d_own, natom, d_atm, d_invR, d_a, d_w, nullptr));
-    HIPCHK(hipMemcpy(w_out, d_w, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(d_pts, pts, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_raw, raw_w, n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_own, owner, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_atm, atom_xyz, 3 * natom * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_invR, invR.data(), invR.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_a, aAB.data(), aAB.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_becke(n, d_pts, d_raw, d_own, natom, d_atm, d_invR, d_a, d_w, st));
+    HIPCHK(hipMemcpyAsync(w_out, d_w, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync(st);
   });
 }
 

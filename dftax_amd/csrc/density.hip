@@ -1,0 +1,255 @@
+// Aufbau density of an orthonormal-basis Fock matrix without an eigensolver.
+//
+// The SCF step needs the projector onto the nocc lowest eigenvectors of
+// F' = X^T F X and a factor of it for the exchange build. A dense FP64
+// eigensolve is launch-latency bound on the GPU (tens of thousands of small
+// kernels for n ~ 10^3), so the projector comes from trace-correcting second-
+// order spectral projection (SP2; Niklasson, Phys. Rev. B 66, 155115 (2002)):
+//
+//   X_0     = (e_max I - F') / (e_max - e_min)        spectrum mapped into [0, 1]
+//   X_{k+1} = X_k^2          if |tr X_k^2 - nocc| < |2 tr X_k - tr X_k^2 - nocc|
+//           = 2 X_k - X_k^2  otherwise
+//
+// with Gershgorin bounds e_min, e_max. Both maps keep the spectrum in [0, 1],
+// so e_k = tr X_k - tr X_k^2 = sum lambda (1 - lambda) >= 0 bounds every
+// eigenvalue's distance from {0, 1}. Once e_k < 1e-9 two forced steps follow
+// (X^2, then 2X - X^2), which square the remaining deviations on both sides,
+// and the iteration freezes. The products X_k^2 are FP64 MFMA GEMMs; the trace
+// test and the update are the small kernels here, and the stop rule runs on
+// the device so the host only reads the state between chunks of iterations.
+//
+// The occupied factor L (X = L L^T, n x nocc) comes from a pivoted Cholesky of
+// the converged projector in panels of kPanel columns: one workgroup picks the
+// panel's pivots (largest residual diagonal) and forms its columns, then a
+// GEMM removes the panel from the trailing residual.
+#include <cstdint>
+
+#include "dfks_common.h"
+
+namespace dfks {
+
+constexpr double kSp2Tol = 1e-9;
+
+__device__ __forceinline__ double block_sum_1024(double v, double* s_red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if (lane == 0) s_red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_red[w];
+  __syncthreads();
+  return t;  // valid in thread 0
+}
+
+// lohi[i] = F_ii - r_i, lohi[n + i] = F_ii + r_i with r_i = sum_{j != i} |F_ij|
+__global__ __launch_bounds__(256) void k_gersh(int n, const double* __restrict__ F, double* __restrict__ lohi) {
+  __shared__ double s[256];
+  const int i = blockIdx.x;
+  double r = 0.0;
+  for (int j = threadIdx.x; j < n; j += 256)
+    if (j != i) r += fabs(F[(int64_t)i * n + j]);
+  s[threadIdx.x] = r;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double d = F[(int64_t)i * n + i];
+    lohi[i] = d - s[0];
+    lohi[n + i] = d + s[0];
+  }
+}
+
+// one block: spectral bounds and a fresh state
+__global__ __launch_bounds__(256) void k_sp2_start(int n, const double* __restrict__ lohi, double* __restrict__ st) {
+  __shared__ double smin[256], smax[256];
+  double lo = 1e300, hi = -1e300;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    lo = fmin(lo, lohi[i]);
+    hi = fmax(hi, lohi[n + i]);
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    lo = smin[0];
+    hi = smax[0];
+    // a zero-width interval only for F' = c I; widen it so the map stays defined
+    if (!(hi - lo > 1e-12 * fmax(1.0, fabs(hi)))) { lo -= 1.0; hi += 1.0; }
+    st[kSpEmin] = lo;
+    st[kSpEmax] = hi;
+    st[kSpFlag] = 0.0;
+    st[kSpCount] = 0.0;
+    st[kSpIter] = 0.0;
+    st[kSpErr] = 1e300;
+    st[kSpTrace] = 0.0;
+    st[kSpResid] = 0.0;
+    st[kSpMinPiv] = 1e300;
+  }
+}
+
+// X = (e_max I - F) / (e_max - e_min)
+__global__ void k_sp2_init(int n, const double* __restrict__ F, const double* __restrict__ st, double* __restrict__ X) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  if (idx >= nn) return;
+  const double lo = st[kSpEmin], hi = st[kSpEmax];
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  X[idx] = ((i == j ? hi : 0.0) - F[idx]) / (hi - lo);
+}
+
+// One block: traces of X and X^2, the branch (or the freeze) for this iteration.
+__global__ __launch_bounds__(1024) void k_sp2_trace(int n, const double* __restrict__ X, const double* __restrict__ X2,
+                                                    int nocc, double* __restrict__ st) {
+  __shared__ double s_red[16];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    a += X[(int64_t)i * n + i];
+    b += X2[(int64_t)i * n + i];
+  }
+  const double t1 = block_sum_1024(a, s_red);
+  const double t2 = block_sum_1024(b, s_red);
+  if (threadIdx.x != 0) return;
+  if (st[kSpFlag] < 0.0) return;  // frozen
+  const int it = (int)st[kSpIter];
+  const double e = t1 - t2;
+  if (it < kSp2MaxIter) st[kSpHist + it] = e;
+  st[kSpIter] = it + 1;
+  st[kSpErr] = e;
+  st[kSpTrace] = t1;
+  const double cnt = st[kSpCount];
+  double flag;
+  if (cnt >= 2.0) {
+    flag = -1.0;  // converged: X is final
+  } else if (cnt == 1.0) {
+    flag = 0.0;   // second forced step: 2X - X^2
+    st[kSpCount] = 2.0;
+  } else if (fabs(e) < kSp2Tol) {
+    flag = 1.0;   // first forced step: X^2
+    st[kSpCount] = 1.0;
+  } else {
+    flag = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc) ? 1.0 : 0.0;
+  }
+  st[kSpFlag] = flag;
+}
+
+__global__ void k_sp2_update(int64_t nn, double* __restrict__ X, const double* __restrict__ X2,
+                             const double* __restrict__ st) {
+  const double flag = st[kSpFlag];
+  if (flag < 0.0) return;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  X[i] = flag > 0.5 ? X2[i] : 2.0 * X[i] - X2[i];
+}
+
+// d[i] = R_ii
+__global__ void k_diag(int n, const double* __restrict__ R, double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = R[(int64_t)i * n + i];
+}
+
+// Pivoted Cholesky panel: columns k0 .. k0+b-1 of L (column-major, L[k*n + i]).
+// R is the residual after the previous panels (row-major, symmetric), d its
+// diagonal minus the columns of this panel formed so far. One workgroup.
+__global__ __launch_bounds__(1024) void k_pchol_panel(int n, int k0, int b, const double* __restrict__ R,
+                                                      double* __restrict__ d, double* __restrict__ L,
+                                                      int32_t* __restrict__ piv, double* __restrict__ st, int last) {
+  __shared__ double s_v[16];
+  __shared__ int s_i[16];
+  __shared__ double s_lp[kPanel];
+  __shared__ int s_p;
+  __shared__ double s_dp;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nw = blockDim.x >> 6;
+  for (int j = 0; j < b; ++j) {
+    const int k = k0 + j;
+    // pivot: largest residual diagonal, lowest index among equals
+    double bv = -1e300;
+    int bi = n;
+    for (int i = tid; i < n; i += blockDim.x) {
+      const double v = d[i];
+      if (v > bv) { bv = v; bi = i; }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_down(bv, off);
+      const int oi = __shfl_down(bi, off);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { s_v[wid] = bv; s_i[wid] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double v = s_v[0];
+      int p = s_i[0];
+      for (int w = 1; w < nw; ++w)
+        if (s_v[w] > v || (s_v[w] == v && s_i[w] < p)) { v = s_v[w]; p = s_i[w]; }
+      if (p >= n) p = 0;
+      s_p = p;
+      s_dp = v;
+      piv[k] = p;
+      st[kSpMinPiv] = fmin(st[kSpMinPiv], v);
+    }
+    __syncthreads();
+    const int p = s_p;
+    const double dp = s_dp;
+    if (tid < j) s_lp[tid] = L[(int64_t)(k0 + tid) * n + p];
+    __syncthreads();
+    const double inv = dp > 0.0 ? 1.0 / sqrt(dp) : 0.0;
+    for (int i = tid; i < n; i += blockDim.x) {
+      double c = R[(int64_t)p * n + i];
+      for (int t = 0; t < j; ++t) c -= L[(int64_t)(k0 + t) * n + i] * s_lp[t];
+      c *= inv;
+      L[(int64_t)k * n + i] = c;
+      d[i] = (i == p) ? 0.0 : d[i] - c * c;
+    }
+    __syncthreads();
+  }
+  if (last) {
+    __shared__ double s_red[16];
+    double a = 0.0;
+    for (int i = tid; i < n; i += blockDim.x) a += d[i];
+    const double r = block_sum_1024(a, s_red);
+    if (tid == 0) st[kSpResid] = r;
+  }
+}
+
+hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_gersh, dim3(n), dim3(256), 0, s, n, F, lohi);
+  hipLaunchKernelGGL(k_sp2_start, dim3(1), dim3(256), 0, s, n, lohi, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, hipStream_t s) {
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(k_sp2_init, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, n, F, st, X);
+  return hipGetLastError();
+}
+
+hipError_t launch_sp2_trace_update(int n, double* X, const double* X2, int nocc, double* st, hipStream_t s) {
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(1024), 0, s, n, X, X2, nocc, st);
+  hipLaunchKernelGGL(k_sp2_update, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, X, X2, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag(int n, const double* R, double* d, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, R, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_pchol_panel(int n, int k0, int b, const double* R, double* d, double* L, int32_t* piv, double* st,
+                              int last, hipStream_t s) {
+  if (b < 1 || b > kPanel) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pchol_panel, dim3(1), dim3(1024), 0, s, n, k0, b, R, d, L, piv, st, last);
+  return hipGetLastError();
+}
+
+}  // namespace dfks

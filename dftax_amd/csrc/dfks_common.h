@@ -16,6 +16,14 @@ constexpr double kBoysDt = 0.1;
 constexpr double kBoysTmax = 40.0;
 constexpr double kPi = 3.14159265358979323846;
 
+// SP2 purification state vector (doubles) and limits, shared by host and kernels
+enum Sp2State {
+  kSpEmin = 0, kSpEmax = 1, kSpFlag = 2, kSpCount = 3, kSpIter = 4, kSpErr = 5, kSpTrace = 6,
+  kSpResid = 7, kSpMinPiv = 8, kSpHist = 16
+};
+constexpr int kSp2MaxIter = 128;
+constexpr int kPanel = 32;            // pivoted Cholesky panel width
+
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 __host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }

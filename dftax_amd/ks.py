@@ -40,7 +40,12 @@ class HipSpec:
     exchange kernel, rocBLAS for the plain products), ``"rocblas"`` or
     ``"mfma"`` (library kernels for all). Mixed steps always use the
     library's split-BF16 kernels. ``metric_cutoff`` is the relative eigenvalue cutoff of the
-    metric pseudo-inverse (terms.py:190-192).
+    metric pseudo-inverse (terms.py:190-192). ``density`` is how each SCF step
+    gets the aufbau density from the orthonormal-basis Fock matrix:
+    ``"purify"`` (SP2 spectral projection with FP64 MFMA products and a
+    pivoted-Cholesky occupied factor; the eigensolver is used only when the
+    projection does not converge, i.e. without a gap at the Fermi level) or
+    ``"eigh"`` (an FP64 eigensolve every step, as scf.py:363-367 does).
     """
 
     precision: str = "fp64"
@@ -48,16 +53,19 @@ class HipSpec:
     rho_thresh: float = 1e-10
     keep_int3c: bool = False
     gemm: str = "auto"
+    density: str = "purify"
 
 
 def hip(*, precision: str = "fp64", metric_cutoff: float = 1e-7, rho_thresh: float = 1e-10,
-        keep_int3c: bool = False, gemm: str = "auto") -> HipSpec:
+        keep_int3c: bool = False, gemm: str = "auto", density: str = "purify") -> HipSpec:
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
     if gemm not in ("auto", "rocblas", "mfma"):
         raise ValueError(f"gemm must be 'auto', 'rocblas' or 'mfma', got {gemm!r}")
+    if density not in ("purify", "eigh"):
+        raise ValueError(f"density must be 'purify' or 'eigh', got {density!r}")
     return HipSpec(precision=precision, metric_cutoff=metric_cutoff, rho_thresh=rho_thresh, keep_int3c=keep_int3c,
-                   gemm=gemm)
+                   gemm=gemm, density=density)
 
 
 @dataclass(frozen=True)
@@ -118,7 +126,8 @@ class HipDFKS:
                                   _native.f64p(self._gw), xc.xc_id, int(nelec))
         opts = _native.Options(1 if spec.precision == "mixed" else 0, float(spec.metric_cutoff),
                                float(spec.rho_thresh), int(spec.keep_int3c),
-                               {"auto": 0, "rocblas": 1, "mfma": 2}[spec.gemm])
+                               {"auto": 0, "rocblas": 1, "mfma": 2}[spec.gemm],
+                               {"purify": 0, "eigh": 1}[spec.density])
         self.precision = spec.precision
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):

@@ -126,7 +126,7 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
     eps = torch.empty(nmo.value, dtype=torch.float64, device=ks.device)
     _native.check(L.dfks_scf_result(h, ctypes.c_void_p(P.data_ptr()), ctypes.c_void_p(C.data_ptr()),
                                     ctypes.c_void_p(eps.data_ptr()), ctypes.byref(nmo), stream))
-    torch.cuda.synchronize(ks.device)
+    torch.cuda.current_stream(ks.device).synchronize()
     res = KSResult(e_tot=float(e), e_elec=float(e) - ks.e_nn, converged=converged, n_iter=it, nocc=ks.nocc,
                    mo_energy=eps[None], mo_coeff=C.T[None], P=P[None], iter_times=times, history=hist)
     if not converged:

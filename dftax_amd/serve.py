@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 
 import threading
 import time
@@ -192,8 +193,6 @@ def latency_stats(reqs: list[Request]) -> dict:
 
 
 def main() -> int:
-    import torch
-    from dftax_amd.system import serving_stream
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--rate", type=float, nargs="+", default=[1.0, 2.0])
@@ -203,6 +202,10 @@ def main() -> int:
     ap.add_argument("--precision", default="mixed")
     ap.add_argument("--isolated", type=int, default=16, help="requests timed alone to set the rate")
     args = ap.parse_args()
+    # one hardware queue per worker stream (HIP's default is 4), set before the runtime starts
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(max(4, args.slots), 16)))
+    import torch
+    from dftax_amd.system import serving_stream
     mols = serving_stream(args.n, seed=args.seed)
     srv = Server(args.xc, precision=args.precision, slots=args.slots)
     # isolated-solve time on the first requests, one at a time

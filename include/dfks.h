@@ -72,6 +72,10 @@ typedef struct {
   int32_t gemm;            /* FP64 products: 0 library symmetric exchange kernel + rocBLAS for the plain
                               T, Z, V products (default); 1 rocBLAS for all; 2 library kernels for all.
                               Mixed steps always use the library's split-BF16 kernels. */
+  int32_t density;         /* SCF density from the orthonormal-basis Fock matrix: 0 spectral projection
+                              (SP2 purification + pivoted Cholesky factor; the eigensolver runs only
+                              when purification does not converge, and once for dfks_scf_result's
+                              orbitals); 1 eigensolver in every step (dftax/ks/scf.py:363-367). */
 } dfks_options;
 
 /* Matrix selectors for dfks_get_matrix. */
@@ -125,7 +129,8 @@ int dfks_scf_begin(dfks_handle h, const double* P0, double lindep, int32_t diis_
                    dfks_stream stream);
 /* One iteration; host out3 = {E_new, E_new - E_prev, |err|}. Synchronises. */
 int dfks_scf_step(dfks_handle h, double* out3, dfks_stream stream);
-/* Current density, orbitals (nao x nmo, column-major), orbital energies. Any may be NULL. */
+/* Current density, orbitals (nao x nmo, column-major), orbital energies of the last
+ * step's extrapolated Fock matrix. Any may be NULL. */
 int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* nmo, dfks_stream stream);
 
 /* Arithmetic of the following Fock builds: 0 FP64; 1 mixed (FP32 operands,
@@ -145,12 +150,14 @@ int dfks_set_precision(dfks_handle h, int32_t mixed);
 int dfks_set_timing(dfks_handle h, int32_t on);
 int dfks_stage_times(dfks_handle h, double* ms_out, int32_t n);
 
-/* Overlap matrix of one basis (host out, nfunc x nfunc); used by the minao guess. */
-int dfks_overlap(const dfks_basis_desc* basis, double* host_out);
+/* Overlap matrix of one basis (host out, nfunc x nfunc); used by the minao guess.
+ * Runs on the stream and synchronises it (only it). */
+int dfks_overlap(const dfks_basis_desc* basis, double* host_out, dfks_stream stream);
 
-/* Becke partition weights for an atom-centred grid (host arrays in and out). */
+/* Becke partition weights for an atom-centred grid (host arrays in and out).
+ * Runs on the stream and synchronises it (only it). */
 int dfks_becke_weights(int64_t n, const double* pts, const double* raw_w, const int32_t* owner, int32_t natom,
-                       const double* atom_xyz, const double* bragg, double* w_out);
+                       const double* atom_xyz, const double* bragg, double* w_out, dfks_stream stream);
 
 #ifdef __cplusplus
 }

@@ -192,10 +192,11 @@ def test_scf_water_pbe0(water_pair):
 @pytest.fixture(scope="module")
 def dimer_backends(torch_cuda, water_dimer_tzvp):
     from dftax_amd import KS, becke, df, hip
-    _say("case: water dimer def2-TZVP PBE0, becke(35,110): rocBLAS vs MFMA vs mixed")
+    _say("case: water dimer def2-TZVP PBE0, becke(35,110): rocBLAS vs MFMA vs mixed vs eigensolver density")
     g = becke(35, 110)
     out = {}
-    for name, eng in (("rocblas", hip(gemm="rocblas")), ("mfma", hip(precision="mixed", gemm="mfma"))):
+    for name, eng in (("rocblas", hip(gemm="rocblas")), ("mfma", hip(precision="mixed", gemm="mfma")),
+                      ("eigh", hip(gemm="rocblas", density="eigh"))):
         out[name] = KS(water_dimer_tzvp, "PBE0", grid=g, coulomb=df(engine=eng))
     return out
 
@@ -244,3 +245,27 @@ def test_mixed_scf_refines_to_fp64(dimer_backends):
     _say(f"fp64 {r64.e_tot:.12f} ({r64.n_iter} it); mixed {rmx.e_tot:.12f} ({rmx.n_iter} it, {nmix} mixed)")
     assert r64.converged and rmx.converged and nmix >= 1
     assert abs(rmx.e_tot - r64.e_tot) < 1e-8
+
+
+def test_purified_density_matches_eigensolver(dimer_backends, torch_cuda):
+    """SP2 purification + pivoted-Cholesky factor against an FP64 eigensolve in every step.
+
+    Both runs start from the same guess; with a HOMO-LUMO gap the purified
+    projector is the aufbau density, so the two SCF trajectories agree step by
+    step and the lazily computed orbitals of the last step match.
+    """
+    from dftax_amd import scf
+    a, e = dimer_backends["rocblas"], dimer_backends["eigh"]
+    ra = scf(a, e_tol=1e-10, d_tol=1e-7)
+    re = scf(e, e_tol=1e-10, d_tol=1e-7)
+    _say(f"purified {ra.e_tot:.12f} ({ra.n_iter} it); eigensolver {re.e_tot:.12f} ({re.n_iter} it)")
+    assert ra.converged and re.converged
+    assert ra.n_iter == re.n_iter
+    dEs = [abs(x[0] - y[0]) for x, y in zip(ra.history, re.history)]
+    assert max(dEs) < 1e-9, dEs
+    assert abs(ra.e_tot - re.e_tot) < 1e-10
+    assert np.abs((ra.P - re.P).cpu().numpy()).max() < 1e-8
+    assert np.abs((ra.mo_energy - re.mo_energy).cpu().numpy()).max() < 1e-8
+    nocc = a.nocc[0]
+    Ca, Ce = ra.mo_coeff[0][:, :nocc], re.mo_coeff[0][:, :nocc]
+    assert np.abs((Ca @ Ca.T - Ce @ Ce.T).cpu().numpy()).max() < 1e-8
