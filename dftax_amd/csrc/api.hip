@@ -30,6 +30,10 @@ void upload_c2s_tables();
 hipError_t launch_eri_wave(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,
                            const ShellSet& A, const ShellSet& C, int b_unit, const double* boys, double* out,
                            int64_t sa, int64_t sb, int tr, hipStream_t stream);
+bool eri_thread_has(int la, int lb, int lc);
+hipError_t launch_eri_thread(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,
+                             const ShellSet& A, const ShellSet& C, const double* boys, double* out, int64_t sa,
+                             int64_t sb, int64_t c0, int tr, hipStream_t st);
 hipError_t launch_int1e(const ShellSet& A, int natm, const double* atm, const double* boys, double* S, double* T,
                         double* V, int64_t nao, hipStream_t stream);
 hipError_t launch_becke(int64_t n, const double* pts, const double* raw, const int32_t* owner, int natm,
@@ -459,9 +463,14 @@ void build_all(dfks_ctx* h, hipStream_t st) {
     for (int lb = 0; lb <= la; ++lb)
       for (int lc = 0; lc <= lx; ++lc) {
         auto& pv = pairs[la * (lo + 1) + lb];
-        HIPCHK(launch_eri_wave(la, lb, lc, d_pairs + pair_off[la * (lo + 1) + lb], (int)pv.size(),
-                               d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A, 0, h->d_boys, d3,
-                               (int64_t)nao * naux, naux, 1, st));
+        if (eri_thread_has(la, lb, lc) && A.spherical)
+          HIPCHK(launch_eri_thread(la, lb, lc, d_pairs + pair_off[la * (lo + 1) + lb], (int)pv.size(),
+                                   d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A, h->d_boys, d3,
+                                   (int64_t)nao * naux, naux, 0, 1, st));
+        else
+          HIPCHK(launch_eri_wave(la, lb, lc, d_pairs + pair_off[la * (lo + 1) + lb], (int)pv.size(),
+                                 d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A, 0, h->d_boys, d3,
+                                 (int64_t)nao * naux, naux, 1, st));
       }
   sync(st);
   h->build_ms[2] = now_ms() - t0;
