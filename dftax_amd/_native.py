@@ -33,7 +33,8 @@ class SystemDesc(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", c_int32), ("metric_cutoff", c_double), ("rho_thresh", c_double),
-                ("keep_int3c", c_int32), ("gemm", c_int32), ("density", c_int32)]
+                ("keep_int3c", c_int32), ("gemm", c_int32), ("density", c_int32), ("aux_chunk", c_int32),
+                ("screen", c_double), ("ao_block", c_int64)]
 
 
 DFKS_S, DFKS_HCORE, DFKS_KINETIC, DFKS_NUCLEAR, DFKS_INT2C, DFKS_INT3C, DFKS_BFIT, DFKS_AO, DFKS_GRID_W = range(9)
@@ -42,7 +43,7 @@ EXPORTS = [
     "dfks_last_error", "dfks_create", "dfks_destroy", "dfks_build", "dfks_sizes", "dfks_get_matrix",
     "dfks_energy_fock", "dfks_energy_fock_factored", "dfks_xc_eval", "dfks_scf_begin", "dfks_scf_step",
     "dfks_scf_result", "dfks_becke_weights", "dfks_overlap", "dfks_set_timing", "dfks_stage_times",
-    "dfks_set_precision",
+    "dfks_set_precision", "dfks_stats",
 ]
 
 STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
@@ -84,6 +85,7 @@ def lib():
     L.dfks_set_timing.argtypes = [c_void_p, c_int32]
     L.dfks_set_precision.argtypes = [c_void_p, c_int32]
     L.dfks_stage_times.argtypes = [c_void_p, P_f64, c_int32]
+    L.dfks_stats.argtypes = [c_void_p, ctypes.POINTER(c_int64), c_int32]
     _lib = L
     return L
 
@@ -149,3 +151,12 @@ def stage_times(handle) -> dict:
     out = (ctypes.c_double * len(STAGES))()
     check(lib().dfks_stage_times(handle, out, len(STAGES)))
     return {k: float(v) for k, v in zip(STAGES, out)}
+
+
+STATS = ["pairs_kept", "pairs_total", "aux_slabs", "max_slab", "ao_block", "eigh_steps", "sp2_iterations"]
+
+
+def stats(handle) -> dict:
+    out = (c_int64 * len(STATS))()
+    check(lib().dfks_stats(handle, out, len(STATS)))
+    return {k: int(v) for k, v in zip(STATS, out)}

@@ -29,7 +29,10 @@ namespace dfks {
 void upload_c2s_tables();
 hipError_t launch_eri_wave(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,
                            const ShellSet& A, const ShellSet& C, int b_unit, const double* boys, double* out,
-                           int64_t sa, int64_t sb, int tr, hipStream_t stream);
+                           int64_t sa, int64_t sb, int tr, int64_t c0, hipStream_t stream);
+hipError_t launch_schwarz(int la, int lb, const int2* pairs, int npair, const ShellSet& A, const double* boys,
+                          double* q_out, hipStream_t st);
+hipError_t launch_jvec_acc(int64_t nrows, int nq, const double* B, const double* g, double* J, hipStream_t st);
 bool eri_thread_has(int la, int lb, int lc);
 hipError_t launch_eri_thread(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,
                              const ShellSet& A, const ShellSet& C, const double* boys, double* out, int64_t sa,
@@ -338,6 +341,22 @@ struct dfks_ctx {
   int sp2_guess = 24;       // iterations the last purification needed (first chunk of the next)
   bool c_stale = false;     // d_C / d_eps not yet computed for the last step (purified density)
   int n_eigh_steps = 0;     // steps that fell back to the eigensolver
+  // recompute mode (options.aux_chunk > 0): screened bra pairs per (la, lb)
+  // class, auxiliary slabs with per-lc shell lists, slab and exchange buffers
+  bool recompute = false;
+  bool slab_zero = false;   // screened pairs leave holes: clear a slab before filling it
+  int2* d_pairs3 = nullptr;
+  std::vector<size_t> pair3_off;
+  std::vector<int> pair3_n;
+  int64_t pairs_kept = 0, pairs_total = 0;
+  int32_t* d_slab_aux = nullptr;
+  std::vector<size_t> slab_aux_off;
+  std::vector<int> slab_aux_n, slab_p0, slab_np;
+  int max_np = 0;
+  double *d_slab = nullptr, *d_Tslab = nullptr, *d_Ttil = nullptr, *d_Lt = nullptr, *d_fitd = nullptr;
+  // AO grid streamed per block (options.ao_block > 0)
+  int64_t ao_block = 0;
+  double *d_aob = nullptr, *d_zwb = nullptr;
 
   ~dfks_ctx() {
     orb.release();
@@ -345,7 +364,8 @@ struct dfks_ctx {
     void* ptrs[] = {d_boys, d_atm, d_S, d_T, d_V, d_H, d_int2c, d_int3c, d_B, d_gxyz, d_gw, d_ao, d_zw, d_exc,
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
                     d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws,
-                    d_sp2, d_lohi, d_dvec, d_L, d_piv};
+                    d_sp2, d_lohi, d_dvec, d_L, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
+                    d_fitd, d_aob, d_zwb};
     for (void* p : ptrs) dfree(p);
     void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, d_T32, d_Cf32, d_P32, d_ws32, d_eps32, d_offd32};
     for (void* p : ptrs32) dfree(p);
@@ -385,6 +405,95 @@ double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Bra pairs of the 3-centre builds: Schwarz factors of every shell pair and
+// the pairs kept per (la, lb) class (all of them when screen = 0).
+void setup_pairs(dfks_ctx* h, const std::vector<std::vector<int2>>& pairs, const int2* d_pairs,
+                 const std::vector<size_t>& pair_off, hipStream_t st) {
+  const int lo = h->orb.maxl();
+  const ShellSet O = h->orb.view();
+  size_t np_all = 0;
+  for (auto& v : pairs) np_all += v.size();
+  h->pair3_off.assign(pairs.size(), 0);
+  h->pair3_n.assign(pairs.size(), 0);
+  h->pairs_total = (int64_t)np_all;
+  if (!(h->opt.screen > 0.0)) {
+    // every pair: the class lists as they are
+    for (size_t c = 0; c < pairs.size(); ++c) {
+      h->pair3_off[c] = pair_off[c];
+      h->pair3_n[c] = (int)pairs[c].size();
+    }
+    h->pairs_kept = (int64_t)np_all;
+    h->slab_zero = false;
+    h->d_pairs3 = dalloc<int2>(std::max<size_t>(np_all, 1));
+    HIPCHK(hipMemcpyAsync(h->d_pairs3, d_pairs, sizeof(int2) * np_all, hipMemcpyDeviceToDevice, st));
+    return;
+  }
+  double* d_q = dalloc<double>(np_all);
+  for (int la = 0; la <= lo; ++la)
+    for (int lb = 0; lb <= la; ++lb) {
+      const int cls = la * (lo + 1) + lb;
+      HIPCHK(launch_schwarz(la, lb, d_pairs + pair_off[cls], (int)pairs[cls].size(), O, h->d_boys,
+                            d_q + pair_off[cls], st));
+    }
+  std::vector<double> q(np_all);
+  HIPCHK(hipMemcpyAsync(q.data(), d_q, sizeof(double) * np_all, hipMemcpyDeviceToHost, st));
+  sync(st);
+  dfree(d_q);
+  double qmax = 0.0;
+  for (double v : q) qmax = std::max(qmax, v);
+  const double cut = h->opt.screen > 0.0 ? h->opt.screen * qmax : -1.0;
+  std::vector<int2> kept;
+  for (size_t c = 0; c < pairs.size(); ++c) {
+    h->pair3_off[c] = kept.size();
+    for (size_t i = 0; i < pairs[c].size(); ++i)
+      if (q[pair_off[c] + i] >= cut) kept.push_back(pairs[c][i]);
+    h->pair3_n[c] = (int)(kept.size() - h->pair3_off[c]);
+  }
+  h->pairs_kept = (int64_t)kept.size();
+  h->slab_zero = h->pairs_kept < h->pairs_total;
+  h->d_pairs3 = dalloc<int2>(std::max<size_t>(kept.size(), 1));
+  if (!kept.empty())
+    HIPCHK(hipMemcpyAsync(h->d_pairs3, kept.data(), kept.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+  sync(st);  // kept lives on the host until the copy is done
+}
+
+// Recompute mode: auxiliary slabs (runs of whole shells of at most aux_chunk
+// functions) with their per-lc shell lists, and the slab-sized buffers.
+void setup_slabs(dfks_ctx* h, hipStream_t st) {
+  const int nao = h->nao, naux = h->naux;
+  const int lx = h->aux.maxl();
+  // slabs of whole shells in function order
+  const int chunk = std::max(1, h->opt.aux_chunk);
+  std::vector<int32_t> flat;
+  int c = 0;
+  while (c < h->aux.n()) {
+    const int p0 = h->aux.ao0[c];
+    int c1 = c, np = 0;
+    while (c1 < h->aux.n()) {
+      const int f = nfun(h->aux.l[c1], h->aux.spherical);
+      if (np > 0 && np + f > chunk) break;
+      np += f;
+      ++c1;
+    }
+    for (int lc = 0; lc <= lx; ++lc) {
+      h->slab_aux_off.push_back(flat.size());
+      for (int k = c; k < c1; ++k)
+        if (h->aux.l[k] == lc) flat.push_back(k);
+      h->slab_aux_n.push_back((int)(flat.size() - h->slab_aux_off.back()));
+    }
+    h->slab_p0.push_back(p0);
+    h->slab_np.push_back(np);
+    h->max_np = std::max(h->max_np, np);
+    c = c1;
+  }
+  h->d_slab_aux = dalloc<int32_t>(std::max<size_t>(flat.size(), 1));
+  HIPCHK(hipMemcpyAsync(h->d_slab_aux, flat.data(), flat.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  h->d_slab = dalloc<double>((size_t)nao * nao * h->max_np);
+  h->d_Tslab = dalloc<double>((size_t)nao * h->max_np * h->nf_chunk);
+  h->d_Ttil = dalloc<double>((size_t)nao * naux * h->nf_chunk);
+  sync(st);
+}
+
 void build_all(dfks_ctx* h, hipStream_t st) {
   h->rb = rb_acquire();
   RBCHK(rocblas_set_stream(h->rb, st));
@@ -420,6 +529,7 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   h->build_ms[0] = now_ms() - t0;
   t0 = now_ms();
 
+  h->nf_chunk = std::max(1, std::min(std::min(nao, 512), std::max(h->nocc, 1)));
   // class lists
   const int lo = h->orb.maxl(), lx = h->aux.maxl();
   std::vector<std::vector<int2>> pairs((lo + 1) * (lo + 1));
@@ -452,27 +562,34 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   for (int la = 0; la <= lx; ++la)
     for (int lc = 0; lc <= lx; ++lc)
       HIPCHK(launch_eri_wave(la, 0, lc, d_pairs + apair_off[la], (int)apairs[la].size(), d_auxl + aux_off[lc],
-                             (int)auxl[lc].size(), A, A, 1, h->d_boys, h->d_int2c, naux, 0, 0, st));
+                             (int)auxl[lc].size(), A, A, 1, h->d_boys, h->d_int2c, naux, 0, 0, 0, st));
   sync(st);
   h->build_ms[1] = now_ms() - t0;
   t0 = now_ms();
-  // three-centre
-  const size_t n3 = (size_t)nao * nao * naux;
-  double* d3 = dalloc<double>(n3);
-  for (int la = 0; la <= lo; ++la)
-    for (int lb = 0; lb <= la; ++lb)
-      for (int lc = 0; lc <= lx; ++lc) {
-        auto& pv = pairs[la * (lo + 1) + lb];
-        if (eri_thread_has(la, lb, lc) && A.spherical)
-          HIPCHK(launch_eri_thread(la, lb, lc, d_pairs + pair_off[la * (lo + 1) + lb], (int)pv.size(),
-                                   d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A, h->d_boys, d3,
-                                   (int64_t)nao * naux, naux, 0, 1, st));
-        else
-          HIPCHK(launch_eri_wave(la, lb, lc, d_pairs + pair_off[la * (lo + 1) + lb], (int)pv.size(),
-                                 d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A, 0, h->d_boys, d3,
-                                 (int64_t)nao * naux, naux, 1, st));
-      }
-  sync(st);
+  // bra pairs (Schwarz-screened when screen > 0)
+  setup_pairs(h, pairs, d_pairs, pair_off, st);
+  // three-centre (materialised mode only)
+  double* d3 = nullptr;
+  if (!h->recompute) {
+    d3 = dalloc<double>((size_t)nao * nao * naux);
+    if (h->slab_zero) HIPCHK(hipMemsetAsync(d3, 0, sizeof(double) * nao * nao * naux, st));
+    for (int la = 0; la <= lo; ++la)
+      for (int lb = 0; lb <= la; ++lb)
+        for (int lc = 0; lc <= lx; ++lc) {
+          const int cls = la * (lo + 1) + lb;
+          const int2* pp = h->d_pairs3 + h->pair3_off[cls];
+          const int npair = h->pair3_n[cls];
+          if (eri_thread_has(la, lb, lc) && A.spherical)
+            HIPCHK(launch_eri_thread(la, lb, lc, pp, npair, d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A,
+                                     h->d_boys, d3, (int64_t)nao * naux, naux, 0, 1, st));
+          else
+            HIPCHK(launch_eri_wave(la, lb, lc, pp, npair, d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A, 0,
+                                   h->d_boys, d3, (int64_t)nao * naux, naux, 1, 0, st));
+        }
+    sync(st);
+  } else {
+    setup_slabs(h, st);
+  }
   h->build_ms[2] = now_ms() - t0;
   t0 = now_ms();
   dfree(d_pairs);
@@ -496,27 +613,38 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   h->nfit = naux - k0;
   double* d_Lt = dalloc<double>((size_t)h->nfit * naux);  // column-major nfit x naux
   HIPCHK(launch_metric_factor(naux, k0, d_U, d_w, d_Lt, st));
-  // B_cm (nfit x nao^2) = Lt (nfit x naux) * int3c_cm (naux x nao^2)
-  h->d_B = dalloc<double>((size_t)nao * nao * h->nfit);
-  const double one = 1.0, zero = 0.0;
-  RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, h->nfit, (int64_t)nao * nao, naux,
-                         &one, d_Lt, h->nfit, d3, naux, &zero, h->d_B, h->nfit));
-  sync(st);
+  if (!h->recompute) {
+    // B_cm (nfit x nao^2) = Lt (nfit x naux) * int3c_cm (naux x nao^2)
+    h->d_B = dalloc<double>((size_t)nao * nao * h->nfit);
+    const double one = 1.0, zero = 0.0;
+    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, h->nfit, (int64_t)nao * nao, naux,
+                           &one, d_Lt, h->nfit, d3, naux, &zero, h->d_B, h->nfit));
+    sync(st);
+    dfree(d_Lt);
+    if (h->opt.keep_int3c) h->d_int3c = d3;
+    else dfree(d3);
+  } else {
+    h->d_Lt = d_Lt;
+    h->d_fitd = dalloc<double>(naux);
+  }
   h->build_ms[3] = now_ms() - t0;
   t0 = now_ms();
-  dfree(d_U); dfree(d_w); dfree(d_e); dfree(d_Lt);
-  if (h->opt.keep_int3c) h->d_int3c = d3;
-  else dfree(d3);
+  dfree(d_U); dfree(d_w); dfree(d_e);
 
-  // AO values and gradients on the grid
+  // AO values and gradients on the grid: resident, or per block in every Fock build
   if (h->ng > 0) {
-    h->d_ao = dalloc<double>((size_t)4 * h->ng * nao);
-    h->d_zw = dalloc<double>((size_t)h->ng * nao);
     h->d_exc = dalloc<double>(h->ng);
-    HIPCHK(launch_ao(h->ng, h->d_gxyz, O, nao, h->d_ao, st));
+    if (h->ao_block == 0) {
+      h->d_ao = dalloc<double>((size_t)4 * h->ng * nao);
+      h->d_zw = dalloc<double>((size_t)h->ng * nao);
+      HIPCHK(launch_ao(h->ng, h->d_gxyz, O, nao, h->d_ao, st));
+    } else {
+      const int64_t gb = std::min<int64_t>(h->ao_block, h->ng);
+      h->d_aob = dalloc<double>((size_t)4 * gb * nao);
+      h->d_zwb = dalloc<double>((size_t)gb * nao);
+    }
   }
   // Fock-build scratch: exchange intermediate for up to nf_chunk factor columns
-  h->nf_chunk = std::max(1, std::min(std::min(nao, 512), std::max(h->nocc, 1)));
   h->d_Tx = dalloc<double>((size_t)nao * h->nfit * h->nf_chunk);
   h->d_gamma = dalloc<double>(h->nfit);
   h->d_J = dalloc<double>((size_t)nao * nao);
@@ -531,7 +659,7 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   h->d_sel = dalloc<int32_t>(nao);
   h->d_gws = dalloc<double>((size_t)64 * nao * nao);
   h->d_gpart = dalloc<double>((size_t)nao * h->nfit);
-  if (h->opt.precision == 1) {
+  if (h->opt.precision == 1 && !h->recompute && h->ao_block == 0) {
     const size_t nB = (size_t)nao * nao * h->nfit;
     h->d_B32 = dalloc<float>(nB);
     HIPCHK(launch_to_f32((int64_t)nB, h->d_B, h->d_B32, st));
@@ -560,6 +688,68 @@ int split_for(int64_t tiles, int64_t K) {
   int64_t s = 2048 / std::max<int64_t>(tiles, 1);
   s = std::min<int64_t>(s, std::max<int64_t>(1, K / 512));
   return (int)std::max<int64_t>(1, std::min<int64_t>(s, 64));
+}
+
+// (mn|P) for the functions of slab s into d_slab, layout [m][n][p] with
+// p = P - p0 (kept bra pairs only; the others stay zero).
+void eri_slab(dfks_ctx* h, int s, hipStream_t st) {
+  const int nao = h->nao;
+  const int lo = h->orb.maxl(), lx = h->aux.maxl();
+  const int np = h->slab_np[s];
+  const int64_t p0 = h->slab_p0[s];
+  const ShellSet O = h->orb.view(), A = h->aux.view();
+  if (h->slab_zero) HIPCHK(hipMemsetAsync(h->d_slab, 0, sizeof(double) * nao * nao * np, st));
+  const int64_t sa = (int64_t)nao * np, sb = np;
+  for (int la = 0; la <= lo; ++la)
+    for (int lb = 0; lb <= la; ++lb) {
+      const int cls = la * (lo + 1) + lb;
+      const int npair = h->pair3_n[cls];
+      if (npair == 0) continue;
+      const int2* pp = h->d_pairs3 + h->pair3_off[cls];
+      for (int lc = 0; lc <= lx; ++lc) {
+        const int sl = s * (lx + 1) + lc;
+        const int nal = h->slab_aux_n[sl];
+        if (nal == 0) continue;
+        const int32_t* aa = h->d_slab_aux + h->slab_aux_off[sl];
+        if (eri_thread_has(la, lb, lc) && A.spherical)
+          HIPCHK(launch_eri_thread(la, lb, lc, pp, npair, aa, nal, O, A, h->d_boys, h->d_slab, sa, sb, p0, 1, st));
+        else
+          HIPCHK(launch_eri_wave(la, lb, lc, pp, npair, aa, nal, O, A, 0, h->d_boys, h->d_slab, sa, sb, 1, p0, st));
+      }
+    }
+}
+
+// Exchange intermediate of nc factor columns from recomputed slabs:
+// T~[m][P][i] = sum_l (l m|P) Cf[l,i] slab by slab, then the fitted
+// T[m][Q][i] = sum_P T~[m][P][i] L[P,Q] (one strided-batched GEMM over m).
+void exchange_recompute(dfks_ctx* h, const double* Cf, int nc, hipStream_t st) {
+  const int nao = h->nao, naux = h->naux, nfit = h->nfit;
+  const double one = 1.0, zero = 0.0;
+  for (size_t s = 0; s < h->slab_np.size(); ++s) {
+    const int np = h->slab_np[s];
+    eri_slab(h, (int)s, st);
+    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc, (int64_t)nao * np,
+                           nao, &one, Cf, nao, h->d_slab, (int64_t)nao * np, &zero, h->d_Tslab, nc));
+    HIPCHK(hipMemcpy2DAsync(h->d_Ttil + (size_t)h->slab_p0[s] * nc, sizeof(double) * naux * nc, h->d_Tslab,
+                            sizeof(double) * np * nc, sizeof(double) * np * nc, nao, hipMemcpyDeviceToDevice, st));
+  }
+  RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_none, rocblas_operation_transpose, nc, nfit, naux, &one,
+                                      h->d_Ttil, nc, (rocblas_stride)naux * nc, h->d_Lt, nfit, 0, &zero, h->d_Tx, nc,
+                                      (rocblas_stride)nfit * nc, nao));
+}
+
+// J = sum_P (mn|P) d_P with d = L gamma, slab by slab.
+void coulomb_recompute(dfks_ctx* h, hipStream_t st) {
+  const int nao = h->nao, naux = h->naux, nfit = h->nfit;
+  const double one = 1.0, zero = 0.0;
+  RBCHK(rocblas_dgemv(h->rb, rocblas_operation_transpose, nfit, naux, &one, h->d_Lt, nfit, h->d_gamma, 1, &zero,
+                      h->d_fitd, 1));
+  for (size_t s = 0; s < h->slab_np.size(); ++s) {
+    const int np = h->slab_np[s];
+    eri_slab(h, (int)s, st);
+    if (s == 0) HIPCHK(launch_jvec((int64_t)nao * nao, np, h->d_slab, h->d_fitd + h->slab_p0[s], h->d_J, st));
+    else HIPCHK(launch_jvec_acc((int64_t)nao * nao, np, h->d_slab, h->d_fitd + h->slab_p0[s], h->d_J, st));
+  }
 }
 
 void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const std::vector<double>& sgn, int nf,
@@ -599,7 +789,10 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
     const double sg = sgn[i0];
     const double beta = first ? 0.0 : 1.0;
     // T[m,Q,i] = sum_l B[l,m,Q] Cf[l,i]
-    if (mixed) {
+    if (h->recompute) {
+      exchange_recompute(h, Cf + (int64_t)i0 * nao, nc, st);
+      HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1, st));
+    } else if (mixed) {
       GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
                  h->d_Cf32 + (int64_t)i0 * nao, nao, true};
       HIPCHK(launch_gemm_f32out(g, h->d_T32, st));
@@ -621,7 +814,7 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
       const int64_t kd = (int64_t)nfit * nc;
       const int ntn = (nao + 127) / 128;
       const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, kd);
-      if (mixed) {
+      if (mixed && !h->recompute) {
         GemmArgs g{nao, nao, kd, h->d_T32, kd, true, h->d_T32, kd, true};
         HIPCHK(launch_gemm(kBF16x3, g, h->d_K, sg, beta, nsplit, 1, h->d_gws, st));
       } else if (lib_k) {
@@ -638,13 +831,32 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
   }
   // E_J = 1/2 |gamma|^2, J = B gamma
   HIPCHK(launch_dot(nfit, h->d_gamma, h->d_gamma, h->d_part, 0.5, E + DFKS_E_J, 0, st));
-  if (mixed) HIPCHK(launch_jvec_f32(nn, nfit, h->d_B32, h->d_gamma, h->d_J, st));
+  if (h->recompute) coulomb_recompute(h, st);
+  else if (mixed) HIPCHK(launch_jvec_f32(nn, nfit, h->d_B32, h->d_gamma, h->d_J, st));
   else HIPCHK(launch_jvec(nn, nfit, h->d_B, h->d_gamma, h->d_J, st));
   h->mark(kStCoulomb + 1, st);
   // E_K = -1/4 a_x sum P o K
   if (h->hf != 0.0) HIPCHK(launch_dot(nn, P, h->d_K, h->d_part, -0.25 * h->hf, E + DFKS_E_K, 0, st));
   // XC: Z = Phi P, W (in place), Vh = Phi^T W; V = Vh + Vh^T in the Fock assembly
-  if (ng > 0) {
+  if (ng > 0 && h->ao_block > 0) {
+    // AO values and gradients per block of points: Z_b = Phi_b P, W_b, Vh += Phi_b^T W_b
+    const ShellSet O = h->orb.view();
+    const int64_t gb = std::min<int64_t>(h->ao_block, ng);
+    for (int64_t g0 = 0; g0 < ng; g0 += gb) {
+      const int64_t nb = std::min(gb, ng - g0);
+      HIPCHK(launch_ao(nb, h->d_gxyz + 3 * g0, O, nao, h->d_aob, st));
+      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, nao, nb, nao, &one, P, nao,
+                             h->d_aob, nao, &zero, h->d_zwb, nao));
+      HIPCHK(launch_xc(nb, nao, h->d_aob, h->d_gw + g0, h->d_zwb, h->xc_id, h->opt.rho_thresh, h->d_exc + g0, nullptr,
+                       st));
+      const double beta = g0 == 0 ? 0.0 : 1.0;
+      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_transpose, nao, nao, nb, &one, h->d_aob,
+                             nao, h->d_zwb, nao, &beta, h->d_Vxc, nao));
+    }
+    h->mark(kStXcKernel + 1, st);
+    HIPCHK(launch_dot(ng, h->d_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
+    h->mark(kStXcSyr2k + 1, st);
+  } else if (ng > 0) {
     const int ntm = (nao + 127) / 128;
     const int nsplit = split_for((int64_t)ntm * ntm, ng);
     if (mixed) {
@@ -947,10 +1159,16 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
     auto* h = new dfks_ctx();
     try {
       if (opt) h->opt = *opt;
-      else h->opt = dfks_options{0, 1e-7, 1e-10, 0, 0, 0};
+      else h->opt = dfks_options{0, 1e-7, 1e-10, 0, 0, 0, 0, 0.0, 0};
       if (h->opt.precision != 0 && h->opt.precision != 1) fail(1, "precision must be 0 (FP64) or 1 (mixed)");
       if (h->opt.gemm < 0 || h->opt.gemm > 2) fail(1, "gemm must be 0 (auto), 1 (rocBLAS) or 2 (library MFMA)");
       if (h->opt.density != 0 && h->opt.density != 1) fail(1, "density must be 0 (purification) or 1 (eigensolver)");
+      if (h->opt.aux_chunk < 0 || h->opt.ao_block < 0 || h->opt.screen < 0.0 || h->opt.screen >= 1.0)
+        fail(1, "aux_chunk and ao_block must be >= 0 and screen in [0, 1)");
+      h->recompute = h->opt.aux_chunk > 0;
+      h->ao_block = h->opt.ao_block;
+      if (h->opt.precision == 1 && (h->recompute || h->ao_block > 0))
+        fail(1, "mixed precision with recomputed slabs or a streamed AO grid is not implemented");
       h->xc_id = sys->xc_id;
       h->hf = hf_coeff(sys->xc_id);
       if (sys->nelec <= 0 || sys->nelec % 2) fail(1, "closed shell needs an even, positive electron count");
@@ -1070,6 +1288,15 @@ int dfks_sizes(dfks_handle h, int64_t* out5) {
   });
 }
 
+int dfks_stats(dfks_handle h, int64_t* out, int32_t n) {
+  return guard([&] {
+    if (!h || !out) fail(1, "null argument");
+    const int64_t v[7] = {h->pairs_kept, h->pairs_total, (int64_t)h->slab_np.size(), h->max_np, h->ao_block,
+                          h->n_eigh_steps, h->sp2_guess};
+    for (int i = 0; i < n; ++i) out[i] = i < 7 ? v[i] : 0;
+  });
+}
+
 int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream stream) {
   return guard([&] {
     if (!h || !dev_out) fail(1, "null argument");
@@ -1091,6 +1318,7 @@ int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream s
       case DFKS_GRID_W: src = h->d_gw; n = (size_t)h->ng; break;
       default: fail(1, "unknown matrix selector");
     }
+    if (!src) fail(1, "this matrix is not held by the handle (recomputed slabs or a streamed AO grid)");
     HIPCHK(hipMemcpyAsync(dev_out, src, n * sizeof(double), hipMemcpyDeviceToDevice, S_(stream)));
   });
 }

@@ -131,7 +131,8 @@ __device__ double* r_ladder(int L, double rho, double pcx, double pcy, double pc
 // Output index: ao_a*sa + ao_b*sb + ao_c (+ transposed bra write if tr).
 __global__ __launch_bounds__(64) void k_eri_wave(
     int la, int lb, int lc, Task3 task, ShellSet A, ShellSet C, int b_unit,
-    const double* __restrict__ boys, double* __restrict__ out, int64_t sa, int64_t sb, int tr, int64_t blk0) {
+    const double* __restrict__ boys, double* __restrict__ out, int64_t sa, int64_t sb, int tr, int64_t blk0,
+    int64_t c0) {
   __shared__ double s_E[3][4 * 4 * 8];
   __shared__ double s_Ec[5 * 11];
   __shared__ double s_F[12];
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(64) void k_eri_wave(
   const int sphA = A.spherical && la >= 2, sphB = A.spherical && lb >= 2 && !b_unit, sphC = C.spherical && lc >= 2;
   const int fa = sphA ? nsph(la) : na, fb = sphB ? nsph(lb) : nb, fc = sphC ? nsph(lc) : nc;
   const int nf = fa * fb * fc;
-  const int64_t ao_a = A.ao0[ia], ao_b = b_unit ? 0 : A.ao0[ib], ao_c = C.ao0[ic];
+  const int64_t ao_a = A.ao0[ia], ao_b = b_unit ? 0 : A.ao0[ib], ao_c = C.ao0[ic] - c0;
   for (int o = lane; o < nf; o += kWave) {
     int Ci = o % fc, Bi = (o / fc) % fb, Ai = o / (fc * fb);
     int nA = sphA ? g_c2s.n[la][Ai] : 1;
@@ -377,6 +378,107 @@ __global__ __launch_bounds__(64) void k_int1e(ShellSet A, int natm, const double
 }
 
 // ---------------------------------------------------------------------------
+// Schwarz factors of bra shell pairs: Q_ab = max over cartesian components
+// (mu nu) of sqrt((mu nu|mu nu)) (dftax/integrals/eri4c.py:206-220, taken per
+// shell pair as in eri3c_bucketed.py:140-169). One wavefront per pair, lanes
+// over the component pairs, primitive quartets in sequence:
+//   (ab|ab) = sum_{p,q} K_p K_q 2 pi^2.5 / (g_p g_q sqrt(g_p + g_q))
+//             sum_{tuv} E^p_{tuv} sum_{tau nu phi} (-1)^{tau+nu+phi} E^q_{tau nu phi} R_{t+tau,u+nu,v+phi}
+// ---------------------------------------------------------------------------
+constexpr int kMaxLQ = 2 * (3 + 3);                                     // (ff|ff)
+constexpr int kNRQ = (kMaxLQ + 1) * (kMaxLQ + 2) * (kMaxLQ + 3) / 6;     // 455
+
+__global__ __launch_bounds__(64) void k_schwarz(int la, int lb, const int2* __restrict__ pairs, int npair, ShellSet A,
+                                                const double* __restrict__ boys, double* __restrict__ q_out) {
+  __shared__ double s_Ep[3][4 * 4 * 7];
+  __shared__ double s_Eq[3][4 * 4 * 7];
+  __shared__ double s_F[kMaxLQ + 1];
+  __shared__ double s_R[2][kNRQ];
+  const int lane = threadIdx.x;
+  const int k = blockIdx.x;
+  if (k >= npair) return;
+  const int2 pr = pairs[k];
+  const int ia = pr.x, ib = pr.y;
+  const int lab = la + lb, L = 2 * lab, mtab = lab + 1, nbb = lb + 1;
+  const int na = ncart(la), nb = ncart(lb);
+  const double Ax = A.centre[3 * ia], Ay = A.centre[3 * ia + 1], Az = A.centre[3 * ia + 2];
+  const double Bx = A.centre[3 * ib], By = A.centre[3 * ib + 1], Bz = A.centre[3 * ib + 2];
+  const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz, ab2 = ABx * ABx + ABy * ABy + ABz * ABz;
+  const int npa = A.nprim[ia], npb = A.nprim[ib], p0a = A.prim0[ia], p0b = A.prim0[ib];
+  const int npp = npa * npb;
+  double acc[2] = {0.0, 0.0};  // components lane and lane + 64 (na * nb <= 100)
+  for (int p = 0; p < npp; ++p) {
+    const double al = A.exps[p0a + p / npb], be = A.exps[p0b + p % npb];
+    const double gp = al + be;
+    const double Px = (al * Ax + be * Bx) / gp, Py = (al * Ay + be * By) / gp, Pz = (al * Az + be * Bz) / gp;
+    const double Kp = A.coefs[p0a + p / npb] * A.coefs[p0b + p % npb] * exp(-al * be / gp * ab2);
+    __syncthreads();
+    if (lane < 3) {
+      const double xab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
+      e_table_1d(la, lb, al, be, xab, mtab, s_Ep[lane]);
+    }
+    for (int q = 0; q < npp; ++q) {
+      const double ga = A.exps[p0a + q / npb], de = A.exps[p0b + q % npb];
+      const double gq = ga + de;
+      const double Qx = (ga * Ax + de * Bx) / gq, Qy = (ga * Ay + de * By) / gq, Qz = (ga * Az + de * Bz) / gq;
+      const double Kq = A.coefs[p0a + q / npb] * A.coefs[p0b + q % npb] * exp(-ga * de / gq * ab2);
+      const double rho = gp * gq / (gp + gq);
+      const double pqx = Px - Qx, pqy = Py - Qy, pqz = Pz - Qz;
+      const double pref = Kp * Kq * 2.0 * 17.493418327624862 / (gp * gq * sqrt(gp + gq));
+      __syncthreads();
+      if (lane < 3) {
+        const double xab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
+        e_table_1d(la, lb, ga, de, xab, mtab, s_Eq[lane]);
+      }
+      if (lane == 0) boys_array_rt(L, rho * (pqx * pqx + pqy * pqy + pqz * pqz), s_F, boys);
+      __syncthreads();
+      const double* R = r_ladder(L, rho, pqx, pqy, pqz, s_F, s_R[0], s_R[1]);
+      for (int h = 0; h < 2; ++h) {
+        const int o = lane + 64 * h;
+        if (o >= na * nb) continue;
+        const int ai = o / nb, bi = o % nb;
+        const int a0 = cexp(la, ai, 0), a1 = cexp(la, ai, 1), a2 = cexp(la, ai, 2);
+        const int b0 = cexp(lb, bi, 0), b1 = cexp(lb, bi, 1), b2 = cexp(lb, bi, 2);
+        const double* Epx = s_Ep[0] + (a0 * nbb + b0) * mtab;
+        const double* Epy = s_Ep[1] + (a1 * nbb + b1) * mtab;
+        const double* Epz = s_Ep[2] + (a2 * nbb + b2) * mtab;
+        const double* Eqx = s_Eq[0] + (a0 * nbb + b0) * mtab;
+        const double* Eqy = s_Eq[1] + (a1 * nbb + b1) * mtab;
+        const double* Eqz = s_Eq[2] + (a2 * nbb + b2) * mtab;
+        double sum = 0.0;
+        for (int t = 0; t <= a0 + b0; ++t)
+          for (int u = 0; u <= a1 + b1; ++u)
+            for (int v = 0; v <= a2 + b2; ++v) {
+              const double e1 = Epx[t] * Epy[u] * Epz[v];
+              if (e1 == 0.0) continue;
+              double inner = 0.0;
+              for (int ta = 0; ta <= a0 + b0; ++ta)
+                for (int nu = 0; nu <= a1 + b1; ++nu)
+                  for (int ph = 0; ph <= a2 + b2; ++ph) {
+                    const double e2 = Eqx[ta] * Eqy[nu] * Eqz[ph];
+                    inner += (((ta + nu + ph) & 1) ? -e2 : e2) * R[ridx(t + ta, u + nu, v + ph)];
+                  }
+              sum += e1 * inner;
+            }
+        acc[h] += pref * sum;
+      }
+      __syncthreads();
+    }
+  }
+  double m = fmax(acc[0], acc[1]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off));
+  if (lane == 0) q_out[k] = sqrt(fmax(m, 0.0));
+}
+
+hipError_t launch_schwarz(int la, int lb, const int2* pairs, int npair, const ShellSet& A, const double* boys,
+                          double* q_out, hipStream_t st) {
+  if (npair == 0) return hipSuccess;
+  if (la > 3 || lb > 3 || ncart(la) * ncart(lb) > 128) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_schwarz, dim3((unsigned)npair), dim3(64), 0, st, la, lb, pairs, npair, A, boys, q_out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
 void upload_c2s_tables() {
@@ -410,7 +512,7 @@ void upload_c2s_tables() {
 
 hipError_t launch_eri_wave(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,
                            const ShellSet& A, const ShellSet& C, int b_unit, const double* boys, double* out,
-                           int64_t sa, int64_t sb, int tr, hipStream_t stream) {
+                           int64_t sa, int64_t sb, int tr, int64_t c0, hipStream_t stream) {
   if (npair == 0 || naux_l == 0) return hipSuccess;
   Task3 t{pairs, aux, npair, naux_l};
   const int64_t nblk = (int64_t)npair * naux_l;
@@ -419,7 +521,7 @@ hipError_t launch_eri_wave(int la, int lb, int lc, const int2* pairs, int npair,
   for (int64_t b0 = 0; b0 < nblk; b0 += chunk) {
     int64_t nb = nblk - b0 < chunk ? nblk - b0 : chunk;
     hipLaunchKernelGGL(k_eri_wave, dim3((unsigned)nb), dim3(kWave), 0, stream, la, lb, lc, t, A, C, b_unit, boys,
-                       out, sa, sb, tr, b0);
+                       out, sa, sb, tr, b0, c0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -120,7 +120,7 @@ hipError_t launch_gamma_f32(int nao, int nq, int nf, const float* T, const doubl
 // J[r] = sum_q B[r*nq + q] g[q]; one wavefront per row, 4 rows per block per step.
 template <typename TB, bool VEC>
 __global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const TB* __restrict__ B,
-                                              const double* __restrict__ g, double* __restrict__ J) {
+                                              const double* __restrict__ g, double* __restrict__ J, int accumulate) {
   extern __shared__ double s_g[];
   for (int i = threadIdx.x; i < nq; i += blockDim.x) s_g[i] = g[i];
   __syncthreads();
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const TB* _
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) J[r] = acc;
+    if (lane == 0) J[r] = accumulate ? J[r] + acc : acc;
   }
 }
 
@@ -152,10 +152,24 @@ hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, 
   const bool vec = (nq % 2 == 0) && ((uintptr_t)B % 16 == 0);
   if (vec)
     hipLaunchKernelGGL((k_jvec<double, true>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g,
-                       J);
+                       J, 0);
   else
     hipLaunchKernelGGL((k_jvec<double, false>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B,
-                       g, J);
+                       g, J, 0);
+  return hipGetLastError();
+}
+
+// J[r] += sum_q B[r, q] g[q] (one auxiliary slab of a recomputed 3-centre tensor)
+hipError_t launch_jvec_acc(int64_t nrows, int nq, const double* B, const double* g, double* J, hipStream_t st) {
+  int64_t nb = (nrows + 3) / 4;
+  if (nb > 65536) nb = 65536;
+  const bool vec = (nq % 2 == 0) && ((uintptr_t)B % 16 == 0);
+  if (vec)
+    hipLaunchKernelGGL((k_jvec<double, true>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g,
+                       J, 1);
+  else
+    hipLaunchKernelGGL((k_jvec<double, false>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B,
+                       g, J, 1);
   return hipGetLastError();
 }
 
@@ -165,10 +179,10 @@ hipError_t launch_jvec_f32(int64_t nrows, int nq, const float* B, const double* 
   const bool vec = (nq % 4 == 0) && ((uintptr_t)B % 16 == 0);
   if (vec)
     hipLaunchKernelGGL((k_jvec<float, true>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g,
-                       J);
+                       J, 0);
   else
     hipLaunchKernelGGL((k_jvec<float, false>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B,
-                       g, J);
+                       g, J, 0);
   return hipGetLastError();
 }
 

@@ -46,6 +46,9 @@ class HipSpec:
     pivoted-Cholesky occupied factor; the eigensolver is used only when the
     projection does not converge, i.e. without a gap at the Fermi level) or
     ``"eigh"`` (an FP64 eigensolve every step, as scf.py:363-367 does).
+    ``ao_block``: ``None`` keeps the AO grid resident when it fits the HBM
+    budget and otherwise recomputes it per block of points in every Fock
+    build; ``0`` forces resident, an int streams with that block.
     """
 
     precision: str = "fp64"
@@ -54,18 +57,22 @@ class HipSpec:
     keep_int3c: bool = False
     gemm: str = "auto"
     density: str = "purify"
+    ao_block: int | None = None
 
 
 def hip(*, precision: str = "fp64", metric_cutoff: float = 1e-7, rho_thresh: float = 1e-10,
-        keep_int3c: bool = False, gemm: str = "auto", density: str = "purify") -> HipSpec:
+        keep_int3c: bool = False, gemm: str = "auto", density: str = "purify",
+        ao_block: int | None = None) -> HipSpec:
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
     if gemm not in ("auto", "rocblas", "mfma"):
         raise ValueError(f"gemm must be 'auto', 'rocblas' or 'mfma', got {gemm!r}")
     if density not in ("purify", "eigh"):
         raise ValueError(f"density must be 'purify' or 'eigh', got {density!r}")
+    if ao_block is not None and (not isinstance(ao_block, int) or ao_block < 0):
+        raise ValueError(f"ao_block must be None or a non-negative int, got {ao_block!r}")
     return HipSpec(precision=precision, metric_cutoff=metric_cutoff, rho_thresh=rho_thresh, keep_int3c=keep_int3c,
-                   gemm=gemm, density=density)
+                   gemm=gemm, density=density, ao_block=ao_block)
 
 
 @dataclass(frozen=True)
@@ -83,19 +90,69 @@ def df(auxbasis: str = "even-tempered-jk", *, chunk="auto", screen: float | None
        spherical: bool | None = None, engine: HipSpec | None = None) -> DFSpec:
     """Density-fitted Coulomb + exchange (terms.py:96-148).
 
-    ``chunk`` and ``screen`` are accepted for interface parity; this engine
-    always materialises B in HBM and computes every bra pair. ``spherical``
-    False is rejected: the auxiliary span is spherical, as on dftax's
-    materialised path (energy.py:308-314).
+    ``chunk``: ``"auto"`` materialises B = (mn|P) L when it fits the HBM
+    budget and otherwise recomputes (mn|P) in auxiliary slabs in every Fock
+    build; ``None`` forces materialised; an int recomputes with slabs of about
+    that many auxiliary functions (energy.py:180-212). ``screen``: relative
+    Schwarz threshold on bra shell pairs, on both paths (pairs with
+    max sqrt((mn|mn)) below screen times the largest are skipped and stay
+    zero). ``spherical`` False is rejected: the auxiliary span is spherical
+    on every path of this engine.
     """
     if spherical is False:
         raise NotImplementedError("the HIP engine fits in the spherical auxiliary span")
-    if screen is not None:
-        raise NotImplementedError("df(screen=...) is not implemented by the HIP engine yet")
-    if isinstance(chunk, int):
-        raise NotImplementedError("streamed df(chunk=<int>) is not implemented by the HIP engine yet")
+    if chunk is not None and chunk != "auto" and (not isinstance(chunk, int) or isinstance(chunk, bool) or chunk <= 0):
+        raise ValueError(f"chunk must be 'auto', None or a positive int, got {chunk!r}")
+    if screen is not None and not (0.0 <= float(screen) < 1.0):
+        raise ValueError(f"screen must be in [0, 1), got {screen!r}")
     return DFSpec(auxbasis=auxbasis, chunk=chunk, screen=screen, spherical=spherical,
                   engine=engine if engine is not None else HipSpec())
+
+
+@dataclass(frozen=True)
+class MemoryPlan:
+    """How one handle lays out its data in HBM."""
+
+    aux_chunk: int      # 0: B materialised; > 0: auxiliary slab size of the recompute path
+    ao_block: int       # 0: AO grid resident; > 0: points per recomputed block
+    bytes_est: int      # estimated device bytes at the peak
+
+
+def memory_plan(nao: int, naux: int, ngrid: int, nocc: int, precision: str, chunk, ao_block,
+                budget: int) -> MemoryPlan:
+    """Pick materialised vs recomputed DF and a resident vs streamed AO grid.
+
+    Word counts (FP64, the mixed path adds FP32 copies): B is nao^2 naux and
+    the build holds (mn|P) and B together; the recompute path holds a slab
+    of nao^2 x chunk, the raw and fitted exchange intermediates (2 nao naux
+    nf) and the metric factor; the AO grid is 4 ng nao values plus ng nao for
+    Z/W. ``budget`` is the device bytes the handle may use.
+    """
+    w = 8
+    f32 = 1.5 if precision == "mixed" else 1.0
+    nf = max(1, min(nao, 512, max(nocc, 1)))
+    exch = nao * naux * nf * w * f32 + 64 * nao * nao * w
+    mat = max(2 * nao * nao * naux * w, nao * nao * naux * w * f32 + exch)
+    if chunk is None:
+        aux_chunk = 0
+    elif isinstance(chunk, int):
+        aux_chunk = chunk
+    else:
+        aux_chunk = 0 if mat + 5 * ngrid * nao * w * f32 <= budget else -1
+    if aux_chunk == -1 or aux_chunk > 0:
+        if aux_chunk == -1:
+            # slab of about 1/8 of the budget, 32 .. 2048 functions
+            aux_chunk = int(min(2048, max(32, budget // 8 // (nao * nao * w))))
+        df_bytes = nao * nao * aux_chunk * w + 2 * nao * naux * nf * w + naux * naux * w + 64 * nao * nao * w
+    else:
+        df_bytes = mat
+    grid = 5 * ngrid * nao * w * f32
+    if ao_block is None:
+        # resident if it fits; else blocks sized to half the memory the DF part leaves
+        spare = max(budget - df_bytes, 0) // 2
+        ao_block = 0 if df_bytes + grid <= budget else int(max(4096, min(ngrid, spare // (5 * nao * w))))
+    grid_bytes = grid if ao_block == 0 else 5 * min(ao_block, ngrid) * nao * w
+    return MemoryPlan(aux_chunk=aux_chunk, ao_block=ao_block, bytes_est=int(df_bytes + grid_bytes))
 
 
 def build_grid(symbols, coords, spec: Becke):
@@ -112,7 +169,7 @@ class HipDFKS:
     """Native DF Coulomb/exchange + grid XC term (one libdfks handle)."""
 
     def __init__(self, basis: BasisSet, aux: BasisSet, mol: Molecule, grid_xyz, grid_w, xc: XCFunctional,
-                 nelec: int, spec: HipSpec, device=None):
+                 nelec: int, spec: HipSpec, device=None, chunk="auto", screen: float | None = None):
         torch = _torch()
         self.L = _native.lib()
         self.device = torch.device("cuda") if device is None else torch.device(device)
@@ -124,10 +181,14 @@ class HipDFKS:
         desc = _native.SystemDesc(self._arrays[0].desc, self._arrays[1].desc, len(self._z), _native.f64p(self._xyz),
                                   _native.f64p(self._z), self._gw.shape[0], _native.f64p(self._gxyz),
                                   _native.f64p(self._gw), xc.xc_id, int(nelec))
+        free, _ = torch.cuda.mem_get_info(self.device)
+        self.plan = memory_plan(basis.nao, aux.nao, self._gw.shape[0], nelec // 2, spec.precision, chunk,
+                                spec.ao_block, int(0.85 * free))
         opts = _native.Options(1 if spec.precision == "mixed" else 0, float(spec.metric_cutoff),
                                float(spec.rho_thresh), int(spec.keep_int3c),
                                {"auto": 0, "rocblas": 1, "mfma": 2}[spec.gemm],
-                               {"purify": 0, "eigh": 1}[spec.density])
+                               {"purify": 0, "eigh": 1}[spec.density], int(self.plan.aux_chunk),
+                               float(screen or 0.0), int(self.plan.ao_block))
         self.precision = spec.precision
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
@@ -159,6 +220,10 @@ class HipDFKS:
         out = torch.empty(shape, dtype=torch.float64, device=self.device)
         _native.check(self.L.dfks_get_matrix(self.handle, which, ctypes.c_void_p(out.data_ptr()), self._stream()))
         return out
+
+    def stats(self) -> dict:
+        """Build and solver counters (kept bra pairs, slabs, AO block, eigensolver fallbacks)."""
+        return _native.stats(self.handle)
 
     def set_precision(self, mixed: bool) -> None:
         _native.check(self.L.dfks_set_precision(self.handle, int(bool(mixed))))
@@ -204,7 +269,7 @@ class KS:
         gspec = grid if grid is not None else becke()
         self.grid_xyz, self.grid_w = build_grid(system.symbols, system.coords, gspec)
         self.coulomb = HipDFKS(self.basis, self.aux_basis, system, self.grid_xyz, self.grid_w, self.xc, nelec,
-                               spec.engine, device=device)
+                               spec.engine, device=device, chunk=spec.chunk, screen=spec.screen)
         self.xc_term = None  # XC is evaluated inside the native term
         n = self.coulomb.nao
         self.S = self.coulomb.matrix(_native.DFKS_S, (n, n))

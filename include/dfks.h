@@ -76,6 +76,15 @@ typedef struct {
                               (SP2 purification + pivoted Cholesky factor; the eigensolver runs only
                               when purification does not converge, and once for dfks_scf_result's
                               orbitals); 1 eigensolver in every step (dftax/ks/scf.py:363-367). */
+  int32_t aux_chunk;       /* 0: materialise B = (mn|P) L (nao^2 x nfit words). > 0: recompute (mn|P) in
+                              auxiliary slabs of about this many functions in every Fock build
+                              (dftax/ks/terms.py:321-493, df(chunk=<int>)): one pass for the exchange
+                              intermediate and gamma, one for J. */
+  double screen;           /* relative Schwarz threshold on bra shell pairs for the recomputed slabs:
+                              pairs with max sqrt((mn|mn)) < screen * max over pairs are skipped
+                              (eri3c_bucketed.py:140-169); 0 keeps every pair. */
+  int64_t ao_block;        /* 0: keep AO values and gradients on the grid resident; > 0: recompute them
+                              per block of this many points in every Fock build (energy.py:158-212). */
 } dfks_options;
 
 /* Matrix selectors for dfks_get_matrix. */
@@ -106,6 +115,12 @@ int dfks_build(dfks_handle h, dfks_stream stream);
 
 /* Sizes: [nao, naux, nfit, nocc] and ngrid. */
 int dfks_sizes(dfks_handle h, int64_t* out5);
+
+/* Build and solver counters: [0] bra shell pairs kept, [1] bra shell pairs in total,
+ * [2] auxiliary slabs (recompute mode, else 0), [3] largest slab in functions,
+ * [4] AO block (0: resident grid), [5] SCF steps that fell back to the eigensolver,
+ * [6] SP2 iterations of the last purified step. Fills n values (zeros past 7). */
+int dfks_stats(dfks_handle h, int64_t* out, int32_t n);
 
 int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream stream);
 

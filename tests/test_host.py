@@ -81,9 +81,33 @@ def test_df_spec_validation():
     from dftax_amd import df, hip
     with pytest.raises(NotImplementedError):
         df(spherical=False)
-    with pytest.raises(NotImplementedError):
-        df(chunk=64)
+    for bad in (0, -3, 1.5, "big", True):
+        with pytest.raises(ValueError):
+            df(chunk=bad)
+    with pytest.raises(ValueError):
+        df(screen=1.0)
     with pytest.raises(ValueError):
         hip(precision="fp16")
+    with pytest.raises(ValueError):
+        hip(ao_block=-1)
     spec = df()
     assert spec.auxbasis == "even-tempered-jk" and spec.engine.precision == "fp64"
+    assert df(chunk=64, screen=1e-12).chunk == 64
+
+
+def test_memory_plan():
+    """Materialise when B and the grid fit; else slabs of ~1/8 of the budget and
+    (if still needed) a streamed grid; explicit values pass through."""
+    from dftax_amd.ks import memory_plan
+    gb = 1 << 30
+    small = memory_plan(24, 115, 36000, 5, "fp64", "auto", None, 200 * gb)
+    assert small.aux_chunk == 0 and small.ao_block == 0
+    # (H2O)_32-like: B ~ 56 GB fits a 245 GB budget
+    c3 = memory_plan(1376, 3680, 1_033_616, 160, "mixed", "auto", None, 245 * gb)
+    assert c3.aux_chunk == 0 and c3.ao_block == 0 and c3.bytes_est < 245 * gb
+    # (H2O)_64-like: B ~ 450 GB does not fit: recompute, grid streamed, estimate within budget
+    c5 = memory_plan(2752, 7360, 2_070_000, 320, "fp64", "auto", None, 245 * gb)
+    assert 32 <= c5.aux_chunk <= 2048 and 4096 <= c5.ao_block < 2_070_000 and c5.bytes_est < 245 * gb
+    forced = memory_plan(24, 115, 36000, 5, "fp64", 16, 100, 200 * gb)
+    assert forced.aux_chunk == 16 and forced.ao_block == 100
+    assert memory_plan(2752, 7360, 2_070_000, 320, "fp64", None, 0, 245 * gb).aux_chunk == 0
