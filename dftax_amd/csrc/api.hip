@@ -34,6 +34,9 @@ hipError_t launch_schwarz(int la, int lb, const int2* pairs, int npair, const Sh
                           double* q_out, hipStream_t st);
 hipError_t launch_jvec_acc(int64_t nrows, int nq, const double* B, const double* g, double* J, hipStream_t st);
 bool eri_thread_has(int la, int lb, int lc);
+hipError_t launch_eri_cls(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,
+                          const ShellSet& A, const ShellSet& C, const double* boys, double* out, int64_t sa,
+                          int64_t sb, int64_t c0, int tr, hipStream_t st);
 hipError_t launch_eri_thread(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,
                              const ShellSet& A, const ShellSet& C, const double* boys, double* out, int64_t sa,
                              int64_t sb, int64_t c0, int tr, hipStream_t st);
@@ -84,6 +87,8 @@ struct GemmArgs {
 hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, double beta, int nsplit, int sym,
                        void* ws, hipStream_t st);
 hipError_t launch_gemm_f32out(const GemmArgs& g, float* C, hipStream_t st);
+hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, int64_t bs_a, int64_t bs_b,
+                                      int64_t bs_c, hipStream_t st);
 hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s);
 hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, hipStream_t s);
 hipError_t launch_sp2_trace_update(int n, double* X, const double* X2, int nocc, double* st, hipStream_t s);
@@ -357,6 +362,12 @@ struct dfks_ctx {
   // AO grid streamed per block (options.ao_block > 0)
   int64_t ao_block = 0;
   double *d_aob = nullptr, *d_zwb = nullptr;
+  // mixed steps of the recompute / streamed paths: FP32 slab, metric factor and
+  // AO block; the FP32 exchange intermediates alias the FP64 ones (a Fock build
+  // uses one precision), so they are not freed separately
+  float *d_slab32 = nullptr, *d_Lt32 = nullptr, *d_Tslab32 = nullptr, *d_aob32 = nullptr, *d_zwb32 = nullptr;
+  float* d_Ttil32 = nullptr;
+  bool t32_alias = false;
 
   ~dfks_ctx() {
     orb.release();
@@ -367,7 +378,8 @@ struct dfks_ctx {
                     d_sp2, d_lohi, d_dvec, d_L, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
                     d_fitd, d_aob, d_zwb};
     for (void* p : ptrs) dfree(p);
-    void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, d_T32, d_Cf32, d_P32, d_ws32, d_eps32, d_offd32};
+    void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, t32_alias ? nullptr : d_T32, d_Cf32, d_P32, d_ws32,
+                      d_eps32, d_offd32, d_slab32, d_Lt32, d_Tslab32, d_aob32, d_zwb32};
     for (void* p : ptrs32) dfree(p);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -583,8 +595,8 @@ void build_all(dfks_ctx* h, hipStream_t st) {
             HIPCHK(launch_eri_thread(la, lb, lc, pp, npair, d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A,
                                      h->d_boys, d3, (int64_t)nao * naux, naux, 0, 1, st));
           else
-            HIPCHK(launch_eri_wave(la, lb, lc, pp, npair, d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A, 0,
-                                   h->d_boys, d3, (int64_t)nao * naux, naux, 1, 0, st));
+            HIPCHK(launch_eri_cls(la, lb, lc, pp, npair, d_auxl + aux_off[lc], (int)auxl[lc].size(), O, A,
+                                  h->d_boys, d3, (int64_t)nao * naux, naux, 0, 1, st));
         }
     sync(st);
   } else {
@@ -659,16 +671,30 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   h->d_sel = dalloc<int32_t>(nao);
   h->d_gws = dalloc<double>((size_t)64 * nao * nao);
   h->d_gpart = dalloc<double>((size_t)nao * h->nfit);
-  if (h->opt.precision == 1 && !h->recompute && h->ao_block == 0) {
-    const size_t nB = (size_t)nao * nao * h->nfit;
-    h->d_B32 = dalloc<float>(nB);
-    HIPCHK(launch_to_f32((int64_t)nB, h->d_B, h->d_B32, st));
-    if (h->ng > 0) {
+  if (h->opt.precision == 1) {
+    if (!h->recompute) {
+      const size_t nB = (size_t)nao * nao * h->nfit;
+      h->d_B32 = dalloc<float>(nB);
+      HIPCHK(launch_to_f32((int64_t)nB, h->d_B, h->d_B32, st));
+      h->d_T32 = dalloc<float>((size_t)nao * h->nfit * h->nf_chunk);
+    } else {
+      h->d_slab32 = dalloc<float>((size_t)nao * nao * h->max_np);
+      h->d_Tslab32 = dalloc<float>((size_t)nao * h->max_np * h->nf_chunk);
+      h->d_Lt32 = dalloc<float>((size_t)h->nfit * naux);
+      HIPCHK(launch_to_f32((int64_t)h->nfit * naux, h->d_Lt, h->d_Lt32, st));
+      h->d_Ttil32 = reinterpret_cast<float*>(h->d_Ttil);
+      h->d_T32 = reinterpret_cast<float*>(h->d_Tx);
+      h->t32_alias = true;
+    }
+    if (h->ng > 0 && h->ao_block == 0) {
       h->d_ao32 = dalloc<float>((size_t)4 * h->ng * nao);
       HIPCHK(launch_to_f32((int64_t)4 * h->ng * nao, h->d_ao, h->d_ao32, st));
       h->d_zw32 = dalloc<float>((size_t)h->ng * nao);
+    } else if (h->ng > 0) {
+      const int64_t gb = std::min<int64_t>(h->ao_block, h->ng);
+      h->d_aob32 = dalloc<float>((size_t)4 * gb * nao);
+      h->d_zwb32 = dalloc<float>((size_t)gb * nao);
     }
-    h->d_T32 = dalloc<float>((size_t)nao * h->nfit * h->nf_chunk);
     h->d_Cf32 = dalloc<float>((size_t)nao * nao);
     h->d_P32 = dalloc<float>((size_t)nao * nao);
     h->d_ws32 = dalloc<float>((size_t)nao * nao);
@@ -714,7 +740,7 @@ void eri_slab(dfks_ctx* h, int s, hipStream_t st) {
         if (eri_thread_has(la, lb, lc) && A.spherical)
           HIPCHK(launch_eri_thread(la, lb, lc, pp, npair, aa, nal, O, A, h->d_boys, h->d_slab, sa, sb, p0, 1, st));
         else
-          HIPCHK(launch_eri_wave(la, lb, lc, pp, npair, aa, nal, O, A, 0, h->d_boys, h->d_slab, sa, sb, 1, p0, st));
+          HIPCHK(launch_eri_cls(la, lb, lc, pp, npair, aa, nal, O, A, h->d_boys, h->d_slab, sa, sb, p0, 1, st));
       }
     }
 }
@@ -722,9 +748,25 @@ void eri_slab(dfks_ctx* h, int s, hipStream_t st) {
 // Exchange intermediate of nc factor columns from recomputed slabs:
 // T~[m][P][i] = sum_l (l m|P) Cf[l,i] slab by slab, then the fitted
 // T[m][Q][i] = sum_P T~[m][P][i] L[P,Q] (one strided-batched GEMM over m).
-void exchange_recompute(dfks_ctx* h, const double* Cf, int nc, hipStream_t st) {
+void exchange_recompute(dfks_ctx* h, const double* Cf, const float* Cf32, int nc, hipStream_t st) {
   const int nao = h->nao, naux = h->naux, nfit = h->nfit;
   const double one = 1.0, zero = 0.0;
+  if (h->mixed_now) {
+    // split-BF16 products: slab (FP32 copy) x Cf, then the metric factor per m (batched)
+    for (size_t s = 0; s < h->slab_np.size(); ++s) {
+      const int np = h->slab_np[s];
+      eri_slab(h, (int)s, st);
+      HIPCHK(launch_to_f32((int64_t)nao * nao * np, h->d_slab, h->d_slab32, st));
+      const GemmArgs g{(int)((int64_t)nao * np), nc, nao, h->d_slab32, (int64_t)nao * np, false, Cf32, nao, true};
+      HIPCHK(launch_gemm_f32out(g, h->d_Tslab32, st));
+      HIPCHK(hipMemcpy2DAsync(h->d_Ttil32 + (size_t)h->slab_p0[s] * nc, sizeof(float) * naux * nc, h->d_Tslab32,
+                              sizeof(float) * np * nc, sizeof(float) * np * nc, nao, hipMemcpyDeviceToDevice, st));
+    }
+    // T_m[Q][i] = sum_P L[P,Q] T~_m[P][i]: A(Q,P) = Lt column-major, B(i,P) = T~_m row-major [P][i]
+    const GemmArgs gm{nfit, nc, naux, h->d_Lt32, nfit, false, h->d_Ttil32, nc, false};
+    HIPCHK(launch_gemm_f32out_batched(gm, h->d_T32, nao, 0, (int64_t)naux * nc, (int64_t)nfit * nc, st));
+    return;
+  }
   for (size_t s = 0; s < h->slab_np.size(); ++s) {
     const int np = h->slab_np[s];
     eri_slab(h, (int)s, st);
@@ -766,7 +808,7 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
   // rocBLAS; 2 runs all four on the library kernel.
   const bool lib = h->opt.gemm == 2;
   const bool lib_k = h->opt.gemm != 1;
-  if (mixed && !h->d_B32) fail(1, "mixed precision needs options.precision = 1 at create");
+  if (mixed && h->opt.precision != 1) fail(1, "mixed precision needs options.precision = 1 at create");
   h->mark(0, st);
   HIPCHK(hipMemsetAsync(E, 0, 8 * sizeof(double), st));
   // one-electron energy
@@ -790,8 +832,13 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
     const double beta = first ? 0.0 : 1.0;
     // T[m,Q,i] = sum_l B[l,m,Q] Cf[l,i]
     if (h->recompute) {
-      exchange_recompute(h, Cf + (int64_t)i0 * nao, nc, st);
-      HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1, st));
+      exchange_recompute(h, Cf + (int64_t)i0 * nao, mixed ? h->d_Cf32 + (int64_t)i0 * nao : nullptr, nc, st);
+      if (mixed)
+        HIPCHK(launch_gamma_f32(nao, nfit, nc, h->d_T32, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma,
+                                h->d_gpart, 1, st));
+      else
+        HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1,
+                            st));
     } else if (mixed) {
       GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
                  h->d_Cf32 + (int64_t)i0 * nao, nao, true};
@@ -814,7 +861,7 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
       const int64_t kd = (int64_t)nfit * nc;
       const int ntn = (nao + 127) / 128;
       const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, kd);
-      if (mixed && !h->recompute) {
+      if (mixed) {
         GemmArgs g{nao, nao, kd, h->d_T32, kd, true, h->d_T32, kd, true};
         HIPCHK(launch_gemm(kBF16x3, g, h->d_K, sg, beta, nsplit, 1, h->d_gws, st));
       } else if (lib_k) {
@@ -842,9 +889,22 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
     // AO values and gradients per block of points: Z_b = Phi_b P, W_b, Vh += Phi_b^T W_b
     const ShellSet O = h->orb.view();
     const int64_t gb = std::min<int64_t>(h->ao_block, ng);
+    if (mixed) HIPCHK(launch_to_f32(nn, P, h->d_P32, st));
+    const int ntm = (nao + 127) / 128;
     for (int64_t g0 = 0; g0 < ng; g0 += gb) {
       const int64_t nb = std::min(gb, ng - g0);
       HIPCHK(launch_ao(nb, h->d_gxyz + 3 * g0, O, nao, h->d_aob, st));
+      if (mixed) {
+        HIPCHK(launch_to_f32(4 * nb * nao, h->d_aob, h->d_aob32, st));
+        const GemmArgs gz{(int)nb, nao, nao, h->d_aob32, nao, true, h->d_P32, nao, true};
+        HIPCHK(launch_gemm_f32out(gz, h->d_zwb32, st));
+        HIPCHK(launch_xc_f32(nb, nao, h->d_aob32, h->d_gw + g0, h->d_zwb32, h->xc_id, h->opt.rho_thresh,
+                             h->d_exc + g0, st));
+        const GemmArgs gv{nao, nao, nb, h->d_aob32, nao, false, h->d_zwb32, nao, false};
+        HIPCHK(launch_gemm(kBF16x3, gv, h->d_Vxc, 1.0, g0 == 0 ? 0.0 : 1.0, split_for((int64_t)ntm * ntm, nb), 0,
+                           h->d_gws, st));
+        continue;
+      }
       RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, nao, nb, nao, &one, P, nao,
                              h->d_aob, nao, &zero, h->d_zwb, nao));
       HIPCHK(launch_xc(nb, nao, h->d_aob, h->d_gw + g0, h->d_zwb, h->xc_id, h->opt.rho_thresh, h->d_exc + g0, nullptr,
@@ -1167,8 +1227,7 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
         fail(1, "aux_chunk and ao_block must be >= 0 and screen in [0, 1)");
       h->recompute = h->opt.aux_chunk > 0;
       h->ao_block = h->opt.ao_block;
-      if (h->opt.precision == 1 && (h->recompute || h->ao_block > 0))
-        fail(1, "mixed precision with recomputed slabs or a streamed AO grid is not implemented");
+
       h->xc_id = sys->xc_id;
       h->hf = hf_coeff(sys->xc_id);
       if (sys->nelec <= 0 || sys->nelec % 2) fail(1, "closed shell needs an even, positive electron count");

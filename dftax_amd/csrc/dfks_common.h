@@ -16,6 +16,13 @@ constexpr double kBoysDt = 0.1;
 constexpr double kBoysTmax = 40.0;
 constexpr double kPi = 3.14159265358979323846;
 
+// Sparse cart->sph lists: for l and spherical index s, up to 12 (cart, coef).
+struct C2SList {
+  int n[7][13];
+  int cart[7][13][12];
+  double coef[7][13][12];
+};
+
 // SP2 purification state vector (doubles) and limits, shared by host and kernels
 enum Sp2State {
   kSpEmin = 0, kSpEmax = 1, kSpFlag = 2, kSpCount = 3, kSpIter = 4, kSpErr = 5, kSpTrace = 6,

@@ -32,12 +32,6 @@ __device__ __forceinline__ int ridx(int t, int u, int v) {
   return s * (s + 1) * (s + 2) / 6 + r * (r + 1) / 2 + v;
 }
 
-// Sparse cart->sph lists: for l and spherical index s, up to 12 (cart, coef).
-struct C2SList {
-  int n[7][13];
-  int cart[7][13][12];
-  double coef[7][13][12];
-};
 __constant__ C2SList g_c2s;
 
 // cartesian exponent table for runtime l (l <= 6)
@@ -481,6 +475,8 @@ hipError_t launch_schwarz(int la, int lb, const int2* pairs, int npair, const Sh
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
+void upload_cls_tables(const C2SList& h, const int (&cart)[7][28][3]);
+
 void upload_c2s_tables() {
   static C2SList h;
   static int cart[7][28][3];
@@ -508,6 +504,7 @@ void upload_c2s_tables() {
   fc(0, kCart0); fc(1, kCart1); fc(2, kCart2); fc(3, kCart3); fc(4, kCart4); fc(5, kCart5); fc(6, kCart6);
   (void)hipMemcpyToSymbol(HIP_SYMBOL(g_c2s), &h, sizeof(h));
   (void)hipMemcpyToSymbol(HIP_SYMBOL(g_cart), cart, sizeof(cart));
+  upload_cls_tables(h, cart);
 }
 
 hipError_t launch_eri_wave(int la, int lb, int lc, const int2* pairs, int npair, const int32_t* aux, int naux_l,

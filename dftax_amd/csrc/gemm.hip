@@ -123,8 +123,13 @@ template <int PREC, bool AK, bool BKC, typename Out>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typename GemmTraits<PREC>::In* __restrict__ A,
                                               int64_t lda, const typename GemmTraits<PREC>::In* __restrict__ B,
                                               int64_t ldb, Out* __restrict__ C, int64_t ldc, int64_t split_stride,
-                                              int64_t kchunk, int sym, int ntiles, int vec_a, int vec_b) {
+                                              int64_t kchunk, int sym, int ntiles, int vec_a, int vec_b, int64_t bs_a,
+                                              int64_t bs_b, int64_t bs_c) {
   using In = typename GemmTraits<PREC>::In;
+  // batch index on gridDim.y (strided operands; a zero stride shares the operand)
+  A += blockIdx.y * bs_a;
+  B += blockIdx.y * bs_b;
+  C += blockIdx.y * bs_c;
   constexpr int BK = GemmTraits<PREC>::BK;
   constexpr int VEC = (128 * BK) / 256;  // values per thread per operand tile
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -295,7 +300,8 @@ struct GemmArgs {
 };
 
 template <int PREC, typename Out>
-static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nsplit, int sym, hipStream_t st) {
+static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nsplit, int sym, hipStream_t st,
+                                int batch = 1, int64_t bs_a = 0, int64_t bs_b = 0, int64_t bs_c = 0) {
   using In = typename GemmTraits<PREC>::In;
   constexpr int BK = GemmTraits<PREC>::BK;
   const int ntm = (g.M + kBM - 1) / kBM, ntn = (g.N + kBN - 1) / kBN;
@@ -304,7 +310,7 @@ static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nspl
   kchunk = (kchunk + BK - 1) / BK * BK;
   const int64_t split_stride = (int64_t)g.M * ldc;
   const int nblk = (ntiles + 7) / 8 * 8;
-  dim3 grid(nblk, 1, nsplit);
+  dim3 grid(nblk, batch, nsplit);
   const In* A = static_cast<const In*>(g.A);
   const In* B = static_cast<const In*>(g.B);
   // 16-byte vector loads need a 16-byte aligned base and row pitch (K-contiguous operands)
@@ -313,16 +319,16 @@ static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nspl
   const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % PER == 0);
   if (g.a_kcontig && g.b_kcontig)
     hipLaunchKernelGGL((k_gemm<PREC, true, true, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb, C,
-                       ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b);
+                       ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
   else if (g.a_kcontig && !g.b_kcontig)
     hipLaunchKernelGGL((k_gemm<PREC, true, false, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
-                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b);
+                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
   else if (!g.a_kcontig && g.b_kcontig)
     hipLaunchKernelGGL((k_gemm<PREC, false, true, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
-                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b);
+                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
   else
     hipLaunchKernelGGL((k_gemm<PREC, false, false, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
-                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b);
+                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
   return hipGetLastError();
 }
 
@@ -358,6 +364,14 @@ hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, dou
 hipError_t launch_gemm_f32out(const GemmArgs& g, float* C, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   return launch_gemm_t<kBF16x3, float>(g, C, g.N, 1, 0, st);
+}
+
+// Batched FP32-output variant: for b < batch, C + b*bs_c = (A + b*bs_a)(B + b*bs_b)^T.
+hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, int64_t bs_a, int64_t bs_b,
+                                      int64_t bs_c, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0 || batch <= 0) return hipSuccess;
+  if (batch > 65535) return hipErrorInvalidValue;
+  return launch_gemm_t<kBF16x3, float>(g, C, g.N, 1, 0, st, batch, bs_a, bs_b, bs_c);
 }
 
 }  // namespace dfks

@@ -144,14 +144,16 @@ def memory_plan(nao: int, naux: int, ngrid: int, nocc: int, precision: str, chun
             # slab of about 1/8 of the budget, 32 .. 2048 functions
             aux_chunk = int(min(2048, max(32, budget // 8 // (nao * nao * w))))
         df_bytes = nao * nao * aux_chunk * w + 2 * nao * naux * nf * w + naux * naux * w + 64 * nao * nao * w
+        if precision == "mixed":
+            df_bytes += nao * nao * aux_chunk * 4 + naux * naux * 4   # FP32 slab and metric factor
     else:
         df_bytes = mat
     grid = 5 * ngrid * nao * w * f32
     if ao_block is None:
         # resident if it fits; else blocks sized to half the memory the DF part leaves
         spare = max(budget - df_bytes, 0) // 2
-        ao_block = 0 if df_bytes + grid <= budget else int(max(4096, min(ngrid, spare // (5 * nao * w))))
-    grid_bytes = grid if ao_block == 0 else 5 * min(ao_block, ngrid) * nao * w
+        ao_block = 0 if df_bytes + grid <= budget else int(max(4096, min(ngrid, spare // (5 * nao * w * f32))))
+    grid_bytes = grid if ao_block == 0 else 5 * min(ao_block, ngrid) * nao * w * f32
     return MemoryPlan(aux_chunk=aux_chunk, ao_block=ao_block, bytes_est=int(df_bytes + grid_bytes))
 
 

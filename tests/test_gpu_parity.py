@@ -286,6 +286,7 @@ def dimer_recompute(torch_cuda, water_dimer_tzvp):
         "rec": df(chunk=48, engine=hip(ao_block=1000, **eng)),
         "scr": df(chunk=48, screen=1e-12, engine=hip(**eng)),
         "scr4": df(chunk=None, screen=1e-4, engine=hip(**eng)),
+        "recmx": df(chunk=48, screen=1e-12, engine=hip(precision="mixed", ao_block=1000, **eng)),
     }
     return {k: KS(water_dimer_tzvp, "PBE0", grid=g, coulomb=v) for k, v in specs.items()}
 
@@ -335,3 +336,30 @@ def test_recompute_scf(dimer_recompute):
     _say(f"materialised {rm.e_tot:.12f} ({rm.n_iter} it); recomputed {rr.e_tot:.12f} ({rr.n_iter} it)")
     assert rm.converged and rr.converged
     assert abs(rm.e_tot - rr.e_tot) < 1e-9
+
+
+def test_recompute_mixed(dimer_recompute, torch_cuda):
+    """Mixed steps on the recompute + streamed-grid path: split-BF16 slab and
+    metric products (batched), FP32 streamed XC; bounded Fock error, and the
+    SCF refined in FP64 lands on the materialised FP64 energy."""
+    from dftax_amd import scf
+    torch = torch_cuda
+    m, r = dimer_recompute["mat"], dimer_recompute["recmx"]
+    P0 = torch.tensor(_p0(m), device="cuda")
+    Em, Fm = m.coulomb.energy_fock(P0)
+    r.coulomb.set_precision(False)
+    E64, F64 = r.coulomb.energy_fock(P0)
+    r.coulomb.set_precision(True)
+    Emx, Fmx = r.coulomb.energy_fock(P0)
+    r.coulomb.set_precision(False)
+    d64 = np.abs((F64 - Fm).cpu().numpy()).max()
+    dmx = np.abs((Fmx - Fm).cpu().numpy()).max()
+    _say(f"recompute mixed: FP64 max|dF| {d64:.2e}, mixed max|dF| {dmx:.2e}")
+    assert d64 < 1e-9
+    assert 0.0 < dmx < 1e-3 and abs(float(Emx[0] - Em[0])) < 1e-3
+    rm = scf(m, e_tol=1e-9)
+    rx = scf(r, precision="mixed", e_tol=1e-9)
+    nmix = sum(1 for h in rx.history if h[3] == "mixed")
+    _say(f"materialised {rm.e_tot:.12f}; recompute mixed {rx.e_tot:.12f} ({rx.n_iter} it, {nmix} mixed)")
+    assert rx.converged and nmix >= 1
+    assert abs(rx.e_tot - rm.e_tot) < 1e-8
