@@ -36,6 +36,19 @@ WORKLOADS = {
 }
 
 
+def measured_traffic(workload: str, stage: str):
+    """HBM bytes per launch of a stage's FP64 kernel, from the committed counter summary
+    (profiles/<workload>_pbe0_hbm_traffic.md: FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_pbe0_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == f"{stage} (FP64)":
+            return round((float(cells[2]) + float(cells[3])) * 1e9)
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,8 +167,8 @@ def main() -> int:
     dom = max((k for k in stage_avg if k in flops), key=lambda k: stage_avg[k])
     ach = flops[dom] / (stage_avg[dom] * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": dom,
-            "per_launch": f"{flops[dom]:.4g} flop, {stage_avg[dom]:.3f} ms"}
+            "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": measured_traffic(args.workload, dom),
+            "kernel": dom, "per_launch": f"{flops[dom]:.4g} flop, {stage_avg[dom]:.3f} ms"}
 
     cpu = None
     if not args.no_cpu_baseline and not recompute:

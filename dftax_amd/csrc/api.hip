@@ -945,8 +945,15 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
       HIPCHK(launch_xc(ng, nao, h->d_ao, h->d_gw, h->d_zw, h->xc_id, h->opt.rho_thresh, h->d_exc, nullptr, st));
       HIPCHK(launch_dot(ng, h->d_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
       h->mark(kStXcKernel + 1, st);
-      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_transpose, nao, nao, ng, &one,
-                             h->d_ao, nao, h->d_zw, nao, &zero, h->d_Vxc, nao));
+      if (ntm <= 6) {
+        // nao <= 768: few output tiles and K = ng; the split-K library kernel fills the chip
+        // where the plain library GEMM cannot (above that size rocBLAS is the faster one)
+        GemmArgs gv{nao, nao, ng, h->d_ao, nao, false, h->d_zw, nao, false};
+        HIPCHK(launch_gemm(kF64, gv, h->d_Vxc, 1.0, 0.0, nsplit, 0, h->d_gws, st));
+      } else {
+        RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_transpose, nao, nao, ng, &one,
+                               h->d_ao, nao, h->d_zw, nao, &zero, h->d_Vxc, nao));
+      }
     }
     h->mark(kStXcSyr2k + 1, st);
   }

@@ -235,10 +235,111 @@ __global__ __launch_bounds__(256) void k_xc(int64_t ng, int64_t nao, const TA* _
   }
 }
 
+// Single-read variant for the FP64 path: one 256-thread block per grid point;
+// each thread keeps its slice of the four AO plane rows (J vectors of 16 bytes
+// per plane) in registers between the rho / grad-rho reduction and the W
+// update, so every AO value is read from HBM once per Fock build instead of
+// twice. Needs the vector layout (nao a multiple of the vector width, 16-byte
+// aligned rows) and nao <= 256 J PER. (The FP32 path keeps the
+// one-wavefront-per-point kernel: its second read costs less than this
+// variant's lower occupancy.)
+template <typename TA, typename TZ, int J>
+__global__ __launch_bounds__(256) void k_xc1(int64_t ng, int64_t nao, const TA* __restrict__ ao,
+                                             const double* __restrict__ wts, TZ* __restrict__ zw, int xc_id,
+                                             double thresh, double* __restrict__ exc) {
+  __shared__ double s_red[4][4];
+  __shared__ double s_coef[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t g = blockIdx.x;
+  const int64_t plane = ng * nao;
+  constexpr int PER = 16 / sizeof(TA);
+  typedef TA vecA __attribute__((ext_vector_type(PER)));
+  typedef TZ vecZ __attribute__((ext_vector_type(PER)));
+  const vecA* p0 = reinterpret_cast<const vecA*>(ao + g * nao);
+  const vecA* p1 = reinterpret_cast<const vecA*>(ao + plane + g * nao);
+  const vecA* p2 = reinterpret_cast<const vecA*>(ao + 2 * plane + g * nao);
+  const vecA* p3 = reinterpret_cast<const vecA*>(ao + 3 * plane + g * nao);
+  vecZ* zrow = reinterpret_cast<vecZ*>(zw + g * nao);
+  const int nv = (int)(nao / PER);
+  vecA a0[J], a1[J], a2[J], a3[J];
+  double r = 0, gx = 0, gy = 0, gz = 0;
+#pragma unroll
+  for (int jv = 0; jv < J; ++jv) {
+    const int n = tid + 256 * jv;
+    if (n < nv) {
+      const vecZ z = zrow[n];
+      a0[jv] = p0[n]; a1[jv] = p1[n]; a2[jv] = p2[n]; a3[jv] = p3[n];
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const double zz = (double)z[q];
+        r += zz * (double)a0[jv][q];
+        gx += zz * (double)a1[jv][q];
+        gy += zz * (double)a2[jv][q];
+        gz += zz * (double)a3[jv][q];
+      }
+    }
+  }
+  r = wave_sum(r);
+  gx = wave_sum(gx);
+  gy = wave_sum(gy);
+  gz = wave_sum(gz);
+  if (lane == 0) { s_red[wave][0] = r; s_red[wave][1] = gx; s_red[wave][2] = gy; s_red[wave][3] = gz; }
+  __syncthreads();
+  if (tid == 0) {
+    r = s_red[0][0] + s_red[1][0] + s_red[2][0] + s_red[3][0];
+    gx = 2.0 * (s_red[0][1] + s_red[1][1] + s_red[2][1] + s_red[3][1]);
+    gy = 2.0 * (s_red[0][2] + s_red[1][2] + s_red[2][2] + s_red[3][2]);
+    gz = 2.0 * (s_red[0][3] + s_red[1][3] + s_red[2][3] + s_red[3][3]);
+    double a = 0.0, bx = 0.0, by = 0.0, bz = 0.0, e = 0.0;
+    if (r > thresh) {
+      const double w = wts[g];
+      D rho{r, 1.0, 0.0};
+      D sig{gx * gx + gy * gy + gz * gz, 0.0, 1.0};
+      D f = eps_xc(xc_id, rho, sig) * rho;
+      e = w * f.v;
+      a = 0.5 * w * f.r;
+      const double t = 2.0 * w * f.s;
+      bx = t * gx; by = t * gy; bz = t * gz;
+    }
+    exc[g] = e;
+    s_coef[0] = a; s_coef[1] = bx; s_coef[2] = by; s_coef[3] = bz;
+  }
+  __syncthreads();
+  const double a = s_coef[0], bx = s_coef[1], by = s_coef[2], bz = s_coef[3];
+#pragma unroll
+  for (int jv = 0; jv < J; ++jv) {
+    const int n = tid + 256 * jv;
+    if (n < nv) {
+      vecZ o;
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+        o[q] = (TZ)(a * (double)a0[jv][q] + bx * (double)a1[jv][q] + by * (double)a2[jv][q] + bz * (double)a3[jv][q]);
+      zrow[n] = o;
+    }
+  }
+}
+
+template <typename TA, typename TZ>
+static bool launch_xc1(int64_t ng, int64_t nao, const TA* ao, const double* w, TZ* zw, int xc_id, double thresh,
+                       double* exc, hipStream_t st) {
+  constexpr int PER = 16 / sizeof(TA);
+  const int64_t j = (nao / PER + 255) / 256;
+  const dim3 grid((unsigned)ng), blk(256);
+  if (j <= 1) hipLaunchKernelGGL((k_xc1<TA, TZ, 1>), grid, blk, 0, st, ng, nao, ao, w, zw, xc_id, thresh, exc);
+  else if (j <= 2) hipLaunchKernelGGL((k_xc1<TA, TZ, 2>), grid, blk, 0, st, ng, nao, ao, w, zw, xc_id, thresh, exc);
+  else if (j <= 3) hipLaunchKernelGGL((k_xc1<TA, TZ, 3>), grid, blk, 0, st, ng, nao, ao, w, zw, xc_id, thresh, exc);
+  else if (j <= 4) hipLaunchKernelGGL((k_xc1<TA, TZ, 4>), grid, blk, 0, st, ng, nao, ao, w, zw, xc_id, thresh, exc);
+  else if (j <= 6) hipLaunchKernelGGL((k_xc1<TA, TZ, 6>), grid, blk, 0, st, ng, nao, ao, w, zw, xc_id, thresh, exc);
+  else return false;
+  return true;
+}
+
 hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w, double* zw, int xc_id, double thresh,
                      double* exc, double* rho_out, hipStream_t st) {
   if (ng == 0) return hipSuccess;
   const bool vec = nao % 2 == 0 && (uintptr_t)ao % 16 == 0 && (uintptr_t)zw % 16 == 0;
+  if (vec && !rho_out && ng < (1LL << 31) && launch_xc1(ng, nao, ao, w, zw, xc_id, thresh, exc, st))
+    return hipGetLastError();
   if (vec)
     hipLaunchKernelGGL((k_xc<double, double, true>), dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w,
                        zw, xc_id, thresh, exc, rho_out);
@@ -252,6 +353,7 @@ hipError_t launch_xc_f32(int64_t ng, int64_t nao, const float* ao, const double*
                          double thresh, double* exc, hipStream_t st) {
   if (ng == 0) return hipSuccess;
   const bool vec = nao % 4 == 0 && (uintptr_t)ao % 16 == 0 && (uintptr_t)zw % 16 == 0;
+
   if (vec)
     hipLaunchKernelGGL((k_xc<float, float, true>), dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w,
                        zw, xc_id, thresh, exc, (double*)nullptr);
