@@ -119,7 +119,10 @@ __device__ __forceinline__ void load_slice(const T* __restrict__ X, int64_t ld, 
   }
 }
 
-template <int PREC, bool AK, bool BKC, typename Out>
+// NJ: 16-wide MFMA blocks per wavefront along N (tile width BN = 32 NJ: 4 ->
+// 128, 5 -> 160 for N = 160-column exchange products); NJ = 5 only for the
+// split-BF16 path without the symmetric mode.
+template <int PREC, bool AK, bool BKC, typename Out, int NJ = 4>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typename GemmTraits<PREC>::In* __restrict__ A,
                                               int64_t lda, const typename GemmTraits<PREC>::In* __restrict__ B,
                                               int64_t ldb, Out* __restrict__ C, int64_t ldc, int64_t split_stride,
@@ -132,12 +135,14 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typ
   C += blockIdx.y * bs_c;
   constexpr int BK = GemmTraits<PREC>::BK;
   constexpr int VEC = (128 * BK) / 256;  // values per thread per operand tile
+  constexpr int BN = 32 * NJ;
+  static_assert(NJ == 4 || (NJ == 5 && PREC == kBF16x3), "160-wide tiles are split-BF16 only");
   const int tile = xcd_tile(blockIdx.x, ntiles);
   if (tile >= ntiles) return;
-  const int ntn = (N + kBN - 1) / kBN;
+  const int ntn = (N + BN - 1) / BN;
   int tm, tn;
   tile_of(tile, ntn, sym != 0, tm, tn);
-  const int m0 = tm * kBM, n0 = tn * kBN;
+  const int m0 = tm * kBM, n0 = tn * BN;
   const int64_t k_begin = (int64_t)blockIdx.z * kchunk;
   const int64_t k_end = k_begin + kchunk < K ? k_begin + kchunk : K;
   Out* Cz = C + (int64_t)blockIdx.z * split_stride;
@@ -145,15 +150,23 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typ
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * (16 * NJ);
   const int li = lane & 15, lk = lane >> 4;
   int ra_r, ra_k, rb_r, rb_k;
   slice_of<AK, VEC>(tid, ra_r, ra_k);
   slice_of<BKC, VEC>(tid, rb_r, rb_k);
   In ra[VEC], rb[VEC];
+  // rows 128 .. BN-1 of the B tile (NJ = 5): threads 0..63, 16 values each
+  constexpr int XR = BN - 128;
+  int rx_r = 0, rx_k = 0;
+  if (BKC) { rx_r = 128 + (tid >> 1); rx_k = (tid & 1) * 16; }
+  else { rx_r = 128 + (tid & 31); rx_k = (tid >> 5) * 16; }
+  const bool has_x = XR > 0 && tid < 64;
+  In rbx[16];
   if (k_begin < k_end) {
     load_slice<AK, VEC>(A, lda, M, m0, ra_r, k_begin, ra_k, k_end, vec_a != 0, ra);
     load_slice<BKC, VEC>(B, ldb, N, n0, rb_r, k_begin, rb_k, k_end, vec_b != 0, rb);
+    if (has_x) load_slice<BKC, 16>(B, ldb, N, n0, rx_r, k_begin, rx_k, k_end, vec_b != 0, rbx);
   }
 
   if constexpr (PREC == kF64) {
@@ -203,13 +216,13 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typ
     constexpr int PADK = BK + 8;  // bf16 per LDS row; 80-byte rows keep 16-byte reads conflict-free
     __shared__ __attribute__((aligned(16))) __bf16 sAh[kBM][PADK];
     __shared__ __attribute__((aligned(16))) __bf16 sAl[kBM][PADK];
-    __shared__ __attribute__((aligned(16))) __bf16 sBh[kBN][PADK];
-    __shared__ __attribute__((aligned(16))) __bf16 sBl[kBN][PADK];
-    flt4 acc[4][4];
+    __shared__ __attribute__((aligned(16))) __bf16 sBh[BN][PADK];
+    __shared__ __attribute__((aligned(16))) __bf16 sBl[BN][PADK];
+    flt4 acc[4][NJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+      for (int j = 0; j < NJ; ++j) acc[i][j] = flt4{0, 0, 0, 0};
     for (int64_t k0 = k_begin; k0 < k_end; k0 += BK) {
       __syncthreads();
       // split into bf16 hi + lo and store 16-byte vectors
@@ -232,26 +245,42 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typ
         *reinterpret_cast<bf16x8*>(&sBh[rb_r][rb_k + 8 * v]) = hb;
         *reinterpret_cast<bf16x8*>(&sBl[rb_r][rb_k + 8 * v]) = lb;
       }
+      if (has_x) {
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          bf16x8 hb, lb;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            float y = rbx[8 * v + j];
+            __bf16 yh = (__bf16)y;
+            hb[j] = yh;
+            lb[j] = (__bf16)(y - (float)yh);
+          }
+          *reinterpret_cast<bf16x8*>(&sBh[rx_r][rx_k + 8 * v]) = hb;
+          *reinterpret_cast<bf16x8*>(&sBl[rx_r][rx_k + 8 * v]) = lb;
+        }
+      }
       __syncthreads();
       if (k0 + BK < k_end) {
         load_slice<AK, VEC>(A, lda, M, m0, ra_r, k0 + BK, ra_k, k_end, vec_a != 0, ra);
         load_slice<BKC, VEC>(B, ldb, N, n0, rb_r, k0 + BK, rb_k, k_end, vec_b != 0, rb);
+        if (has_x) load_slice<BKC, 16>(B, ldb, N, n0, rx_r, k0 + BK, rx_k, k_end, vec_b != 0, rbx);
       }
-      bf16x8 ah[4], al[4], bh[4], bl[4];
+      bf16x8 ah[4], al[4], bh[NJ], bl[NJ];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         ah[i] = *reinterpret_cast<const bf16x8*>(&sAh[wm + 16 * i + li][8 * lk]);
         al[i] = *reinterpret_cast<const bf16x8*>(&sAl[wm + 16 * i + li][8 * lk]);
       }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < NJ; ++j) {
         bh[j] = *reinterpret_cast<const bf16x8*>(&sBh[wn + 16 * j + li][8 * lk]);
         bl[j] = *reinterpret_cast<const bf16x8*>(&sBl[wn + 16 * j + li][8 * lk]);
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NJ; ++j) {
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
@@ -261,7 +290,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typ
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < NJ; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           int gm = m0 + wm + 16 * i + 4 * lk + r;
@@ -299,12 +328,14 @@ struct GemmArgs {
   bool b_kcontig;
 };
 
-template <int PREC, typename Out>
+template <int PREC, typename Out, int NJ = 4>
 static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nsplit, int sym, hipStream_t st,
                                 int batch = 1, int64_t bs_a = 0, int64_t bs_b = 0, int64_t bs_c = 0) {
   using In = typename GemmTraits<PREC>::In;
   constexpr int BK = GemmTraits<PREC>::BK;
-  const int ntm = (g.M + kBM - 1) / kBM, ntn = (g.N + kBN - 1) / kBN;
+  constexpr int BN = 32 * NJ;
+  if (sym && NJ != 4) return hipErrorInvalidValue;
+  const int ntm = (g.M + kBM - 1) / kBM, ntn = (g.N + BN - 1) / BN;
   const int ntiles = sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
   int64_t kchunk = (g.K + nsplit - 1) / nsplit;
   kchunk = (kchunk + BK - 1) / BK * BK;
@@ -318,16 +349,16 @@ static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nspl
   const int vec_a = ((uintptr_t)A % 16 == 0) && (g.lda % PER == 0);
   const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % PER == 0);
   if (g.a_kcontig && g.b_kcontig)
-    hipLaunchKernelGGL((k_gemm<PREC, true, true, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb, C,
+    hipLaunchKernelGGL((k_gemm<PREC, true, true, Out, NJ>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb, C,
                        ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
   else if (g.a_kcontig && !g.b_kcontig)
-    hipLaunchKernelGGL((k_gemm<PREC, true, false, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
+    hipLaunchKernelGGL((k_gemm<PREC, true, false, Out, NJ>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
                        C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
   else if (!g.a_kcontig && g.b_kcontig)
-    hipLaunchKernelGGL((k_gemm<PREC, false, true, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
+    hipLaunchKernelGGL((k_gemm<PREC, false, true, Out, NJ>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
                        C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
   else
-    hipLaunchKernelGGL((k_gemm<PREC, false, false, Out>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
+    hipLaunchKernelGGL((k_gemm<PREC, false, false, Out, NJ>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
                        C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
   return hipGetLastError();
 }
@@ -361,8 +392,13 @@ hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, dou
 }
 
 // FP32 output variant (mixed path intermediates): C = A B^T, no split, no sym.
+// N of 129..160 (the exchange product at 160 occupied orbitals) or a multiple
+// of 160 takes 160-wide tiles: one tile per row block instead of a padded pair.
 hipError_t launch_gemm_f32out(const GemmArgs& g, float* C, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  const int t128 = (g.N + 127) / 128, t160 = (g.N + 159) / 160;
+  if (t160 < t128 && (int64_t)t160 * 160 - g.N < (int64_t)t128 * 128 - g.N)
+    return launch_gemm_t<kBF16x3, float, 5>(g, C, g.N, 1, 0, st);
   return launch_gemm_t<kBF16x3, float>(g, C, g.N, 1, 0, st);
 }
 
@@ -371,6 +407,9 @@ hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, in
                                       int64_t bs_c, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0 || batch <= 0) return hipSuccess;
   if (batch > 65535) return hipErrorInvalidValue;
+  const int t128 = (g.N + 127) / 128, t160 = (g.N + 159) / 160;
+  if (t160 < t128 && (int64_t)t160 * 160 - g.N < (int64_t)t128 * 128 - g.N)
+    return launch_gemm_t<kBF16x3, float, 5>(g, C, g.N, 1, 0, st, batch, bs_a, bs_b, bs_c);
   return launch_gemm_t<kBF16x3, float>(g, C, g.N, 1, 0, st, batch, bs_a, bs_b, bs_c);
 }
 
