@@ -344,6 +344,9 @@ struct dfks_ctx {
   double *d_sp2 = nullptr, *d_lohi = nullptr, *d_dvec = nullptr, *d_L = nullptr;
   int32_t* d_piv = nullptr;
   int sp2_guess = 24;       // iterations the last purification needed (first chunk of the next)
+  hipGraphExec_t sp2_exec = nullptr;  // kSp2Block purification iterations as one graph (per nmo)
+  int sp2_exec_n = -1;
+  hipStream_t cap_stream = nullptr;   // private non-blocking stream the graph is captured on
   bool c_stale = false;     // d_C / d_eps not yet computed for the last step (purified density)
   int n_eigh_steps = 0;     // steps that fell back to the eigensolver
   // recompute mode (options.aux_chunk > 0): screened bra pairs per (la, lb)
@@ -385,6 +388,8 @@ struct dfks_ctx {
       if (e) (void)hipEventDestroy(e);
     for (auto& e : scf_ev)
       if (e) (void)hipEventDestroy(e);
+    if (sp2_exec) (void)hipGraphExecDestroy(sp2_exec);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     rb_release(rb);
   }
 };
@@ -1073,33 +1078,70 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
 // factor L (pivoted Cholesky) and Cf = sqrt(2) X L. Returns false when the
 // purification does not reach an idempotent projector of trace nocc (no gap
 // at the Fermi level); the caller then takes the eigensolver path.
+// kSp2Block purification iterations (product, trace, update), launched eagerly
+hipError_t sp2_block(dfks_ctx* h, hipStream_t st) {
+  const int n = h->nmo;
+  const int ntn = (n + 127) / 128;
+  const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, n);
+  const GemmArgs g{n, n, n, h->d_W1, n, true, h->d_W1, n, true};
+  for (int c = 0; c < kSp2Block; ++c) {
+    hipError_t e = launch_gemm(kF64, g, h->d_W2, 1.0, 0.0, nsplit, 1, h->d_gws, st);
+    if (e == hipSuccess) e = launch_sp2_trace_update(n, h->d_W1, h->d_W2, h->nocc, h->d_sp2, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// The same block captured once per handle and nmo as a HIP graph (kernels
+// only: fixed buffers, no memset or copy nodes); one graph launch replaces
+// 4 kSp2Block kernel launches. Captured on a private non-blocking stream
+// (capture is not allowed on the legacy default stream) and launched on the
+// caller's stream. Falls back to eager launches if capture fails.
+void sp2_run_blocks(dfks_ctx* h, int nblocks, hipStream_t st) {
+  if (h->sp2_exec_n != h->nmo) {
+    if (h->sp2_exec) (void)hipGraphExecDestroy(h->sp2_exec);
+    h->sp2_exec = nullptr;
+    h->sp2_exec_n = h->nmo;
+    hipGraph_t gr = nullptr;
+    if (!h->cap_stream && hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess)
+      h->cap_stream = nullptr;
+    if (h->cap_stream && hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      const hipError_t e1 = sp2_block(h, h->cap_stream);
+      const hipError_t e2 = hipStreamEndCapture(h->cap_stream, &gr);
+      if (e1 == hipSuccess && e2 == hipSuccess && gr) {
+        if (hipGraphInstantiate(&h->sp2_exec, gr, nullptr, nullptr, 0) != hipSuccess) h->sp2_exec = nullptr;
+      }
+      if (gr) (void)hipGraphDestroy(gr);
+      (void)hipGetLastError();
+    }
+  }
+  for (int b = 0; b < nblocks; ++b) {
+    if (h->sp2_exec) HIPCHK(hipGraphLaunch(h->sp2_exec, st));
+    else HIPCHK(sp2_block(h, st));
+  }
+}
+
 bool purify_density(dfks_ctx* h, hipStream_t st) {
   const int n = h->nmo, nocc = h->nocc, nao = h->nao;
   double* X = h->d_W1;
-  double* X2 = h->d_W2;
   HIPCHK(launch_gersh(n, h->d_Ws, h->d_lohi, h->d_sp2, st));
   HIPCHK(launch_sp2_init(n, h->d_Ws, h->d_sp2, X, st));
-  const int ntn = (n + 127) / 128;
-  const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, n);
-  const GemmArgs g{n, n, n, X, n, true, X, n, true};
   int done = 0;
   int chunk = h->sp2_guess + 1;
   double stv[kSpHist];
   bool frozen = false;
   while (done < kSp2MaxIter) {
-    chunk = std::min(chunk, kSp2MaxIter - done);
-    for (int c = 0; c < chunk; ++c) {
-      HIPCHK(launch_gemm(kF64, g, X2, 1.0, 0.0, nsplit, 1, h->d_gws, st));
-      HIPCHK(launch_sp2_trace_update(n, X, X2, nocc, h->d_sp2, st));
-    }
-    done += chunk;
+    int nblocks = (chunk + kSp2Block - 1) / kSp2Block;
+    nblocks = std::min(nblocks, (kSp2MaxIter - done) / kSp2Block);
+    sp2_run_blocks(h, nblocks, st);
+    done += nblocks * kSp2Block;
     HIPCHK(hipMemcpyAsync(stv, h->d_sp2, sizeof(stv), hipMemcpyDeviceToHost, st));
     sync(st);
     if (stv[kSpFlag] < 0.0) {
       frozen = true;
       break;
     }
-    chunk = 4;
+    chunk = kSp2Block;
   }
   if (!frozen) return false;
   h->sp2_guess = std::max(8, (int)stv[kSpIter]);

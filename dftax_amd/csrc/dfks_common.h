@@ -29,6 +29,7 @@ enum Sp2State {
   kSpResid = 7, kSpMinPiv = 8, kSpHist = 16
 };
 constexpr int kSp2MaxIter = 128;
+constexpr int kSp2Block = 4;          // purification iterations per graph launch (divides kSp2MaxIter)
 constexpr int kPanel = 32;            // pivoted Cholesky panel width
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
