@@ -38,8 +38,8 @@ WORKLOADS = {
 
 def measured_traffic(workload: str, stage: str):
     """HBM bytes per launch of a stage's FP64 kernel, from the committed counter summary
-    (profiles/<workload>_pbe0_hbm_traffic.md: FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_pbe0_hbm_traffic.md")
+    (profiles/<workload>_pbe0_hbm_traffic.txt: FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_pbe0_hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -47,6 +47,11 @@ def measured_traffic(workload: str, stage: str):
         if len(cells) == 6 and cells[0] == f"{stage} (FP64)":
             return round((float(cells[2]) + float(cells[3])) * 1e9)
     return None
+
+
+def note(msg: str) -> None:
+    """Progress line on stderr (stdout carries only the JSON result)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def main() -> int:
@@ -90,6 +95,7 @@ def main() -> int:
     ks = KS(mol, args.xc, coulomb=df(screen=wl["screen"], engine=hip(precision="mixed")))
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
+    note(f"built {args.workload} in {t_build:.1f} s")
     c = ks.coulomb
     L, h = c.L, c.handle
     stream = c._stream()
@@ -114,7 +120,9 @@ def main() -> int:
         return time.perf_counter() - t, {k: v / args.steps for k, v in acc.items()}
 
     dt, stage_avg = timed(False)          # FP64: the headline value
+    note(f"FP64 steps: {1e3 * dt / args.steps:.1f} ms each")
     dt_mx, stage_mx = timed(True)         # mixed: split-BF16 MFMA products
+    note(f"mixed steps: {1e3 * dt_mx / args.steps:.1f} ms each")
     build = {k: v for k, v in N.stage_times(h).items() if k.startswith("build")}
     if args.counters:
         print(json.dumps({"counters_run": args.workload, "steps": args.steps, "copy_bytes": 1 << 30,
@@ -130,9 +138,11 @@ def main() -> int:
         t = time.perf_counter()
         r64 = scf(ks, precision="fp64")
         t64 = time.perf_counter() - t
+        note(f"FP64 SCF: {r64.n_iter} iterations, {t64:.1f} s")
         t = time.perf_counter()
         rmx = scf(ks, precision="mixed")
         tmx = time.perf_counter() - t
+        note(f"mixed SCF: {rmx.n_iter} iterations, {tmx:.1f} s")
 
     nao, nfit, nocc, ng = c.nao, c.nfit, c.nocc, c.ngrid
     recompute = c.plan.aux_chunk > 0 or c.plan.ao_block > 0

@@ -114,12 +114,15 @@ class Server:
         with torch.cuda.device(self.device), torch.cuda.stream(stream):
             ks = None
             try:
+                t0 = time.perf_counter()
                 ks = KS(req.mol, self.xc, grid=self.grid,
                         coulomb=df(self.auxbasis, engine=hip(precision=self.precision)), device=self.device)
+                t1 = time.perf_counter()
                 r = scf(ks, e_tol=self.e_tol, d_tol=self.d_tol, precision=self.precision)
                 req.e_tot, req.n_iter, req.converged = r.e_tot, r.n_iter, r.converged
                 req.extra = {"nao": ks.nao, "ngrid": int(ks.grid_w.shape[0]),
-                             "n_mixed": sum(1 for h in r.history if h[3] == "mixed")}
+                             "n_mixed": sum(1 for h in r.history if h[3] == "mixed"),
+                             "build_s": t1 - t0, "scf_s": time.perf_counter() - t1}
             except Exception as exc:  # reported per request, the server keeps going
                 req.error = f"{type(exc).__name__}: {exc}"
             finally:
@@ -182,8 +185,14 @@ def latency_stats(reqs: list[Request]) -> dict:
     ok = [r for r in reqs if not r.error]
     lat = np.array([r.end - r.arrival for r in ok])
     span = max(r.end for r in ok) - min(r.arrival for r in ok) if ok else float("nan")
+    build = np.array([r.extra.get("build_s", np.nan) for r in ok])
+    scf_t = np.array([r.extra.get("scf_s", np.nan) for r in ok])
+    wait = np.array([r.start - r.arrival for r in ok])
     return {
         "n": len(reqs), "ok": len(ok), "converged": sum(r.converged for r in ok),
+        "mean_build_s": float(np.nanmean(build)) if ok else None,
+        "mean_scf_s": float(np.nanmean(scf_t)) if ok else None,
+        "mean_queue_wait_s": float(wait.mean()) if ok else None,
         "solves_per_min": 60.0 * len(ok) / span if ok else 0.0,
         "p50_s": float(np.percentile(lat, 50)) if ok else None,
         "p99_s": float(np.percentile(lat, 99)) if ok else None,
@@ -209,15 +218,19 @@ def main() -> int:
     mols = serving_stream(args.n, seed=args.seed)
     srv = Server(args.xc, precision=args.precision, slots=args.slots)
     # isolated-solve time on the first requests, one at a time
-    iso = []
+    iso, iso_req = [], []
     for k in range(min(args.isolated, len(mols))):
         t = time.perf_counter()
         r = srv.solve(Request(rid=k, mol=mols[k]))
         iso.append(time.perf_counter() - t)
+        iso_req.append(r)
         if r.error:
             raise RuntimeError(r.error)
     tau = float(np.mean(iso[1:])) if len(iso) > 1 else iso[0]
-    out = {"isolated_solve_s_mean": tau, "isolated_solve_s_p99": float(np.percentile(iso[1:] or iso, 99))}
+    body = iso_req[1:] or iso_req
+    out = {"isolated_solve_s_mean": tau, "isolated_solve_s_p99": float(np.percentile(iso[1:] or iso, 99)),
+           "isolated_build_s_mean": float(np.mean([r.extra["build_s"] for r in body])),
+           "isolated_scf_s_mean": float(np.mean([r.extra["scf_s"] for r in body]))}
     rng = np.random.default_rng(args.seed + 1)
     for rate in args.rate:
         lam = rate / tau
