@@ -43,7 +43,7 @@ EXPORTS = [
     "dfks_last_error", "dfks_create", "dfks_destroy", "dfks_build", "dfks_sizes", "dfks_get_matrix",
     "dfks_energy_fock", "dfks_energy_fock_factored", "dfks_xc_eval", "dfks_scf_begin", "dfks_scf_step",
     "dfks_scf_result", "dfks_becke_weights", "dfks_overlap", "dfks_set_timing", "dfks_stage_times",
-    "dfks_set_precision", "dfks_stats",
+    "dfks_set_precision", "dfks_stats", "dfks_xc_eval2", "dfks_response_begin", "dfks_response_fock",
 ]
 
 STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
@@ -77,6 +77,9 @@ def lib():
     L.dfks_energy_fock_factored.argtypes = [c_void_p, c_void_p, c_void_p, P_f64, c_int32, c_void_p, c_void_p,
                                             c_void_p]
     L.dfks_xc_eval.argtypes = [c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.dfks_xc_eval2.argtypes = [c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.dfks_response_begin.argtypes = [c_void_p, c_void_p, c_void_p]
+    L.dfks_response_fock.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]
     L.dfks_scf_begin.argtypes = [c_void_p, c_void_p, c_double, c_int32, c_double, c_void_p]
     L.dfks_scf_step.argtypes = [c_void_p, P_f64, c_void_p]
     L.dfks_scf_result.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, P_i32, c_void_p]

@@ -49,6 +49,11 @@ hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w,
                      double* exc, double* rho_out, hipStream_t st);
 hipError_t launch_xc_pointwise(int64_t n, int xc_id, const double* rho, const double* sigma, double* out,
                                hipStream_t st);
+hipError_t launch_xc_pointwise2(int64_t n, int xc_id, const double* rho, const double* sigma, double* out,
+                                hipStream_t st);
+hipError_t launch_fxc_coef(int64_t ng, const double* rho0, const double* w, int xc_id, double thresh, double* rec,
+                           hipStream_t st);
+hipError_t launch_fxc(int64_t ng, int64_t nao, const double* ao, double* zw, const double* rec, hipStream_t st);
 hipError_t launch_dot(int64_t n, const double* x, const double* y, double* part, double scale, double* out,
                       int accumulate, hipStream_t st);
 hipError_t launch_gamma(int nao, int nq, int nf, const double* T, const double* Cf, const double* sgn, double* gamma,
@@ -329,7 +334,7 @@ struct dfks_ctx {
   double* d_gpart = nullptr;  // per-row partial sums of gamma (nao x nfit)
   // mixed-precision operands (precision = 1)
   bool mixed_now = false;
-  bool refresh_fock = false;  // rebuild F in FP64 at the current P before the next step
+  bool restart_diis = false;  // restart the DIIS history once the next step has run
   int diis_base = 0;          // iteration index where the DIIS history restarted
   int last_nf = 0;            // factor columns of the last Fock build (signs in h_sgn)
   float *d_B32 = nullptr, *d_ao32 = nullptr, *d_zw32 = nullptr, *d_T32 = nullptr, *d_Cf32 = nullptr, *d_P32 = nullptr;
@@ -343,6 +348,11 @@ struct dfks_ctx {
   // purification: state, Gershgorin bounds, residual diagonal, occupied factor, pivots
   double *d_sp2 = nullptr, *d_lohi = nullptr, *d_dvec = nullptr, *d_L = nullptr;
   int32_t* d_piv = nullptr;
+  // response (jvp of the Fock matrix): per-point XC records at the reference
+  // density (8 x ng), rho0 and grad rho0 (4 x ng), and the signed factor of dP
+  double *d_fxc = nullptr, *d_rho0 = nullptr, *d_Rf = nullptr;
+  int rf_cap = 0;
+  bool resp_ready = false;
   int sp2_guess = 24;       // iterations the last purification needed (first chunk of the next)
   hipGraphExec_t sp2_exec = nullptr;  // kSp2Block purification iterations as one graph (per nmo)
   int sp2_exec_n = -1;
@@ -379,7 +389,7 @@ struct dfks_ctx {
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
                     d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws,
                     d_sp2, d_lohi, d_dvec, d_L, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
-                    d_fitd, d_aob, d_zwb};
+                    d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf};
     for (void* p : ptrs) dfree(p);
     void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, t32_alias ? nullptr : d_T32, d_Cf32, d_P32, d_ws32,
                       d_eps32, d_offd32, d_slab32, d_Lt32, d_Tslab32, d_aob32, d_zwb32};
@@ -668,7 +678,7 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   h->d_K = dalloc<double>((size_t)nao * nao);
   h->d_Vxc = dalloc<double>((size_t)nao * nao);
   h->d_Cf = dalloc<double>((size_t)nao * nao);
-  h->d_sgn = dalloc<double>(nao);
+  h->d_sgn = dalloc<double>(2 * (size_t)nao);
   h->d_W1 = dalloc<double>((size_t)nao * nao);
   h->d_W2 = dalloc<double>((size_t)nao * nao);
   h->d_evals = dalloc<double>(nao);
@@ -799,27 +809,17 @@ void coulomb_recompute(dfks_ctx* h, hipStream_t st) {
   }
 }
 
-void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const std::vector<double>& sgn, int nf,
-                          double* E, double* F, hipStream_t st) {
-  if (!h->built) fail(6, "dfks_build has not been called");
-  RBCHK(rocblas_set_stream(h->rb, st));
+// gamma, J and K of P = sum_i s_i c_i c_i^T from its factor alone:
+// gamma = sum_i s_i T_i c_i, J = B gamma, K = sum_i s_i T_i^T T_i.
+void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, int nf, hipStream_t st) {
   const int nao = h->nao, nfit = h->nfit;
   const int64_t nn = (int64_t)nao * nao;
-  const int64_t ng = h->ng;
   const double one = 1.0, zero = 0.0;
   const bool mixed = h->mixed_now;
-  // FP64 products: gemm 0 (default) runs the symmetric exchange product on the
-  // library kernel and the plain T, Z, V products on rocBLAS; 1 runs all four on
-  // rocBLAS; 2 runs all four on the library kernel.
   const bool lib = h->opt.gemm == 2;
   const bool lib_k = h->opt.gemm != 1;
-  if (mixed && h->opt.precision != 1) fail(1, "mixed precision needs options.precision = 1 at create");
-  h->mark(0, st);
-  HIPCHK(hipMemsetAsync(E, 0, 8 * sizeof(double), st));
-  // one-electron energy
-  HIPCHK(launch_dot(nn, P, h->d_H, h->d_part, 1.0, E + DFKS_E_ONE, 0, st));
   // signs to the device (host copy kept in the handle until the next call)
-  if (nf > nao) fail(1, "more factor columns than basis functions");
+  if (nf > (mixed ? nao : 2 * nao)) fail(1, "too many factor columns");
   h->h_sgn.assign(sgn.begin(), sgn.end());
   h->last_nf = nf;
   HIPCHK(hipMemcpyAsync(h->d_sgn, h->h_sgn.data(), sizeof(double) * nf, hipMemcpyHostToDevice, st));
@@ -881,14 +881,21 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
     first = false;
     i0 = i1;
   }
-  // E_J = 1/2 |gamma|^2, J = B gamma
-  HIPCHK(launch_dot(nfit, h->d_gamma, h->d_gamma, h->d_part, 0.5, E + DFKS_E_J, 0, st));
+  // J = B gamma
   if (h->recompute) coulomb_recompute(h, st);
   else if (mixed) HIPCHK(launch_jvec_f32(nn, nfit, h->d_B32, h->d_gamma, h->d_J, st));
   else HIPCHK(launch_jvec(nn, nfit, h->d_B, h->d_gamma, h->d_J, st));
   h->mark(kStCoulomb + 1, st);
-  // E_K = -1/4 a_x sum P o K
-  if (h->hf != 0.0) HIPCHK(launch_dot(nn, P, h->d_K, h->d_part, -0.25 * h->hf, E + DFKS_E_K, 0, st));
+}
+
+// XC energy (E + DFKS_E_XC) and the half potential Vh (d_Vxc, Vxc = Vh + Vh^T) at P.
+void xc_potential(dfks_ctx* h, const double* P, double* E, hipStream_t st) {
+  const int nao = h->nao;
+  const int64_t nn = (int64_t)nao * nao;
+  const int64_t ng = h->ng;
+  const double one = 1.0, zero = 0.0;
+  const bool mixed = h->mixed_now;
+  const bool lib = h->opt.gemm == 2;
   // XC: Z = Phi P, W (in place), Vh = Phi^T W; V = Vh + Vh^T in the Fock assembly
   if (ng > 0 && h->ao_block > 0) {
     // AO values and gradients per block of points: Z_b = Phi_b P, W_b, Vh += Phi_b^T W_b
@@ -962,6 +969,26 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
     }
     h->mark(kStXcSyr2k + 1, st);
   }
+}
+
+void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const std::vector<double>& sgn, int nf,
+                          double* E, double* F, hipStream_t st) {
+  if (!h->built) fail(6, "dfks_build has not been called");
+  RBCHK(rocblas_set_stream(h->rb, st));
+  const int nao = h->nao, nfit = h->nfit;
+  const int64_t nn = (int64_t)nao * nao;
+  const int64_t ng = h->ng;
+  if (h->mixed_now && h->opt.precision != 1) fail(1, "mixed precision needs options.precision = 1 at create");
+  h->mark(0, st);
+  HIPCHK(hipMemsetAsync(E, 0, 8 * sizeof(double), st));
+  // one-electron energy
+  HIPCHK(launch_dot(nn, P, h->d_H, h->d_part, 1.0, E + DFKS_E_ONE, 0, st));
+  jk_factored(h, Cf, sgn, nf, st);
+  // E_J = 1/2 |gamma|^2
+  HIPCHK(launch_dot(nfit, h->d_gamma, h->d_gamma, h->d_part, 0.5, E + DFKS_E_J, 0, st));
+  // E_K = -1/4 a_x sum P o K
+  if (h->hf != 0.0) HIPCHK(launch_dot(nn, P, h->d_K, h->d_part, -0.25 * h->hf, E + DFKS_E_K, 0, st));
+  xc_potential(h, P, E, st);
   HIPCHK(launch_fock(nao, h->d_H, h->d_J, h->hf != 0.0 ? h->d_K : nullptr, -0.5 * h->hf,
                      ng > 0 ? h->d_Vxc : nullptr, F, st));
   // totals: E_tot = e1 + eJ + eK + exc + enn
@@ -1016,6 +1043,88 @@ void gemm(dfks_ctx* h, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double
                          &beta, C, ldc));
 }
 
+// ---------------------------------------------------------------------------
+// Response: R = J[dP] - a_x/2 K[dP] + f_xc(P0)[dP], the derivative of
+// F = dE/dP at P0 along dP (jax.jvp(jax.grad(E)) of dftax/ks/newton.py:134-136)
+// ---------------------------------------------------------------------------
+// rho0 and grad rho0 at the reference density P0, then the per-point records.
+void response_begin(dfks_ctx* h, const double* P, hipStream_t st) {
+  if (!h->built) fail(6, "dfks_build has not been called");
+  if (h->mixed_now) fail(1, "the response runs in FP64: call dfks_set_precision(h, 0) first");
+  RBCHK(rocblas_set_stream(h->rb, st));
+  h->resp_ready = false;
+  const int nao = h->nao;
+  const int64_t ng = h->ng;
+  const double thresh = h->opt.rho_thresh;
+  if (ng > 0) {
+    if (!h->d_rho0) h->d_rho0 = dalloc<double>(4 * (size_t)ng);
+    if (!h->d_fxc) h->d_fxc = dalloc<double>(8 * (size_t)ng);
+    if (h->ao_block == 0) {
+      gemm(h, false, false, nao, ng, nao, 1.0, P, nao, h->d_ao, nao, 0.0, h->d_zw, nao);
+      HIPCHK(launch_xc(ng, nao, h->d_ao, h->d_gw, h->d_zw, h->xc_id, thresh, h->d_exc, h->d_rho0, st));
+    } else {
+      const ShellSet O = h->orb.view();
+      const int64_t gb = std::min<int64_t>(h->ao_block, ng);
+      for (int64_t g0 = 0; g0 < ng; g0 += gb) {
+        const int64_t nb = std::min(gb, ng - g0);
+        HIPCHK(launch_ao(nb, h->d_gxyz + 3 * g0, O, nao, h->d_aob, st));
+        gemm(h, false, false, nao, nb, nao, 1.0, P, nao, h->d_aob, nao, 0.0, h->d_zwb, nao);
+        HIPCHK(launch_xc(nb, nao, h->d_aob, h->d_gw + g0, h->d_zwb, h->xc_id, thresh, h->d_exc + g0,
+                         h->d_rho0 + 4 * g0, st));
+      }
+    }
+    HIPCHK(launch_fxc_coef(ng, h->d_rho0, h->d_gw, h->xc_id, thresh, h->d_fxc, st));
+  }
+  h->resp_ready = true;
+}
+
+// R for dP = U V^T + V U^T (U, V column-major nao x r). J and K come from the
+// signed factor dP = 1/2 (U+V)(U+V)^T - 1/2 (U-V)(U-V)^T; the XC part from
+// Z1 = Phi dP and the per-point records.
+void response_fock(dfks_ctx* h, const double* U, const double* V, int r, double* R, hipStream_t st) {
+  if (!h->resp_ready) fail(6, "dfks_response_begin has not been called");
+  if (h->mixed_now) fail(1, "the response runs in FP64: call dfks_set_precision(h, 0) first");
+  const int nao = h->nao;
+  const int64_t ng = h->ng;
+  if (r < 1 || r > nao) fail(1, "r must be in 1..nao");
+  RBCHK(rocblas_set_stream(h->rb, st));
+  if (h->rf_cap < 2 * r) {
+    dfree(h->d_Rf);
+    h->d_Rf = dalloc<double>((size_t)nao * 2 * r);
+    h->rf_cap = 2 * r;
+  }
+  const double c = std::sqrt(0.5), mc = -c;
+  RBCHK(rocblas_dgeam(h->rb, rocblas_operation_none, rocblas_operation_none, nao, r, &c, U, nao, &c, V, nao, h->d_Rf,
+                      nao));
+  RBCHK(rocblas_dgeam(h->rb, rocblas_operation_none, rocblas_operation_none, nao, r, &c, U, nao, &mc, V, nao,
+                      h->d_Rf + (int64_t)nao * r, nao));
+  std::vector<double> sg(2 * r, 1.0);
+  for (int i = r; i < 2 * r; ++i) sg[i] = -1.0;
+  jk_factored(h, h->d_Rf, sg, 2 * r, st);
+  if (ng > 0) {
+    // dense dP, then V1 = Phi^T W1 (the assembly adds its transpose)
+    gemm(h, false, true, nao, nao, r, 1.0, U, nao, V, nao, 0.0, h->d_W1, nao);
+    gemm(h, false, true, nao, nao, r, 1.0, V, nao, U, nao, 1.0, h->d_W1, nao);
+    if (h->ao_block == 0) {
+      gemm(h, false, false, nao, ng, nao, 1.0, h->d_W1, nao, h->d_ao, nao, 0.0, h->d_zw, nao);
+      HIPCHK(launch_fxc(ng, nao, h->d_ao, h->d_zw, h->d_fxc, st));
+      gemm(h, false, true, nao, nao, ng, 1.0, h->d_ao, nao, h->d_zw, nao, 0.0, h->d_Vxc, nao);
+    } else {
+      const ShellSet O = h->orb.view();
+      const int64_t gb = std::min<int64_t>(h->ao_block, ng);
+      for (int64_t g0 = 0; g0 < ng; g0 += gb) {
+        const int64_t nb = std::min(gb, ng - g0);
+        HIPCHK(launch_ao(nb, h->d_gxyz + 3 * g0, O, nao, h->d_aob, st));
+        gemm(h, false, false, nao, nb, nao, 1.0, h->d_W1, nao, h->d_aob, nao, 0.0, h->d_zwb, nao);
+        HIPCHK(launch_fxc(nb, nao, h->d_aob, h->d_zwb, h->d_fxc + 8 * g0, st));
+        gemm(h, false, true, nao, nao, nb, 1.0, h->d_aob, nao, h->d_zwb, nao, g0 == 0 ? 0.0 : 1.0, h->d_Vxc, nao);
+      }
+    }
+  }
+  HIPCHK(launch_fock(nao, nullptr, h->d_J, h->hf != 0.0 ? h->d_K : nullptr, -0.5 * h->hf,
+                     ng > 0 ? h->d_Vxc : nullptr, R, st));
+}
+
 void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level_shift, hipStream_t st) {
   if (!h->built) fail(6, "dfks_build has not been called");
   if (m < 1 || m > 16) fail(1, "diis_space must be in 1..16");
@@ -1037,7 +1146,7 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   h->m = m;
   h->it = 0;
   h->diis_base = 0;
-  h->refresh_fock = false;
+  h->restart_diis = false;
   h->level_shift = level_shift;
   auto realloc = [](double*& p, size_t n) { dfree(p); p = dalloc<double>(n); };
   realloc(h->d_X, (size_t)nao * nmo);
@@ -1168,16 +1277,6 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
   RBCHK(rocblas_set_stream(h->rb, st));
   const int nao = h->nao, nmo = h->nmo, m = h->m;
   const int64_t nn = (int64_t)nao * nao, mm = (int64_t)nmo * nmo;
-  if (h->refresh_fock) {
-    // the stored F came from a mixed-precision build: rebuild it in FP64 at the current P
-    std::vector<double> sg(h->h_sgn.begin(), h->h_sgn.begin() + h->last_nf);
-    energy_fock_factored(h, h->d_P, h->d_Cf, sg, h->last_nf, h->d_Edev, h->d_F, st);
-    double e[8];
-    HIPCHK(hipMemcpyAsync(e, h->d_Edev, sizeof(e), hipMemcpyDeviceToHost, st));
-    sync(st);
-    h->e_prev = e[0];
-    h->refresh_fock = false;
-  }
   const int slot = (h->it - h->diis_base) % m;
   const int count = std::min(h->it - h->diis_base + 1, m);
   double* err = h->d_Es + slot * mm;
@@ -1244,6 +1343,12 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
   out3[1] = e[0] - h->e_prev;
   out3[2] = std::sqrt(e[7]);
   h->e_prev = e[0];
+  if (h->restart_diis) {
+    // the history so far holds mixed-precision Fock matrices; from the next
+    // step on it holds only FP64 ones, starting with the F just built
+    h->diis_base = h->it + 1;
+    h->restart_diis = false;
+  }
   h->it += 1;
 }
 
@@ -1326,11 +1431,9 @@ int dfks_set_precision(dfks_handle h, int32_t mixed) {
     if (mixed && h->opt.precision != 1) fail(1, "mixed steps need options.precision = 1 at create");
     const bool was = h->mixed_now;
     h->mixed_now = mixed != 0;
-    if (was && !h->mixed_now && h->d_X) {
-      // leaving mixed steps inside an SCF: FP64 Fock at the current density, fresh DIIS history
-      h->refresh_fock = true;
-      h->diis_base = h->it;
-    }
+    // leaving mixed steps inside an SCF: the next step extrapolates over the
+    // mixed history and builds F in FP64; the DIIS history restarts after it
+    if (was && !h->mixed_now && h->d_X) h->restart_diis = true;
   });
 }
 
@@ -1456,6 +1559,30 @@ int dfks_xc_eval(int32_t xc_id, int64_t n, const double* rho, const double* sigm
   return guard([&] {
     hf_coeff(xc_id);
     HIPCHK(launch_xc_pointwise(n, xc_id, rho, sigma, out3, S_(stream)));
+  });
+}
+
+int dfks_xc_eval2(int32_t xc_id, int64_t n, const double* rho, const double* sigma, double* out6,
+                  dfks_stream stream) {
+  return guard([&] {
+    hf_coeff(xc_id);
+    HIPCHK(launch_xc_pointwise2(n, xc_id, rho, sigma, out6, S_(stream)));
+  });
+}
+
+int dfks_response_begin(dfks_handle h, const double* P, dfks_stream stream) {
+  return guard([&] {
+    if (!h || !P) fail(1, "null argument");
+    StreamScope ss(h->st = S_(stream));
+    response_begin(h, P, S_(stream));
+  });
+}
+
+int dfks_response_fock(dfks_handle h, const double* U, const double* V, int32_t r, double* R, dfks_stream stream) {
+  return guard([&] {
+    if (!h || !U || !V || !R) fail(1, "null argument");
+    StreamScope ss(h->st = S_(stream));
+    response_fock(h, U, V, r, R, S_(stream));
   });
 }
 

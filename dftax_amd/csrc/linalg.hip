@@ -4,7 +4,7 @@
 //               gamma_Q = sum_{m,i} s_i Cf[m,i] T[m,Q,i]   (fitted density, from the
 //               exchange intermediate T = B^T Cf, so B is not re-read for it)
 //  * k_jvec     J[mn] = sum_Q B[mn,Q] gamma_Q              (one read of B, HBM bound)
-//  * k_fock     F = hcore + J + cK sym(K) + sym(Vxc)        (upper-triangle inputs)
+//  * k_fock     F = hcore + J + cK sym(K) + sym(Vxc)        (upper-triangle inputs; hcore may be absent)
 //  * deterministic two-pass reductions for the energy terms
 //  * DIIS: Gram row, bordered solve (one lane, FP64, partial pivoting),
 //    extrapolation (dftax/ks/scf.py:252-276)
@@ -219,7 +219,7 @@ __global__ void k_fock(int n, const double* __restrict__ H, const double* __rest
   int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= (int64_t)n * n) return;
   int r = (int)(idx / n), c = (int)(idx % n);
-  double f = H[idx] + J[idx];
+  double f = (H ? H[idx] : 0.0) + J[idx];
   if (K) f += cK * 0.5 * (K[idx] + K[(int64_t)c * n + r]);
   if (Vh) f += Vh[idx] + Vh[(int64_t)c * n + r];
   F[idx] = f;

@@ -21,127 +21,186 @@ namespace dfks {
 
 enum XcId { kXcPBE = 1, kXcPBE0 = 2, kXcB3LYP = 3 };
 
-struct D {
-  double v, r, s;  // value, d/drho, d/dsigma
+// Forward-mode dual number over the two functional arguments: value, d/drho,
+// d/dsigma. Dual<double> gives f, f_rho, f_sigma; Dual<Dual<double>> (a dual
+// whose parts are duals) also carries the second derivatives f_rho rho,
+// f_rho sigma, f_sigma sigma for the response kernel. The functionals below
+// are written once, generic over the number type.
+template <class T>
+struct Dual {
+  T v, r, s;  // value, d/drho, d/dsigma
 };
-__device__ __forceinline__ D dc(double c) { return {c, 0.0, 0.0}; }
-__device__ __forceinline__ D operator+(D a, D b) { return {a.v + b.v, a.r + b.r, a.s + b.s}; }
-__device__ __forceinline__ D operator-(D a, D b) { return {a.v - b.v, a.r - b.r, a.s - b.s}; }
-__device__ __forceinline__ D operator-(D a) { return {-a.v, -a.r, -a.s}; }
-__device__ __forceinline__ D operator*(D a, D b) { return {a.v * b.v, a.r * b.v + a.v * b.r, a.s * b.v + a.v * b.s}; }
-__device__ __forceinline__ D operator/(D a, D b) {
-  double iv = 1.0 / b.v, q = a.v * iv;
+using D = Dual<double>;
+using D2 = Dual<Dual<double>>;
+
+__device__ __forceinline__ double val(double x) { return x; }
+template <class T>
+__device__ __forceinline__ double val(const Dual<T>& a) { return val(a.v); }
+
+template <class T>
+__device__ __forceinline__ Dual<T> operator+(const Dual<T>& a, const Dual<T>& b) { return {a.v + b.v, a.r + b.r, a.s + b.s}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator-(const Dual<T>& a, const Dual<T>& b) { return {a.v - b.v, a.r - b.r, a.s - b.s}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator-(const Dual<T>& a) { return {-a.v, -a.r, -a.s}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator*(const Dual<T>& a, const Dual<T>& b) {
+  return {a.v * b.v, a.r * b.v + a.v * b.r, a.s * b.v + a.v * b.s};
+}
+template <class T>
+__device__ __forceinline__ Dual<T> operator/(const Dual<T>& a, const Dual<T>& b) {
+  const T iv = 1.0 / b.v, q = a.v * iv;
   return {q, (a.r - q * b.r) * iv, (a.s - q * b.s) * iv};
 }
-__device__ __forceinline__ D operator+(D a, double c) { return {a.v + c, a.r, a.s}; }
-__device__ __forceinline__ D operator+(double c, D a) { return {a.v + c, a.r, a.s}; }
-__device__ __forceinline__ D operator-(D a, double c) { return {a.v - c, a.r, a.s}; }
-__device__ __forceinline__ D operator-(double c, D a) { return {c - a.v, -a.r, -a.s}; }
-__device__ __forceinline__ D operator*(D a, double c) { return {a.v * c, a.r * c, a.s * c}; }
-__device__ __forceinline__ D operator*(double c, D a) { return {a.v * c, a.r * c, a.s * c}; }
-__device__ __forceinline__ D operator/(D a, double c) { return {a.v / c, a.r / c, a.s / c}; }
-__device__ __forceinline__ D operator/(double c, D a) {
-  double q = c / a.v;
-  double dq = -q / a.v;
+template <class T>
+__device__ __forceinline__ Dual<T> operator+(const Dual<T>& a, double c) { return {a.v + c, a.r, a.s}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator+(double c, const Dual<T>& a) { return {a.v + c, a.r, a.s}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator-(const Dual<T>& a, double c) { return {a.v - c, a.r, a.s}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator-(double c, const Dual<T>& a) { return {c - a.v, -a.r, -a.s}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator*(const Dual<T>& a, double c) { return {a.v * c, a.r * c, a.s * c}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator*(double c, const Dual<T>& a) { return {a.v * c, a.r * c, a.s * c}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator/(const Dual<T>& a, double c) { return {a.v / c, a.r / c, a.s / c}; }
+template <class T>
+__device__ __forceinline__ Dual<T> operator/(double c, const Dual<T>& a) {
+  const T q = c / a.v;
+  const T dq = -q / a.v;
   return {q, dq * a.r, dq * a.s};
 }
-__device__ __forceinline__ D chain(D a, double f, double df) { return {f, df * a.r, df * a.s}; }
-__device__ __forceinline__ D dsqrt(D a) { double f = sqrt(a.v); return chain(a, f, 0.5 / f); }
-__device__ __forceinline__ D dexp(D a) { double f = exp(a.v); return chain(a, f, f); }
-__device__ __forceinline__ D dlog(D a) { return chain(a, log(a.v), 1.0 / a.v); }
-__device__ __forceinline__ D dlog1p(D a) { return chain(a, log1p(a.v), 1.0 / (1.0 + a.v)); }
-__device__ __forceinline__ D dexpm1(D a) { return chain(a, expm1(a.v), exp(a.v)); }
-__device__ __forceinline__ D datan(D a) { return chain(a, atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
-__device__ __forceinline__ D dasinh(D a) { return chain(a, asinh(a.v), 1.0 / sqrt(1.0 + a.v * a.v)); }
-__device__ __forceinline__ D dpow(D a, double p) {
-  double f = pow(a.v, p);
+
+// f(a) from f(a.v) and f'(a.v), both evaluated in the inner number type
+template <class T>
+__device__ __forceinline__ Dual<T> chain(const Dual<T>& a, const T& f, const T& df) { return {f, df * a.r, df * a.s}; }
+
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double dexp(double x) { return exp(x); }
+__device__ __forceinline__ double dlog(double x) { return log(x); }
+__device__ __forceinline__ double dlog1p(double x) { return log1p(x); }
+__device__ __forceinline__ double dexpm1(double x) { return expm1(x); }
+__device__ __forceinline__ double datan(double x) { return atan(x); }
+__device__ __forceinline__ double dasinh(double x) { return asinh(x); }
+__device__ __forceinline__ double dpow(double x, double p) { return pow(x, p); }
+
+template <class T>
+__device__ __forceinline__ Dual<T> dsqrt(const Dual<T>& a) { const T f = dsqrt(a.v); return chain(a, f, 0.5 / f); }
+template <class T>
+__device__ __forceinline__ Dual<T> dexp(const Dual<T>& a) { const T f = dexp(a.v); return chain(a, f, f); }
+template <class T>
+__device__ __forceinline__ Dual<T> dlog(const Dual<T>& a) { return chain(a, dlog(a.v), 1.0 / a.v); }
+template <class T>
+__device__ __forceinline__ Dual<T> dlog1p(const Dual<T>& a) { return chain(a, dlog1p(a.v), 1.0 / (1.0 + a.v)); }
+template <class T>
+__device__ __forceinline__ Dual<T> dexpm1(const Dual<T>& a) { return chain(a, dexpm1(a.v), dexp(a.v)); }
+template <class T>
+__device__ __forceinline__ Dual<T> datan(const Dual<T>& a) { return chain(a, datan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+template <class T>
+__device__ __forceinline__ Dual<T> dasinh(const Dual<T>& a) {
+  return chain(a, dasinh(a.v), 1.0 / dsqrt(1.0 + a.v * a.v));
+}
+template <class T>
+__device__ __forceinline__ Dual<T> dpow(const Dual<T>& a, double p) {
+  const T f = dpow(a.v, p);
   return chain(a, f, p * f / a.v);
 }
 // jnp.clip(x, lo): derivative passes where x >= lo
-__device__ __forceinline__ D dclip(D a, double lo) { return a.v >= lo ? a : dc(lo); }
+template <class N>
+__device__ __forceinline__ N dclip(const N& a, double lo) { return val(a) >= lo ? a : N{} + lo; }
 
 constexpr double kPi2 = kPi * kPi;
 
-__device__ D lda_x(D rho) {
-  D kF = dpow(3.0 * kPi2 * rho + 1e-30, 1.0 / 3.0);
+template <class N>
+__device__ N lda_x(N rho) {
+  N kF = dpow(3.0 * kPi2 * rho + 1e-30, 1.0 / 3.0);
   return -3.0 * kF / (4.0 * kPi);
 }
 
 // |grad + 1e-30| of dftax's PBE/B88 exchange, written on sigma
-__device__ __forceinline__ D grad_norm_x(D sig) { return dsqrt(sig + 3e-60); }
+template <class N>
+__device__ __forceinline__ N grad_norm_x(N sig) { return dsqrt(sig + 3e-60); }
 
-__device__ D pbe_x(D rho, D sig) {
+template <class N>
+__device__ N pbe_x(N rho, N sig) {
   const double kappa = 0.804, mu = 0.2195149727645171;
-  D gn = grad_norm_x(sig);
-  D kF = dpow(3.0 * kPi2 * rho + 1e-30, 1.0 / 3.0);
-  D s = gn / dclip(2.0 * kF * rho, 1e-30);
-  D eps_unif = -3.0 * kF / (4.0 * kPi);
-  D f = 1.0 + kappa - kappa / (1.0 + mu * s * s / kappa);
+  N gn = grad_norm_x(sig);
+  N kF = dpow(3.0 * kPi2 * rho + 1e-30, 1.0 / 3.0);
+  N s = gn / dclip(2.0 * kF * rho, 1e-30);
+  N eps_unif = -3.0 * kF / (4.0 * kPi);
+  N f = 1.0 + kappa - kappa / (1.0 + mu * s * s / kappa);
   return eps_unif * f;
 }
 
-__device__ D pbe_c(D rho, D sig) {
+template <class N>
+__device__ N pbe_c(N rho, N sig) {
   const double gamma = (1.0 - 0.69314718055994530942) / kPi2;
   const double beta = 0.06672455060314922;
   const double A = 0.0310907, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
-  D rt = dclip(rho, 1e-20);
-  D rs = dpow(3.0 / (4.0 * kPi * rt) + 1e-20, 1.0 / 3.0);
-  D kf = dpow(3.0 * kPi2 * rt + 1e-20, 1.0 / 3.0);
-  D ks = dsqrt(4.0 * kf / kPi);
-  D gn = dsqrt(sig + 1e-30);
-  D t = gn / dclip(2.0 * ks * rt, 1e-20);
-  D rs_sqrt = dsqrt(rs);
-  D denom = 2.0 * A * (b1 * rs_sqrt + b2 * rs + b3 * rs * rs_sqrt + b4 * rs * rs);
+  N rt = dclip(rho, 1e-20);
+  N rs = dpow(3.0 / (4.0 * kPi * rt) + 1e-20, 1.0 / 3.0);
+  N kf = dpow(3.0 * kPi2 * rt + 1e-20, 1.0 / 3.0);
+  N ks = dsqrt(4.0 * kf / kPi);
+  N gn = dsqrt(sig + 1e-30);
+  N t = gn / dclip(2.0 * ks * rt, 1e-20);
+  N rs_sqrt = dsqrt(rs);
+  N denom = 2.0 * A * (b1 * rs_sqrt + b2 * rs + b3 * rs * rs_sqrt + b4 * rs * rs);
   // zeta = 0: the PW92 spin interpolation reduces to the paramagnetic branch
-  D eps_pw = -2.0 * A * (1.0 + a1 * rs) * dlog1p(1.0 / dclip(denom, 1e-20));
-  D denom_A = gamma * dexpm1(-eps_pw / gamma);
-  D Ac = beta / dclip(denom_A, 1e-20);
-  D t2 = t * t;
-  D f1 = t2 + Ac * t2 * t2;
-  D frac = (beta * f1) / dclip(gamma * (1.0 + Ac * f1), 1e-20);
+  N eps_pw = -2.0 * A * (1.0 + a1 * rs) * dlog1p(1.0 / dclip(denom, 1e-20));
+  N denom_A = gamma * dexpm1(-eps_pw / gamma);
+  N Ac = beta / dclip(denom_A, 1e-20);
+  N t2 = t * t;
+  N f1 = t2 + Ac * t2 * t2;
+  N frac = (beta * f1) / dclip(gamma * (1.0 + Ac * f1), 1e-20);
   return eps_pw + gamma * dlog1p(frac);
 }
 
-__device__ D b88_x(D rho, D sig) {
+template <class N>
+__device__ N b88_x(N rho, N sig) {
   const double beta = 0.0042;
-  D gn = grad_norm_x(sig);
-  D r = dclip(rho * 0.5, 1e-20);
-  D r13 = dpow(r, 1.0 / 3.0);
+  N gn = grad_norm_x(sig);
+  N r = dclip(rho * 0.5, 1e-20);
+  N r13 = dpow(r, 1.0 / 3.0);
   const double C = 0.75 * 1.2407009817988000;  // (6/pi)^(1/3)
-  D x = (gn * 0.5) / dclip(dpow(r, 4.0 / 3.0), 1e-20);
-  D den = 1.0 + 6.0 * beta * x * dasinh(x);
+  N x = (gn * 0.5) / dclip(dpow(r, 4.0 / 3.0), 1e-20);
+  N den = 1.0 + 6.0 * beta * x * dasinh(x);
   return -C * r13 - beta * r13 * x * x / dclip(den, 1e-20);
 }
 
-__device__ D vwn_rpa_c(D rho) {
+template <class N>
+__device__ N vwn_rpa_c(N rho) {
   const double A = 0.0310907, b = 13.0720, c = 42.7198, x0 = -0.409286;
-  D r = dclip(rho, 1e-20);
-  D rs = dpow(3.0 / (4.0 * kPi * r), 1.0 / 3.0);
-  D x = dsqrt(rs);
-  D Xx = x * x + b * x + c;
+  N r = dclip(rho, 1e-20);
+  N rs = dpow(3.0 / (4.0 * kPi * r), 1.0 / 3.0);
+  N x = dsqrt(rs);
+  N Xx = x * x + b * x + c;
   double Xx0 = x0 * x0 + b * x0 + c;
   double Q = sqrt(4.0 * c - b * b);
-  D at = datan(Q / dclip(2.0 * x + b, 1e-20));
-  D term1 = dlog(x * x / dclip(Xx, 1e-20));
-  D term2 = (2.0 * b / Q) * at;
-  D xm = x - x0;
-  D term3 = -(b * x0 / Xx0) * (dlog(xm * xm / dclip(Xx, 1e-20)) + (2.0 * (b + 2.0 * x0) / Q) * at);
+  N at = datan(Q / dclip(2.0 * x + b, 1e-20));
+  N term1 = dlog(x * x / dclip(Xx, 1e-20));
+  N term2 = (2.0 * b / Q) * at;
+  N xm = x - x0;
+  N term3 = -(b * x0 / Xx0) * (dlog(xm * xm / dclip(Xx, 1e-20)) + (2.0 * (b + 2.0 * x0) / Q) * at);
   return A * (term1 + term2 + term3);
 }
 
-__device__ D lyp_c(D rho, D sig) {
+template <class N>
+__device__ N lyp_c(N rho, N sig) {
   const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
   const double C_F = 0.3 * 9.5707800006273;  // (3 pi^2)^(2/3)
-  D r = dclip(rho, 1e-20);
-  D ri3 = dpow(r, -1.0 / 3.0);
-  D g = 1.0 / (1.0 + d * ri3);
-  D om = dexp(-c * ri3) * g * dpow(r, -11.0 / 3.0);
-  D delta = c * ri3 + d * ri3 * g;
-  D term_ff = C_F * dpow(r, 8.0 / 3.0) - sig * (3.0 + 7.0 * delta) / 72.0;
+  N r = dclip(rho, 1e-20);
+  N ri3 = dpow(r, -1.0 / 3.0);
+  N g = 1.0 / (1.0 + d * ri3);
+  N om = dexp(-c * ri3) * g * dpow(r, -11.0 / 3.0);
+  N delta = c * ri3 + d * ri3 * g;
+  N term_ff = C_F * dpow(r, 8.0 / 3.0) - sig * (3.0 + 7.0 * delta) / 72.0;
   return -a * g - a * b * om * r * term_ff;
 }
 
-__device__ D eps_xc(int id, D rho, D sig) {
+template <class N>
+__device__ N eps_xc(int id, N rho, N sig) {
   switch (id) {
     case kXcPBE: return pbe_x(rho, sig) + pbe_c(rho, sig);
     case kXcPBE0: return 0.75 * pbe_x(rho, sig) + pbe_c(rho, sig);
@@ -380,6 +439,133 @@ hipError_t launch_xc_pointwise(int64_t n, int xc_id, const double* rho, const do
                                hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_xc_pointwise, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, xc_id, rho, sigma, out);
+  return hipGetLastError();
+}
+
+// f and its first and second partials at (rho, sigma) from a dual of duals:
+// o = {f, f_r, f_s, f_rr, f_rs, f_ss}. The outer dual differentiates the inner
+// one, so f.r = (f_r, f_rr, f_rs) and f.s = (f_s, f_sr, f_ss).
+__device__ __forceinline__ void xc_second(int xc_id, double r, double s, double* o) {
+  const D2 rho{D{r, 1.0, 0.0}, D{1.0, 0.0, 0.0}, D{0.0, 0.0, 0.0}};
+  const D2 sig{D{s, 0.0, 1.0}, D{0.0, 0.0, 0.0}, D{1.0, 0.0, 0.0}};
+  const D2 f = eps_xc(xc_id, rho, sig) * rho;
+  o[0] = f.v.v; o[1] = f.v.r; o[2] = f.v.s; o[3] = f.r.r; o[4] = f.r.s; o[5] = f.s.s;
+}
+
+// Pointwise second derivatives (parity tests): out[6 i ..] = xc_second.
+__global__ __launch_bounds__(128) void k_xc_pointwise2(int64_t n, int xc_id, const double* __restrict__ rho,
+                                const double* __restrict__ sigma, double* __restrict__ out) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double o[6];
+  xc_second(xc_id, rho[i], sigma[i], o);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) out[6 * i + k] = o[k];
+}
+
+hipError_t launch_xc_pointwise2(int64_t n, int xc_id, const double* rho, const double* sigma, double* out,
+                                hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xc_pointwise2, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, n, xc_id, rho, sigma, out);
+  return hipGetLastError();
+}
+
+// XC response at a reference density P0 (the jvp of Vxc along dP):
+//   rho1 = sum_mn phi_m dP_mn phi_n, grad rho1 = 2 sum phi_m dP_mn grad phi_n,
+//   sigma1 = 2 grad rho0 . grad rho1,
+//   W1_gn = w [1/2 (f_rr rho1 + f_rs sigma1) phi_gn
+//              + 2 ((f_sr rho1 + f_ss sigma1) grad rho0 + f_s grad rho1) . grad phi_gn],
+// V1 = Phi^T W1 + W1^T Phi: the derivative of the Vxc of dftax/ks/terms.py:791-828
+// along dP, i.e. the XC part of jax.jvp(jax.grad(E)) in dftax/ks/newton.py:134-136.
+//
+// Per-point record at P0 (8 doubles): w {f_rr, f_rs, f_ss, f_s}, grad rho0, 0;
+// all zero where rho0 <= thresh (the masked points carry no derivative).
+__global__ __launch_bounds__(128) void k_fxc_coef(int64_t ng, const double* __restrict__ rho0, const double* __restrict__ wts, int xc_id,
+                           double thresh, double* __restrict__ rec) {
+  int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const double r = rho0[4 * g], gx = rho0[4 * g + 1], gy = rho0[4 * g + 2], gz = rho0[4 * g + 3];
+  double c[8] = {};
+  if (r > thresh) {
+    double o[6];
+    xc_second(xc_id, r, gx * gx + gy * gy + gz * gz, o);
+    const double w = wts[g];
+    c[0] = w * o[3]; c[1] = w * o[4]; c[2] = w * o[5]; c[3] = w * o[2];
+    c[4] = gx; c[5] = gy; c[6] = gz;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) rec[8 * g + k] = c[k];
+}
+
+hipError_t launch_fxc_coef(int64_t ng, const double* rho0, const double* w, int xc_id, double thresh, double* rec,
+                           hipStream_t st) {
+  if (ng == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fxc_coef, dim3((unsigned)((ng + 127) / 128)), dim3(128), 0, st, ng, rho0, w, xc_id, thresh,
+                     rec);
+  return hipGetLastError();
+}
+
+// One wavefront per grid point: zw holds Z1 = Phi dP on entry and W1 on exit.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_fxc(int64_t ng, int64_t nao, const double* __restrict__ ao,
+                                             double* __restrict__ zw, const double* __restrict__ rec) {
+  const int lane = threadIdx.x & 63;
+  const int64_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ng) return;
+  const int64_t plane = ng * nao;
+  const double* p0 = ao + g * nao;
+  const double* p1 = p0 + plane;
+  const double* p2 = p0 + 2 * plane;
+  const double* p3 = p0 + 3 * plane;
+  double* zrow = zw + g * nao;
+  typedef double vec2 __attribute__((ext_vector_type(2)));
+  double r = 0, gx = 0, gy = 0, gz = 0;
+  if (VEC) {
+    for (int64_t n = lane; n < nao / 2; n += 64) {
+      const vec2 z = reinterpret_cast<const vec2*>(zrow)[n];
+      const vec2 a0 = reinterpret_cast<const vec2*>(p0)[n], a1 = reinterpret_cast<const vec2*>(p1)[n];
+      const vec2 a2 = reinterpret_cast<const vec2*>(p2)[n], a3 = reinterpret_cast<const vec2*>(p3)[n];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        r += z[j] * a0[j]; gx += z[j] * a1[j]; gy += z[j] * a2[j]; gz += z[j] * a3[j];
+      }
+    }
+  } else {
+    for (int64_t n = lane; n < nao; n += 64) {
+      const double z = zrow[n];
+      r += z * p0[n]; gx += z * p1[n]; gy += z * p2[n]; gz += z * p3[n];
+    }
+  }
+  r = wave_sum(r);
+  gx = 2.0 * wave_sum(gx);
+  gy = 2.0 * wave_sum(gy);
+  gz = 2.0 * wave_sum(gz);
+  const double* c = rec + 8 * g;
+  const double gx0 = c[4], gy0 = c[5], gz0 = c[6];
+  const double s1 = 2.0 * (gx0 * gx + gy0 * gy + gz0 * gz);
+  const double a = 0.5 * (c[0] * r + c[1] * s1);
+  const double t = 2.0 * (c[1] * r + c[2] * s1);
+  const double bx = t * gx0 + 2.0 * c[3] * gx, by = t * gy0 + 2.0 * c[3] * gy, bz = t * gz0 + 2.0 * c[3] * gz;
+  if (VEC) {
+    for (int64_t n = lane; n < nao / 2; n += 64) {
+      const vec2 a0 = reinterpret_cast<const vec2*>(p0)[n], a1 = reinterpret_cast<const vec2*>(p1)[n];
+      const vec2 a2 = reinterpret_cast<const vec2*>(p2)[n], a3 = reinterpret_cast<const vec2*>(p3)[n];
+      vec2 o;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) o[j] = a * a0[j] + bx * a1[j] + by * a2[j] + bz * a3[j];
+      reinterpret_cast<vec2*>(zrow)[n] = o;
+    }
+  } else {
+    for (int64_t n = lane; n < nao; n += 64) zrow[n] = a * p0[n] + bx * p1[n] + by * p2[n] + bz * p3[n];
+  }
+}
+
+hipError_t launch_fxc(int64_t ng, int64_t nao, const double* ao, double* zw, const double* rec, hipStream_t st) {
+  if (ng == 0) return hipSuccess;
+  const bool vec = nao % 2 == 0 && (uintptr_t)ao % 16 == 0 && (uintptr_t)zw % 16 == 0;
+  const dim3 grid((unsigned)((ng + 3) / 4)), blk(256);
+  if (vec) hipLaunchKernelGGL(k_fxc<true>, grid, blk, 0, st, ng, nao, ao, zw, rec);
+  else hipLaunchKernelGGL(k_fxc<false>, grid, blk, 0, st, ng, nao, ao, zw, rec);
   return hipGetLastError();
 }
 

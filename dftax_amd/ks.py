@@ -240,6 +240,43 @@ class HipDFKS:
                                               self._stream()))
         return self._E.clone(), F
 
+    def energy_fock_factored(self, Cf):
+        """(E components [8], F, P) at P = Cf Cf^T for a device factor Cf (nao x nf): the
+        exchange runs over the nf factor columns, with no eigen-factorisation of P."""
+        torch = _torch()
+        Cf = Cf.to(device=self.device, dtype=torch.float64)
+        P = (Cf @ Cf.T).contiguous()
+        cm = Cf.T.contiguous()                       # column-major nao x nf
+        nf = int(cm.shape[0])
+        signs = (ctypes.c_double * nf)(*([1.0] * nf))
+        F = torch.empty_like(P)
+        _native.check(self.L.dfks_energy_fock_factored(self.handle, ctypes.c_void_p(P.data_ptr()),
+                                                       ctypes.c_void_p(cm.data_ptr()), signs, nf,
+                                                       ctypes.c_void_p(self._E.data_ptr()),
+                                                       ctypes.c_void_p(F.data_ptr()), self._stream()))
+        return self._E.clone(), F, P
+
+    def response_begin(self, P) -> None:
+        """Fix the reference density P0 of :meth:`response_fock` (device nao x nao)."""
+        torch = _torch()
+        self._P0 = P.to(device=self.device, dtype=torch.float64).contiguous()
+        _native.check(self.L.dfks_response_begin(self.handle, ctypes.c_void_p(self._P0.data_ptr()),
+                                                 self._stream()))
+
+    def response_fock(self, U, V):
+        """R = J[dP] - a_x/2 K[dP] + f_xc(P0)[dP] for dP = U V^T + V U^T (U, V: nao x r),
+        the derivative of F = dE/dP at P0 along dP."""
+        torch = _torch()
+        Uc = U.to(device=self.device, dtype=torch.float64).T.contiguous()   # column-major nao x r
+        Vc = V.to(device=self.device, dtype=torch.float64).T.contiguous()
+        if Uc.shape != Vc.shape or Uc.shape[1] != self.nao:
+            raise ValueError(f"U and V must both be ({self.nao}, r); got {tuple(U.shape)} and {tuple(V.shape)}")
+        R = torch.empty((self.nao, self.nao), dtype=torch.float64, device=self.device)
+        _native.check(self.L.dfks_response_fock(self.handle, ctypes.c_void_p(Uc.data_ptr()),
+                                                ctypes.c_void_p(Vc.data_ptr()), int(Uc.shape[0]),
+                                                ctypes.c_void_p(R.data_ptr()), self._stream()))
+        return R
+
     def energy(self, P, S=None, nocc=None):
         """dftax's CoulombTerm contract: the J + K + XC energy of a spin-stacked P."""
         E, _ = self.energy_fock(P[0] if P.dim() == 3 else P)

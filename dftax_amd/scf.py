@@ -71,18 +71,28 @@ def _core_density(ks):
 
 def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, diis_space: int = 8,
         lindep_thresh: float = 1e-7, level_shift: float = 0.0, guess=None, verbose: bool = False,
-        precision: str | None = None, mixed_d_tol: float = 1e-4, mixed_max_iter: int = 30) -> KSResult:
+        precision: str | None = None, mixed_d_tol: float = 1e-4, mixed_max_iter: int = 30,
+        refine: int | None = None) -> KSResult:
     """Run the KS SCF to self-consistency (closed shell).
 
     ``precision="mixed"`` (needs a KS built with ``hip(precision="mixed")``)
     iterates with split-BF16 Fock builds until the orbital gradient
     ``|err| < mixed_d_tol`` (or ``mixed_max_iter`` mixed steps), then switches
-    to FP64: the Fock matrix is rebuilt in FP64 at that density, the DIIS
-    history restarts, and the FP64 steps run to the usual ``e_tol``/``d_tol``.
+    to FP64: the next step extrapolates over the mixed DIIS history and builds
+    its Fock matrix in FP64, the DIIS history restarts after it, and the FP64
+    steps run to the usual ``e_tol``/``d_tol``.
     The switch keys on the gradient only: the split-BF16 energy carries noise
     of order 1e-5 relative, so |dE| is not a usable test in that phase. The
     default is the precision the KS was built for, FP64 unless it was built
     mixed.
+
+    ``refine=n`` (mixed only, n >= 2) converges the energy only: the FP64
+    steps stop at the first one (from the second on) whose energy change is
+    below ``e_tol`` -- ``converged`` -- or after n of them. The energy is
+    second order in the density error, so it lands within ~1e-9 Ha of the
+    fully converged FP64 energy while the orbital gradient is left at the
+    size the split-BF16 Fock error imposes (~1e-4); use the default for
+    densities and orbitals converged to ``d_tol``.
     """
     torch = __import__("torch")
     L = ks.coulomb.L
@@ -93,6 +103,8 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
     mixed = precision == "mixed"
+    if refine is not None and (not mixed or int(refine) < 2):
+        raise ValueError("refine needs precision='mixed' and at least 2 FP64 steps")
     P0 = initial_density(ks, guess)
     ks.coulomb.set_precision(mixed)
     _native.check(L.dfks_scf_begin(h, ctypes.c_void_p(P0.data_ptr()), float(lindep_thresh), int(diis_space),
@@ -116,7 +128,15 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
                 ks.coulomb.set_precision(False)
                 mixed = False
         else:
-            converged = abs(de) < e_tol and derr < d_tol
+            n_fp64 = sum(1 for h in hist if h[3] == "fp64")
+            if refine is not None:
+                if n_fp64 >= 2 and abs(de) < e_tol:
+                    converged = True
+                if converged or n_fp64 >= int(refine):
+                    it += 1
+                    break
+            else:
+                converged = abs(de) < e_tol and derr < d_tol
         it += 1
     n = ks.nao
     P = torch.empty((n, n), dtype=torch.float64, device=ks.device)

@@ -137,6 +137,22 @@ int dfks_energy_fock_factored(dfks_handle h, const double* P, const double* Cf, 
 int dfks_xc_eval(int32_t xc_id, int64_t n, const double* rho, const double* sigma, double* out3,
                  dfks_stream stream);
 
+/* Pointwise second derivatives: out6[6 i ..] = {f, f_rho, f_sigma, f_rho rho,
+ * f_rho sigma, f_sigma sigma} (the XC kernel of the response below). */
+int dfks_xc_eval2(int32_t xc_id, int64_t n, const double* rho, const double* sigma, double* out6,
+                  dfks_stream stream);
+
+/* Response Fock matrix, the derivative of F = dE/dP at P0 along dP:
+ *   R = J[dP] - a_x/2 K[dP] + f_xc(P0)[dP],
+ * which replaces jax.jvp(jax.grad(E), (P0,), (dP,)) in the Hessian-vector
+ * products of dftax/ks/newton.py:134-136. dfks_response_begin fixes P0 (device
+ * nao x nao) and caches the functional's second derivatives on the grid;
+ * dfks_response_fock then takes dP = U V^T + V U^T (U, V device, column-major
+ * nao x r, 1 <= r <= nao; the orbital-rotation tangent of P = 2 C_o C_o^T has
+ * U = C_o, V = 2 dC_o) and writes R (device nao x nao). FP64 builds only. */
+int dfks_response_begin(dfks_handle h, const double* P0, dfks_stream stream);
+int dfks_response_fock(dfks_handle h, const double* U, const double* V, int32_t r, double* R, dfks_stream stream);
+
 /* Device SCF (dftax/ks/scf.py:279-391): canonical orthonormaliser of S with
  * eigenvalue cutoff lindep, Pulay DIIS of depth m, level shift. P0: device
  * nao x nao initial density. Synchronises the stream. */
@@ -152,8 +168,9 @@ int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* n
  * three-term split-BF16 MFMA products with FP32 accumulation for the
  * exchange and XC GEMMs; FP64 reductions, functional, J sum, DIIS, eigh).
  * Needs options.precision = 1 at create. Switching from mixed to FP64 inside
- * an SCF rebuilds the Fock matrix in FP64 at the current density before the
- * next step and restarts the DIIS history. */
+ * an SCF: the next step still extrapolates over the mixed DIIS history, then
+ * builds its Fock matrix in FP64; the DIIS history restarts after that step,
+ * so it holds FP64 Fock matrices only. */
 int dfks_set_precision(dfks_handle h, int32_t mixed);
 
 /* Stage timers (HIP events on the work stream). Stages of the last Fock

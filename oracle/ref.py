@@ -18,7 +18,9 @@ CPU FP64 restatement of dftax's density-fitted Kohn-Sham SCF iteration:
   from dftax/energy/xc.py:64-601 in torch float64, with the rho > 1e-10
   double-where mask of dftax/energy/grid.py:10-44;
 * the Fock matrix ``F = sym(dE/dP)`` by torch autograd, the same definition
-  as dftax/ks/scf.py:42-46 (plus an analytic J/K/Vxc form checked against it);
+  as dftax/ks/scf.py:42-46 (plus an analytic J/K/Vxc form checked against it),
+  and its derivative along dP (the response Fock matrix of the Newton
+  Hessian-vector product, dftax/ks/newton.py:134-136) by forward-over-reverse;
 * the SCF loop of dftax/ks/scf.py:252-391 (Pulay DIIS m=8, ridge 1e-10,
   canonical orthonormaliser, aufbau fill) and the minao guess of
   dftax/ks/guess.py:296-337, 529-545, 602-616.
@@ -404,6 +406,18 @@ class OracleKS:
         (g,) = t.autograd.grad(e, Pt)
         g = g.numpy()
         return float(e), 0.5 * (g + g.T), {k: float(v) for k, v in parts.items()}
+
+    def response_fock(self, P, dP):
+        """d/dt F(P + t dP) at t = 0, F = sym(dE/dP): forward-over-reverse through
+        energy_t, the quantity dftax/ks/newton.py:134-136 forms as jax.jvp(jax.grad(E))."""
+        t = _torch()
+
+        def fock(X):
+            return t.func.grad(lambda Y: self.energy_t(Y)[0])(X)
+
+        _, R = t.func.jvp(fock, (t.tensor(P, dtype=t.float64),), (t.tensor(dP, dtype=t.float64),))
+        R = R.numpy()
+        return 0.5 * (R + R.T)
 
     def energy_fock(self, P):
         """Analytic form: F = hcore + J - 1/2 a_x K + Vxc (checked against autograd)."""

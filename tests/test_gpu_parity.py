@@ -363,3 +363,151 @@ def test_recompute_mixed(dimer_recompute, torch_cuda):
     _say(f"materialised {rm.e_tot:.12f}; recompute mixed {rx.e_tot:.12f} ({rx.n_iter} it, {nmix} mixed)")
     assert rx.converged and nmix >= 1
     assert abs(rx.e_tot - rm.e_tot) < 1e-8
+    # mixed to the switch, then FP64 steps until the energy settles: within the 1e-6 Ha budget
+    rr = scf(r, precision="mixed", e_tol=1e-9, refine=3)
+    nfp = sum(1 for h in rr.history if h[3] == "fp64")
+    _say(f"mixed + {nfp} FP64 steps {rr.e_tot:.12f} ({rr.n_iter} it), |dE| {abs(rr.e_tot - rm.e_tot):.1e}")
+    assert rr.converged and 2 <= nfp <= 3 and abs(rr.e_tot - rm.e_tot) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# Response Fock matrix and the Newton solver (dftax/ks/newton.py)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("xc", ["PBE", "PBE0", "B3LYP"])
+def test_xc_second_derivatives(torch_cuda, xc):
+    """f_rr, f_rs, f_ss (dual of duals) against double autograd of the oracle's
+    functional written on sigma."""
+    import ctypes
+    from dftax_amd import _native as N
+    from dftax_amd.xc import resolve
+    from oracle import ref
+    torch = torch_cuda
+    rng = np.random.default_rng(11)
+    n = 3000
+    rho = 10.0 ** rng.uniform(-6, 2, size=n)
+    sigma = (rho ** (8 / 3)) * 10.0 ** rng.uniform(-3, 1, size=n)
+    r = torch.tensor(rho, device="cuda")
+    s = torch.tensor(sigma, device="cuda")
+    out = torch.empty((n, 6), dtype=torch.float64, device="cuda")
+    N.check(N.lib().dfks_xc_eval2(resolve(xc).xc_id, n, ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(s.data_ptr()),
+                                  ctypes.c_void_p(out.data_ptr()), None))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    rt = torch.tensor(rho, requires_grad=True)
+    st = torch.tensor(sigma, requires_grad=True)
+    grad = torch.stack([torch.sqrt(st), torch.zeros_like(st), torch.zeros_like(st)], dim=1)
+    f = ref.eps_xc(xc, rt, grad) * rt
+    fr, fs = torch.autograd.grad(f.sum(), (rt, st), create_graph=True)
+    frr, frs = torch.autograd.grad(fr.sum(), (rt, st), retain_graph=True)
+    fsr, fss = torch.autograd.grad(fs.sum(), (rt, st))
+    want = [f.detach().numpy(), fr.detach().numpy(), fs.detach().numpy(), frr.numpy(), frs.numpy(), fss.numpy()]
+    assert np.abs(frs.numpy() - fsr.numpy()).max() <= 1e-9 * np.abs(frs.numpy()).max()
+    # value to 1e-12; derivatives to 1e-7 / 1e-6 relative: in PBE the exchange and
+    # correlation gradient terms cancel at small reduced gradient (mu = beta pi^2 / 3),
+    # so the two FP64 evaluation orders differ by ~2e-8 of f_sigma there
+    for col, (b, tol) in enumerate(zip(want, (1e-12, 1e-7, 1e-7, 1e-6, 1e-6, 1e-6))):
+        a = got[:, col]
+        err = np.abs(a - b) / np.maximum(np.abs(b), 1e-12 * np.abs(b).max())
+        i = int(np.argmax(err))
+        _say(f"{xc} col {col}: max rel err {err[i]:.2e}")
+        assert err[i] <= tol, (col, i, rho[i], sigma[i], a[i], b[i])
+
+
+def _uv(n, r, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((n, r)), rng.standard_normal((n, r))
+
+
+@pytest.mark.parametrize("case", ["water_pair", "dimer_pair"])
+def test_response_fock_matches_oracle(case, request, torch_cuda):
+    """R = J[dP] - a_x/2 K[dP] + f_xc(P)[dP] against the oracle's forward-over-reverse
+    derivative of F = sym(dE/dP), at the minao density, dP = U V^T + V U^T."""
+    torch = torch_cuda
+    o, k = request.getfixturevalue(case)
+    P = _p0(k)
+    U, V = _uv(k.nao, 4, 5)
+    R_ref = o.response_fock(P, U @ V.T + V @ U.T)
+    k.coulomb.response_begin(torch.tensor(P, device="cuda"))
+    R = k.coulomb.response_fock(torch.tensor(U, device="cuda"), torch.tensor(V, device="cuda")).cpu().numpy()
+    err = np.abs(R - R_ref).max()
+    _say(f"{case}: response max|dR| {err:.2e} (max|R| {np.abs(R_ref).max():.2e})")
+    assert err < 1e-9 * max(1.0, np.abs(R_ref).max())
+
+
+def test_response_fock_recompute(dimer_recompute, torch_cuda):
+    """Recomputed slabs + streamed grid blocks give the materialised response."""
+    torch = torch_cuda
+    m, r = dimer_recompute["mat"], dimer_recompute["rec"]
+    P = torch.tensor(_p0(m), device="cuda")
+    U, V = (torch.tensor(a, device="cuda") for a in _uv(m.nao, 7, 6))
+    out = []
+    for ks in (m, r):
+        ks.coulomb.set_precision(False)
+        ks.coulomb.response_begin(P)
+        out.append(ks.coulomb.response_fock(U, V).cpu().numpy())
+    err = np.abs(out[0] - out[1]).max()
+    _say(f"recompute response max|dR| {err:.2e}")
+    assert err < 1e-10 * max(1.0, np.abs(out[0]).max())
+
+
+def test_newton_hessian_quadratic_form(water_pair, torch_cuda):
+    """<v, H v> from the response against the second difference of E(C exp(K(v)));
+    H symmetric: <w, H v> = <v, H w>."""
+    import math
+    from dftax_amd import scf
+    from dftax_amd.newton import _rotate
+    torch = torch_cuda
+    _, k = water_pair
+    res = scf(k, e_tol=1e-10)
+    C = res.mo_coeff[0]
+    no = k.nocc[0]
+    c = k.coulomb
+    E0, F, P = c.energy_fock_factored(math.sqrt(2.0) * C[:, :no])
+    Ft = C.T @ F @ C
+    Co, Cv = C[:, :no], C[:, no:]
+    c.response_begin(P)
+
+    def hvp(v):
+        R = c.response_fock(Co, -2.0 * (Cv @ v.T))
+        return 4.0 * (v @ Ft[no:, no:] - Ft[:no, :no] @ v) - 4.0 * (Co.T @ R @ Cv)
+
+    rng = np.random.default_rng(2)
+    v = torch.tensor(rng.standard_normal((no, C.shape[1] - no)), device="cuda")
+    v = v / torch.linalg.norm(v)
+    w = torch.tensor(rng.standard_normal((no, C.shape[1] - no)), device="cuda")
+    Hv, Hw = hvp(v), hvp(w)
+    a, b = float(torch.sum(w * Hv)), float(torch.sum(v * Hw))
+    assert abs(a - b) < 1e-9 * max(1.0, abs(a))
+
+    def energy(kappa):
+        Cr = _rotate(C, kappa, no)
+        E, _, _ = c.energy_fock_factored(math.sqrt(2.0) * Cr[:, :no])
+        return float(E[0])
+
+    eps = 1e-3
+    quad = (energy(eps * v) - 2.0 * float(E0[0]) + energy(-eps * v)) / eps ** 2
+    vHv = float(torch.sum(v * Hv))
+    _say(f"<v,Hv> = {vHv:.8f}, second difference {quad:.8f}")
+    # |eps v| = 1e-3: quartic terms ~1e-6 relative, rounding of E ~1e-7 relative
+    assert abs(quad - vHv) < 1e-4 * abs(vHv)
+
+
+@pytest.mark.parametrize("case", ["water_pair", "dimer_pair"])
+def test_newton_matches_scf(case, request, torch_cuda):
+    """Trust-region Newton lands on the DIIS SCF minimum and on the oracle's energy."""
+    from dftax_amd import scf
+    from dftax_amd.newton import newton
+    from oracle import ref
+    o, k = request.getfixturevalue(case)
+    rs = scf(k, e_tol=1e-10)
+    rn = newton(k)
+    ro = ref.scf(o, e_tol=1e-10, d_tol=1e-7)
+    nh = sum(h[5] for h in rn.history)
+    _say(f"{case}: newton {rn.e_tot:.12f} ({rn.n_iter} it, {nh} Hessian products); scf {rs.e_tot:.12f}; "
+          f"oracle {ro['e_tot']:.12f}")
+    assert rn.converged
+    assert abs(rn.e_tot - rs.e_tot) < 1e-8
+    assert abs(rn.e_tot - ro["e_tot"]) < TOL_E
+    Pn = rn.P[0].cpu().numpy()
+    assert np.abs(Pn - rs.P[0].cpu().numpy()).max() < 1e-5

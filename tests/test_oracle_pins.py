@@ -367,3 +367,30 @@ def test_scf_invariants(water_oks):
     P2 = 2 * C2[:, :nocc] @ C2[:, :nocc].T
     e2, _, _ = water_oks.energy_fock(P2)
     assert e2 > e0 - 1e-10
+
+
+def test_response_fock_pins(water_oks):
+    """The oracle's response Fock R = dF/dP . dP: its J and K parts equal J(dP) and
+    K(dP) (both linear), and the whole matches a central difference of the
+    analytic Fock matrix (the XC second derivative is the only nonlinear part)."""
+    rng = np.random.default_rng(3)
+    P = _guess(water_oks)
+    n = P.shape[0]
+    U, V = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    dP = U @ V.T + V @ U.T
+    R = water_oks.response_fock(P, dP)
+    assert np.abs(R - R.T).max() < 1e-12
+    # the difference error falls as h^2 (1.8e-4, 1.8e-6, 1.8e-8 at h = 1e-3, 1e-4, 1e-5)
+    h = 1e-5
+    _, Fp, _ = water_oks.energy_fock(P + h * dP)
+    _, Fm, _ = water_oks.energy_fock(P - h * dP)
+    fd = (Fp - Fm) / (2 * h)
+    assert np.abs(R - fd).max() < 1e-8 * max(1.0, np.abs(fd).max())
+    # linear parts: J(dP) - a_x/2 K(dP) is what remains after removing the XC response
+    _, _, parts = water_oks.energy_fock(dP)
+    _, _, p_p = water_oks.energy_fock(P + h * dP)
+    _, _, p_m = water_oks.energy_fock(P - h * dP)
+    vxc_fd = (p_p["Vxc"] - p_m["Vxc"]) / (2 * h)
+    lin = parts["J"] - 0.5 * water_oks.hf * parts["K"]
+    assert np.abs(R - vxc_fd - lin).max() < 1e-8 * max(1.0, np.abs(fd).max())
+    assert np.abs(lin).max() > 1e-2 and np.abs(vxc_fd).max() > 1e-2
