@@ -149,7 +149,7 @@ def main() -> int:
         note(f"mixed SCF + FP64 energy refinement: {rrf.n_iter} iterations, {trf:.1f} s")
 
     nao, nfit, nocc, ng = c.nao, c.nfit, c.nocc, c.ngrid
-    recompute = c.plan.aux_chunk > 0 or c.plan.ao_block > 0
+    recompute = c.plan.aux_chunk > 0 or c.stats()["ao_block"] > 0
     flops = {
         "exchange_gemm": 2.0 * nao * nao * nfit * nocc,
         # the symmetric product computes the upper tiles only: nao (nao+1) / 2 dot products
@@ -157,20 +157,27 @@ def main() -> int:
         "xc_gemm": 2.0 * ng * nao * nao,
         "xc_syr2k": 2.0 * ng * nao * nao,
     }
+    compact = c.stats().get("xc_blocks", 0) > 0
+    if compact:
+        # compact XC grid: Z, the pointwise kernel and the per-block products share one stage
+        # (xc_kernel) and the row assembly is xc_syr2k; no dense GEMM counts apply
+        flops.pop("xc_gemm")
+        flops.pop("xc_syr2k")
     if recompute:
-        # stages hold integral recomputation and streamed AO work: only K keeps a flop count
+        # stages hold integral recomputation: only K keeps a flop count
         flops = {"exchange_syrk": flops["exchange_syrk"]}
 
     def stage_table(avg, elem_bytes, peak_tf):
-        bytes_ = {} if recompute else {"coulomb_j": elem_bytes * nao * nao * nfit,
-                                       "xc_kernel": elem_bytes * ng * nao * 6}
+        bytes_ = {} if recompute else {"coulomb_j": elem_bytes * nao * nao * nfit}
+        if not recompute and not compact:
+            bytes_["xc_kernel"] = elem_bytes * ng * nao * 6
         tab = {}
         for k, ms in avg.items():
             ent = {"ms": round(ms, 4)}
-            if flops.get(k):
+            if flops.get(k) and ms > 0:
                 a = flops[k] / (ms * 1e-3) / 1e12
                 ent.update(bound="mfma", achieved_tflops=round(a, 2), frac=round(a / peak_tf, 3))
-            if k in bytes_:
+            if k in bytes_ and ms > 0:
                 a = bytes_[k] / (ms * 1e-3) / 1e9
                 ent.update(bound="hbm", achieved_gbs=round(a, 1), frac=round(a / PEAK_HBM_GBS, 3))
             tab[k] = ent
@@ -178,7 +185,7 @@ def main() -> int:
 
     stages = stage_table(stage_avg, 8.0, PEAK_FP64_TFLOPS)
     stages_mx = stage_table(stage_mx, 4.0, PEAK_BF16X3_TFLOPS)
-    dom = max((k for k in stage_avg if k in flops), key=lambda k: stage_avg[k])
+    dom = max((k for k in stage_avg if k in flops and stage_avg[k] > 0), key=lambda k: stage_avg[k])
     ach = flops[dom] / (stage_avg[dom] * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": measured_traffic(args.workload, dom),

@@ -34,7 +34,7 @@ class SystemDesc(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("precision", c_int32), ("metric_cutoff", c_double), ("rho_thresh", c_double),
                 ("keep_int3c", c_int32), ("gemm", c_int32), ("density", c_int32), ("aux_chunk", c_int32),
-                ("screen", c_double), ("ao_block", c_int64)]
+                ("screen", c_double), ("ao_block", c_int64), ("xc_blocks", c_int32)]
 
 
 DFKS_S, DFKS_HCORE, DFKS_KINETIC, DFKS_NUCLEAR, DFKS_INT2C, DFKS_INT3C, DFKS_BFIT, DFKS_AO, DFKS_GRID_W = range(9)
@@ -156,7 +156,8 @@ def stage_times(handle) -> dict:
     return {k: float(v) for k, v in zip(STAGES, out)}
 
 
-STATS = ["pairs_kept", "pairs_total", "aux_slabs", "max_slab", "ao_block", "eigh_steps", "sp2_iterations"]
+STATS = ["pairs_kept", "pairs_total", "aux_slabs", "max_slab", "ao_block", "eigh_steps", "sp2_iterations",
+         "xc_blocks", "xc_bins", "xc_width_sum"]
 
 
 def stats(handle) -> dict:

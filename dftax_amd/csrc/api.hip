@@ -45,6 +45,14 @@ hipError_t launch_int1e(const ShellSet& A, int natm, const double* atm, const do
 hipError_t launch_becke(int64_t n, const double* pts, const double* raw, const int32_t* owner, int natm,
                         const double* atm, const double* invR, const double* aAB, double* w, hipStream_t st);
 hipError_t launch_ao(int64_t ng, const double* pts, const ShellSet& S, int64_t nao, double* ao, hipStream_t st);
+hipError_t launch_ao_cmp(int nblk, int G, int n, const double* pts, const int2* shl, const int64_t* off,
+                         const ShellSet& S, double* ao, hipStream_t st);
+template <typename T>
+hipError_t launch_gather_sym(int64_t nblk, int n, const int32_t* cols, const double* P, int64_t nao, T* out,
+                             hipStream_t st);
+template <typename T>
+hipError_t launch_vassemble(int nao, const int64_t* rowptr, const int64_t* prow, const int64_t* pcol,
+                            const int32_t* plen, const T* q, const int32_t* cols, double* V, hipStream_t st);
 hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w, double* zw, int xc_id, double thresh,
                      double* exc, double* rho_out, hipStream_t st);
 hipError_t launch_xc_pointwise(int64_t n, int xc_id, const double* rho, const double* sigma, double* out,
@@ -348,6 +356,22 @@ struct dfks_ctx {
   // purification: state, Gershgorin bounds, residual diagonal, occupied factor, pivots
   double *d_sp2 = nullptr, *d_lohi = nullptr, *d_dvec = nullptr, *d_L = nullptr;
   int32_t* d_piv = nullptr;
+  // compact XC grid: bins of blocks of equal padded width (see plan_xc_blocks)
+  struct XcBin {
+    int n;                 // padded compact width (multiple of 32)
+    int64_t nblk, blk0;    // blocks in the bin, first block (bin order)
+    int64_t pt0;           // first padded point of the bin
+    int64_t ao_off, q_off, col_off;
+  };
+  bool xcb = false;
+  std::vector<XcBin> xbins;
+  int64_t xc_npts = 0, xc_zw = 0, xc_nblk = 0;
+  double *d_xc_pts = nullptr, *d_xc_w = nullptr, *d_xc_exc = nullptr, *d_xc_ao = nullptr, *d_xc_zw = nullptr;
+  double* d_xc_q = nullptr;
+  float* d_xc_ao32 = nullptr;
+  int32_t *d_xc_cols = nullptr, *d_xc_vlen = nullptr;
+  int2* d_xc_shl = nullptr;
+  int64_t *d_xc_shoff = nullptr, *d_xc_vrow = nullptr, *d_xc_vprow = nullptr, *d_xc_vpcol = nullptr;
   // response (jvp of the Fock matrix): per-point XC records at the reference
   // density (8 x ng), rho0 and grad rho0 (4 x ng), and the signed factor of dP
   double *d_fxc = nullptr, *d_rho0 = nullptr, *d_Rf = nullptr;
@@ -389,7 +413,9 @@ struct dfks_ctx {
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
                     d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws,
                     d_sp2, d_lohi, d_dvec, d_L, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
-                    d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf};
+                    d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf, d_xc_pts, d_xc_w, d_xc_exc, d_xc_ao,
+                    d_xc_zw, d_xc_q, d_xc_ao32, d_xc_cols, d_xc_vlen, d_xc_shl, d_xc_shoff, d_xc_vrow,
+                    d_xc_vprow, d_xc_vpcol};
     for (void* p : ptrs) dfree(p);
     void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, t32_alias ? nullptr : d_T32, d_Cf32, d_P32, d_ws32,
                       d_eps32, d_offd32, d_slab32, d_Lt32, d_Tslab32, d_aob32, d_zwb32};
@@ -519,6 +545,207 @@ void setup_slabs(dfks_ctx* h, hipStream_t st) {
   h->d_Tslab = dalloc<double>((size_t)nao * h->max_np * h->nf_chunk);
   h->d_Ttil = dalloc<double>((size_t)nao * naux * h->nf_chunk);
   sync(st);
+}
+
+// ---------------------------------------------------------------------------
+// Compact XC grid
+// ---------------------------------------------------------------------------
+constexpr int kXcG = 128;          // points per block
+constexpr double kXcTau = 1e-14;   // AO value / gradient bound below which a shell is dropped in a block
+
+// max over r >= d of r^p exp(-a r^2)
+double tail_max(int p, double d, double a) {
+  if (p == 0) return std::exp(-a * d * d);
+  const double rs = std::sqrt(p / (2.0 * a));
+  const double r = std::max(d, rs);
+  return std::pow(r, p) * std::exp(-a * r * r);
+}
+
+// Bound on |phi| and each |d phi / dx| of a shell's functions at distance >= d
+// from its centre: sum_k |c_k| (r^l + l r^(l-1) + 2 a_k r^(l+1)) e^(-a_k r^2),
+// times 4 for the cart->sph combination.
+double shell_bound(const Basis& B, int s, double d) {
+  const int l = B.l[s];
+  double acc = 0.0;
+  for (int k = B.prim0[s]; k < B.prim0[s] + B.nprim[s]; ++k) {
+    const double a = B.exps[k], c = std::fabs(B.coefs[k]);
+    double t = tail_max(l, d, a) + 2.0 * a * tail_max(l + 1, d, a);
+    if (l > 0) t += l * tail_max(l - 1, d, a);
+    acc += c * t;
+  }
+  return 4.0 * acc;
+}
+
+// Radius beyond which the shell's bound stays below kXcTau (-1: nowhere above).
+double shell_radius(const Basis& B, int s) {
+  if (shell_bound(B, s, 0.0) < kXcTau) return -1.0;
+  double lo = 0.0, hi = 1.0;
+  while (shell_bound(B, s, hi) >= kXcTau && hi < 1e4) hi *= 2.0;
+  for (int it = 0; it < 60; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    (shell_bound(B, s, mid) >= kXcTau ? lo : hi) = mid;
+  }
+  return hi;
+}
+
+uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
+  v &= 0x3ff;
+  v = (v | (v << 16)) & 0x030000ff;
+  v = (v | (v << 8)) & 0x0300f00f;
+  v = (v | (v << 4)) & 0x030c30c3;
+  v = (v | (v << 2)) & 0x09249249;
+  return v;
+}
+
+// Blocks of kXcG grid points in Morton order; per block the shells whose bound
+// reaches kXcTau anywhere in the block's bounding sphere; blocks binned by the
+// padded count of their functions (multiples of 32). Uploads the binned points,
+// weights, column maps, shell lists and the row lists of the Vxc assembly, and
+// evaluates the compact AO values and gradients (resident).
+void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
+  const int64_t ng = h->ng;
+  const int nao = h->nao;
+  const Basis& O = h->orb;
+  const int nsh = O.n();
+  const double* X = h->gxyz_host.data();
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t g = 0; g < ng; ++g)
+    for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], X[3 * g + d]); hi[d] = std::max(hi[d], X[3 * g + d]); }
+  std::vector<std::pair<uint32_t, int64_t>> key((size_t)ng);
+  for (int64_t g = 0; g < ng; ++g) {
+    uint32_t c = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double t = (X[3 * g + d] - lo[d]) / (hi[d] - lo[d] + 1e-12);
+      c |= spread3((uint32_t)std::min(1023.0, std::max(0.0, t * 1023.0))) << d;
+    }
+    key[g] = {c, g};
+  }
+  std::sort(key.begin(), key.end());
+  const int64_t nblk = (ng + kXcG - 1) / kXcG;
+  std::vector<double> rcut(nsh);
+  for (int s = 0; s < nsh; ++s) rcut[s] = shell_radius(O, s);
+  std::vector<std::vector<int>> bsh((size_t)nblk);
+  std::vector<int> npad((size_t)nblk);
+  for (int64_t b = 0; b < nblk; ++b) {
+    const int64_t p0 = b * kXcG, p1 = std::min(ng, p0 + kXcG);
+    double c[3] = {0, 0, 0};
+    for (int64_t p = p0; p < p1; ++p)
+      for (int d = 0; d < 3; ++d) c[d] += X[3 * key[p].second + d];
+    for (int d = 0; d < 3; ++d) c[d] /= (double)(p1 - p0);
+    double rad = 0.0;
+    for (int64_t p = p0; p < p1; ++p) {
+      const double* x = X + 3 * key[p].second;
+      rad = std::max(rad, std::sqrt((x[0] - c[0]) * (x[0] - c[0]) + (x[1] - c[1]) * (x[1] - c[1]) +
+                                    (x[2] - c[2]) * (x[2] - c[2])));
+    }
+    int nb = 0;
+    for (int s = 0; s < nsh; ++s) {
+      if (rcut[s] < 0.0) continue;
+      const double* A = &O.centre[3 * s];
+      const double dist = std::sqrt((A[0] - c[0]) * (A[0] - c[0]) + (A[1] - c[1]) * (A[1] - c[1]) +
+                                    (A[2] - c[2]) * (A[2] - c[2]));
+      if (dist - rad <= rcut[s]) {
+        bsh[b].push_back(s);
+        nb += nfun(O.l[s], O.spherical);
+      }
+    }
+    npad[b] = std::max(32, (nb + 31) / 32 * 32);
+  }
+  // bins of equal padded width, blocks ordered by (width, position)
+  std::vector<int64_t> order((size_t)nblk);
+  for (int64_t b = 0; b < nblk; ++b) order[b] = b;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return npad[a] < npad[b]; });
+  h->xbins.clear();
+  int64_t ao_tot = 0, q_tot = 0, col_tot = 0, zw_max = 0;
+  for (int64_t i = 0; i < nblk;) {
+    int64_t j = i;
+    while (j < nblk && npad[order[j]] == npad[order[i]]) ++j;
+    dfks_ctx::XcBin bn{npad[order[i]], j - i, i, i * kXcG, ao_tot, q_tot, col_tot};
+    ao_tot += 4 * bn.nblk * kXcG * bn.n;
+    q_tot += bn.nblk * (int64_t)bn.n * bn.n;
+    col_tot += bn.nblk * bn.n;
+    zw_max = std::max(zw_max, bn.nblk * kXcG * bn.n);
+    h->xbins.push_back(bn);
+    i = j;
+  }
+  const int64_t npts = nblk * kXcG;
+  std::vector<double> pts((size_t)(3 * npts)), w((size_t)npts, 0.0);
+  std::vector<int32_t> cols((size_t)col_tot, -1);
+  std::vector<int64_t> shoff((size_t)nblk + 1, 0);
+  std::vector<int2> shl;
+  std::vector<std::vector<std::pair<int64_t, int64_t>>> rows((size_t)nao);  // (q offset of the row, col offset)
+  std::vector<std::vector<int32_t>> rlen((size_t)nao);
+  for (const auto& bn : h->xbins) {
+    for (int64_t k = 0; k < bn.nblk; ++k) {
+      const int64_t ob = bn.blk0 + k;             // block in bin order
+      const int64_t b = order[ob];                // Morton block
+      const int64_t p0 = b * kXcG, p1 = std::min(ng, p0 + kXcG);
+      for (int64_t p = 0; p < kXcG; ++p) {
+        const int64_t src = key[std::min(p0 + p, p1 - 1)].second;
+        for (int d = 0; d < 3; ++d) pts[3 * (ob * kXcG + p) + d] = X[3 * src + d];
+        w[ob * kXcG + p] = p0 + p < p1 ? h->gw_host[src] : 0.0;
+      }
+      int col = 0;
+      int32_t* cb = cols.data() + bn.col_off + k * bn.n;
+      for (int s : bsh[b]) {
+        shl.push_back(make_int2(s, col));
+        const int nf = nfun(O.l[s], O.spherical);
+        for (int f = 0; f < nf; ++f) cb[col + f] = O.ao0[s] + f;
+        col += nf;
+      }
+      shoff[ob + 1] = (int64_t)shl.size();
+      const int64_t qb = bn.q_off + k * (int64_t)bn.n * bn.n, cbo = bn.col_off + k * bn.n;
+      for (int i = 0; i < col; ++i) {
+        rows[cb[i]].push_back({qb + (int64_t)i * bn.n, cbo});
+        rlen[cb[i]].push_back(bn.n);
+      }
+    }
+  }
+  std::vector<int64_t> vrow((size_t)nao + 1, 0), vprow, vpcol;
+  std::vector<int32_t> vlen;
+  for (int m = 0; m < nao; ++m) {
+    for (size_t e = 0; e < rows[m].size(); ++e) {
+      vprow.push_back(rows[m][e].first);
+      vpcol.push_back(rows[m][e].second);
+      vlen.push_back(rlen[m][e]);
+    }
+    vrow[m + 1] = (int64_t)vprow.size();
+  }
+  if (vprow.empty()) { vprow.push_back(0); vpcol.push_back(0); vlen.push_back(0); }
+  auto up = [&](auto& v) {
+    using T = typename std::decay_t<decltype(v)>::value_type;
+    T* p = dalloc<T>(v.size());
+    HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return p;
+  };
+  h->d_xc_pts = up(pts);
+  h->d_xc_w = up(w);
+  h->d_xc_cols = up(cols);
+  h->d_xc_shoff = up(shoff);
+  if (shl.empty()) shl.push_back(make_int2(0, 0));
+  h->d_xc_shl = up(shl);
+  h->d_xc_vrow = up(vrow);
+  h->d_xc_vprow = up(vprow);
+  h->d_xc_vpcol = up(vpcol);
+  h->d_xc_vlen = up(vlen);
+  h->xc_npts = npts;
+  h->xc_nblk = nblk;
+  h->xc_zw = zw_max;
+  h->d_xc_exc = dalloc<double>(npts);
+  h->d_xc_zw = dalloc<double>(zw_max);
+  h->d_xc_q = dalloc<double>(q_tot);
+  h->d_xc_ao = dalloc<double>(ao_tot);
+  HIPCHK(hipMemsetAsync(h->d_xc_ao, 0, sizeof(double) * ao_tot, st));
+  const ShellSet S = O.view();
+  for (const auto& bn : h->xbins)
+    HIPCHK(launch_ao_cmp((int)bn.nblk, kXcG, bn.n, h->d_xc_pts + 3 * bn.pt0, h->d_xc_shl, h->d_xc_shoff + bn.blk0, S,
+                         h->d_xc_ao + bn.ao_off, st));
+  if (h->opt.precision == 1) {
+    h->d_xc_ao32 = dalloc<float>(ao_tot);
+    HIPCHK(launch_to_f32(ao_tot, h->d_xc_ao, h->d_xc_ao32, st));
+  }
+  sync(st);  // the uploads read host vectors that go out of scope here
+  h->xcb = true;
 }
 
 void build_all(dfks_ctx* h, hipStream_t st) {
@@ -658,8 +885,11 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   t0 = now_ms();
   dfree(d_U); dfree(d_w); dfree(d_e);
 
-  // AO values and gradients on the grid: resident, or per block in every Fock build
-  if (h->ng > 0) {
+  // AO values and gradients on the grid: compact blocks, resident, or per block in every Fock build
+  if (h->ng > 0 && h->opt.xc_blocks) {
+    plan_xc_blocks(h, st);
+    h->ao_block = 0;
+  } else if (h->ng > 0) {
     h->d_exc = dalloc<double>(h->ng);
     if (h->ao_block == 0) {
       h->d_ao = dalloc<double>((size_t)4 * h->ng * nao);
@@ -701,7 +931,9 @@ void build_all(dfks_ctx* h, hipStream_t st) {
       h->d_T32 = reinterpret_cast<float*>(h->d_Tx);
       h->t32_alias = true;
     }
-    if (h->ng > 0 && h->ao_block == 0) {
+    if (h->xcb) {
+      // compact AO rows were converted in plan_xc_blocks
+    } else if (h->ng > 0 && h->ao_block == 0) {
       h->d_ao32 = dalloc<float>((size_t)4 * h->ng * nao);
       HIPCHK(launch_to_f32((int64_t)4 * h->ng * nao, h->d_ao, h->d_ao32, st));
       h->d_zw32 = dalloc<float>((size_t)h->ng * nao);
@@ -888,6 +1120,77 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
   h->mark(kStCoulomb + 1, st);
 }
 
+// XC on the compact grid, bin by bin: Pg = P gathered to each block's columns,
+// Z_b = Phi_b Pg_b (batched), the pointwise kernel on the compact rows, then
+// prod_b = Phi_b^T W_b (batched, over Pg's storage) and the row assembly of
+// Vh = sum_b prod_b scattered to the blocks' columns.
+//   kXcEnergy:   k_xc, E_xc into E + DFKS_E_XC (FP64 or mixed)
+//   kXcRho:      rho0 and grad rho0 per padded point into d_rho0 (FP64, no Vh)
+//   kXcResponse: Z1 = Phi dP, the response kernel with the records in d_fxc (FP64)
+enum XcMode { kXcEnergy = 0, kXcRho = 1, kXcResponse = 2 };
+
+void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t st) {
+  const int nao = h->nao;
+  const bool mixed = h->mixed_now && mode == kXcEnergy;
+  const double one = 1.0, zero = 0.0;
+  const double thresh = h->opt.rho_thresh;
+  for (const auto& bn : h->xbins) {
+    const int n = bn.n;
+    const int64_t nb = bn.nblk, rows = nb * kXcG;
+    const int32_t* cols = h->d_xc_cols + bn.col_off;
+    if (mixed) {
+      float* q = reinterpret_cast<float*>(h->d_xc_q) + bn.q_off;
+      const float* ao = h->d_xc_ao32 + bn.ao_off;
+      float* zw = reinterpret_cast<float*>(h->d_xc_zw);
+      HIPCHK(launch_gather_sym<float>(nb, n, cols, P, nao, q, st));
+      for (int64_t c0 = 0; c0 < nb; c0 += 65535) {
+        const int cnt = (int)std::min<int64_t>(65535, nb - c0);
+        const GemmArgs gz{kXcG, n, n, ao + c0 * kXcG * n, n, true, q + c0 * n * n, n, true};
+        HIPCHK(launch_gemm_f32out_batched(gz, zw + c0 * kXcG * n, cnt, (int64_t)kXcG * n, (int64_t)n * n,
+                                          (int64_t)kXcG * n, st));
+      }
+      HIPCHK(launch_xc_f32(rows, n, ao, h->d_xc_w + bn.pt0, zw, h->xc_id, thresh, h->d_xc_exc + bn.pt0, st));
+      for (int64_t c0 = 0; c0 < nb; c0 += 65535) {
+        const int cnt = (int)std::min<int64_t>(65535, nb - c0);
+        const GemmArgs gv{n, n, kXcG, ao + c0 * kXcG * n, n, false, zw + c0 * kXcG * n, n, false};
+        HIPCHK(launch_gemm_f32out_batched(gv, q + c0 * n * n, cnt, (int64_t)kXcG * n, (int64_t)kXcG * n,
+                                          (int64_t)n * n, st));
+      }
+      continue;
+    }
+    double* q = h->d_xc_q + bn.q_off;
+    const double* ao = h->d_xc_ao + bn.ao_off;
+    double* zw = h->d_xc_zw;
+    HIPCHK(launch_gather_sym<double>(nb, n, cols, P, nao, q, st));
+    // column-major: Z_b^T (n x G) = Pg_b^T (n x n) Phi_b^T (n x G)
+    RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_transpose, rocblas_operation_none, n, kXcG, n, &one,
+                                        q, n, (rocblas_stride)n * n, ao, n, (rocblas_stride)kXcG * n, &zero, zw, n,
+                                        (rocblas_stride)kXcG * n, (int)nb));
+    if (mode == kXcEnergy)
+      HIPCHK(launch_xc(rows, n, ao, h->d_xc_w + bn.pt0, zw, h->xc_id, thresh, h->d_xc_exc + bn.pt0, nullptr, st));
+    else if (mode == kXcRho)
+      HIPCHK(launch_xc(rows, n, ao, h->d_xc_w + bn.pt0, zw, h->xc_id, thresh, h->d_xc_exc + bn.pt0,
+                       h->d_rho0 + 4 * bn.pt0, st));
+    else
+      HIPCHK(launch_fxc(rows, n, ao, zw, h->d_fxc + 8 * bn.pt0, st));
+    if (mode == kXcRho) continue;
+    // column-major: prod_b^T (n x n) = W_b^T (n x G) Phi_b (G x n), i.e. prod_b[i][j] = sum_g Phi[g][i] W[g][j]
+    RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_none, rocblas_operation_transpose, n, n, kXcG, &one,
+                                        zw, n, (rocblas_stride)kXcG * n, ao, n, (rocblas_stride)kXcG * n, &zero, q, n,
+                                        (rocblas_stride)n * n, (int)nb));
+  }
+  if (mode == kXcRho) return;
+  h->mark(kStXcKernel + 1, st);
+  if (mixed)
+    HIPCHK(launch_vassemble<float>(nao, h->d_xc_vrow, h->d_xc_vprow, h->d_xc_vpcol, h->d_xc_vlen,
+                                   reinterpret_cast<const float*>(h->d_xc_q), h->d_xc_cols, h->d_Vxc, st));
+  else
+    HIPCHK(launch_vassemble<double>(nao, h->d_xc_vrow, h->d_xc_vprow, h->d_xc_vpcol, h->d_xc_vlen, h->d_xc_q,
+                                    h->d_xc_cols, h->d_Vxc, st));
+  if (mode == kXcEnergy) HIPCHK(launch_dot(h->xc_npts, h->d_xc_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
+  h->mark(kStXcSyr2k + 1, st);
+}
+
 // XC energy (E + DFKS_E_XC) and the half potential Vh (d_Vxc, Vxc = Vh + Vh^T) at P.
 void xc_potential(dfks_ctx* h, const double* P, double* E, hipStream_t st) {
   const int nao = h->nao;
@@ -896,6 +1199,10 @@ void xc_potential(dfks_ctx* h, const double* P, double* E, hipStream_t st) {
   const double one = 1.0, zero = 0.0;
   const bool mixed = h->mixed_now;
   const bool lib = h->opt.gemm == 2;
+  if (h->xcb) {
+    xc_compact(h, P, E, kXcEnergy, st);
+    return;
+  }
   // XC: Z = Phi P, W (in place), Vh = Phi^T W; V = Vh + Vh^T in the Fock assembly
   if (ng > 0 && h->ao_block > 0) {
     // AO values and gradients per block of points: Z_b = Phi_b P, W_b, Vh += Phi_b^T W_b
@@ -1057,8 +1364,15 @@ void response_begin(dfks_ctx* h, const double* P, hipStream_t st) {
   const int64_t ng = h->ng;
   const double thresh = h->opt.rho_thresh;
   if (ng > 0) {
-    if (!h->d_rho0) h->d_rho0 = dalloc<double>(4 * (size_t)ng);
-    if (!h->d_fxc) h->d_fxc = dalloc<double>(8 * (size_t)ng);
+    const int64_t npt = h->xcb ? h->xc_npts : ng;
+    if (!h->d_rho0) h->d_rho0 = dalloc<double>(4 * (size_t)npt);
+    if (!h->d_fxc) h->d_fxc = dalloc<double>(8 * (size_t)npt);
+    if (h->xcb) {
+      xc_compact(h, P, nullptr, kXcRho, st);
+      HIPCHK(launch_fxc_coef(npt, h->d_rho0, h->d_xc_w, h->xc_id, thresh, h->d_fxc, st));
+      h->resp_ready = true;
+      return;
+    }
     if (h->ao_block == 0) {
       gemm(h, false, false, nao, ng, nao, 1.0, P, nao, h->d_ao, nao, 0.0, h->d_zw, nao);
       HIPCHK(launch_xc(ng, nao, h->d_ao, h->d_gw, h->d_zw, h->xc_id, thresh, h->d_exc, h->d_rho0, st));
@@ -1105,7 +1419,9 @@ void response_fock(dfks_ctx* h, const double* U, const double* V, int r, double*
     // dense dP, then V1 = Phi^T W1 (the assembly adds its transpose)
     gemm(h, false, true, nao, nao, r, 1.0, U, nao, V, nao, 0.0, h->d_W1, nao);
     gemm(h, false, true, nao, nao, r, 1.0, V, nao, U, nao, 1.0, h->d_W1, nao);
-    if (h->ao_block == 0) {
+    if (h->xcb) {
+      xc_compact(h, h->d_W1, nullptr, kXcResponse, st);
+    } else if (h->ao_block == 0) {
       gemm(h, false, false, nao, ng, nao, 1.0, h->d_W1, nao, h->d_ao, nao, 0.0, h->d_zw, nao);
       HIPCHK(launch_fxc(ng, nao, h->d_ao, h->d_zw, h->d_fxc, st));
       gemm(h, false, true, nao, nao, ng, 1.0, h->d_ao, nao, h->d_zw, nao, 0.0, h->d_Vxc, nao);
@@ -1373,12 +1689,13 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
     auto* h = new dfks_ctx();
     try {
       if (opt) h->opt = *opt;
-      else h->opt = dfks_options{0, 1e-7, 1e-10, 0, 0, 0, 0, 0.0, 0};
+      else h->opt = dfks_options{0, 1e-7, 1e-10, 0, 0, 0, 0, 0.0, 0, 1};
       if (h->opt.precision != 0 && h->opt.precision != 1) fail(1, "precision must be 0 (FP64) or 1 (mixed)");
       if (h->opt.gemm < 0 || h->opt.gemm > 2) fail(1, "gemm must be 0 (auto), 1 (rocBLAS) or 2 (library MFMA)");
       if (h->opt.density != 0 && h->opt.density != 1) fail(1, "density must be 0 (purification) or 1 (eigensolver)");
       if (h->opt.aux_chunk < 0 || h->opt.ao_block < 0 || h->opt.screen < 0.0 || h->opt.screen >= 1.0)
         fail(1, "aux_chunk and ao_block must be >= 0 and screen in [0, 1)");
+      if (h->opt.xc_blocks != 0 && h->opt.xc_blocks != 1) fail(1, "xc_blocks must be 0 (dense rows) or 1 (compact blocks)");
       h->recompute = h->opt.aux_chunk > 0;
       h->ao_block = h->opt.ao_block;
 
@@ -1502,9 +1819,11 @@ int dfks_sizes(dfks_handle h, int64_t* out5) {
 int dfks_stats(dfks_handle h, int64_t* out, int32_t n) {
   return guard([&] {
     if (!h || !out) fail(1, "null argument");
-    const int64_t v[7] = {h->pairs_kept, h->pairs_total, (int64_t)h->slab_np.size(), h->max_np, h->ao_block,
-                          h->n_eigh_steps, h->sp2_guess};
-    for (int i = 0; i < n; ++i) out[i] = i < 7 ? v[i] : 0;
+    int64_t width = 0;
+    for (const auto& bn : h->xbins) width += bn.nblk * bn.n;
+    const int64_t v[10] = {h->pairs_kept, h->pairs_total, (int64_t)h->slab_np.size(), h->max_np, h->ao_block,
+                           h->n_eigh_steps, h->sp2_guess, h->xc_nblk, (int64_t)h->xbins.size(), width};
+    for (int i = 0; i < n; ++i) out[i] = i < 10 ? v[i] : 0;
   });
 }
 
@@ -1525,7 +1844,13 @@ int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream s
         if (!h->d_int3c) fail(1, "int3c was not kept (options.keep_int3c = 0)");
         src = h->d_int3c; n = nn * h->naux; break;
       case DFKS_BFIT: src = h->d_B; n = nn * h->nfit; break;
-      case DFKS_AO: src = h->d_ao; n = 4 * (size_t)h->ng * h->nao; break;
+      case DFKS_AO:
+        if (!h->d_ao && h->xcb) {
+          // compact grid: evaluate the dense rows on demand in the caller's buffer
+          HIPCHK(launch_ao(h->ng, h->d_gxyz, h->orb.view(), h->nao, dev_out, S_(stream)));
+          return;
+        }
+        src = h->d_ao; n = 4 * (size_t)h->ng * h->nao; break;
       case DFKS_GRID_W: src = h->d_gw; n = (size_t)h->ng; break;
       default: fail(1, "unknown matrix selector");
     }

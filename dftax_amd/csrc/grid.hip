@@ -8,6 +8,8 @@
 // followed by the per-shell cart->sph block; gradients are analytic.
 // Layout of the AO block: ao[4][ng][nao] (value, d/dx, d/dy, d/dz), row-major
 // per point so a wavefront reading one point's row is contiguous.
+#include <algorithm>
+
 #include "dfks_common.h"
 
 namespace dfks {
@@ -129,6 +131,123 @@ __global__ __launch_bounds__(256) void k_ao(int64_t npts, int64_t g0, int64_t ng
     default: ao_shell<4>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
   }
 }
+
+// Compacted AO values for blocks of grid points (one bin: blocks of equal
+// padded width n). Block b holds G points, rows b*G .. b*G+G-1 of the bin's
+// planes ao[4][nblk*G][n]; its significant shells are shl[off[b] .. off[b+1])
+// as (shell, first compact column). One workgroup per block; padded columns
+// are left as they are (zeroed once by the caller).
+__global__ __launch_bounds__(256) void k_ao_cmp(int nblk, int G, int n, const double* __restrict__ pts,
+                                                 const int2* __restrict__ shl, const int64_t* __restrict__ off,
+                                                 ShellSet S, double* __restrict__ ao) {
+  const int b = blockIdx.x;
+  if (b >= nblk) return;
+  const int64_t s0 = off[b], ns = off[b + 1] - s0;
+  const int64_t plane = (int64_t)nblk * G * n;
+  for (int64_t idx = threadIdx.x; idx < ns * G; idx += blockDim.x) {
+    const int p = (int)(idx % G);
+    const int2 e = shl[s0 + idx / G];
+    const int ish = e.x;
+    const int64_t row = (int64_t)b * G + p;
+    const double dx = pts[3 * row] - S.centre[3 * ish];
+    const double dy = pts[3 * row + 1] - S.centre[3 * ish + 1];
+    const double dz = pts[3 * row + 2] - S.centre[3 * ish + 2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double R = 0.0, dR = 0.0;
+    const int p0 = S.prim0[ish], np = S.nprim[ish];
+    for (int k = 0; k < np; ++k) {
+      const double a = S.exps[p0 + k];
+      const double ev = S.coefs[p0 + k] * exp(-a * r2);
+      R += ev;
+      dR += a * ev;
+    }
+    dR *= -2.0;
+    double* r0 = ao + row * n + e.y;
+    switch (S.l[ish]) {
+      case 0: ao_shell<0>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+      case 1: ao_shell<1>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+      case 2: ao_shell<2>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+      case 3: ao_shell<3>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+      default: ao_shell<4>(dx, dy, dz, R, dR, S.spherical, r0, r0 + plane, r0 + 2 * plane, r0 + 3 * plane); break;
+    }
+  }
+}
+
+hipError_t launch_ao_cmp(int nblk, int G, int n, const double* pts, const int2* shl, const int64_t* off,
+                         const ShellSet& S, double* ao, hipStream_t st) {
+  if (nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ao_cmp, dim3((unsigned)nblk), dim3(256), 0, st, nblk, G, n, pts, shl, off, S, ao);
+  return hipGetLastError();
+}
+
+// Pg[b][k][j] = P[c_k][c_j] for the compact columns c = cols[b][0..n) of each
+// block of a bin (0 where either column is padding, c < 0).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_sym(int64_t nblk, int n, const int32_t* __restrict__ cols,
+                                                     const double* __restrict__ P, int64_t nao, T* __restrict__ out) {
+  const int64_t tot = nblk * n * n;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / ((int64_t)n * n);
+    const int r = (int)((i / n) % n), c = (int)(i % n);
+    const int32_t cr = cols[b * n + r], cc = cols[b * n + c];
+    out[i] = (cr >= 0 && cc >= 0) ? (T)P[(int64_t)cr * nao + cc] : (T)0;
+  }
+}
+
+template <typename T>
+hipError_t launch_gather_sym(int64_t nblk, int n, const int32_t* cols, const double* P, int64_t nao, T* out,
+                             hipStream_t st) {
+  const int64_t tot = nblk * n * n;
+  if (tot == 0) return hipSuccess;
+  const int64_t nb = std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_gather_sym<T>, dim3((unsigned)nb), dim3(256), 0, st, nblk, n, cols, P, nao, out);
+  return hipGetLastError();
+}
+template hipError_t launch_gather_sym<double>(int64_t, int, const int32_t*, const double*, int64_t, double*,
+                                             hipStream_t);
+template hipError_t launch_gather_sym<float>(int64_t, int, const int32_t*, const double*, int64_t, float*,
+                                            hipStream_t);
+
+// V[m][:] = sum over the (block, local column i) entries of row m of the
+// block products prod_b[i][:], scattered to the blocks' columns. One workgroup
+// per row; the row accumulates in LDS (FP64 LDS atomics: entries processed by
+// different wavefronts may hit the same column).
+template <typename T>
+__global__ __launch_bounds__(256) void k_vassemble(int nao, const int64_t* __restrict__ rowptr,
+                                                    const int64_t* __restrict__ prow, const int64_t* __restrict__ pcol,
+                                                    const int32_t* __restrict__ plen, const T* __restrict__ q,
+                                                    const int32_t* __restrict__ cols, double* __restrict__ V) {
+  extern __shared__ double acc[];
+  const int m = blockIdx.x;
+  if (m >= nao) return;
+  for (int j = threadIdx.x; j < nao; j += blockDim.x) acc[j] = 0.0;
+  __syncthreads();
+  for (int64_t e = rowptr[m]; e < rowptr[m + 1]; ++e) {
+    const T* src = q + prow[e];
+    const int32_t* dst = cols + pcol[e];
+    const int n = plen[e];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      const int32_t c = dst[j];
+      if (c >= 0) atomicAdd(&acc[c], (double)src[j]);
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nao; j += blockDim.x) V[(int64_t)m * nao + j] = acc[j];
+}
+
+template <typename T>
+hipError_t launch_vassemble(int nao, const int64_t* rowptr, const int64_t* prow, const int64_t* pcol,
+                            const int32_t* plen, const T* q, const int32_t* cols, double* V, hipStream_t st) {
+  if (nao <= 0) return hipSuccess;
+  if ((size_t)nao * sizeof(double) > 64 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_vassemble<T>, dim3((unsigned)nao), dim3(256), nao * sizeof(double), st, nao, rowptr, prow,
+                     pcol, plen, q, cols, V);
+  return hipGetLastError();
+}
+template hipError_t launch_vassemble<double>(int, const int64_t*, const int64_t*, const int64_t*, const int32_t*,
+                                            const double*, const int32_t*, double*, hipStream_t);
+template hipError_t launch_vassemble<float>(int, const int64_t*, const int64_t*, const int64_t*, const int32_t*,
+                                           const float*, const int32_t*, double*, hipStream_t);
 
 hipError_t launch_becke(int64_t n, const double* pts, const double* raw, const int32_t* owner, int natm,
                         const double* atm, const double* invR, const double* aAB, double* w, hipStream_t st) {

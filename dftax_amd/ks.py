@@ -49,6 +49,11 @@ class HipSpec:
     ``ao_block``: ``None`` keeps the AO grid resident when it fits the HBM
     budget and otherwise recomputes it per block of points in every Fock
     build; ``0`` forces resident, an int streams with that block.
+    ``xc_grid``: ``"blocks"`` evaluates the XC term on blocks of 128
+    spatially sorted grid points, each holding only the AOs whose value or
+    gradient bound reaches 1e-14 there (resident compact rows, batched
+    per-block products; ``ao_block`` is then unused); ``"dense"`` keeps full
+    AO rows per point.
     """
 
     precision: str = "fp64"
@@ -58,11 +63,12 @@ class HipSpec:
     gemm: str = "auto"
     density: str = "purify"
     ao_block: int | None = None
+    xc_grid: str = "blocks"
 
 
 def hip(*, precision: str = "fp64", metric_cutoff: float = 1e-7, rho_thresh: float = 1e-10,
         keep_int3c: bool = False, gemm: str = "auto", density: str = "purify",
-        ao_block: int | None = None) -> HipSpec:
+        ao_block: int | None = None, xc_grid: str = "blocks") -> HipSpec:
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
     if gemm not in ("auto", "rocblas", "mfma"):
@@ -71,8 +77,10 @@ def hip(*, precision: str = "fp64", metric_cutoff: float = 1e-7, rho_thresh: flo
         raise ValueError(f"density must be 'purify' or 'eigh', got {density!r}")
     if ao_block is not None and (not isinstance(ao_block, int) or ao_block < 0):
         raise ValueError(f"ao_block must be None or a non-negative int, got {ao_block!r}")
+    if xc_grid not in ("blocks", "dense"):
+        raise ValueError(f"xc_grid must be 'blocks' or 'dense', got {xc_grid!r}")
     return HipSpec(precision=precision, metric_cutoff=metric_cutoff, rho_thresh=rho_thresh, keep_int3c=keep_int3c,
-                   gemm=gemm, density=density, ao_block=ao_block)
+                   gemm=gemm, density=density, ao_block=ao_block, xc_grid=xc_grid)
 
 
 @dataclass(frozen=True)
@@ -190,7 +198,7 @@ class HipDFKS:
                                float(spec.rho_thresh), int(spec.keep_int3c),
                                {"auto": 0, "rocblas": 1, "mfma": 2}[spec.gemm],
                                {"purify": 0, "eigh": 1}[spec.density], int(self.plan.aux_chunk),
-                               float(screen or 0.0), int(self.plan.ao_block))
+                               float(screen or 0.0), int(self.plan.ao_block), int(spec.xc_grid == "blocks"))
         self.precision = spec.precision
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):

@@ -85,6 +85,9 @@ typedef struct {
                               (eri3c_bucketed.py:140-169); 0 keeps every pair. */
   int64_t ao_block;        /* 0: keep AO values and gradients on the grid resident; > 0: recompute them
                               per block of this many points in every Fock build (energy.py:158-212). */
+  int32_t xc_blocks;       /* 1: XC on blocks of 128 spatially sorted points, each with only the AOs
+                              whose value or gradient bound reaches 1e-14 there (compact rows, batched
+                              per-block products); 0: dense AO rows (grid.py:10-44 as written). */
 } dfks_options;
 
 /* Matrix selectors for dfks_get_matrix. */
@@ -119,7 +122,9 @@ int dfks_sizes(dfks_handle h, int64_t* out5);
 /* Build and solver counters: [0] bra shell pairs kept, [1] bra shell pairs in total,
  * [2] auxiliary slabs (recompute mode, else 0), [3] largest slab in functions,
  * [4] AO block (0: resident grid), [5] SCF steps that fell back to the eigensolver,
- * [6] SP2 iterations of the last purified step. Fills n values (zeros past 7). */
+ * [6] SP2 iterations of the last purified step, [7] compact XC blocks (0: dense
+ * rows), [8] their bins of equal width, [9] the sum of their padded widths.
+ * Fills n values (zeros past 10). */
 int dfks_stats(dfks_handle h, int64_t* out, int32_t n);
 
 int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream stream);

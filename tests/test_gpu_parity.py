@@ -282,11 +282,13 @@ def dimer_recompute(torch_cuda, water_dimer_tzvp):
     g = becke(35, 110)
     eng = dict(gemm="rocblas")
     specs = {
-        "mat": df(chunk=None, engine=hip(ao_block=0, **eng)),
-        "rec": df(chunk=48, engine=hip(ao_block=1000, **eng)),
+        "mat": df(chunk=None, engine=hip(ao_block=0, xc_grid="dense", **eng)),
+        "rec": df(chunk=48, engine=hip(ao_block=1000, xc_grid="dense", **eng)),
         "scr": df(chunk=48, screen=1e-12, engine=hip(**eng)),
         "scr4": df(chunk=None, screen=1e-4, engine=hip(**eng)),
-        "recmx": df(chunk=48, screen=1e-12, engine=hip(precision="mixed", ao_block=1000, **eng)),
+        "recmx": df(chunk=48, screen=1e-12, engine=hip(precision="mixed", ao_block=1000, xc_grid="dense", **eng)),
+        "blk": df(chunk=None, engine=hip(ao_block=0, **eng)),
+        "blkmx": df(chunk=48, screen=1e-12, engine=hip(precision="mixed", **eng)),
     }
     return {k: KS(water_dimer_tzvp, "PBE0", grid=g, coulomb=v) for k, v in specs.items()}
 
@@ -364,7 +366,7 @@ def test_recompute_mixed(dimer_recompute, torch_cuda):
     assert rx.converged and nmix >= 1
     assert abs(rx.e_tot - rm.e_tot) < 1e-8
     # mixed to the switch, then FP64 steps until the energy settles: within the 1e-6 Ha budget
-    rr = scf(r, precision="mixed", e_tol=1e-9, refine=3)
+    rr = scf(r, precision="mixed", refine=3)
     nfp = sum(1 for h in rr.history if h[3] == "fp64")
     _say(f"mixed + {nfp} FP64 steps {rr.e_tot:.12f} ({rr.n_iter} it), |dE| {abs(rr.e_tot - rm.e_tot):.1e}")
     assert rr.converged and 2 <= nfp <= 3 and abs(rr.e_tot - rm.e_tot) < 1e-6
@@ -511,3 +513,53 @@ def test_newton_matches_scf(case, request, torch_cuda):
     assert abs(rn.e_tot - ro["e_tot"]) < TOL_E
     Pn = rn.P[0].cpu().numpy()
     assert np.abs(Pn - rs.P[0].cpu().numpy()).max() < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Compact XC grid (blocks of 128 sorted points, significant AOs only)
+# ---------------------------------------------------------------------------
+
+def test_xc_blocks_match_dense(dimer_recompute, torch_cuda):
+    """Compact-block XC against dense AO rows: energy, Fock and response; the
+    dropped AO values are bounded by 1e-14."""
+    torch = torch_cuda
+    d, b = dimer_recompute["mat"], dimer_recompute["blk"]
+    st = b.coulomb.stats()
+    _say(f"compact XC: {st['xc_blocks']} blocks in {st['xc_bins']} bins, mean width "
+         f"{st['xc_width_sum'] / max(st['xc_blocks'], 1):.0f} of nao {b.nao}")
+    assert st["xc_blocks"] > 0 and d.coulomb.stats()["xc_blocks"] == 0
+    P = torch.tensor(_p0(d), device="cuda")
+    Ed, Fd = d.coulomb.energy_fock(P)
+    Eb, Fb = b.coulomb.energy_fock(P)
+    dF = np.abs((Fd - Fb).cpu().numpy()).max()
+    dE = abs(float(Ed[4] - Eb[4]))
+    _say(f"compact vs dense: max|dF| {dF:.2e}, |dE_xc| {dE:.2e}")
+    assert dF < 1e-10 and dE < 1e-10
+    U, V = (torch.tensor(a, device="cuda") for a in _uv(d.nao, 5, 8))
+    out = []
+    for ks in (d, b):
+        ks.coulomb.response_begin(P)
+        out.append(ks.coulomb.response_fock(U, V).cpu().numpy())
+    dR = np.abs(out[0] - out[1]).max()
+    _say(f"compact vs dense response: max|dR| {dR:.2e}")
+    assert dR < 1e-10 * max(1.0, np.abs(out[0]).max())
+
+
+def test_xc_blocks_mixed_scf(dimer_recompute, torch_cuda):
+    """Mixed precision on the compact grid with recomputed, screened slabs: the
+    refined SCF lands on the dense materialised FP64 energy."""
+    from dftax_amd import scf
+    torch = torch_cuda
+    d, bm = dimer_recompute["mat"], dimer_recompute["blkmx"]
+    P = torch.tensor(_p0(d), device="cuda")
+    _, Fd = d.coulomb.energy_fock(P)
+    bm.coulomb.set_precision(True)
+    _, Fm = bm.coulomb.energy_fock(P)
+    bm.coulomb.set_precision(False)
+    dmx = np.abs((Fm - Fd).cpu().numpy()).max()
+    _say(f"compact mixed Fock max|dF| {dmx:.2e}")
+    assert 0.0 < dmx < 1e-3
+    rd = scf(d, e_tol=1e-9)
+    rb = scf(bm, precision="mixed", e_tol=1e-9)
+    _say(f"dense FP64 {rd.e_tot:.12f}; compact mixed + FP64 {rb.e_tot:.12f} ({rb.n_iter} it)")
+    assert rb.converged and abs(rb.e_tot - rd.e_tot) < 1e-8
