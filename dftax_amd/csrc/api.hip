@@ -52,7 +52,8 @@ hipError_t launch_gather_sym(int64_t nblk, int n, const int32_t* cols, const dou
                              hipStream_t st);
 template <typename T>
 hipError_t launch_vassemble(int nao, const int64_t* rowptr, const int64_t* prow, const int64_t* pcol,
-                            const int32_t* plen, const T* q, const int32_t* cols, double* V, hipStream_t st);
+                            const int32_t* plen, const T* q, const int32_t* cols, double* V, double* ws, int nslice,
+                            hipStream_t st);
 hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w, double* zw, int xc_id, double thresh,
                      double* exc, double* rho_out, hipStream_t st);
 hipError_t launch_xc_pointwise(int64_t n, int xc_id, const double* rho, const double* sigma, double* out,
@@ -1139,23 +1140,20 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
     const int64_t nb = bn.nblk, rows = nb * kXcG;
     const int32_t* cols = h->d_xc_cols + bn.col_off;
     if (mixed) {
+      // FP32 operands and FP32 MFMA products (rocBLAS strided batched): on these short
+      // per-block products (K = n or 128) they beat the split-BF16 kernel and are exact to FP32
+      const float onef = 1.0f, zerof = 0.0f;
       float* q = reinterpret_cast<float*>(h->d_xc_q) + bn.q_off;
       const float* ao = h->d_xc_ao32 + bn.ao_off;
       float* zw = reinterpret_cast<float*>(h->d_xc_zw);
       HIPCHK(launch_gather_sym<float>(nb, n, cols, P, nao, q, st));
-      for (int64_t c0 = 0; c0 < nb; c0 += 65535) {
-        const int cnt = (int)std::min<int64_t>(65535, nb - c0);
-        const GemmArgs gz{kXcG, n, n, ao + c0 * kXcG * n, n, true, q + c0 * n * n, n, true};
-        HIPCHK(launch_gemm_f32out_batched(gz, zw + c0 * kXcG * n, cnt, (int64_t)kXcG * n, (int64_t)n * n,
-                                          (int64_t)kXcG * n, st));
-      }
+      RBCHK(rocblas_sgemm_strided_batched(h->rb, rocblas_operation_transpose, rocblas_operation_none, n, kXcG, n,
+                                          &onef, q, n, (rocblas_stride)n * n, ao, n, (rocblas_stride)kXcG * n,
+                                          &zerof, zw, n, (rocblas_stride)kXcG * n, (int)nb));
       HIPCHK(launch_xc_f32(rows, n, ao, h->d_xc_w + bn.pt0, zw, h->xc_id, thresh, h->d_xc_exc + bn.pt0, st));
-      for (int64_t c0 = 0; c0 < nb; c0 += 65535) {
-        const int cnt = (int)std::min<int64_t>(65535, nb - c0);
-        const GemmArgs gv{n, n, kXcG, ao + c0 * kXcG * n, n, false, zw + c0 * kXcG * n, n, false};
-        HIPCHK(launch_gemm_f32out_batched(gv, q + c0 * n * n, cnt, (int64_t)kXcG * n, (int64_t)kXcG * n,
-                                          (int64_t)n * n, st));
-      }
+      RBCHK(rocblas_sgemm_strided_batched(h->rb, rocblas_operation_none, rocblas_operation_transpose, n, n, kXcG,
+                                          &onef, zw, n, (rocblas_stride)kXcG * n, ao, n, (rocblas_stride)kXcG * n,
+                                          &zerof, q, n, (rocblas_stride)n * n, (int)nb));
       continue;
     }
     double* q = h->d_xc_q + bn.q_off;
@@ -1181,12 +1179,16 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
   }
   if (mode == kXcRho) return;
   h->mark(kStXcKernel + 1, st);
+  // slices of each row's entries so that about 32768 workgroups share the assembly (at most 32 x nao^2 of
+  // partial rows in the split-K workspace, which holds 64 x nao^2)
+  const int nslice = std::max(1, std::min(32, (32768 + nao - 1) / nao));
   if (mixed)
     HIPCHK(launch_vassemble<float>(nao, h->d_xc_vrow, h->d_xc_vprow, h->d_xc_vpcol, h->d_xc_vlen,
-                                   reinterpret_cast<const float*>(h->d_xc_q), h->d_xc_cols, h->d_Vxc, st));
+                                   reinterpret_cast<const float*>(h->d_xc_q), h->d_xc_cols, h->d_Vxc, h->d_gws,
+                                   nslice, st));
   else
     HIPCHK(launch_vassemble<double>(nao, h->d_xc_vrow, h->d_xc_vprow, h->d_xc_vpcol, h->d_xc_vlen, h->d_xc_q,
-                                    h->d_xc_cols, h->d_Vxc, st));
+                                    h->d_xc_cols, h->d_Vxc, h->d_gws, nslice, st));
   if (mode == kXcEnergy) HIPCHK(launch_dot(h->xc_npts, h->d_xc_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
   h->mark(kStXcSyr2k + 1, st);
 }

@@ -181,25 +181,32 @@ hipError_t launch_ao_cmp(int nblk, int G, int n, const double* pts, const int2* 
 }
 
 // Pg[b][k][j] = P[c_k][c_j] for the compact columns c = cols[b][0..n) of each
-// block of a bin (0 where either column is padding, c < 0).
+// block of a bin (0 where either column is padding, c < 0). One wavefront per
+// row (b, k), grid-stride over the rows.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gather_sym(int64_t nblk, int n, const int32_t* __restrict__ cols,
                                                      const double* __restrict__ P, int64_t nao, T* __restrict__ out) {
-  const int64_t tot = nblk * n * n;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = i / ((int64_t)n * n);
-    const int r = (int)((i / n) % n), c = (int)(i % n);
-    const int32_t cr = cols[b * n + r], cc = cols[b * n + c];
-    out[i] = (cr >= 0 && cc >= 0) ? (T)P[(int64_t)cr * nao + cc] : (T)0;
+  const int lane = threadIdx.x & 63;
+  const int64_t nrow = nblk * n;
+  for (int64_t row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrow; row += (int64_t)gridDim.x * 4) {
+    const int64_t b = row / n;
+    const int32_t* cb = cols + b * n;
+    const int32_t cr = cols[row];
+    T* o = out + row * n;
+    const double* pr = P + (int64_t)(cr >= 0 ? cr : 0) * nao;
+    for (int c = lane; c < n; c += 64) {
+      const int32_t cc = cb[c];
+      o[c] = (cr >= 0 && cc >= 0) ? (T)pr[cc] : (T)0;
+    }
   }
 }
 
 template <typename T>
 hipError_t launch_gather_sym(int64_t nblk, int n, const int32_t* cols, const double* P, int64_t nao, T* out,
                              hipStream_t st) {
-  const int64_t tot = nblk * n * n;
-  if (tot == 0) return hipSuccess;
-  const int64_t nb = std::min<int64_t>((tot + 255) / 256, 65536);
+  const int64_t nrow = nblk * n;
+  if (nrow == 0) return hipSuccess;
+  const int64_t nb = std::min<int64_t>((nrow + 3) / 4, 65536);
   hipLaunchKernelGGL(k_gather_sym<T>, dim3((unsigned)nb), dim3(256), 0, st, nblk, n, cols, P, nao, out);
   return hipGetLastError();
 }
@@ -208,46 +215,85 @@ template hipError_t launch_gather_sym<double>(int64_t, int, const int32_t*, cons
 template hipError_t launch_gather_sym<float>(int64_t, int, const int32_t*, const double*, int64_t, float*,
                                             hipStream_t);
 
-// V[m][:] = sum over the (block, local column i) entries of row m of the
-// block products prod_b[i][:], scattered to the blocks' columns. One workgroup
-// per row; the row accumulates in LDS (FP64 LDS atomics: entries processed by
-// different wavefronts may hit the same column).
+// Vh[m][:] = sum over the (block, local column i) entries of row m of the
+// block products prod_b[i][:], scattered to the blocks' columns. Workgroup
+// (m, s) takes slice s of row m's entries, one entry per wavefront at a time,
+// and accumulates the row in LDS (FP64 LDS atomics: wavefronts may hit the
+// same column); slice s writes out + s * nao^2 (the slices are summed after).
 template <typename T>
-__global__ __launch_bounds__(256) void k_vassemble(int nao, const int64_t* __restrict__ rowptr,
+__global__ __launch_bounds__(256) void k_vassemble(int nao, int nslice, const int64_t* __restrict__ rowptr,
                                                     const int64_t* __restrict__ prow, const int64_t* __restrict__ pcol,
                                                     const int32_t* __restrict__ plen, const T* __restrict__ q,
-                                                    const int32_t* __restrict__ cols, double* __restrict__ V) {
+                                                    const int32_t* __restrict__ cols, double* __restrict__ out) {
   extern __shared__ double acc[];
-  const int m = blockIdx.x;
-  if (m >= nao) return;
+  const int m = blockIdx.x, s = blockIdx.y;
+  if (m >= nao || s >= nslice) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int j = threadIdx.x; j < nao; j += blockDim.x) acc[j] = 0.0;
   __syncthreads();
-  for (int64_t e = rowptr[m]; e < rowptr[m + 1]; ++e) {
+  const int64_t r0 = rowptr[m], cnt = rowptr[m + 1] - r0;
+  const int64_t e0 = r0 + cnt * s / nslice, e1 = r0 + cnt * (s + 1) / nslice;
+  // two entries per wavefront in flight
+  int64_t e = e0 + wave;
+  for (; e + 4 < e1; e += 8) {
+    const T* s0 = q + prow[e];
+    const T* s1 = q + prow[e + 4];
+    const int32_t* d0 = cols + pcol[e];
+    const int32_t* d1 = cols + pcol[e + 4];
+    const int n0 = plen[e], n1 = plen[e + 4];
+    const int nmax = n0 > n1 ? n0 : n1;
+    for (int j = lane; j < nmax; j += 64) {
+      const int32_t c0 = j < n0 ? d0[j] : -1;
+      const int32_t c1 = j < n1 ? d1[j] : -1;
+      const double v0 = c0 >= 0 ? (double)s0[j] : 0.0;
+      const double v1 = c1 >= 0 ? (double)s1[j] : 0.0;
+      if (c0 >= 0) atomicAdd(&acc[c0], v0);
+      if (c1 >= 0) atomicAdd(&acc[c1], v1);
+    }
+  }
+  for (; e < e1; e += 4) {
     const T* src = q + prow[e];
     const int32_t* dst = cols + pcol[e];
     const int n = plen[e];
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    for (int j = lane; j < n; j += 64) {
       const int32_t c = dst[j];
       if (c >= 0) atomicAdd(&acc[c], (double)src[j]);
     }
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < nao; j += blockDim.x) V[(int64_t)m * nao + j] = acc[j];
+  double* o = out + (int64_t)s * nao * nao + (int64_t)m * nao;
+  for (int j = threadIdx.x; j < nao; j += blockDim.x) o[j] = acc[j];
 }
 
+// V[i] = sum_s ws[s][i]
+__global__ __launch_bounds__(256) void k_sum_slices(int64_t nn, int nslice, const double* __restrict__ ws,
+                                                     double* __restrict__ V) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  double v = 0.0;
+  for (int s = 0; s < nslice; ++s) v += ws[(int64_t)s * nn + i];
+  V[i] = v;
+}
+
+// ws: nslice * nao^2 doubles when nslice > 1 (unused otherwise).
 template <typename T>
 hipError_t launch_vassemble(int nao, const int64_t* rowptr, const int64_t* prow, const int64_t* pcol,
-                            const int32_t* plen, const T* q, const int32_t* cols, double* V, hipStream_t st) {
+                            const int32_t* plen, const T* q, const int32_t* cols, double* V, double* ws, int nslice,
+                            hipStream_t st) {
   if (nao <= 0) return hipSuccess;
-  if ((size_t)nao * sizeof(double) > 64 * 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_vassemble<T>, dim3((unsigned)nao), dim3(256), nao * sizeof(double), st, nao, rowptr, prow,
-                     pcol, plen, q, cols, V);
+  if ((size_t)nao * sizeof(double) > 64 * 1024 || nslice < 1 || (nslice > 1 && !ws)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_vassemble<T>, dim3((unsigned)nao, (unsigned)nslice), dim3(256), nao * sizeof(double), st, nao,
+                     nslice, rowptr, prow, pcol, plen, q, cols, nslice > 1 ? ws : V);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nslice == 1) return e;
+  const int64_t nn = (int64_t)nao * nao;
+  hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, nn, nslice, ws, V);
   return hipGetLastError();
 }
 template hipError_t launch_vassemble<double>(int, const int64_t*, const int64_t*, const int64_t*, const int32_t*,
-                                            const double*, const int32_t*, double*, hipStream_t);
+                                            const double*, const int32_t*, double*, double*, int, hipStream_t);
 template hipError_t launch_vassemble<float>(int, const int64_t*, const int64_t*, const int64_t*, const int32_t*,
-                                           const float*, const int32_t*, double*, hipStream_t);
+                                           const float*, const int32_t*, double*, double*, int, hipStream_t);
 
 hipError_t launch_becke(int64_t n, const double* pts, const double* raw, const int32_t* owner, int natm,
                         const double* atm, const double* invR, const double* aAB, double* w, hipStream_t st) {

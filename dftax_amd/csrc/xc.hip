@@ -397,7 +397,9 @@ hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w,
                      double* exc, double* rho_out, hipStream_t st) {
   if (ng == 0) return hipSuccess;
   const bool vec = nao % 2 == 0 && (uintptr_t)ao % 16 == 0 && (uintptr_t)zw % 16 == 0;
-  if (vec && !rho_out && ng < (1LL << 31) && launch_xc1(ng, nao, ao, w, zw, xc_id, thresh, exc, st))
+  // one block per point reads each AO value once, which pays off on long rows; short
+  // rows (the compact grid's blocks) keep one wavefront per point and four points per block
+  if (vec && !rho_out && nao >= 768 && ng < (1LL << 31) && launch_xc1(ng, nao, ao, w, zw, xc_id, thresh, exc, st))
     return hipGetLastError();
   if (vec)
     hipLaunchKernelGGL((k_xc<double, double, true>), dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w,
