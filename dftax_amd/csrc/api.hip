@@ -115,8 +115,8 @@ using namespace dfks;
 
 // Fock-build / SCF stages timed by dfks_stage_times (event i marks the end of stage i-1).
 enum Stage {
-  kStExchangeGemm = 0,  // T = B^T Cf (all chunks, includes gamma)
-  kStExchangeSyrk = 1,  // K = T^T T
+  kStExchangeGemm = 0,  // T = B^T Cf
+  kStExchangeSyrk = 1,  // gamma from T, and K = T^T T
   kStCoulomb = 2,       // E_J and J = B gamma
   kStXcGemm = 3,        // Z = Phi P
   kStXcKernel = 4,      // rho, functional, W
@@ -1071,29 +1071,27 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
     // T[m,Q,i] = sum_l B[l,m,Q] Cf[l,i]
     if (h->recompute) {
       exchange_recompute(h, Cf + (int64_t)i0 * nao, mixed ? h->d_Cf32 + (int64_t)i0 * nao : nullptr, nc, st);
-      if (mixed)
-        HIPCHK(launch_gamma_f32(nao, nfit, nc, h->d_T32, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma,
-                                h->d_gpart, 1, st));
-      else
-        HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1,
-                            st));
     } else if (mixed) {
       GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
                  h->d_Cf32 + (int64_t)i0 * nao, nao, true};
       HIPCHK(launch_gemm_f32out(g, h->d_T32, st));
-      HIPCHK(launch_gamma_f32(nao, nfit, nc, h->d_T32, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1, st));
     } else if (lib) {
       GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B, (int64_t)nao * nfit, false, Cf + (int64_t)i0 * nao,
                  nao, true};
       HIPCHK(launch_gemm(kF64, g, h->d_Tx, 1.0, 0.0, 1, 0, nullptr, st));
-      HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1, st));
     } else {
       RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc,
                              (int64_t)nao * nfit, nao, &one, Cf + (int64_t)i0 * nao, nao, h->d_B,
                              (int64_t)nao * nfit, &zero, h->d_Tx, nc));
-      HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1, st));
     }
     h->mark(kStExchangeGemm + 1, st);
+    // gamma += sum_i s_i T_i c_i (timed with the exchange product K below)
+    if (mixed)
+      HIPCHK(launch_gamma_f32(nao, nfit, nc, h->d_T32, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart,
+                              1, st));
+    else
+      HIPCHK(launch_gamma(nao, nfit, nc, h->d_Tx, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart, 1,
+                          st));
     if (h->hf != 0.0) {
       // K += s T T^T (symmetric: upper tiles, mirrored by the split reduction)
       const int64_t kd = (int64_t)nfit * nc;

@@ -184,14 +184,16 @@ class Server:
         return requests
 
 
-def latency_stats(reqs: list[Request]) -> dict:
+def latency_stats(reqs: list[Request], alone: dict | None = None) -> dict:
+    """Latency percentiles; with ``alone`` (request id -> isolated solve seconds) also the
+    per-request slowdown latency / isolated time of the same molecule."""
     ok = [r for r in reqs if not r.error]
     lat = np.array([r.end - r.arrival for r in ok])
     span = max(r.end for r in ok) - min(r.arrival for r in ok) if ok else float("nan")
     build = np.array([r.extra.get("build_s", np.nan) for r in ok])
     scf_t = np.array([r.extra.get("scf_s", np.nan) for r in ok])
     wait = np.array([r.start - r.arrival for r in ok])
-    return {
+    out = {
         "n": len(reqs), "ok": len(ok), "converged": sum(r.converged for r in ok),
         "mean_build_s": float(np.nanmean(build)) if ok else None,
         "mean_scf_s": float(np.nanmean(scf_t)) if ok else None,
@@ -202,6 +204,12 @@ def latency_stats(reqs: list[Request]) -> dict:
         "mean_s": float(lat.mean()) if ok else None,
         "span_s": span,
     }
+    if alone and ok:
+        slow = np.array([(r.end - r.arrival) / alone[r.rid] for r in ok if r.rid in alone])
+        if slow.size:
+            out["slowdown_p50"] = float(np.percentile(slow, 50))
+            out["slowdown_p99"] = float(np.percentile(slow, 99))
+    return out
 
 
 def main() -> int:
@@ -212,7 +220,8 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--xc", default="B3LYP")
     ap.add_argument("--precision", default="mixed")
-    ap.add_argument("--isolated", type=int, default=16, help="requests timed alone to set the rate")
+    ap.add_argument("--isolated", type=int, default=0,
+                    help="requests timed alone to set the rate (0: all, which also gives per-request slowdowns)")
     ap.add_argument("--refine", type=int, default=3,
                     help="at most this many FP64 steps after the mixed phase, stopping when the energy "
                          "has converged (0: run them to the density tolerance as well)")
@@ -223,20 +232,24 @@ def main() -> int:
     from dftax_amd.system import serving_stream
     mols = serving_stream(args.n, seed=args.seed)
     srv = Server(args.xc, precision=args.precision, slots=args.slots, refine=args.refine or None)
-    # isolated-solve time on the first requests, one at a time
-    iso, iso_req = [], []
-    for k in range(min(args.isolated, len(mols))):
+    # isolated-solve time of every request, one at a time (the first solve also warms up
+    # the library and is repeated)
+    n_iso = len(mols) if args.isolated <= 0 else min(args.isolated, len(mols))
+    srv.solve(Request(rid=-1, mol=mols[0]))
+    alone, iso_req = {}, []
+    for k in range(n_iso):
         t = time.perf_counter()
         r = srv.solve(Request(rid=k, mol=mols[k]))
-        iso.append(time.perf_counter() - t)
+        alone[k] = time.perf_counter() - t
         iso_req.append(r)
         if r.error:
             raise RuntimeError(r.error)
-    tau = float(np.mean(iso[1:])) if len(iso) > 1 else iso[0]
-    body = iso_req[1:] or iso_req
-    out = {"isolated_solve_s_mean": tau, "isolated_solve_s_p99": float(np.percentile(iso[1:] or iso, 99)),
-           "isolated_build_s_mean": float(np.mean([r.extra["build_s"] for r in body])),
-           "isolated_scf_s_mean": float(np.mean([r.extra["scf_s"] for r in body]))}
+    iso = list(alone.values())
+    tau = float(np.mean(iso))
+    out = {"isolated_solve_s_mean": tau, "isolated_solve_s_p99": float(np.percentile(iso, 99)),
+           "isolated_build_s_mean": float(np.mean([r.extra["build_s"] for r in iso_req])),
+           "isolated_scf_s_mean": float(np.mean([r.extra["scf_s"] for r in iso_req])),
+           "isolated_requests": n_iso}
     rng = np.random.default_rng(args.seed + 1)
     for rate in args.rate:
         lam = rate / tau
@@ -244,8 +257,9 @@ def main() -> int:
         arr = np.cumsum(gaps) - gaps[0]
         reqs = [Request(rid=k, mol=m, arrival=float(a)) for k, (m, a) in enumerate(zip(mols, arr))]
         srv.run(reqs)
-        st = latency_stats(reqs)
+        st = latency_stats(reqs, alone)
         st["p99_over_isolated"] = st["p99_s"] / tau if st["p99_s"] else None
+        st["p99_over_isolated_p99"] = st["p99_s"] / out["isolated_solve_s_p99"] if st["p99_s"] else None
         out[f"rate_{rate:g}x"] = st
     out.update({"n_requests": len(mols), "xc": args.xc, "basis": "def2-svp / even-tempered-jk",
                 "precision": args.precision, "refine_fp64_steps": srv.refine, "slots": args.slots,

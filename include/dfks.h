@@ -179,11 +179,13 @@ int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* n
 int dfks_set_precision(dfks_handle h, int32_t mixed);
 
 /* Stage timers (HIP events on the work stream). Stages of the last Fock
- * build / SCF step, in ms: 0 exchange GEMM T = B^T Cf (+ gamma), 1 exchange
- * SYRK K = T^T T, 2 Coulomb J = B gamma, 3 XC GEMM Z = Phi P, 4 XC kernel,
+ * build / SCF step, in ms: 0 exchange GEMM T = B^T Cf, 1 gamma from T and the
+ * exchange SYRK K = T^T T, 2 Coulomb J = B gamma, 3 XC GEMM Z = Phi P, 4 XC kernel,
  * 5 XC SYR2K, 6 Fock assembly + energies, 7 commutator + DIIS, 8 eigensolve +
  * density; then build wall times 9 one-electron, 10 (P|Q), 11 (mn|P),
- * 12 metric factor + B, 13 AO grid. */
+ * 12 metric factor + B, 13 AO grid. On the compact XC grid stage 3 is empty,
+ * 4 holds the per-bin gathers, products and pointwise kernel, and 5 the Vxc
+ * row assembly. */
 int dfks_set_timing(dfks_handle h, int32_t on);
 int dfks_stage_times(dfks_handle h, double* ms_out, int32_t n);
 
