@@ -1075,7 +1075,9 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
       GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
                  h->d_Cf32 + (int64_t)i0 * nao, nao, true};
       HIPCHK(launch_gemm_f32out(g, h->d_T32, st));
-    } else if (lib) {
+    } else if (lib || (h->opt.gemm == 0 && nc > 128 && nc <= 160)) {
+      // the library kernel's 160-wide tile covers 129..160 factor columns in one N tile
+      // (rocBLAS pads them to two 128-wide tiles and reads B twice)
       GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B, (int64_t)nao * nfit, false, Cf + (int64_t)i0 * nao,
                  nao, true};
       HIPCHK(launch_gemm(kF64, g, h->d_Tx, 1.0, 0.0, 1, 0, nullptr, st));

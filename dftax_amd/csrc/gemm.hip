@@ -120,10 +120,18 @@ __device__ __forceinline__ void load_slice(const T* __restrict__ X, int64_t ld, 
 }
 
 // NJ: 16-wide MFMA blocks per wavefront along N (tile width BN = 32 NJ: 4 ->
-// 128, 5 -> 160 for N = 160-column exchange products); NJ = 5 only for the
-// split-BF16 path without the symmetric mode.
+// 128, 5 -> 160 for N = 160-column exchange products); NJ = 5 only without the
+// symmetric mode.
+// Two workgroups per CU (the register budget of 2 waves per SIMD) where the
+// tile fits it without spilling; the FP64 160-wide tile and the 160-wide
+// split-BF16 tiles with an N-contiguous B keep one.
+template <int PREC, bool BKC, int NJ>
+constexpr int gemm_blocks_per_cu() {
+  return (NJ == 5 && (PREC == kF64 || !BKC)) ? 1 : 2;
+}
+
 template <int PREC, bool AK, bool BKC, typename Out, int NJ = 4>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typename GemmTraits<PREC>::In* __restrict__ A,
+__global__ __launch_bounds__(256, (gemm_blocks_per_cu<PREC, BKC, NJ>())) void k_gemm(int M, int N, int64_t K, const typename GemmTraits<PREC>::In* __restrict__ A,
                                               int64_t lda, const typename GemmTraits<PREC>::In* __restrict__ B,
                                               int64_t ldb, Out* __restrict__ C, int64_t ldc, int64_t split_stride,
                                               int64_t kchunk, int sym, int ntiles, int vec_a, int vec_b, int64_t bs_a,
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typ
   constexpr int BK = GemmTraits<PREC>::BK;
   constexpr int VEC = (128 * BK) / 256;  // values per thread per operand tile
   constexpr int BN = 32 * NJ;
-  static_assert(NJ == 4 || (NJ == 5 && PREC == kBF16x3), "160-wide tiles are split-BF16 only");
+  static_assert(NJ == 4 || NJ == 5, "128- or 160-wide tiles");
   const int tile = xcd_tile(blockIdx.x, ntiles);
   if (tile >= ntiles) return;
   const int ntn = (N + BN - 1) / BN;
@@ -156,56 +164,62 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typ
   slice_of<AK, VEC>(tid, ra_r, ra_k);
   slice_of<BKC, VEC>(tid, rb_r, rb_k);
   In ra[VEC], rb[VEC];
-  // rows 128 .. BN-1 of the B tile (NJ = 5): threads 0..63, 16 values each
+  // rows 128 .. BN-1 of the B tile (NJ = 5): threads 0..63, XV = 32 BK / 64 values each
   constexpr int XR = BN - 128;
+  constexpr int XV = BK / 2;
   int rx_r = 0, rx_k = 0;
-  if (BKC) { rx_r = 128 + (tid >> 1); rx_k = (tid & 1) * 16; }
-  else { rx_r = 128 + (tid & 31); rx_k = (tid >> 5) * 16; }
+  if (BKC) { rx_r = 128 + (tid >> 1); rx_k = (tid & 1) * XV; }
+  else { rx_r = 128 + (tid & 31); rx_k = (tid >> 5) * XV; }
   const bool has_x = XR > 0 && tid < 64;
-  In rbx[16];
+  In rbx[XV];
   if (k_begin < k_end) {
     load_slice<AK, VEC>(A, lda, M, m0, ra_r, k_begin, ra_k, k_end, vec_a != 0, ra);
     load_slice<BKC, VEC>(B, ldb, N, n0, rb_r, k_begin, rb_k, k_end, vec_b != 0, rb);
-    if (has_x) load_slice<BKC, 16>(B, ldb, N, n0, rx_r, k_begin, rx_k, k_end, vec_b != 0, rbx);
+    if (has_x) load_slice<BKC, XV>(B, ldb, N, n0, rx_r, k_begin, rx_k, k_end, vec_b != 0, rbx);
   }
 
   if constexpr (PREC == kF64) {
     __shared__ double sA[kBM][BK + 1];
-    __shared__ double sB[kBN][BK + 1];
-    dbl4 acc[4][4];
+    __shared__ double sB[BN][BK + 1];
+    dbl4 acc[4][NJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
+      for (int j = 0; j < NJ; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
     for (int64_t k0 = k_begin; k0 < k_end; k0 += BK) {
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < VEC; ++j) sA[ra_r][ra_k + j] = ra[j];
 #pragma unroll
       for (int j = 0; j < VEC; ++j) sB[rb_r][rb_k + j] = rb[j];
+      if (has_x) {
+#pragma unroll
+        for (int j = 0; j < XV; ++j) sB[rx_r][rx_k + j] = rbx[j];
+      }
       __syncthreads();
       if (k0 + BK < k_end) {  // prefetch the next tile while this one computes
         load_slice<AK, VEC>(A, lda, M, m0, ra_r, k0 + BK, ra_k, k_end, vec_a != 0, ra);
         load_slice<BKC, VEC>(B, ldb, N, n0, rb_r, k0 + BK, rb_k, k_end, vec_b != 0, rb);
+        if (has_x) load_slice<BKC, XV>(B, ldb, N, n0, rx_r, k0 + BK, rx_k, k_end, vec_b != 0, rbx);
       }
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
-        double a[4], b[4];
+        double a[4], b[NJ];
 #pragma unroll
         for (int i = 0; i < 4; ++i) a[i] = sA[wm + 16 * i + li][kk + lk];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = sB[wn + 16 * j + li][kk + lk];
+        for (int j = 0; j < NJ; ++j) b[j] = sB[wn + 16 * j + li][kk + lk];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < NJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
       }
     }
     // v_mfma_f64_16x16x4f64 result: lane l, register r holds row (l/16) + 4r, column l%16
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < NJ; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           int gm = m0 + wm + 16 * i + lk + 4 * r;
@@ -247,7 +261,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typ
       }
       if (has_x) {
 #pragma unroll
-        for (int v = 0; v < 2; ++v) {
+        for (int v = 0; v < XV / 8; ++v) {
           bf16x8 hb, lb;
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
@@ -264,7 +278,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int64_t K, const typ
       if (k0 + BK < k_end) {
         load_slice<AK, VEC>(A, lda, M, m0, ra_r, k0 + BK, ra_k, k_end, vec_a != 0, ra);
         load_slice<BKC, VEC>(B, ldb, N, n0, rb_r, k0 + BK, rb_k, k_end, vec_b != 0, rb);
-        if (has_x) load_slice<BKC, 16>(B, ldb, N, n0, rx_r, k0 + BK, rx_k, k_end, vec_b != 0, rbx);
+        if (has_x) load_slice<BKC, XV>(B, ldb, N, n0, rx_r, k0 + BK, rx_k, k_end, vec_b != 0, rbx);
       }
       bf16x8 ah[4], al[4], bh[NJ], bl[NJ];
 #pragma unroll
@@ -372,7 +386,12 @@ hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, dou
   if (nsplit < 1) nsplit = 1;
   const bool direct = nsplit == 1 && !sym && alpha == 1.0 && beta == 0.0 && prec == kF64;
   hipError_t e;
-  if (direct) return launch_gemm_t<kF64, double>(g, C, g.N, 1, 0, st);
+  if (direct) {
+    const int t128 = (g.N + 127) / 128, t160 = (g.N + 159) / 160;
+    if (t160 < t128 && (int64_t)t160 * 160 - g.N < (int64_t)t128 * 128 - g.N)
+      return launch_gemm_t<kF64, double, 5>(g, C, g.N, 1, 0, st);
+    return launch_gemm_t<kF64, double>(g, C, g.N, 1, 0, st);
+  }
   if (!ws) return hipErrorInvalidValue;
   const int64_t stride = (int64_t)g.M * g.N;
   const int64_t tot = (int64_t)g.M * g.N;

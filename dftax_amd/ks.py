@@ -37,9 +37,11 @@ class HipSpec:
     FP32 copies of B and the AO grid so the SCF can run its early iterations
     with split-BF16 MFMA GEMMs (FP32 accumulate) and refine in FP64.
     ``gemm`` picks the FP64 GEMM kernels: ``"auto"`` (library symmetric
-    exchange kernel, rocBLAS for the plain products), ``"rocblas"`` or
-    ``"mfma"`` (library kernels for all). Mixed steps always use the
-    library's split-BF16 kernels. ``metric_cutoff`` is the relative eigenvalue cutoff of the
+    exchange kernel, the library T = B^T C when 129..160 occupied orbitals fit
+    its 160-wide tile, rocBLAS for the other plain products), ``"rocblas"`` or
+    ``"mfma"`` (library kernels for all). Mixed steps use the library's
+    split-BF16 kernels for the exchange products and FP32 rocBLAS products on
+    the compact XC grid. ``metric_cutoff`` is the relative eigenvalue cutoff of the
     metric pseudo-inverse (terms.py:190-192). ``density`` is how each SCF step
     gets the aufbau density from the orthonormal-basis Fock matrix:
     ``"purify"`` (SP2 spectral projection with FP64 MFMA products and a

@@ -563,3 +563,22 @@ def test_xc_blocks_mixed_scf(dimer_recompute, torch_cuda):
     rb = scf(bm, precision="mixed", e_tol=1e-9)
     _say(f"dense FP64 {rd.e_tot:.12f}; compact mixed + FP64 {rb.e_tot:.12f} ({rb.n_iter} it)")
     assert rb.converged and abs(rb.e_tot - rd.e_tot) < 1e-8
+
+
+def test_exchange_t160_matches_rocblas(torch_cuda):
+    """The 160-wide FP64 tile of the exchange product T = B^T C (129..160 occupied
+    orbitals: (H2O)_28 / def2-SVP has 140) against rocBLAS: same Fock matrix."""
+    from dftax_amd import KS, becke, df, hip, water_cluster
+    from dftax_amd.scf import initial_density
+    torch = torch_cuda
+    mol = water_cluster((4, 7, 1), seed=0, basis="def2-svp")
+    F = {}
+    for g in ("rocblas", "auto"):
+        ks = KS(mol, "PBE0", grid=becke(30, 110), coulomb=df(engine=hip(gemm=g)))
+        assert ks.nocc[0] == 140
+        E, F[g] = ks.coulomb.energy_fock(initial_density(ks, None))
+        F[g] = F[g].cpu().numpy()
+        ks.coulomb.close()
+    d = np.abs(F["auto"] - F["rocblas"]).max()
+    _say(f"(H2O)_28 SVP: 160-wide FP64 T vs rocBLAS max|dF| {d:.2e}")
+    assert d < 1e-11
