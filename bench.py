@@ -148,6 +148,30 @@ def main() -> int:
         trf = time.perf_counter() - t
         note(f"mixed SCF + FP64 energy refinement: {rrf.n_iter} iterations, {trf:.1f} s")
 
+    # response Fock matrix of the Newton Hessian-vector product at the converged density:
+    # dP = U V^T + V U^T with U = C_occ and V = -2 C_virt v^T (one CG direction); materialised
+    # workloads only (with recomputed slabs one product costs two slab passes per factor sign)
+    resp = None
+    if r64 is not None and c.plan.aux_chunk == 0:
+        c.set_precision(False)
+        C = r64.mo_coeff[0]
+        no = ks.nocc[0]
+        g = torch.Generator(device=C.device).manual_seed(0)
+        v = torch.randn((no, C.shape[1] - no), dtype=torch.float64, device=C.device, generator=g)
+        U, V = C[:, :no].contiguous(), (-2.0 * (C[:, no:] @ v.T)).contiguous()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        c.response_begin(r64.P[0])
+        torch.cuda.synchronize()
+        t_begin = time.perf_counter() - t
+        t = time.perf_counter()
+        for _ in range(3):
+            c.response_fock(U, V)
+        torch.cuda.synchronize()
+        resp = {"begin_ms": round(1e3 * t_begin, 3), "response_fock_ms": round(1e3 * (time.perf_counter() - t) / 3, 3),
+                "rank": int(no)}
+        note(f"response Fock: {resp['response_fock_ms']:.1f} ms per Hessian-vector product")
+
     nao, nfit, nocc, ng = c.nao, c.nfit, c.nocc, c.ngrid
     recompute = c.plan.aux_chunk > 0 or c.stats()["ao_block"] > 0
     flops = {
@@ -247,6 +271,7 @@ def main() -> int:
         "stages": stages,
         "mixed": mixed,
         "memory_plan": {"aux_chunk": c.plan.aux_chunk, "ao_block": c.plan.ao_block, **c.stats()},
+        "newton_response": resp,
         "build_s": round(t_build, 3),
         "build_stages_ms": {k: round(v, 1) for k, v in build.items()},
     }
