@@ -111,3 +111,53 @@ def test_memory_plan():
     forced = memory_plan(24, 115, 36000, 5, "fp64", 16, 100, 200 * gb)
     assert forced.aux_chunk == 16 and forced.ao_block == 100
     assert memory_plan(2752, 7360, 2_070_000, 320, "fp64", None, 0, 245 * gb).aux_chunk == 0
+
+
+def test_steihaug_cg_host_logic():
+    """The Newton step solver (newton.py:44-94 semantics) on explicit quadratics:
+    interior solution H p = -g for a positive definite H and a large radius, a
+    boundary step of norm radius when the radius binds, and the boundary along
+    negative curvature for an indefinite H."""
+    import torch
+    from dftax_amd.newton import _steihaug_cg
+    torch.manual_seed(0)
+    n = 12
+    Q, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64))
+    H = Q @ torch.diag(torch.linspace(0.5, 4.0, n, dtype=torch.float64)) @ Q.T
+    g = torch.randn(n, dtype=torch.float64)
+    hvp = lambda v: H @ v
+    p, nh = _steihaug_cg(hvp, g, 1e3, 64)
+    assert torch.allclose(p, torch.linalg.solve(H, -g), atol=1e-10)
+    p, _ = _steihaug_cg(hvp, g, 0.05, 64)
+    assert abs(float(torch.linalg.norm(p)) - 0.05) < 1e-12 and float(g @ p) < 0.0
+    Hn = Q @ torch.diag(torch.linspace(-1.0, 4.0, n, dtype=torch.float64)) @ Q.T
+    p, _ = _steihaug_cg(lambda v: Hn @ v, g, 0.7, 64)
+    assert abs(float(torch.linalg.norm(p)) - 0.7) < 1e-12
+    assert float(g @ p + 0.5 * p @ (Hn @ p)) < 0.0
+
+
+def test_rotate_is_orthogonal_and_density_preserving():
+    """C exp(K) keeps C^T S C = 1 for S-orthonormal C (newton.py:97-106)."""
+    import torch
+    from dftax_amd.newton import _rotate
+    torch.manual_seed(1)
+    n, no = 9, 3
+    C = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64))[0]
+    k = 0.3 * torch.randn(no, n - no, dtype=torch.float64)
+    C2 = _rotate(C, k, no)
+    assert torch.allclose(C2.T @ C2, torch.eye(n, dtype=torch.float64), atol=1e-12)
+    assert torch.allclose(_rotate(C, torch.zeros_like(k), no), C, atol=1e-14)
+
+
+def test_serving_latency_slowdowns():
+    """Per-request slowdown against each molecule's own isolated time."""
+    from dftax_amd.serve import Request, latency_stats
+    reqs = []
+    for i, (arr, start, end) in enumerate([(0.0, 0.0, 1.0), (0.5, 0.5, 2.5), (1.0, 1.2, 1.6)]):
+        r = Request(rid=i, mol=None, arrival=arr)
+        r.start, r.end, r.converged = start, end, True
+        reqs.append(r)
+    st = latency_stats(reqs, {0: 1.0, 1: 1.0, 2: 0.2})
+    assert st["ok"] == 3 and abs(st["p50_s"] - 1.0) < 1e-12
+    assert abs(st["slowdown_p50"] - 2.0) < 1e-12 and st["slowdown_p99"] > 2.9
+    assert abs(st["mean_queue_wait_s"] - 0.2 / 3) < 1e-12
