@@ -378,6 +378,95 @@ __global__ __launch_bounds__(256) void k_xc1(int64_t ng, int64_t nao, const TA* 
   }
 }
 
+// Single-read variant for short rows (the compact grid's blocks): one
+// wavefront per point, four points per block; each lane keeps its J vectors of
+// the four AO plane rows in registers between the rho / grad-rho reduction and
+// the W update, so every AO value is read once. Needs the vector layout and
+// nao <= 64 J PER.
+template <typename TA, typename TZ, int J>
+__global__ __launch_bounds__(256) void k_xcw(int64_t ng, int64_t nao, const TA* __restrict__ ao,
+                                             const double* __restrict__ wts, TZ* __restrict__ zw, int xc_id,
+                                             double thresh, double* __restrict__ exc, double* __restrict__ rho_out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ng) return;
+  constexpr int PER = 16 / sizeof(TA);
+  typedef TA vecA __attribute__((ext_vector_type(PER)));
+  typedef TZ vecZ __attribute__((ext_vector_type(PER)));
+  const int64_t plane = ng * nao;
+  const vecA* p0 = reinterpret_cast<const vecA*>(ao + g * nao);
+  const vecA* p1 = reinterpret_cast<const vecA*>(ao + plane + g * nao);
+  const vecA* p2 = reinterpret_cast<const vecA*>(ao + 2 * plane + g * nao);
+  const vecA* p3 = reinterpret_cast<const vecA*>(ao + 3 * plane + g * nao);
+  vecZ* zrow = reinterpret_cast<vecZ*>(zw + g * nao);
+  const int nv = (int)(nao / PER);
+  vecA a0[J], a1[J], a2[J], a3[J];
+  double r = 0, gx = 0, gy = 0, gz = 0;
+#pragma unroll
+  for (int jv = 0; jv < J; ++jv) {
+    const int n = lane + 64 * jv;
+    if (n < nv) {
+      const vecZ z = zrow[n];
+      a0[jv] = p0[n]; a1[jv] = p1[n]; a2[jv] = p2[n]; a3[jv] = p3[n];
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const double zz = (double)z[q];
+        r += zz * (double)a0[jv][q];
+        gx += zz * (double)a1[jv][q];
+        gy += zz * (double)a2[jv][q];
+        gz += zz * (double)a3[jv][q];
+      }
+    }
+  }
+  r = wave_sum(r);
+  gx = 2.0 * wave_sum(gx);
+  gy = 2.0 * wave_sum(gy);
+  gz = 2.0 * wave_sum(gz);
+  if (rho_out && lane == 0) {
+    rho_out[4 * g] = r; rho_out[4 * g + 1] = gx; rho_out[4 * g + 2] = gy; rho_out[4 * g + 3] = gz;
+  }
+  double a = 0.0, bx = 0.0, by = 0.0, bz = 0.0, e = 0.0;
+  if (r > thresh) {
+    const double w = wts[g];
+    D rho{r, 1.0, 0.0};
+    D sig{gx * gx + gy * gy + gz * gz, 0.0, 1.0};
+    D f = eps_xc(xc_id, rho, sig) * rho;
+    e = w * f.v;
+    a = 0.5 * w * f.r;
+    const double t = 2.0 * w * f.s;
+    bx = t * gx; by = t * gy; bz = t * gz;
+  }
+  if (lane == 0) exc[g] = e;
+#pragma unroll
+  for (int jv = 0; jv < J; ++jv) {
+    const int n = lane + 64 * jv;
+    if (n < nv) {
+      vecZ o;
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+        o[q] = (TZ)(a * (double)a0[jv][q] + bx * (double)a1[jv][q] + by * (double)a2[jv][q] + bz * (double)a3[jv][q]);
+      zrow[n] = o;
+    }
+  }
+}
+
+template <typename TA, typename TZ>
+static bool launch_xcw(int64_t ng, int64_t nao, const TA* ao, const double* w, TZ* zw, int xc_id, double thresh,
+                       double* exc, double* rho_out, hipStream_t st) {
+  constexpr int PER = 16 / sizeof(TA);
+  const int64_t j = (nao / PER + 63) / 64;
+  const dim3 grid((unsigned)((ng + 3) / 4)), blk(256);
+#define DFKS_XCW(JJ)                                                                                               \
+  hipLaunchKernelGGL((k_xcw<TA, TZ, JJ>), grid, blk, 0, st, ng, nao, ao, w, zw, xc_id, thresh, exc, rho_out)
+  if (j <= 1) DFKS_XCW(1);
+  else if (j <= 2) DFKS_XCW(2);
+  else if (j <= 3) DFKS_XCW(3);
+  else if (j <= 4) DFKS_XCW(4);
+  else return false;
+#undef DFKS_XCW
+  return true;
+}
+
 template <typename TA, typename TZ>
 static bool launch_xc1(int64_t ng, int64_t nao, const TA* ao, const double* w, TZ* zw, int xc_id, double thresh,
                        double* exc, hipStream_t st) {
@@ -397,10 +486,11 @@ hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w,
                      double* exc, double* rho_out, hipStream_t st) {
   if (ng == 0) return hipSuccess;
   const bool vec = nao % 2 == 0 && (uintptr_t)ao % 16 == 0 && (uintptr_t)zw % 16 == 0;
-  // one block per point reads each AO value once, which pays off on long rows; short
-  // rows (the compact grid's blocks) keep one wavefront per point and four points per block
+  // every AO value read once: one block per point on long rows, one wavefront per
+  // point with the row slices in registers on short rows (the compact grid's blocks)
   if (vec && !rho_out && nao >= 768 && ng < (1LL << 31) && launch_xc1(ng, nao, ao, w, zw, xc_id, thresh, exc, st))
     return hipGetLastError();
+  if (vec && nao < 768 && launch_xcw(ng, nao, ao, w, zw, xc_id, thresh, exc, rho_out, st)) return hipGetLastError();
   if (vec)
     hipLaunchKernelGGL((k_xc<double, double, true>), dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w,
                        zw, xc_id, thresh, exc, rho_out);
@@ -414,7 +504,7 @@ hipError_t launch_xc_f32(int64_t ng, int64_t nao, const float* ao, const double*
                          double thresh, double* exc, hipStream_t st) {
   if (ng == 0) return hipSuccess;
   const bool vec = nao % 4 == 0 && (uintptr_t)ao % 16 == 0 && (uintptr_t)zw % 16 == 0;
-
+  if (vec && launch_xcw(ng, nao, ao, w, zw, xc_id, thresh, exc, (double*)nullptr, st)) return hipGetLastError();
   if (vec)
     hipLaunchKernelGGL((k_xc<float, float, true>), dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w,
                        zw, xc_id, thresh, exc, (double*)nullptr);
