@@ -18,7 +18,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -1042,14 +1044,65 @@ void coulomb_recompute(dfks_ctx* h, hipStream_t st) {
   }
 }
 
+// FP64 exchange product T[m,Q,i] = sum_l B[l,m,Q] Cf[l,i], autotuned: the first
+// product of each (device, M, N, K) shape runs both candidates -- the library
+// kernel (128- or 160-wide N tiles, whichever pads less) and rocBLAS -- timed
+// with HIP events on the work stream, and the process keeps the faster one for
+// that shape. Both compute the same T to rounding.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int64_t, int, int>, int> g_tune;  // -> 0 rocBLAS, 1 library
+
+void t_product_run(dfks_ctx* h, int choice, const double* C, int nc, hipStream_t st) {
+  const int nao = h->nao, nfit = h->nfit;
+  const double one = 1.0, zero = 0.0;
+  if (choice == 1) {
+    GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B, (int64_t)nao * nfit, false, C, nao, true};
+    HIPCHK(launch_gemm(kF64, g, h->d_Tx, 1.0, 0.0, 1, 0, nullptr, st));
+  } else {
+    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc, (int64_t)nao * nfit,
+                           nao, &one, C, nao, h->d_B, (int64_t)nao * nfit, &zero, h->d_Tx, nc));
+  }
+}
+
+void t_product(dfks_ctx* h, const double* C, int nc, hipStream_t st) {
+  if (h->opt.gemm != 0) {
+    t_product_run(h, h->opt.gemm == 2 ? 1 : 0, C, nc, st);
+    return;
+  }
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, (int64_t)h->nao * h->nfit, nc, h->nao);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      const int c = it->second;
+      t_product_run(h, c, C, nc, st);
+      return;
+    }
+  }
+  hipEvent_t ev[3];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(ev[0], st));
+  t_product_run(h, 0, C, nc, st);
+  HIPCHK(hipEventRecord(ev[1], st));
+  t_product_run(h, 1, C, nc, st);
+  HIPCHK(hipEventRecord(ev[2], st));
+  HIPCHK(hipEventSynchronize(ev[2]));
+  float t_rb = 0.f, t_lib = 0.f;
+  HIPCHK(hipEventElapsedTime(&t_rb, ev[0], ev[1]));
+  HIPCHK(hipEventElapsedTime(&t_lib, ev[1], ev[2]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.emplace(key, t_lib < t_rb ? 1 : 0);
+}
+
 // gamma, J and K of P = sum_i s_i c_i c_i^T from its factor alone:
 // gamma = sum_i s_i T_i c_i, J = B gamma, K = sum_i s_i T_i^T T_i.
 void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, int nf, hipStream_t st) {
   const int nao = h->nao, nfit = h->nfit;
   const int64_t nn = (int64_t)nao * nao;
-  const double one = 1.0, zero = 0.0;
   const bool mixed = h->mixed_now;
-  const bool lib = h->opt.gemm == 2;
   const bool lib_k = h->opt.gemm != 1;
   // signs to the device (host copy kept in the handle until the next call)
   if (nf > (mixed ? nao : 2 * nao)) fail(1, "too many factor columns");
@@ -1075,16 +1128,8 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
       GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
                  h->d_Cf32 + (int64_t)i0 * nao, nao, true};
       HIPCHK(launch_gemm_f32out(g, h->d_T32, st));
-    } else if (lib || (h->opt.gemm == 0 && nc > 128 && nc <= 160)) {
-      // the library kernel's 160-wide tile covers 129..160 factor columns in one N tile
-      // (rocBLAS pads them to two 128-wide tiles and reads B twice)
-      GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B, (int64_t)nao * nfit, false, Cf + (int64_t)i0 * nao,
-                 nao, true};
-      HIPCHK(launch_gemm(kF64, g, h->d_Tx, 1.0, 0.0, 1, 0, nullptr, st));
     } else {
-      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc,
-                             (int64_t)nao * nfit, nao, &one, Cf + (int64_t)i0 * nao, nao, h->d_B,
-                             (int64_t)nao * nfit, &zero, h->d_Tx, nc));
+      t_product(h, Cf + (int64_t)i0 * nao, nc, st);
     }
     h->mark(kStExchangeGemm + 1, st);
     // gamma += sum_i s_i T_i c_i (timed with the exchange product K below)

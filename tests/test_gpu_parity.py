@@ -567,18 +567,20 @@ def test_xc_blocks_mixed_scf(dimer_recompute, torch_cuda):
 
 def test_exchange_t160_matches_rocblas(torch_cuda):
     """The 160-wide FP64 tile of the exchange product T = B^T C (129..160 occupied
-    orbitals: (H2O)_28 / def2-SVP has 140) against rocBLAS: same Fock matrix."""
+    orbitals: (H2O)_28 / def2-SVP has 140) and the autotuned choice against
+    rocBLAS: same Fock matrix."""
     from dftax_amd import KS, becke, df, hip, water_cluster
     from dftax_amd.scf import initial_density
     torch = torch_cuda
     mol = water_cluster((4, 7, 1), seed=0, basis="def2-svp")
     F = {}
-    for g in ("rocblas", "auto"):
+    for g in ("rocblas", "mfma", "auto"):
         ks = KS(mol, "PBE0", grid=becke(30, 110), coulomb=df(engine=hip(gemm=g)))
         assert ks.nocc[0] == 140
         E, F[g] = ks.coulomb.energy_fock(initial_density(ks, None))
         F[g] = F[g].cpu().numpy()
         ks.coulomb.close()
-    d = np.abs(F["auto"] - F["rocblas"]).max()
-    _say(f"(H2O)_28 SVP: 160-wide FP64 T vs rocBLAS max|dF| {d:.2e}")
-    assert d < 1e-11
+    d = np.abs(F["mfma"] - F["rocblas"]).max()
+    da = np.abs(F["auto"] - F["rocblas"]).max()
+    _say(f"(H2O)_28 SVP: 160-wide FP64 T vs rocBLAS max|dF| {d:.2e}; autotuned {da:.2e}")
+    assert d < 1e-11 and da < 1e-11
