@@ -105,12 +105,16 @@ hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, dou
 hipError_t launch_gemm_f32out(const GemmArgs& g, float* C, hipStream_t st);
 hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, int64_t bs_a, int64_t bs_b,
                                       int64_t bs_c, hipStream_t st);
+hipError_t launch_gemm_f64_batched(const GemmArgs& g, double* C, int batch, int64_t bs_a, int64_t bs_b,
+                                   int64_t bs_c, hipStream_t st);
+hipError_t launch_gemm_w8(const GemmArgs& g, double* C, hipStream_t st);
 hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s);
 hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, hipStream_t s);
 hipError_t launch_sp2_trace_update(int n, double* X, const double* X2, int nocc, double* st, hipStream_t s);
 hipError_t launch_diag(int n, const double* R, double* d, hipStream_t s);
 hipError_t launch_pchol_panel(int n, int k0, int b, const double* R, double* d, double* L, int32_t* piv, double* st,
                               int last, hipStream_t s);
+thread_local GraphRec* g_rec = nullptr;
 }  // namespace dfks
 
 using namespace dfks;
@@ -383,7 +387,6 @@ struct dfks_ctx {
   int sp2_guess = 24;       // iterations the last purification needed (first chunk of the next)
   hipGraphExec_t sp2_exec = nullptr;  // kSp2Block purification iterations as one graph (per nmo)
   int sp2_exec_n = -1;
-  hipStream_t cap_stream = nullptr;   // private non-blocking stream the graph is captured on
   bool c_stale = false;     // d_C / d_eps not yet computed for the last step (purified density)
   int n_eigh_steps = 0;     // steps that fell back to the eigensolver
   // recompute mode (options.aux_chunk > 0): screened bra pairs per (la, lb)
@@ -428,7 +431,6 @@ struct dfks_ctx {
     for (auto& e : scf_ev)
       if (e) (void)hipEventDestroy(e);
     if (sp2_exec) (void)hipGraphExecDestroy(sp2_exec);
-    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     rb_release(rb);
   }
 };
@@ -1049,52 +1051,75 @@ void coulomb_recompute(dfks_ctx* h, hipStream_t st) {
 // kernel (128- or 160-wide N tiles, whichever pads less) and rocBLAS -- timed
 // with HIP events on the work stream, and the process keeps the faster one for
 // that shape. Both compute the same T to rounding.
+// Per-shape choice among the variants of an FP64 product: 0 rocBLAS, 1 the
+// library's MFMA kernel, 2 (where the product has it) the library's 8-wave
+// kernel. The first call of a shape on a device times every variant with HIP
+// events and keeps the fastest; the output of that call is recomputed by the
+// kept variant when it is not the one that ran last, so every call of a shape
+// returns the kept variant's result. The key holds a tag for the product (0:
+// exchange T, 1: XC Z_b, 2: XC Phi_b^T W_b). options.gemm 1, 2 and 3 force
+// variant 0, 1 and 2 (1 where there is no third).
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int64_t, int, int>, int> g_tune;  // -> 0 rocBLAS, 1 library
+std::map<std::tuple<int, int, int64_t, int64_t, int64_t>, int> g_tune;
 
-void t_product_run(dfks_ctx* h, int choice, const double* C, int nc, hipStream_t st) {
-  const int nao = h->nao, nfit = h->nfit;
-  const double one = 1.0, zero = 0.0;
-  if (choice == 1) {
-    GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B, (int64_t)nao * nfit, false, C, nao, true};
-    HIPCHK(launch_gemm(kF64, g, h->d_Tx, 1.0, 0.0, 1, 0, nullptr, st));
-  } else {
-    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc, (int64_t)nao * nfit,
-                           nao, &one, C, nao, h->d_B, (int64_t)nao * nfit, &zero, h->d_Tx, nc));
-  }
-}
-
-void t_product(dfks_ctx* h, const double* C, int nc, hipStream_t st) {
+template <typename Run>
+void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int ncand, Run run, hipStream_t st) {
   if (h->opt.gemm != 0) {
-    t_product_run(h, h->opt.gemm == 2 ? 1 : 0, C, nc, st);
+    run(std::min(h->opt.gemm - 1, ncand - 1));
     return;
   }
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
-  const auto key = std::make_tuple(dev, (int64_t)h->nao * h->nfit, nc, h->nao);
+  const auto key = std::make_tuple(dev, tag, d0, d1, d2);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
       const int c = it->second;
-      t_product_run(h, c, C, nc, st);
+      run(c);
       return;
     }
   }
-  hipEvent_t ev[3];
-  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  hipEvent_t ev[4];
+  for (int c = 0; c <= ncand; ++c) HIPCHK(hipEventCreate(&ev[c]));
   HIPCHK(hipEventRecord(ev[0], st));
-  t_product_run(h, 0, C, nc, st);
-  HIPCHK(hipEventRecord(ev[1], st));
-  t_product_run(h, 1, C, nc, st);
-  HIPCHK(hipEventRecord(ev[2], st));
-  HIPCHK(hipEventSynchronize(ev[2]));
-  float t_rb = 0.f, t_lib = 0.f;
-  HIPCHK(hipEventElapsedTime(&t_rb, ev[0], ev[1]));
-  HIPCHK(hipEventElapsedTime(&t_lib, ev[1], ev[2]));
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tune.emplace(key, t_lib < t_rb ? 1 : 0);
+  for (int c = 0; c < ncand; ++c) {
+    run(c);
+    HIPCHK(hipEventRecord(ev[c + 1], st));
+  }
+  HIPCHK(hipEventSynchronize(ev[ncand]));
+  int best = 0;
+  float t_best = 0.f;
+  for (int c = 0; c < ncand; ++c) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, ev[c], ev[c + 1]));
+    if (c == 0 || t < t_best) { best = c; t_best = t; }
+  }
+  for (int c = 0; c <= ncand; ++c) (void)hipEventDestroy(ev[c]);
+  int kept;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    kept = g_tune.emplace(key, best).first->second;
+  }
+  if (kept != ncand - 1) run(kept);
+}
+
+// T = B^T C (nao*nfit x nc), FP64
+void t_product(dfks_ctx* h, const double* C, int nc, hipStream_t st) {
+  const int nao = h->nao, nfit = h->nfit;
+  const double one = 1.0, zero = 0.0;
+  tuned_product(h, 0, (int64_t)nao * nfit, nc, nao, 3, [&](int choice) {
+    const GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B, (int64_t)nao * nfit, false, C, nao, true};
+    if (choice == 2) {
+      HIPCHK(launch_gemm_w8(g, h->d_Tx, st));
+    } else if (choice == 1) {
+      HIPCHK(launch_gemm(kF64, g, h->d_Tx, 1.0, 0.0, 1, 0, nullptr, st));
+    } else {
+      RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc,
+                             (int64_t)nao * nfit, nao, &one, C, nao, h->d_B, (int64_t)nao * nfit, &zero, h->d_Tx,
+                             nc));
+    }
+  }, st);
 }
 
 // gamma, J and K of P = sum_i s_i c_i c_i^T from its factor alone:
@@ -1205,10 +1230,17 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
     const double* ao = h->d_xc_ao + bn.ao_off;
     double* zw = h->d_xc_zw;
     HIPCHK(launch_gather_sym<double>(nb, n, cols, P, nao, q, st));
-    // column-major: Z_b^T (n x G) = Pg_b^T (n x n) Phi_b^T (n x G)
-    RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_transpose, rocblas_operation_none, n, kXcG, n, &one,
-                                        q, n, (rocblas_stride)n * n, ao, n, (rocblas_stride)kXcG * n, &zero, zw, n,
-                                        (rocblas_stride)kXcG * n, (int)nb));
+    // Z_b (G x n, row-major) = Phi_b Pg_b; rocBLAS column-major: Z_b^T = Pg_b^T Phi_b^T
+    tuned_product(h, 1, n, nb, 0, 2, [&](int choice) {
+      if (choice == 1) {
+        const GemmArgs g{kXcG, n, n, ao, n, true, q, n, true};
+        HIPCHK(launch_gemm_f64_batched(g, zw, (int)nb, (int64_t)kXcG * n, (int64_t)n * n, (int64_t)kXcG * n, st));
+      } else {
+        RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_transpose, rocblas_operation_none, n, kXcG, n,
+                                            &one, q, n, (rocblas_stride)n * n, ao, n, (rocblas_stride)kXcG * n, &zero,
+                                            zw, n, (rocblas_stride)kXcG * n, (int)nb));
+      }
+    }, st);
     if (mode == kXcEnergy)
       HIPCHK(launch_xc(rows, n, ao, h->d_xc_w + bn.pt0, zw, h->xc_id, thresh, h->d_xc_exc + bn.pt0, nullptr, st));
     else if (mode == kXcRho)
@@ -1217,10 +1249,17 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
     else
       HIPCHK(launch_fxc(rows, n, ao, zw, h->d_fxc + 8 * bn.pt0, st));
     if (mode == kXcRho) continue;
-    // column-major: prod_b^T (n x n) = W_b^T (n x G) Phi_b (G x n), i.e. prod_b[i][j] = sum_g Phi[g][i] W[g][j]
-    RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_none, rocblas_operation_transpose, n, n, kXcG, &one,
-                                        zw, n, (rocblas_stride)kXcG * n, ao, n, (rocblas_stride)kXcG * n, &zero, q, n,
-                                        (rocblas_stride)n * n, (int)nb));
+    // prod_b[i][j] = sum_g Phi[g][i] W[g][j]; rocBLAS column-major: prod_b^T = W_b^T Phi_b
+    tuned_product(h, 2, n, nb, 0, 2, [&](int choice) {
+      if (choice == 1) {
+        const GemmArgs g{n, n, kXcG, ao, n, false, zw, n, false};
+        HIPCHK(launch_gemm_f64_batched(g, q, (int)nb, (int64_t)kXcG * n, (int64_t)kXcG * n, (int64_t)n * n, st));
+      } else {
+        RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_none, rocblas_operation_transpose, n, n, kXcG,
+                                            &one, zw, n, (rocblas_stride)kXcG * n, ao, n, (rocblas_stride)kXcG * n,
+                                            &zero, q, n, (rocblas_stride)n * n, (int)nb));
+      }
+    }, st);
   }
   if (mode == kXcRho) return;
   h->mark(kStXcKernel + 1, st);
@@ -1245,7 +1284,7 @@ void xc_potential(dfks_ctx* h, const double* P, double* E, hipStream_t st) {
   const int64_t ng = h->ng;
   const double one = 1.0, zero = 0.0;
   const bool mixed = h->mixed_now;
-  const bool lib = h->opt.gemm == 2;
+  const bool lib = h->opt.gemm >= 2;
   if (h->xcb) {
     xc_compact(h, P, E, kXcEnergy, st);
     return;
@@ -1564,26 +1603,28 @@ hipError_t sp2_block(dfks_ctx* h, hipStream_t st) {
   return hipSuccess;
 }
 
-// The same block captured once per handle and nmo as a HIP graph (kernels
-// only: fixed buffers, no memset or copy nodes); one graph launch replaces
-// 4 kSp2Block kernel launches. Captured on a private non-blocking stream
-// (capture is not allowed on the legacy default stream) and launched on the
-// caller's stream. Falls back to eager launches if capture fails.
+// The same block built once per handle and nmo as a HIP graph of kernel
+// nodes (fixed buffers, no memset or copy nodes); one graph launch replaces
+// 4 kSp2Block kernel launches. The graph is recorded node by node (launch_k
+// with a GraphRec installed), not by stream capture: a stream in capture mode
+// would make other threads' legacy-stream work fail while it lasts. Falls
+// back to eager launches if the graph cannot be built.
 void sp2_run_blocks(dfks_ctx* h, int nblocks, hipStream_t st) {
   if (h->sp2_exec_n != h->nmo) {
     if (h->sp2_exec) (void)hipGraphExecDestroy(h->sp2_exec);
     h->sp2_exec = nullptr;
     h->sp2_exec_n = h->nmo;
     hipGraph_t gr = nullptr;
-    if (!h->cap_stream && hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess)
-      h->cap_stream = nullptr;
-    if (h->cap_stream && hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-      const hipError_t e1 = sp2_block(h, h->cap_stream);
-      const hipError_t e2 = hipStreamEndCapture(h->cap_stream, &gr);
-      if (e1 == hipSuccess && e2 == hipSuccess && gr) {
-        if (hipGraphInstantiate(&h->sp2_exec, gr, nullptr, nullptr, 0) != hipSuccess) h->sp2_exec = nullptr;
-      }
-      if (gr) (void)hipGraphDestroy(gr);
+    if (hipGraphCreate(&gr, 0) == hipSuccess && gr) {
+      GraphRec rec;
+      rec.g = gr;
+      g_rec = &rec;
+      const hipError_t e1 = sp2_block(h, st);
+      g_rec = nullptr;
+      if (e1 == hipSuccess && rec.err == hipSuccess &&
+          hipGraphInstantiate(&h->sp2_exec, gr, nullptr, nullptr, 0) != hipSuccess)
+        h->sp2_exec = nullptr;
+      (void)hipGraphDestroy(gr);
       (void)hipGetLastError();
     }
   }
@@ -1738,7 +1779,8 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
       if (opt) h->opt = *opt;
       else h->opt = dfks_options{0, 1e-7, 1e-10, 0, 0, 0, 0, 0.0, 0, 1};
       if (h->opt.precision != 0 && h->opt.precision != 1) fail(1, "precision must be 0 (FP64) or 1 (mixed)");
-      if (h->opt.gemm < 0 || h->opt.gemm > 2) fail(1, "gemm must be 0 (auto), 1 (rocBLAS) or 2 (library MFMA)");
+      if (h->opt.gemm < 0 || h->opt.gemm > 3)
+        fail(1, "gemm must be 0 (auto), 1 (rocBLAS), 2 (library MFMA) or 3 (library 8-wave exchange tile)");
       if (h->opt.density != 0 && h->opt.density != 1) fail(1, "density must be 0 (purification) or 1 (eigensolver)");
       if (h->opt.aux_chunk < 0 || h->opt.ao_block < 0 || h->opt.screen < 0.0 || h->opt.screen >= 1.0)
         fail(1, "aux_chunk and ao_block must be >= 0 and screen in [0, 1)");

@@ -235,9 +235,9 @@ hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, 
 
 hipError_t launch_sp2_trace_update(int n, double* X, const double* X2, int nocc, double* st, hipStream_t s) {
   const int64_t nn = (int64_t)n * n;
-  hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(1024), 0, s, n, X, X2, nocc, st);
-  hipLaunchKernelGGL(k_sp2_update, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, X, X2, st);
-  return hipGetLastError();
+  const hipError_t e = launch_k(k_sp2_trace, dim3(1), dim3(1024), s, n, X, X2, nocc, st);
+  if (e != hipSuccess) return e;
+  return launch_k(k_sp2_update, dim3((unsigned)((nn + 255) / 256)), dim3(256), s, nn, X, X2, st);
 }
 
 hipError_t launch_diag(int n, const double* R, double* d, hipStream_t s) {

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <tuple>
+#include <utility>
+
 #include "c2s_tables.h"
 
 namespace dfks {
@@ -31,6 +34,47 @@ enum Sp2State {
 constexpr int kSp2MaxIter = 128;
 constexpr int kSp2Block = 4;          // purification iterations per graph launch (divides kSp2MaxIter)
 constexpr int kPanel = 32;            // pivoted Cholesky panel width
+
+// Graph recording without stream capture. While a thread has a GraphRec
+// installed in g_rec, launch_k appends each kernel as a node of g_rec->g that
+// depends on the previous one, instead of launching it. Building a graph this
+// way puts no stream into capture mode, so other threads' work on other
+// streams (including library calls that touch the legacy stream) cannot
+// collide with it.
+struct GraphRec {
+  hipGraph_t g = nullptr;
+  hipGraphNode_t last = nullptr;
+  hipError_t err = hipSuccess;
+};
+extern thread_local GraphRec* g_rec;
+
+template <typename... P, typename... A>
+inline hipError_t launch_k(void (*k)(P...), dim3 grid, dim3 block, hipStream_t st, A&&... a) {
+  static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+  if (!g_rec) {
+    hipLaunchKernelGGL(k, grid, block, 0, st, static_cast<P>(std::forward<A>(a))...);
+    return hipGetLastError();
+  }
+  std::tuple<P...> vals(static_cast<P>(std::forward<A>(a))...);
+  void* args[sizeof...(P)];
+  std::apply([&](auto&... v) {
+    int i = 0;
+    ((args[i++] = static_cast<void*>(&v)), ...);
+  }, vals);
+  hipKernelNodeParams p{};
+  p.func = reinterpret_cast<void*>(k);
+  p.gridDim = grid;
+  p.blockDim = block;
+  p.sharedMemBytes = 0;
+  p.kernelParams = args;
+  p.extra = nullptr;
+  hipGraphNode_t node = nullptr;
+  const hipError_t e = hipGraphAddKernelNode(&node, g_rec->g, g_rec->last ? &g_rec->last : nullptr,
+                                             g_rec->last ? 1 : 0, &p);
+  if (e == hipSuccess) g_rec->last = node;
+  else g_rec->err = e;
+  return e;
+}
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 __host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }

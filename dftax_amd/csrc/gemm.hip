@@ -314,6 +314,105 @@ __global__ __launch_bounds__(256, (gemm_blocks_per_cu<PREC, BKC, NJ>())) void k_
   }
 }
 
+// FP64 product C = A B^T for A M-contiguous (A(m, k) = A[k*lda + m]) and B
+// K-contiguous (B(n, k) = B[n*ldb + k]), C row-major with ldc = N: the
+// exchange product T = B^T C over the fitted tensor. One 128 x 160 tile per
+// workgroup of 512 threads = 8 wavefronts (4 along M x 2 along N, each 32 x 80
+// = 2 x 5 MFMA 16x16 blocks), so every SIMD holds two wavefronts and one
+// wavefront's MFMAs cover the other's barrier and LDS waits. The LDS tiles
+// are double-buffered: the next k-step's global loads are issued before the
+// current step's MFMAs, stored to the other buffer after them, and one
+// barrier per k-step separates the two. N tiles on gridDim.y.
+constexpr int kW8BM = 128, kW8BN = 160, kW8BK = 16;
+
+__global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int K, const double* __restrict__ A,
+                                                   int64_t lda, const double* __restrict__ B, int64_t ldb,
+                                                   double* __restrict__ C, int vec_b) {
+  __shared__ double sA[2][kW8BM][kW8BK + 1];
+  __shared__ double sB[2][kW8BN][kW8BK + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t m0 = (int64_t)blockIdx.x * kW8BM;
+  const int n0 = blockIdx.y * kW8BN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 80;
+  const int li = lane & 15, lk = lane >> 4;
+  // A slice: row tid % 128, k offsets 4 (tid / 128) .. +3 (coalesced along m)
+  const int ar = tid & 127, ak = (tid >> 7) * 4;
+  // B slice (threads < 320): row tid / 2, k offsets 8 (tid % 2) .. +7
+  const int br = tid >> 1, bk = (tid & 1) * 8;
+  const bool has_b = tid < 2 * kW8BN;
+  const bool a_in = m0 + ar < M;
+  const bool b_in = has_b && n0 + br < N;
+  const double* __restrict__ pa = A + m0 + ar;
+  const double* __restrict__ pb = B + (int64_t)(n0 + (b_in ? br : 0)) * ldb;
+  double ra[4], rb[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = k0 + ak + j;
+      ra[j] = (a_in && k < K) ? pa[(int64_t)k * lda] : 0.0;
+    }
+    if (!has_b) return;
+    const int kb = k0 + bk;
+    if (b_in && vec_b && kb + 8 <= K) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double2 x = *reinterpret_cast<const double2*>(pb + kb + 2 * v);
+        rb[2 * v] = x.x;
+        rb[2 * v + 1] = x.y;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) rb[j] = (b_in && kb + j < K) ? pb[kb + j] : 0.0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sA[buf][ar][ak + j] = ra[j];
+    if (has_b) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sB[buf][br][bk + j] = rb[j];
+    }
+  };
+  dbl4 acc[2][5];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
+  const int nk = (K + kW8BK - 1) / kW8BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int t = 0; t < nk; ++t) {
+    const int cur = t & 1;
+    if (t + 1 < nk) load((t + 1) * kW8BK);
+#pragma unroll
+    for (int kk = 0; kk < kW8BK; kk += 4) {
+      double a[2], b[5];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = sA[cur][wm + 16 * i + li][kk + lk];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) b[j] = sB[cur][wn + 16 * j + li][kk + lk];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (t + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+  // v_mfma_f64_16x16x4f64 result: lane l, register r holds row (l/16) + 4r, column l%16
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm + 16 * i + lk + 4 * r;
+        const int gn = n0 + wn + 16 * j + li;
+        if (gm < M && gn < N) C[gm * N + gn] = acc[i][j][r];
+      }
+}
+
 // out[m*ldo + n] = alpha * sum_z ws[z][m][n] (+ beta * out); symmetric mode reads
 // the upper triangle (m <= n) and mirrors it. Fixed summation order.
 template <typename W, typename Out>
@@ -362,19 +461,14 @@ static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nspl
   constexpr int PER = 16 / sizeof(In);
   const int vec_a = ((uintptr_t)A % 16 == 0) && (g.lda % PER == 0);
   const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % PER == 0);
-  if (g.a_kcontig && g.b_kcontig)
-    hipLaunchKernelGGL((k_gemm<PREC, true, true, Out, NJ>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb, C,
-                       ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
-  else if (g.a_kcontig && !g.b_kcontig)
-    hipLaunchKernelGGL((k_gemm<PREC, true, false, Out, NJ>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
-                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
-  else if (!g.a_kcontig && g.b_kcontig)
-    hipLaunchKernelGGL((k_gemm<PREC, false, true, Out, NJ>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
-                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
-  else
-    hipLaunchKernelGGL((k_gemm<PREC, false, false, Out, NJ>), grid, dim3(256), 0, st, g.M, g.N, g.K, A, g.lda, B, g.ldb,
-                       C, ldc, split_stride, kchunk, sym, ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
-  return hipGetLastError();
+  const auto run = [&](auto kern) {
+    return launch_k(kern, grid, dim3(256), st, g.M, g.N, g.K, A, g.lda, B, g.ldb, C, ldc, split_stride, kchunk, sym,
+                    ntiles, vec_a, vec_b, bs_a, bs_b, bs_c);
+  };
+  if (g.a_kcontig && g.b_kcontig) return run(k_gemm<PREC, true, true, Out, NJ>);
+  if (g.a_kcontig && !g.b_kcontig) return run(k_gemm<PREC, true, false, Out, NJ>);
+  if (!g.a_kcontig && g.b_kcontig) return run(k_gemm<PREC, false, true, Out, NJ>);
+  return run(k_gemm<PREC, false, false, Out, NJ>);
 }
 
 // C = alpha * A B^T (+ beta C), C row-major M x N (ldc = N), FP64 output.
@@ -399,15 +493,13 @@ hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, dou
   if (prec == kF64) {
     e = launch_gemm_t<kF64, double>(g, static_cast<double*>(ws), g.N, nsplit, sym, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_gemm_reduce<double, double>), dim3(nb), dim3(256), 0, st, g.M, g.N, nsplit,
-                       static_cast<const double*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
-  } else {
-    e = launch_gemm_t<kBF16x3, float>(g, static_cast<float*>(ws), g.N, nsplit, sym, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_gemm_reduce<float, double>), dim3(nb), dim3(256), 0, st, g.M, g.N, nsplit,
-                       static_cast<const float*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
+    return launch_k(k_gemm_reduce<double, double>, dim3(nb), dim3(256), st, g.M, g.N, nsplit,
+                    static_cast<const double*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
   }
-  return hipGetLastError();
+  e = launch_gemm_t<kBF16x3, float>(g, static_cast<float*>(ws), g.N, nsplit, sym, st);
+  if (e != hipSuccess) return e;
+  return launch_k(k_gemm_reduce<float, double>, dim3(nb), dim3(256), st, g.M, g.N, nsplit,
+                  static_cast<const float*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
 }
 
 // FP32 output variant (mixed path intermediates): C = A B^T, no split, no sym.
@@ -430,6 +522,30 @@ hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, in
   if (t160 < t128 && (int64_t)t160 * 160 - g.N < (int64_t)t128 * 128 - g.N)
     return launch_gemm_t<kBF16x3, float, 5>(g, C, g.N, 1, 0, st, batch, bs_a, bs_b, bs_c);
   return launch_gemm_t<kBF16x3, float>(g, C, g.N, 1, 0, st, batch, bs_a, bs_b, bs_c);
+}
+
+// C = A B^T in FP64 by k_gemm_w8 (A M-contiguous, B K-contiguous, ldc = N).
+hipError_t launch_gemm_w8(const GemmArgs& g, double* C, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (g.a_kcontig || !g.b_kcontig || g.K > 0x7fffffff) return hipErrorInvalidValue;
+  const double* A = static_cast<const double*>(g.A);
+  const double* B = static_cast<const double*>(g.B);
+  const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % 2 == 0);
+  const dim3 grid((unsigned)(((int64_t)g.M + kW8BM - 1) / kW8BM), (unsigned)((g.N + kW8BN - 1) / kW8BN));
+  return launch_k(k_gemm_w8, grid, dim3(512), st, (int64_t)g.M, g.N, (int)g.K, A, g.lda, B, g.ldb, C, vec_b);
+}
+
+// Batched FP64 variant (no split, FP64 accumulate): for b < batch,
+// C + b*bs_c = (A + b*bs_a)(B + b*bs_b)^T with ldc = N; 128- or 160-wide N
+// tiles, whichever pads N less (the per-block products of the compact XC grid).
+hipError_t launch_gemm_f64_batched(const GemmArgs& g, double* C, int batch, int64_t bs_a, int64_t bs_b,
+                                   int64_t bs_c, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0 || batch <= 0) return hipSuccess;
+  if (batch > 65535) return hipErrorInvalidValue;
+  const int t128 = (g.N + 127) / 128, t160 = (g.N + 159) / 160;
+  if (t160 < t128 && (int64_t)t160 * 160 - g.N < (int64_t)t128 * 128 - g.N)
+    return launch_gemm_t<kF64, double, 5>(g, C, g.N, 1, 0, st, batch, bs_a, bs_b, bs_c);
+  return launch_gemm_t<kF64, double>(g, C, g.N, 1, 0, st, batch, bs_a, bs_b, bs_c);
 }
 
 }  // namespace dfks

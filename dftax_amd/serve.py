@@ -72,7 +72,7 @@ class Server:
     """Admission by HBM budget, one worker thread and HIP stream per slot."""
 
     def __init__(self, xc: str = "B3LYP", *, auxbasis: str = "even-tempered-jk", grid: Becke | None = None,
-                 precision: str = "mixed", slots: int = 8, budget_bytes: int | None = None, e_tol: float = 1e-8,
+                 precision: str = "mixed", slots: int = 16, budget_bytes: int | None = None, e_tol: float = 1e-8,
                  d_tol: float = 1e-6, device: int = 0, refine: int | None = 3):
         self.xc = xc
         self.auxbasis = auxbasis
@@ -217,7 +217,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--rate", type=float, nargs="+", default=[1.0, 2.0])
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--xc", default="B3LYP")
     ap.add_argument("--precision", default="mixed")
     ap.add_argument("--isolated", type=int, default=0,

@@ -69,9 +69,11 @@ typedef struct {
   double metric_cutoff;    /* relative eigenvalue cutoff of the metric pseudo-inverse (1e-7) */
   double rho_thresh;       /* density mask of the XC quadrature (1e-10) */
   int32_t keep_int3c;      /* 1: keep the unfitted (mn|P) tensor for inspection */
-  int32_t gemm;            /* FP64 products: 0 library symmetric exchange kernel + rocBLAS for the plain
-                              T, Z, V products (default); 1 rocBLAS for all; 2 library kernels for all.
-                              Mixed steps always use the library's split-BF16 kernels. */
+  int32_t gemm;            /* FP64 products: 0 library symmetric exchange kernel, and for the exchange
+                              product T and the compact-grid XC products the fastest of rocBLAS and the
+                              library kernels, timed once per shape and device (default); 1 rocBLAS
+                              for all; 2 library kernels for all; 3 as 2 with the 8-wavefront tile for
+                              T. Mixed steps always use the library's split-BF16 kernels. */
   int32_t density;         /* SCF density from the orthonormal-basis Fock matrix: 0 spectral projection
                               (SP2 purification + pivoted Cholesky factor; the eigensolver runs only
                               when purification does not converge, and once for dfks_scf_result's

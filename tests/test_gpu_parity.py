@@ -566,21 +566,25 @@ def test_xc_blocks_mixed_scf(dimer_recompute, torch_cuda):
 
 
 def test_exchange_t160_matches_rocblas(torch_cuda):
-    """The 160-wide FP64 tile of the exchange product T = B^T C (129..160 occupied
-    orbitals: (H2O)_28 / def2-SVP has 140) and the autotuned choice against
-    rocBLAS: same Fock matrix."""
+    """The library kernels of the exchange product T = B^T C against rocBLAS:
+    the 160-wide FP64 tile, the 8-wavefront 128 x 160 tile and the autotuned
+    choice give the same Fock matrix. (H2O)_28 / def2-SVP has 140 occupied
+    orbitals (one partial N tile, nao = 672 a multiple of the k-step);
+    (H2O)_33 has 165 (two N tiles, the second partial) and nao = 792 (a
+    partial last k-step)."""
     from dftax_amd import KS, becke, df, hip, water_cluster
     from dftax_amd.scf import initial_density
     torch = torch_cuda
-    mol = water_cluster((4, 7, 1), seed=0, basis="def2-svp")
-    F = {}
-    for g in ("rocblas", "mfma", "auto"):
-        ks = KS(mol, "PBE0", grid=becke(30, 110), coulomb=df(engine=hip(gemm=g)))
-        assert ks.nocc[0] == 140
-        E, F[g] = ks.coulomb.energy_fock(initial_density(ks, None))
-        F[g] = F[g].cpu().numpy()
-        ks.coulomb.close()
-    d = np.abs(F["mfma"] - F["rocblas"]).max()
-    da = np.abs(F["auto"] - F["rocblas"]).max()
-    _say(f"(H2O)_28 SVP: 160-wide FP64 T vs rocBLAS max|dF| {d:.2e}; autotuned {da:.2e}")
-    assert d < 1e-11 and da < 1e-11
+    for shape, nocc in (((4, 7, 1), 140), ((3, 11, 1), 165)):
+        mol = water_cluster(shape, seed=0, basis="def2-svp")
+        F = {}
+        for g in ("rocblas", "mfma", "mfma8", "auto"):
+            ks = KS(mol, "PBE0", grid=becke(30, 110), coulomb=df(engine=hip(gemm=g)))
+            assert ks.nocc[0] == nocc
+            E, F[g] = ks.coulomb.energy_fock(initial_density(ks, None))
+            F[g] = F[g].cpu().numpy()
+            ks.coulomb.close()
+        d = {g: float(np.abs(F[g] - F["rocblas"]).max()) for g in ("mfma", "mfma8", "auto")}
+        _say(f"(H2O)_{len(mol.symbols) // 3} SVP, nocc {nocc}: T vs rocBLAS max|dF| 160-wide {d['mfma']:.2e}, "
+             f"8-wave {d['mfma8']:.2e}, autotuned {d['auto']:.2e}")
+        assert max(d.values()) < 1e-11
