@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -108,6 +109,11 @@ hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, in
 hipError_t launch_gemm_f64_batched(const GemmArgs& g, double* C, int batch, int64_t bs_a, int64_t bs_b,
                                    int64_t bs_c, hipStream_t st);
 hipError_t launch_gemm_w8(const GemmArgs& g, double* C, hipStream_t st);
+hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, hipStream_t st);
+hipError_t launch_gemm_partials(int prec, const GemmArgs& g, int nsplit, int sym, void* ws, int variant,
+                                hipStream_t st);
+hipError_t launch_gemm_reduce(int prec, const GemmArgs& g, double* C, double alpha, double beta, int nsplit, int sym,
+                              const void* ws, hipStream_t st);
 hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s);
 hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, hipStream_t s);
 hipError_t launch_sp2_trace_update(int n, double* X, const double* X2, int nocc, double* st, hipStream_t s);
@@ -1051,21 +1057,26 @@ void coulomb_recompute(dfks_ctx* h, hipStream_t st) {
 // kernel (128- or 160-wide N tiles, whichever pads less) and rocBLAS -- timed
 // with HIP events on the work stream, and the process keeps the faster one for
 // that shape. Both compute the same T to rounding.
-// Per-shape choice among the variants of an FP64 product: 0 rocBLAS, 1 the
-// library's MFMA kernel, 2 (where the product has it) the library's 8-wave
-// kernel. The first call of a shape on a device times every variant with HIP
-// events and keeps the fastest; the output of that call is recomputed by the
-// kept variant when it is not the one that ran last, so every call of a shape
-// returns the kept variant's result. The key holds a tag for the product (0:
-// exchange T, 1: XC Z_b, 2: XC Phi_b^T W_b). options.gemm 1, 2 and 3 force
-// variant 0, 1 and 2 (1 where there is no third).
+// Per-shape choice among the variants of a product. Tags and variants:
+//   0  FP64 exchange T = B^T C:      0 rocBLAS, 1 k_gemm, 2 k_gemm_w8
+//   1  FP64 XC Z_b = Phi_b Pg_b:     0 rocBLAS, 1 k_gemm (batched)
+//   2  FP64 XC Phi_b^T W_b:          0 rocBLAS, 1 k_gemm (batched)
+//   3  FP64 exchange K partials:     0 k_gemm, 1 k_gemm_w8   (base 1)
+//   4  split-BF16 exchange T:        0 k_gemm, 1 k_gemm_b3w8 (base 1)
+// The first call of a shape on a device times every variant with HIP events
+// and keeps the fastest; the output of that call is recomputed by the kept
+// variant when it is not the one that ran last, so every call of a shape
+// returns the kept variant's result. options.gemm 1, 2 and 3 force variant
+// gemm - 1 - base, clamped to the variants there are (gemm 2: k_gemm, 3: the
+// 8-wavefront tiles).
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int64_t, int64_t, int64_t>, int> g_tune;
 
 template <typename Run>
-void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int ncand, Run run, hipStream_t st) {
+void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int ncand, int base, Run run,
+                   hipStream_t st) {
   if (h->opt.gemm != 0) {
-    run(std::min(h->opt.gemm - 1, ncand - 1));
+    run(std::max(0, std::min(h->opt.gemm - 1 - base, ncand - 1)));
     return;
   }
   int dev = 0;
@@ -1089,12 +1100,18 @@ void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int
   }
   HIPCHK(hipEventSynchronize(ev[ncand]));
   int best = 0;
-  float t_best = 0.f;
+  float t_best = 0.f, t_all[3] = {0.f, 0.f, 0.f};
   for (int c = 0; c < ncand; ++c) {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, ev[c], ev[c + 1]));
+    t_all[c] = t;
     if (c == 0 || t < t_best) { best = c; t_best = t; }
   }
+  static const bool verbose = std::getenv("DFKS_TUNE_LOG") != nullptr;
+  if (verbose)
+    std::fprintf(stderr, "dfks tune: product %d shape (%lld, %lld, %lld): %.3f / %.3f / %.3f ms -> variant %d\n",
+                 tag, (long long)d0, (long long)d1, (long long)d2, t_all[0], t_all[1], ncand > 2 ? t_all[2] : 0.f,
+                 best);
   for (int c = 0; c <= ncand; ++c) (void)hipEventDestroy(ev[c]);
   int kept;
   {
@@ -1108,7 +1125,7 @@ void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int
 void t_product(dfks_ctx* h, const double* C, int nc, hipStream_t st) {
   const int nao = h->nao, nfit = h->nfit;
   const double one = 1.0, zero = 0.0;
-  tuned_product(h, 0, (int64_t)nao * nfit, nc, nao, 3, [&](int choice) {
+  tuned_product(h, 0, (int64_t)nao * nfit, nc, nao, 3, 0, [&](int choice) {
     const GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B, (int64_t)nao * nfit, false, C, nao, true};
     if (choice == 2) {
       HIPCHK(launch_gemm_w8(g, h->d_Tx, st));
@@ -1150,9 +1167,13 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
     if (h->recompute) {
       exchange_recompute(h, Cf + (int64_t)i0 * nao, mixed ? h->d_Cf32 + (int64_t)i0 * nao : nullptr, nc, st);
     } else if (mixed) {
-      GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
-                 h->d_Cf32 + (int64_t)i0 * nao, nao, true};
-      HIPCHK(launch_gemm_f32out(g, h->d_T32, st));
+      // split-BF16 T: the 128-row tile of k_gemm or the 8-wavefront 256-row tile, whichever is faster here
+      const GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
+                       h->d_Cf32 + (int64_t)i0 * nao, nao, true};
+      tuned_product(h, 4, (int64_t)nao * nfit, nc, nao, 2, 1, [&](int c) {
+        if (c == 1) HIPCHK(launch_gemm_b3w8(g, h->d_T32, st));
+        else HIPCHK(launch_gemm_f32out(g, h->d_T32, st));
+      }, st);
     } else {
       t_product(h, Cf + (int64_t)i0 * nao, nc, st);
     }
@@ -1173,8 +1194,12 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
         GemmArgs g{nao, nao, kd, h->d_T32, kd, true, h->d_T32, kd, true};
         HIPCHK(launch_gemm(kBF16x3, g, h->d_K, sg, beta, nsplit, 1, h->d_gws, st));
       } else if (lib_k) {
-        GemmArgs g{nao, nao, kd, h->d_Tx, kd, true, h->d_Tx, kd, true};
-        HIPCHK(launch_gemm(kF64, g, h->d_K, sg, beta, nsplit, 1, h->d_gws, st));
+        // split-K partial tiles by the faster of the two library tiles for this shape, then the reduction
+        const GemmArgs g{nao, nao, kd, h->d_Tx, kd, true, h->d_Tx, kd, true};
+        tuned_product(h, 3, nao, kd, nsplit, 2, 1, [&](int c) {
+          HIPCHK(launch_gemm_partials(kF64, g, nsplit, 1, h->d_gws, c, st));
+        }, st);
+        HIPCHK(launch_gemm_reduce(kF64, g, h->d_K, sg, beta, nsplit, 1, h->d_gws, st));
       } else {
         RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_none, nao, nao, kd, &sg,
                                h->d_Tx, kd, h->d_Tx, kd, &beta, h->d_K, nao));
@@ -1231,7 +1256,7 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
     double* zw = h->d_xc_zw;
     HIPCHK(launch_gather_sym<double>(nb, n, cols, P, nao, q, st));
     // Z_b (G x n, row-major) = Phi_b Pg_b; rocBLAS column-major: Z_b^T = Pg_b^T Phi_b^T
-    tuned_product(h, 1, n, nb, 0, 2, [&](int choice) {
+    tuned_product(h, 1, n, nb, 0, 2, 0, [&](int choice) {
       if (choice == 1) {
         const GemmArgs g{kXcG, n, n, ao, n, true, q, n, true};
         HIPCHK(launch_gemm_f64_batched(g, zw, (int)nb, (int64_t)kXcG * n, (int64_t)n * n, (int64_t)kXcG * n, st));
@@ -1250,7 +1275,7 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
       HIPCHK(launch_fxc(rows, n, ao, zw, h->d_fxc + 8 * bn.pt0, st));
     if (mode == kXcRho) continue;
     // prod_b[i][j] = sum_g Phi[g][i] W[g][j]; rocBLAS column-major: prod_b^T = W_b^T Phi_b
-    tuned_product(h, 2, n, nb, 0, 2, [&](int choice) {
+    tuned_product(h, 2, n, nb, 0, 2, 0, [&](int choice) {
       if (choice == 1) {
         const GemmArgs g{n, n, kXcG, ao, n, false, zw, n, false};
         HIPCHK(launch_gemm_f64_batched(g, q, (int)nb, (int64_t)kXcG * n, (int64_t)kXcG * n, (int64_t)n * n, st));

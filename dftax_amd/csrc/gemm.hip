@@ -29,6 +29,7 @@ namespace dfks {
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 enum GemmPrec { kF64 = 0, kBF16x3 = 1 };
 
@@ -314,46 +315,67 @@ __global__ __launch_bounds__(256, (gemm_blocks_per_cu<PREC, BKC, NJ>())) void k_
   }
 }
 
-// FP64 product C = A B^T for A M-contiguous (A(m, k) = A[k*lda + m]) and B
-// K-contiguous (B(n, k) = B[n*ldb + k]), C row-major with ldc = N: the
-// exchange product T = B^T C over the fitted tensor. One 128 x 160 tile per
-// workgroup of 512 threads = 8 wavefronts (4 along M x 2 along N, each 32 x 80
-// = 2 x 5 MFMA 16x16 blocks), so every SIMD holds two wavefronts and one
-// wavefront's MFMAs cover the other's barrier and LDS waits. The LDS tiles
-// are double-buffered: the next k-step's global loads are issued before the
-// current step's MFMAs, stored to the other buffer after them, and one
-// barrier per k-step separates the two. N tiles on gridDim.y.
-constexpr int kW8BM = 128, kW8BN = 160, kW8BK = 16;
+// FP64 product C = A B^T with B K-contiguous (B(n, k) = B[n*ldb + k]) and A
+// either M-contiguous (AK = false: A(m, k) = A[k*lda + m], the exchange
+// product T = B^T C over the fitted tensor) or K-contiguous (AK = true: the
+// symmetric exchange product K = T T^T). One 128 x 32NJ tile per workgroup of
+// 512 threads = 8 wavefronts (4 along M x 2 along N, each 32 x 16NJ = 2 x NJ
+// MFMA 16x16 blocks), so every SIMD holds two wavefronts and one wavefront's
+// MFMAs cover the other's barrier and LDS waits. The LDS tiles are
+// double-buffered: the next k-step's global loads are issued before the
+// current step's MFMAs and stored to the other buffer after them, with one
+// barrier per k-step. Tiles, the symmetric mode (upper tiles only) and the
+// split of K over gridDim.z (partial tiles at C + z * split_stride) follow
+// k_gemm.
+constexpr int kW8BM = 128, kW8BK = 16;
 
-__global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int K, const double* __restrict__ A,
+template <bool AK, int NJ>
+__global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K, const double* __restrict__ A,
                                                    int64_t lda, const double* __restrict__ B, int64_t ldb,
-                                                   double* __restrict__ C, int vec_b) {
+                                                   double* __restrict__ C, int64_t ldc, int64_t split_stride,
+                                                   int64_t kchunk, int sym, int ntiles, int vec_a, int vec_b) {
+  constexpr int BN = 32 * NJ;
   __shared__ double sA[2][kW8BM][kW8BK + 1];
-  __shared__ double sB[2][kW8BN][kW8BK + 1];
+  __shared__ double sB[2][BN][kW8BK + 1];
+  if ((int)blockIdx.x >= ntiles) return;
+  const int ntn = (N + BN - 1) / BN;
+  int tm, tn;
+  tile_of(blockIdx.x, ntn, sym != 0, tm, tn);
+  const int64_t m0 = (int64_t)tm * kW8BM;
+  const int n0 = tn * BN;
+  const int64_t k_begin = (int64_t)blockIdx.z * kchunk;
+  const int64_t k_end = k_begin + kchunk < K ? k_begin + kchunk : K;
+  double* __restrict__ Cz = C + (int64_t)blockIdx.z * split_stride;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t m0 = (int64_t)blockIdx.x * kW8BM;
-  const int n0 = blockIdx.y * kW8BN;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 80;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * (16 * NJ);
   const int li = lane & 15, lk = lane >> 4;
-  // A slice: row tid % 128, k offsets 4 (tid / 128) .. +3 (coalesced along m)
-  const int ar = tid & 127, ak = (tid >> 7) * 4;
-  // B slice (threads < 320): row tid / 2, k offsets 8 (tid % 2) .. +7
+  // A slice, 4 values per thread: M-contiguous row tid % 128, k 4 (tid / 128) .. +3
+  // (coalesced along m); K-contiguous row tid / 4, k 4 (tid % 4) .. +3
+  const int ar = AK ? tid >> 2 : tid & 127, ak = AK ? (tid & 3) * 4 : (tid >> 7) * 4;
+  // B slice (threads < 2 BN): row tid / 2, k 8 (tid % 2) .. +7
   const int br = tid >> 1, bk = (tid & 1) * 8;
-  const bool has_b = tid < 2 * kW8BN;
+  const bool has_b = tid < 2 * BN;
   const bool a_in = m0 + ar < M;
   const bool b_in = has_b && n0 + br < N;
-  const double* __restrict__ pa = A + m0 + ar;
-  const double* __restrict__ pb = B + (int64_t)(n0 + (b_in ? br : 0)) * ldb;
+  const double* __restrict__ pa = AK ? A + (a_in ? m0 + ar : 0) * lda : A + (a_in ? m0 + ar : 0);
+  const double* __restrict__ pb = B + (int64_t)(b_in ? n0 + br : 0) * ldb;
   double ra[4], rb[8];
-  auto load = [&](int k0) {
+  auto load = [&](int64_t k0) {
+    const int64_t ka = k0 + ak;
+    if (AK && a_in && vec_a && ka + 4 <= k_end) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = k0 + ak + j;
-      ra[j] = (a_in && k < K) ? pa[(int64_t)k * lda] : 0.0;
+      for (int v = 0; v < 2; ++v) {
+        const double2 x = *reinterpret_cast<const double2*>(pa + ka + 2 * v);
+        ra[2 * v] = x.x;
+        ra[2 * v + 1] = x.y;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ra[j] = (a_in && ka + j < k_end) ? (AK ? pa[ka + j] : pa[(ka + j) * lda]) : 0.0;
     }
     if (!has_b) return;
-    const int kb = k0 + bk;
-    if (b_in && vec_b && kb + 8 <= K) {
+    const int64_t kb = k0 + bk;
+    if (b_in && vec_b && kb + 8 <= k_end) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const double2 x = *reinterpret_cast<const double2*>(pb + kb + 2 * v);
@@ -362,7 +384,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int K, con
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) rb[j] = (b_in && kb + j < K) ? pb[kb + j] : 0.0;
+      for (int j = 0; j < 8; ++j) rb[j] = (b_in && kb + j < k_end) ? pb[kb + j] : 0.0;
     }
   };
   auto store = [&](int buf) {
@@ -373,29 +395,30 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int K, con
       for (int j = 0; j < 8; ++j) sB[buf][br][bk + j] = rb[j];
     }
   };
-  dbl4 acc[2][5];
+  dbl4 acc[2][NJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
-  const int nk = (K + kW8BK - 1) / kW8BK;
-  load(0);
+    for (int j = 0; j < NJ; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
+  const int64_t nk = k_end > k_begin ? (k_end - k_begin + kW8BK - 1) / kW8BK : 0;
+  load(k_begin);
   store(0);
   __syncthreads();
-  for (int t = 0; t < nk; ++t) {
-    const int cur = t & 1;
-    if (t + 1 < nk) load((t + 1) * kW8BK);
+  for (int64_t t = 0; t < nk; ++t) {
+    const int cur = (int)(t & 1);
+    if (t + 1 < nk) load(k_begin + (t + 1) * kW8BK);
 #pragma unroll
     for (int kk = 0; kk < kW8BK; kk += 4) {
-      double a[2], b[5];
+      double a[2], b[NJ];
 #pragma unroll
       for (int i = 0; i < 2; ++i) a[i] = sA[cur][wm + 16 * i + li][kk + lk];
 #pragma unroll
-      for (int j = 0; j < 5; ++j) b[j] = sB[cur][wn + 16 * j + li][kk + lk];
+      for (int j = 0; j < NJ; ++j) b[j] = sB[cur][wn + 16 * j + li][kk + lk];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 5; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NJ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (t + 1 < nk) store(cur ^ 1);
     __syncthreads();
@@ -404,10 +427,165 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int K, con
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 5; ++j)
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t gm = m0 + wm + 16 * i + lk + 4 * r;
+        const int gn = n0 + wn + 16 * j + li;
+        if (gm < M && gn < N) Cz[gm * ldc + gn] = acc[i][j][r];
+      }
+}
+
+// Split-BF16 (three-term) product C = A B^T with FP32 accumulate for A
+// M-contiguous (A(m, k) = A[k*lda + m]) and B K-contiguous, C row-major FP32
+// with ldc = N: the mixed-precision exchange product T = B^T C. One 256 x 160
+// tile per workgroup of 512 threads = 8 wavefronts (4 along M x 2 along N, each
+// 64 x 80 = 4 x 5 MFMA 16x16x32 blocks). Each operand value is split into bf16
+// hi + lo once, while it is staged to LDS; the LDS tiles are double-buffered
+// with one barrier per k-step, the next step's global loads issued before the
+// current step's MFMAs. The 256-row tile halves the re-reads of the small
+// operand per row of the large one against a 128-row tile.
+constexpr int kB3BM = 256, kB3BN = 160, kB3BK = 32, kB3PAD = kB3BK + 8;
+
+__device__ __forceinline__ void split_bf16x8(const float* x, bf16x8& h, bf16x8& l) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const __bf16 xh = (__bf16)x[j];
+    h[j] = xh;
+    l[j] = (__bf16)(x[j] - (float)xh);
+  }
+}
+
+__global__ __launch_bounds__(512, 1) void k_gemm_b3w8(int64_t M, int N, int K, const float* __restrict__ A,
+                                                     int64_t lda, const float* __restrict__ B, int64_t ldb,
+                                                     float* __restrict__ C, int vec_a, int vec_b) {
+  __shared__ __attribute__((aligned(16))) __bf16 sAh[2][kB3BM][kB3PAD];
+  __shared__ __attribute__((aligned(16))) __bf16 sAl[2][kB3BM][kB3PAD];
+  __shared__ __attribute__((aligned(16))) __bf16 sBh[2][kB3BN][kB3PAD];
+  __shared__ __attribute__((aligned(16))) __bf16 sBl[2][kB3BN][kB3PAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t m0 = (int64_t)blockIdx.x * kB3BM;
+  const int n0 = blockIdx.y * kB3BN;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 80;
+  const int li = lane & 15, lk = lane >> 4;
+  // A slice: rows 4 (tid % 64) .. +3 (one 16-byte load per k, coalesced along
+  // m) at k 4 (tid / 64) .. +3
+  const int ar = (tid & 63) * 4, ak = (tid >> 6) * 4;
+  // B slice (threads < 320): row tid / 2, k 16 (tid % 2) .. +15
+  const int br = tid >> 1, bk = (tid & 1) * 16;
+  const bool has_b = tid < 2 * kB3BN;
+  const bool a_in = m0 + ar < M;
+  const bool b_in = has_b && n0 + br < N;
+  const float* __restrict__ pa = A + (a_in ? m0 + ar : 0);
+  const float* __restrict__ pb = B + (int64_t)(b_in ? n0 + br : 0) * ldb;
+  // A and B are loaded one k-step ahead into registers and split into bf16
+  // hi + lo when stored to LDS. ra[4 q + c]: row ar + c, k ak + q
+  float ra[16], rb[16];
+  auto load_a = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + ak + q;
+      const float* src = pa + (int64_t)k * lda;
+      if (vec_a && a_in && k < K) {
+        const float4 x = *reinterpret_cast<const float4*>(src);
+        ra[4 * q] = x.x;
+        ra[4 * q + 1] = x.y;
+        ra[4 * q + 2] = x.z;
+        ra[4 * q + 3] = x.w;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ra[4 * q + c] = (m0 + ar + c < M && k < K) ? src[c] : 0.f;
+      }
+    }
+  };
+  auto load_b = [&](int k0) {
+    if (!has_b) return;
+    const int kb = k0 + bk;
+    if (b_in && vec_b && kb + 16 <= K) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float4 x = *reinterpret_cast<const float4*>(pb + kb + 4 * v);
+        rb[4 * v] = x.x;
+        rb[4 * v + 1] = x.y;
+        rb[4 * v + 2] = x.z;
+        rb[4 * v + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) rb[j] = (b_in && kb + j < K) ? pb[kb + j] : 0.f;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      bf16x4 h, l;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float x = ra[4 * q + c];
+        const __bf16 xh = (__bf16)x;
+        h[q] = xh;
+        l[q] = (__bf16)(x - (float)xh);
+      }
+      *reinterpret_cast<bf16x4*>(&sAh[buf][ar + c][ak]) = h;
+      *reinterpret_cast<bf16x4*>(&sAl[buf][ar + c][ak]) = l;
+    }
+    if (has_b) {
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        bf16x8 h, l;
+        split_bf16x8(rb + 8 * v, h, l);
+        *reinterpret_cast<bf16x8*>(&sBh[buf][br][bk + 8 * v]) = h;
+        *reinterpret_cast<bf16x8*>(&sBl[buf][br][bk + 8 * v]) = l;
+      }
+    }
+  };
+  flt4 acc[4][5];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+  auto compute = [&](int cur) {
+    bf16x8 bh[5], bl[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      bh[j] = *reinterpret_cast<const bf16x8*>(&sBh[cur][wn + 16 * j + li][8 * lk]);
+      bl[j] = *reinterpret_cast<const bf16x8*>(&sBl[cur][wn + 16 * j + li][8 * lk]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&sAh[cur][wm + 16 * i + li][8 * lk]);
+      const bf16x8 al = *reinterpret_cast<const bf16x8*>(&sAl[cur][wm + 16 * i + li][8 * lk]);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  };
+  const int nk = (K + kB3BK - 1) / kB3BK;
+  load_a(0);
+  load_b(0);
+  store(0);
+  __syncthreads();
+  for (int t = 0; t < nk; ++t) {
+    const int cur = t & 1;
+    if (t + 1 < nk) {
+      load_a((t + 1) * kB3BK);
+      load_b((t + 1) * kB3BK);
+    }
+    compute(cur);
+    if (t + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+  // v_mfma_f32_16x16x32_bf16 result: lane l, register r holds row 4(l/16) + r, column l%16
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm + 16 * i + 4 * lk + r;
         const int gn = n0 + wn + 16 * j + li;
         if (gm < M && gn < N) C[gm * N + gn] = acc[i][j][r];
       }
@@ -471,6 +649,68 @@ static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nspl
   return run(k_gemm<PREC, false, false, Out, NJ>);
 }
 
+// k_gemm_w8 launch: ldc, split and symmetric mode as launch_gemm_t (NJ = 5
+// only without the symmetric mode); B must be K-contiguous.
+template <int NJ>
+static hipError_t launch_w8_t(const GemmArgs& g, double* C, int64_t ldc, int nsplit, int sym, hipStream_t st) {
+  constexpr int BN = 32 * NJ;
+  if (!g.b_kcontig || (sym && NJ != 4)) return hipErrorInvalidValue;
+  const int64_t ntm = ((int64_t)g.M + kW8BM - 1) / kW8BM, ntn = (g.N + BN - 1) / BN;
+  const int64_t ntl = sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
+  if (ntl > 0x7fffffff || nsplit > 65535) return hipErrorInvalidValue;
+  int64_t kchunk = (g.K + nsplit - 1) / nsplit;
+  kchunk = (kchunk + kW8BK - 1) / kW8BK * kW8BK;
+  const double* A = static_cast<const double*>(g.A);
+  const double* B = static_cast<const double*>(g.B);
+  const int vec_a = ((uintptr_t)A % 16 == 0) && (g.lda % 2 == 0);
+  const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % 2 == 0);
+  const dim3 grid((unsigned)ntl, 1, (unsigned)nsplit);
+  const int64_t split_stride = (int64_t)g.M * ldc;
+  if (g.a_kcontig)
+    return launch_k(k_gemm_w8<true, NJ>, grid, dim3(512), st, (int64_t)g.M, g.N, g.K, A, g.lda, B, g.ldb, C, ldc,
+                    split_stride, kchunk, sym, (int)ntl, vec_a, vec_b);
+  return launch_k(k_gemm_w8<false, NJ>, grid, dim3(512), st, (int64_t)g.M, g.N, g.K, A, g.lda, B, g.ldb, C, ldc,
+                  split_stride, kchunk, sym, (int)ntl, vec_a, vec_b);
+}
+
+// C = A B^T in FP64 by k_gemm_w8, no split (ldc = N): 128- or 160-wide N
+// tiles, whichever pads N less. B must be K-contiguous.
+hipError_t launch_gemm_w8(const GemmArgs& g, double* C, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  const int t128 = (g.N + 127) / 128, t160 = (g.N + 159) / 160;
+  if (t160 < t128 && (int64_t)t160 * 160 - g.N < (int64_t)t128 * 128 - g.N)
+    return launch_w8_t<5>(g, C, g.N, 1, 0, st);
+  return launch_w8_t<4>(g, C, g.N, 1, 0, st);
+}
+
+// Split-K partial tiles of A B^T into ws (nsplit x M x N, FP64 for kF64 and
+// FP32 for kBF16x3; upper tiles only in the symmetric mode). variant 1 takes
+// the 8-wavefront FP64 tile where it applies (B K-contiguous), 0 k_gemm.
+hipError_t launch_gemm_partials(int prec, const GemmArgs& g, int nsplit, int sym, void* ws, int variant,
+                                hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (!ws || nsplit < 1) return hipErrorInvalidValue;
+  if (prec == kF64) {
+    if (variant == 1 && g.b_kcontig) return launch_w8_t<4>(g, static_cast<double*>(ws), g.N, nsplit, sym, st);
+    return launch_gemm_t<kF64, double>(g, static_cast<double*>(ws), g.N, nsplit, sym, st);
+  }
+  return launch_gemm_t<kBF16x3, float>(g, static_cast<float*>(ws), g.N, nsplit, sym, st);
+}
+
+// C = alpha * sum of the nsplit partial tiles in ws (+ beta C), mirrored in the
+// symmetric mode; fixed summation order.
+hipError_t launch_gemm_reduce(int prec, const GemmArgs& g, double* C, double alpha, double beta, int nsplit, int sym,
+                              const void* ws, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  const int64_t stride = (int64_t)g.M * g.N;
+  const unsigned nb = (unsigned)((stride + 255) / 256);
+  if (prec == kF64)
+    return launch_k(k_gemm_reduce<double, double>, dim3(nb), dim3(256), st, g.M, g.N, nsplit,
+                    static_cast<const double*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
+  return launch_k(k_gemm_reduce<float, double>, dim3(nb), dim3(256), st, g.M, g.N, nsplit,
+                  static_cast<const float*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
+}
+
 // C = alpha * A B^T (+ beta C), C row-major M x N (ldc = N), FP64 output.
 // ws: workspace of nsplit * M * N doubles (FP64) or floats (BF16x3), used when
 // nsplit > 1 or sym; may be null otherwise.
@@ -479,27 +719,15 @@ hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, dou
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (nsplit < 1) nsplit = 1;
   const bool direct = nsplit == 1 && !sym && alpha == 1.0 && beta == 0.0 && prec == kF64;
-  hipError_t e;
   if (direct) {
     const int t128 = (g.N + 127) / 128, t160 = (g.N + 159) / 160;
     if (t160 < t128 && (int64_t)t160 * 160 - g.N < (int64_t)t128 * 128 - g.N)
       return launch_gemm_t<kF64, double, 5>(g, C, g.N, 1, 0, st);
     return launch_gemm_t<kF64, double>(g, C, g.N, 1, 0, st);
   }
-  if (!ws) return hipErrorInvalidValue;
-  const int64_t stride = (int64_t)g.M * g.N;
-  const int64_t tot = (int64_t)g.M * g.N;
-  const unsigned nb = (unsigned)((tot + 255) / 256);
-  if (prec == kF64) {
-    e = launch_gemm_t<kF64, double>(g, static_cast<double*>(ws), g.N, nsplit, sym, st);
-    if (e != hipSuccess) return e;
-    return launch_k(k_gemm_reduce<double, double>, dim3(nb), dim3(256), st, g.M, g.N, nsplit,
-                    static_cast<const double*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
-  }
-  e = launch_gemm_t<kBF16x3, float>(g, static_cast<float*>(ws), g.N, nsplit, sym, st);
+  const hipError_t e = launch_gemm_partials(prec, g, nsplit, sym, ws, 0, st);
   if (e != hipSuccess) return e;
-  return launch_k(k_gemm_reduce<float, double>, dim3(nb), dim3(256), st, g.M, g.N, nsplit,
-                  static_cast<const float*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
+  return launch_gemm_reduce(prec, g, C, alpha, beta, nsplit, sym, ws, st);
 }
 
 // FP32 output variant (mixed path intermediates): C = A B^T, no split, no sym.
@@ -524,15 +752,18 @@ hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, in
   return launch_gemm_t<kBF16x3, float>(g, C, g.N, 1, 0, st, batch, bs_a, bs_b, bs_c);
 }
 
-// C = A B^T in FP64 by k_gemm_w8 (A M-contiguous, B K-contiguous, ldc = N).
-hipError_t launch_gemm_w8(const GemmArgs& g, double* C, hipStream_t st) {
+// C = A B^T in FP32 by k_gemm_b3w8 (split-BF16, A M-contiguous, B K-contiguous, ldc = N).
+hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (g.a_kcontig || !g.b_kcontig || g.K > 0x7fffffff) return hipErrorInvalidValue;
-  const double* A = static_cast<const double*>(g.A);
-  const double* B = static_cast<const double*>(g.B);
-  const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % 2 == 0);
-  const dim3 grid((unsigned)(((int64_t)g.M + kW8BM - 1) / kW8BM), (unsigned)((g.N + kW8BN - 1) / kW8BN));
-  return launch_k(k_gemm_w8, grid, dim3(512), st, (int64_t)g.M, g.N, (int)g.K, A, g.lda, B, g.ldb, C, vec_b);
+  const float* A = static_cast<const float*>(g.A);
+  const float* B = static_cast<const float*>(g.B);
+  // 16-byte loads of 4 consecutive rows of A; 4 floats at a time along k of B
+  const int vec_a = ((uintptr_t)A % 16 == 0) && (g.lda % 4 == 0) && (g.M % 4 == 0);
+  const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % 4 == 0);
+  const dim3 grid((unsigned)(((int64_t)g.M + kB3BM - 1) / kB3BM), (unsigned)((g.N + kB3BN - 1) / kB3BN));
+  return launch_k(k_gemm_b3w8, grid, dim3(512), st, (int64_t)g.M, g.N, (int)g.K, A, g.lda, B, g.ldb, C, vec_a,
+                  vec_b);
 }
 
 // Batched FP64 variant (no split, FP64 accumulate): for b < batch,

@@ -588,3 +588,15 @@ def test_exchange_t160_matches_rocblas(torch_cuda):
         _say(f"(H2O)_{len(mol.symbols) // 3} SVP, nocc {nocc}: T vs rocBLAS max|dF| 160-wide {d['mfma']:.2e}, "
              f"8-wave {d['mfma8']:.2e}, autotuned {d['auto']:.2e}")
         assert max(d.values()) < 1e-11
+        # mixed steps: the split-BF16 T of k_gemm and of the 8-wavefront 256-row tile
+        Fm = {}
+        for g in ("mfma", "mfma8"):
+            ks = KS(mol, "PBE0", grid=becke(30, 110), coulomb=df(engine=hip(precision="mixed", gemm=g)))
+            ks.coulomb.set_precision(True)
+            _, Fm[g] = ks.coulomb.energy_fock(initial_density(ks, None))
+            Fm[g] = Fm[g].cpu().numpy()
+            ks.coulomb.close()
+        dm = {g: float(np.abs(Fm[g] - F["rocblas"]).max()) for g in Fm}
+        dmm = float(np.abs(Fm["mfma8"] - Fm["mfma"]).max())
+        _say(f"  mixed vs FP64 max|dF| k_gemm {dm['mfma']:.2e}, 8-wave {dm['mfma8']:.2e}; between them {dmm:.2e}")
+        assert max(dm.values()) < 1e-3 and dmm < 1e-5
