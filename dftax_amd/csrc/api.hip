@@ -116,7 +116,8 @@ hipError_t launch_gemm_reduce(int prec, const GemmArgs& g, double* C, double alp
                               const void* ws, hipStream_t st);
 hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s);
 hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, hipStream_t s);
-hipError_t launch_sp2_trace_update(int n, double* X, const double* X2, int nocc, double* st, hipStream_t s);
+hipError_t launch_sp2_begin(int n, const double* X, int nocc, double* st, hipStream_t s);
+hipError_t launch_sp2_update(int n, int nsplit, const double* ws, double* X, int nocc, double* st, hipStream_t s);
 hipError_t launch_diag(int n, const double* R, double* d, hipStream_t s);
 hipError_t launch_pchol_panel(int n, int k0, int b, const double* R, double* d, double* L, int32_t* piv, double* st,
                               int last, hipStream_t s);
@@ -393,6 +394,7 @@ struct dfks_ctx {
   int sp2_guess = 24;       // iterations the last purification needed (first chunk of the next)
   hipGraphExec_t sp2_exec = nullptr;  // kSp2Block purification iterations as one graph (per nmo)
   int sp2_exec_n = -1;
+  int sp2_variant = 0;      // tile of the purification products (autotuned when the graph is built)
   bool c_stale = false;     // d_C / d_eps not yet computed for the last step (purified density)
   int n_eigh_steps = 0;     // steps that fell back to the eigensolver
   // recompute mode (options.aux_chunk > 0): screened bra pairs per (la, lb)
@@ -926,6 +928,9 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   h->d_offd = dalloc<double>(nao);
   h->d_sel = dalloc<int32_t>(nao);
   h->d_gws = dalloc<double>((size_t)64 * nao * nao);
+  // purification state: fixed size, allocated once, because the purification
+  // graph keeps its address (as it keeps those of d_W1 and d_gws)
+  h->d_sp2 = dalloc<double>(kSp2State);
   h->d_gpart = dalloc<double>((size_t)nao * h->nfit);
   if (h->opt.precision == 1) {
     if (!h->recompute) {
@@ -1063,6 +1068,7 @@ void coulomb_recompute(dfks_ctx* h, hipStream_t st) {
 //   2  FP64 XC Phi_b^T W_b:          0 rocBLAS, 1 k_gemm (batched)
 //   3  FP64 exchange K partials:     0 k_gemm, 1 k_gemm_w8   (base 1)
 //   4  split-BF16 exchange T:        0 k_gemm, 1 k_gemm_b3w8 (base 1)
+//   5  SP2 product partials X X:     0 k_gemm, 1 k_gemm_w8   (base 1)
 // The first call of a shape on a device times every variant with HIP events
 // and keeps the fastest; the output of that call is recomputed by the kept
 // variant when it is not the one that ran last, so every call of a shape
@@ -1073,11 +1079,12 @@ std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int64_t, int64_t, int64_t>, int> g_tune;
 
 template <typename Run>
-void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int ncand, int base, Run run,
-                   hipStream_t st) {
+int tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int ncand, int base, Run run,
+                  hipStream_t st) {
   if (h->opt.gemm != 0) {
-    run(std::max(0, std::min(h->opt.gemm - 1 - base, ncand - 1)));
-    return;
+    const int c = std::max(0, std::min(h->opt.gemm - 1 - base, ncand - 1));
+    run(c);
+    return c;
   }
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -1088,10 +1095,10 @@ void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int
     if (it != g_tune.end()) {
       const int c = it->second;
       run(c);
-      return;
+      return c;
     }
   }
-  hipEvent_t ev[4];
+  hipEvent_t ev[5];
   for (int c = 0; c <= ncand; ++c) HIPCHK(hipEventCreate(&ev[c]));
   HIPCHK(hipEventRecord(ev[0], st));
   for (int c = 0; c < ncand; ++c) {
@@ -1100,7 +1107,7 @@ void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int
   }
   HIPCHK(hipEventSynchronize(ev[ncand]));
   int best = 0;
-  float t_best = 0.f, t_all[3] = {0.f, 0.f, 0.f};
+  float t_best = 0.f, t_all[4] = {0.f, 0.f, 0.f, 0.f};
   for (int c = 0; c < ncand; ++c) {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, ev[c], ev[c + 1]));
@@ -1109,9 +1116,8 @@ void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int
   }
   static const bool verbose = std::getenv("DFKS_TUNE_LOG") != nullptr;
   if (verbose)
-    std::fprintf(stderr, "dfks tune: product %d shape (%lld, %lld, %lld): %.3f / %.3f / %.3f ms -> variant %d\n",
-                 tag, (long long)d0, (long long)d1, (long long)d2, t_all[0], t_all[1], ncand > 2 ? t_all[2] : 0.f,
-                 best);
+    std::fprintf(stderr, "dfks tune: product %d shape (%lld, %lld, %lld): %.3f / %.3f / %.3f / %.3f ms -> variant %d\n",
+                 tag, (long long)d0, (long long)d1, (long long)d2, t_all[0], t_all[1], t_all[2], t_all[3], best);
   for (int c = 0; c <= ncand; ++c) (void)hipEventDestroy(ev[c]);
   int kept;
   {
@@ -1119,6 +1125,7 @@ void tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int
     kept = g_tune.emplace(key, best).first->second;
   }
   if (kept != ncand - 1) run(kept);
+  return kept;
 }
 
 // T = B^T C (nao*nfit x nc), FP64
@@ -1589,7 +1596,6 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   realloc(h->d_C, (size_t)nao * nmo);
   realloc(h->d_eps, nmo);
   realloc(h->d_Ws, (size_t)nao * nao);
-  realloc(h->d_sp2, kSpHist + kSp2MaxIter);
   realloc(h->d_lohi, 2 * (size_t)nmo);
   realloc(h->d_dvec, nmo);
   realloc(h->d_L, (size_t)nmo * h->nocc);
@@ -1621,8 +1627,8 @@ hipError_t sp2_block(dfks_ctx* h, hipStream_t st) {
   const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, n);
   const GemmArgs g{n, n, n, h->d_W1, n, true, h->d_W1, n, true};
   for (int c = 0; c < kSp2Block; ++c) {
-    hipError_t e = launch_gemm(kF64, g, h->d_W2, 1.0, 0.0, nsplit, 1, h->d_gws, st);
-    if (e == hipSuccess) e = launch_sp2_trace_update(n, h->d_W1, h->d_W2, h->nocc, h->d_sp2, st);
+    hipError_t e = launch_gemm_partials(kF64, g, nsplit, 1, h->d_gws, h->sp2_variant, st);
+    if (e == hipSuccess) e = launch_sp2_update(n, nsplit, h->d_gws, h->d_W1, h->nocc, h->d_sp2, st);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -1639,6 +1645,14 @@ void sp2_run_blocks(dfks_ctx* h, int nblocks, hipStream_t st) {
     if (h->sp2_exec) (void)hipGraphExecDestroy(h->sp2_exec);
     h->sp2_exec = nullptr;
     h->sp2_exec_n = h->nmo;
+    // the products' tile, timed eagerly on the current X (the partials are recomputed by the graph)
+    const int n = h->nmo;
+    const int ntn = (n + 127) / 128;
+    const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, n);
+    const GemmArgs g{n, n, n, h->d_W1, n, true, h->d_W1, n, true};
+    h->sp2_variant = tuned_product(h, 5, n, n, nsplit, 2, 1, [&](int c) {
+      HIPCHK(launch_gemm_partials(kF64, g, nsplit, 1, h->d_gws, c, st));
+    }, st);
     hipGraph_t gr = nullptr;
     if (hipGraphCreate(&gr, 0) == hipSuccess && gr) {
       GraphRec rec;
@@ -1664,6 +1678,7 @@ bool purify_density(dfks_ctx* h, hipStream_t st) {
   double* X = h->d_W1;
   HIPCHK(launch_gersh(n, h->d_Ws, h->d_lohi, h->d_sp2, st));
   HIPCHK(launch_sp2_init(n, h->d_Ws, h->d_sp2, X, st));
+  HIPCHK(launch_sp2_begin(n, X, nocc, h->d_sp2, st));
   int done = 0;
   int chunk = h->sp2_guess + 1;
   double stv[kSpHist];

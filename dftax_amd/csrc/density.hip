@@ -14,9 +14,11 @@
 // so e_k = tr X_k - tr X_k^2 = sum lambda (1 - lambda) >= 0 bounds every
 // eigenvalue's distance from {0, 1}. Once e_k < 1e-9 two forced steps follow
 // (X^2, then 2X - X^2), which square the remaining deviations on both sides,
-// and the iteration freezes. The products X_k^2 are FP64 MFMA GEMMs; the trace
-// test and the update are the small kernels here, and the stop rule runs on
-// the device so the host only reads the state between chunks of iterations.
+// and the iteration freezes. The products X_k^2 are FP64 MFMA GEMMs (split-K
+// partial tiles); one kernel here sums the partials and applies the branch
+// while it accumulates tr X_{k+1} and sum_ij (X_{k+1})_ij^2 = tr X_{k+1}^2,
+// which decide the next branch before its product. The stop rule runs on the
+// device, so the host only reads the state between chunks of iterations.
 //
 // The occupied factor L (X = L L^T, n x nocc) comes from a pivoted Cholesky of
 // the converged projector in panels of kPanel columns: one workgroup picks the
@@ -107,18 +109,9 @@ __global__ void k_sp2_init(int n, const double* __restrict__ F, const double* __
   X[idx] = ((i == j ? hi : 0.0) - F[idx]) / (hi - lo);
 }
 
-// One block: traces of X and X^2, the branch (or the freeze) for this iteration.
-__global__ __launch_bounds__(1024) void k_sp2_trace(int n, const double* __restrict__ X, const double* __restrict__ X2,
-                                                    int nocc, double* __restrict__ st) {
-  __shared__ double s_red[16];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    a += X[(int64_t)i * n + i];
-    b += X2[(int64_t)i * n + i];
-  }
-  const double t1 = block_sum_1024(a, s_red);
-  const double t2 = block_sum_1024(b, s_red);
-  if (threadIdx.x != 0) return;
+// The branch (or the freeze) of one iteration from t1 = tr X and t2 = tr X^2
+// (thread 0 of k_sp2_decide).
+__device__ void sp2_decide(double t1, double t2, int nocc, double* __restrict__ st) {
   if (st[kSpFlag] < 0.0) return;  // frozen
   const int it = (int)st[kSpIter];
   const double e = t1 - t2;
@@ -142,13 +135,76 @@ __global__ __launch_bounds__(1024) void k_sp2_trace(int n, const double* __restr
   st[kSpFlag] = flag;
 }
 
-__global__ void k_sp2_update(int64_t nn, double* __restrict__ X, const double* __restrict__ X2,
-                             const double* __restrict__ st) {
+// Per-workgroup partial sums of tr X and of sum_ij X_ij^2 (= tr X^2 for the
+// symmetric X) into st[kSpPart + 2 b], st[kSpPart + 2 b + 1]; kSp2Groups
+// workgroups of 256 threads stride over the n^2 elements.
+__device__ void sp2_partials(double a, double b, double* __restrict__ st) {
+  __shared__ double s_a[256], s_b[256];
+  s_a[threadIdx.x] = a;
+  s_b[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_a[threadIdx.x] += s_a[threadIdx.x + w];
+      s_b[threadIdx.x] += s_b[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    st[kSpPart + 2 * blockIdx.x] = s_a[0];
+    st[kSpPart + 2 * blockIdx.x + 1] = s_b[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sp2_norms(int n, const double* __restrict__ X, double* __restrict__ st) {
+  const int64_t nn = (int64_t)n * n;
+  double a = 0.0, b = 0.0;
+  for (int64_t idx = blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)kSp2Groups * 256) {
+    const double x = X[idx];
+    if (idx / n == idx % n) a += x;
+    b += x * x;
+  }
+  sp2_partials(a, b, st);
+}
+
+// One block: t1, t2 from the kSp2Groups partial pairs (fixed order), then the branch.
+__global__ __launch_bounds__(kSp2Groups) void k_sp2_decide(int nocc, double* __restrict__ st) {
+  __shared__ double s_a[kSp2Groups], s_b[kSp2Groups];
+  s_a[threadIdx.x] = st[kSpPart + 2 * threadIdx.x];
+  s_b[threadIdx.x] = st[kSpPart + 2 * threadIdx.x + 1];
+  __syncthreads();
+  for (int w = kSp2Groups / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_a[threadIdx.x] += s_a[threadIdx.x + w];
+      s_b[threadIdx.x] += s_b[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sp2_decide(s_a[0], s_b[0], nocc, st);
+}
+
+// X <- X^2 or 2X - X^2 by the decided branch, X^2 summed from the nsplit
+// partial upper tiles of the symmetric product (mirrored), with the partial
+// sums of the new X's trace and squared norm for the next decision. Nothing
+// changes once the iteration is frozen.
+__global__ __launch_bounds__(256) void k_sp2_reduce_update(int n, int nsplit, const double* __restrict__ ws,
+                                                           int64_t split_stride, double* __restrict__ X,
+                                                           double* __restrict__ st) {
   const double flag = st[kSpFlag];
   if (flag < 0.0) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nn) return;
-  X[i] = flag > 0.5 ? X2[i] : 2.0 * X[i] - X2[i];
+  const int64_t nn = (int64_t)n * n;
+  double a = 0.0, b = 0.0;
+  for (int64_t idx = blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)kSp2Groups * 256) {
+    const int m = (int)(idx / n), c = (int)(idx % n);
+    const int64_t off = m <= c ? (int64_t)m * n + c : (int64_t)c * n + m;
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z) s += ws[z * split_stride + off];
+    const double x = flag > 0.5 ? s : 2.0 * X[idx] - s;
+    X[idx] = x;
+    if (m == c) a += x;
+    b += x * x;
+  }
+  sp2_partials(a, b, st);
 }
 
 // d[i] = R_ii
@@ -233,11 +289,19 @@ hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, 
   return hipGetLastError();
 }
 
-hipError_t launch_sp2_trace_update(int n, double* X, const double* X2, int nocc, double* st, hipStream_t s) {
-  const int64_t nn = (int64_t)n * n;
-  const hipError_t e = launch_k(k_sp2_trace, dim3(1), dim3(1024), s, n, X, X2, nocc, st);
+// The first decision, from X_0 (after launch_sp2_init).
+hipError_t launch_sp2_begin(int n, const double* X, int nocc, double* st, hipStream_t s) {
+  const hipError_t e = launch_k(k_sp2_norms, dim3(kSp2Groups), dim3(256), s, n, X, st);
   if (e != hipSuccess) return e;
-  return launch_k(k_sp2_update, dim3((unsigned)((nn + 255) / 256)), dim3(256), s, nn, X, X2, st);
+  return launch_k(k_sp2_decide, dim3(1), dim3(kSp2Groups), s, nocc, st);
+}
+
+// One iteration after the product's split-K partials are in ws: update X, decide the next branch.
+hipError_t launch_sp2_update(int n, int nsplit, const double* ws, double* X, int nocc, double* st, hipStream_t s) {
+  const hipError_t e = launch_k(k_sp2_reduce_update, dim3(kSp2Groups), dim3(256), s, n, nsplit, ws,
+                                (int64_t)n * n, X, st);
+  if (e != hipSuccess) return e;
+  return launch_k(k_sp2_decide, dim3(1), dim3(kSp2Groups), s, nocc, st);
 }
 
 hipError_t launch_diag(int n, const double* R, double* d, hipStream_t s) {

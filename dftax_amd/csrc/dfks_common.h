@@ -32,6 +32,9 @@ enum Sp2State {
   kSpResid = 7, kSpMinPiv = 8, kSpHist = 16
 };
 constexpr int kSp2MaxIter = 128;
+constexpr int kSp2Groups = 512;       // workgroups of the SP2 update (partial trace sums)
+constexpr int kSpPart = kSpHist + kSp2MaxIter;  // their partial sums: 2 kSp2Groups doubles
+constexpr int kSp2State = kSpPart + 2 * kSp2Groups;
 constexpr int kSp2Block = 4;          // purification iterations per graph launch (divides kSp2MaxIter)
 constexpr int kPanel = 32;            // pivoted Cholesky panel width
 

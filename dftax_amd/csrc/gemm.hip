@@ -359,8 +359,9 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K,
   const bool b_in = has_b && n0 + br < N;
   const double* __restrict__ pa = AK ? A + (a_in ? m0 + ar : 0) * lda : A + (a_in ? m0 + ar : 0);
   const double* __restrict__ pb = B + (int64_t)(b_in ? n0 + br : 0) * ldb;
+  // A and B are loaded one k-step ahead into registers.
   double ra[4], rb[8];
-  auto load = [&](int64_t k0) {
+  auto load_a = [&](int64_t k0) {
     const int64_t ka = k0 + ak;
     if (AK && a_in && vec_a && ka + 4 <= k_end) {
 #pragma unroll
@@ -373,6 +374,8 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K,
 #pragma unroll
       for (int j = 0; j < 4; ++j) ra[j] = (a_in && ka + j < k_end) ? (AK ? pa[ka + j] : pa[(ka + j) * lda]) : 0.0;
     }
+  };
+  auto load_b = [&](int64_t k0) {
     if (!has_b) return;
     const int64_t kb = k0 + bk;
     if (b_in && vec_b && kb + 8 <= k_end) {
@@ -400,13 +403,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K,
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < NJ; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
-  const int64_t nk = k_end > k_begin ? (k_end - k_begin + kW8BK - 1) / kW8BK : 0;
-  load(k_begin);
-  store(0);
-  __syncthreads();
-  for (int64_t t = 0; t < nk; ++t) {
-    const int cur = (int)(t & 1);
-    if (t + 1 < nk) load(k_begin + (t + 1) * kW8BK);
+  auto compute = [&](int cur) {
 #pragma unroll
     for (int kk = 0; kk < kW8BK; kk += 4) {
       double a[2], b[NJ];
@@ -420,6 +417,19 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K,
         for (int j = 0; j < NJ; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+  };
+  const int64_t nk = k_end > k_begin ? (k_end - k_begin + kW8BK - 1) / kW8BK : 0;
+  load_a(k_begin);
+  load_b(k_begin);
+  store(0);
+  __syncthreads();
+  for (int64_t t = 0; t < nk; ++t) {
+    const int cur = (int)(t & 1);
+    if (t + 1 < nk) {
+      load_a(k_begin + (t + 1) * kW8BK);
+      load_b(k_begin + (t + 1) * kW8BK);
+    }
+    compute(cur);
     if (t + 1 < nk) store(cur ^ 1);
     __syncthreads();
   }

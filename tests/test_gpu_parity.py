@@ -247,6 +247,22 @@ def test_mixed_scf_refines_to_fp64(dimer_backends):
     assert abs(rmx.e_tot - r64.e_tot) < 1e-8
 
 
+def test_purification_graph_survives_scf_restart(dimer_backends):
+    """A second SCF on the same handle (a new dfks_scf_begin) purifies every
+    step again and repeats the first SCF: the recorded purification graph only
+    refers to buffers that live as long as the handle. (Repeated builds agree
+    to rounding, not bitwise: the Vxc row assembly sums in LDS with atomics.)"""
+    from dftax_amd import scf
+    k = dimer_backends["rocblas"]
+    r1 = scf(k, precision="fp64", e_tol=1e-9)
+    r2 = scf(k, precision="fp64", e_tol=1e-9)
+    st = k.coulomb.stats()
+    _say(f"SCF twice on one handle: {r1.e_tot:.12f} ({r1.n_iter} it), {r2.e_tot:.12f} ({r2.n_iter} it), "
+         f"eigensolver steps {st['eigh_steps']}")
+    assert r1.converged and r2.converged and st["eigh_steps"] == 0
+    assert r2.n_iter == r1.n_iter and abs(r2.e_tot - r1.e_tot) < 1e-12
+
+
 def test_purified_density_matches_eigensolver(dimer_backends, torch_cuda):
     """SP2 purification + pivoted-Cholesky factor against an FP64 eigensolve in every step.
 
