@@ -50,8 +50,8 @@ hipError_t launch_becke(int64_t n, const double* pts, const double* raw, const i
 hipError_t launch_ao(int64_t ng, const double* pts, const ShellSet& S, int64_t nao, double* ao, hipStream_t st);
 hipError_t launch_ao_cmp(int nblk, int G, int n, const double* pts, const int2* shl, const int64_t* off,
                          const ShellSet& S, double* ao, hipStream_t st);
-template <typename T>
-hipError_t launch_gather_sym(int64_t nblk, int n, const int32_t* cols, const double* P, int64_t nao, T* out,
+template <typename TP, typename T>
+hipError_t launch_gather_sym(int64_t nblk, int n, const int32_t* cols, const TP* P, int64_t nao, T* out,
                              hipStream_t st);
 template <typename T>
 hipError_t launch_vassemble(int nao, const int64_t* rowptr, const int64_t* prow, const int64_t* pcol,
@@ -1237,6 +1237,8 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
   const bool mixed = h->mixed_now && mode == kXcEnergy;
   const double one = 1.0, zero = 0.0;
   const double thresh = h->opt.rho_thresh;
+  // mixed: the bins gather from an FP32 copy of P (the same rounding as converting each gathered element)
+  if (mixed) HIPCHK(launch_to_f32((int64_t)nao * nao, P, h->d_P32, st));
   for (const auto& bn : h->xbins) {
     const int n = bn.n;
     const int64_t nb = bn.nblk, rows = nb * kXcG;
@@ -1248,7 +1250,7 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
       float* q = reinterpret_cast<float*>(h->d_xc_q) + bn.q_off;
       const float* ao = h->d_xc_ao32 + bn.ao_off;
       float* zw = reinterpret_cast<float*>(h->d_xc_zw);
-      HIPCHK(launch_gather_sym<float>(nb, n, cols, P, nao, q, st));
+      HIPCHK((launch_gather_sym<float, float>(nb, n, cols, h->d_P32, nao, q, st)));
       RBCHK(rocblas_sgemm_strided_batched(h->rb, rocblas_operation_transpose, rocblas_operation_none, n, kXcG, n,
                                           &onef, q, n, (rocblas_stride)n * n, ao, n, (rocblas_stride)kXcG * n,
                                           &zerof, zw, n, (rocblas_stride)kXcG * n, (int)nb));
@@ -1261,7 +1263,7 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
     double* q = h->d_xc_q + bn.q_off;
     const double* ao = h->d_xc_ao + bn.ao_off;
     double* zw = h->d_xc_zw;
-    HIPCHK(launch_gather_sym<double>(nb, n, cols, P, nao, q, st));
+    HIPCHK((launch_gather_sym<double, double>(nb, n, cols, P, nao, q, st)));
     // Z_b (G x n, row-major) = Phi_b Pg_b; rocBLAS column-major: Z_b^T = Pg_b^T Phi_b^T
     tuned_product(h, 1, n, nb, 0, 2, 0, [&](int choice) {
       if (choice == 1) {

@@ -183,9 +183,9 @@ hipError_t launch_ao_cmp(int nblk, int G, int n, const double* pts, const int2* 
 // Pg[b][k][j] = P[c_k][c_j] for the compact columns c = cols[b][0..n) of each
 // block of a bin (0 where either column is padding, c < 0). One wavefront per
 // row (b, k), grid-stride over the rows.
-template <typename T>
+template <typename TP, typename T>
 __global__ __launch_bounds__(256) void k_gather_sym(int64_t nblk, int n, const int32_t* __restrict__ cols,
-                                                     const double* __restrict__ P, int64_t nao, T* __restrict__ out) {
+                                                     const TP* __restrict__ P, int64_t nao, T* __restrict__ out) {
   const int lane = threadIdx.x & 63;
   const int64_t nrow = nblk * n;
   for (int64_t row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrow; row += (int64_t)gridDim.x * 4) {
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_gather_sym(int64_t nblk, int n, const i
     const int32_t* cb = cols + b * n;
     const int32_t cr = cols[row];
     T* o = out + row * n;
-    const double* pr = P + (int64_t)(cr >= 0 ? cr : 0) * nao;
+    const TP* pr = P + (int64_t)(cr >= 0 ? cr : 0) * nao;
     for (int c = lane; c < n; c += 64) {
       const int32_t cc = cb[c];
       o[c] = (cr >= 0 && cc >= 0) ? (T)pr[cc] : (T)0;
@@ -201,19 +201,19 @@ __global__ __launch_bounds__(256) void k_gather_sym(int64_t nblk, int n, const i
   }
 }
 
-template <typename T>
-hipError_t launch_gather_sym(int64_t nblk, int n, const int32_t* cols, const double* P, int64_t nao, T* out,
+template <typename TP, typename T>
+hipError_t launch_gather_sym(int64_t nblk, int n, const int32_t* cols, const TP* P, int64_t nao, T* out,
                              hipStream_t st) {
   const int64_t nrow = nblk * n;
   if (nrow == 0) return hipSuccess;
   const int64_t nb = std::min<int64_t>((nrow + 3) / 4, 65536);
-  hipLaunchKernelGGL(k_gather_sym<T>, dim3((unsigned)nb), dim3(256), 0, st, nblk, n, cols, P, nao, out);
+  hipLaunchKernelGGL((k_gather_sym<TP, T>), dim3((unsigned)nb), dim3(256), 0, st, nblk, n, cols, P, nao, out);
   return hipGetLastError();
 }
-template hipError_t launch_gather_sym<double>(int64_t, int, const int32_t*, const double*, int64_t, double*,
-                                             hipStream_t);
-template hipError_t launch_gather_sym<float>(int64_t, int, const int32_t*, const double*, int64_t, float*,
-                                            hipStream_t);
+template hipError_t launch_gather_sym<double, double>(int64_t, int, const int32_t*, const double*, int64_t, double*,
+                                                     hipStream_t);
+template hipError_t launch_gather_sym<float, float>(int64_t, int, const int32_t*, const float*, int64_t, float*,
+                                                   hipStream_t);
 
 // Vh[m][:] = sum over the (block, local column i) entries of row m of the
 // block products prod_b[i][:], scattered to the blocks' columns. Workgroup

@@ -131,12 +131,27 @@ __global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const TB* _
     const TB* row = B + r * nq;
     double acc = 0.0;
     if (VEC) {
+      // four 16-byte loads in flight per lane, four partial sums (fixed order)
       const vecT* rv = reinterpret_cast<const vecT*>(row);
-      for (int q = lane; q < nq / PER; q += 64) {
-        vecT x = rv[q];
+      const int nv = nq / PER;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int q = lane;
+      for (; q + 192 < nv; q += 256) {
+        const vecT x0 = rv[q], x1 = rv[q + 64], x2 = rv[q + 128], x3 = rv[q + 192];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) acc += (double)x[j] * s_g[q * PER + j];
+        for (int j = 0; j < PER; ++j) {
+          a0 += (double)x0[j] * s_g[q * PER + j];
+          a1 += (double)x1[j] * s_g[(q + 64) * PER + j];
+          a2 += (double)x2[j] * s_g[(q + 128) * PER + j];
+          a3 += (double)x3[j] * s_g[(q + 192) * PER + j];
+        }
       }
+      for (; q < nv; q += 64) {
+        const vecT x = rv[q];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) a0 += (double)x[j] * s_g[q * PER + j];
+      }
+      acc = (a0 + a1) + (a2 + a3);
     } else {
       for (int q = lane; q < nq; q += 64) acc += (double)row[q] * s_g[q];
     }

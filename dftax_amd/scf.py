@@ -69,16 +69,36 @@ def _core_density(ks):
     return (2.0 * Co @ Co.T).contiguous()
 
 
+def mixed_phase_done(derrs, mixed_d_tol: float = 1e-4, mixed_max_iter: int = 30, stall: int = 3) -> bool:
+    """Whether the mixed phase ends after the steps whose orbital gradients are
+    ``derrs`` (oldest first): the gradient is below ``mixed_d_tol``, the phase
+    has run ``mixed_max_iter`` steps, or it has stalled at the split-BF16 floor
+    -- the last ``stall`` steps all stay above 0.7 x the best gradient before
+    them while that best is already below 10 x ``mixed_d_tol`` (the floor of
+    the split-BF16 Fock error grows with the system: ~1e-4 for (H2O)_64)."""
+    n = len(derrs)
+    if n == 0:
+        return False
+    if derrs[-1] < mixed_d_tol or n >= mixed_max_iter:
+        return True
+    if stall > 0 and n > stall:
+        best = min(derrs[:-stall])
+        if best < 10.0 * mixed_d_tol and min(derrs[-stall:]) > 0.7 * best:
+            return True
+    return False
+
+
 def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, diis_space: int = 8,
         lindep_thresh: float = 1e-7, level_shift: float = 0.0, guess=None, verbose: bool = False,
         precision: str | None = None, mixed_d_tol: float = 1e-4, mixed_max_iter: int = 30,
-        refine: int | None = None) -> KSResult:
+        mixed_stall: int = 3, refine: int | None = None) -> KSResult:
     """Run the KS SCF to self-consistency (closed shell).
 
     ``precision="mixed"`` (needs a KS built with ``hip(precision="mixed")``)
     iterates with split-BF16 Fock builds until the orbital gradient
-    ``|err| < mixed_d_tol`` (or ``mixed_max_iter`` mixed steps), then switches
-    to FP64: the next step extrapolates over the mixed DIIS history and builds
+    ``|err| < mixed_d_tol``, ``mixed_max_iter`` mixed steps, or a stall of
+    ``mixed_stall`` steps at the split-BF16 floor (``mixed_phase_done``), then
+    switches to FP64: the next step extrapolates over the mixed DIIS history and builds
     its Fock matrix in FP64, the DIIS history restarts after it, and the FP64
     steps run to the usual ``e_tol``/``d_tol``.
     The switch keys on the gradient only: the split-BF16 energy carries noise
@@ -123,8 +143,7 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
         if verbose:
             print(f"  scf {it} [{'mixed' if mixed else 'fp64'}]: E={e:.10f} dE={de:+.2e} |[F,P]|={derr:.2e}")
         if mixed:
-            n_mixed = sum(1 for h in hist if h[3] == "mixed")
-            if derr < mixed_d_tol or n_mixed >= mixed_max_iter:
+            if mixed_phase_done([h[2] for h in hist if h[3] == "mixed"], mixed_d_tol, mixed_max_iter, mixed_stall):
                 ks.coulomb.set_precision(False)
                 mixed = False
         else:

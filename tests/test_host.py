@@ -161,3 +161,22 @@ def test_serving_latency_slowdowns():
     assert st["ok"] == 3 and abs(st["p50_s"] - 1.0) < 1e-12
     assert abs(st["slowdown_p50"] - 2.0) < 1e-12 and st["slowdown_p99"] > 2.9
     assert abs(st["mean_queue_wait_s"] - 0.2 / 3) < 1e-12
+
+
+def test_mixed_phase_switch_rule():
+    """The mixed phase of scf ends at the gradient threshold, at the step cap, or
+    on a stall at the split-BF16 floor -- never on an early, large-gradient stall."""
+    from dftax_amd.scf import mixed_phase_done
+    steady = [1e-1, 3e-2, 1e-2, 3e-3, 1e-3, 3e-4, 1.2e-4, 9e-5]
+    assert [mixed_phase_done(steady[:k]) for k in range(1, len(steady) + 1)] == [False] * 7 + [True]
+    # a plateau just above the threshold: switch once three steps fail to improve on the best by 30 %
+    plateau = [1e-1, 1e-2, 1e-3, 3e-4, 2.0e-4, 1.9e-4, 2.2e-4, 1.8e-4]
+    done = [mixed_phase_done(plateau[:k]) for k in range(1, len(plateau) + 1)]
+    assert done == [False] * 7 + [True]
+    # early oscillation far above the floor does not count as a stall
+    early = [1e-1, 2e-1, 1.5e-1, 1.2e-1, 1.1e-1, 9e-2]
+    assert not any(mixed_phase_done(early[:k]) for k in range(1, len(early) + 1))
+    # the cap
+    assert mixed_phase_done([1e-2] * 30) and not mixed_phase_done([1e-2] * 29)
+    # stall detection off
+    assert not mixed_phase_done(plateau, stall=0)
