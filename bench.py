@@ -60,6 +60,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="h2o32", choices=sorted(WORKLOADS))
+    ap.add_argument("--refine", type=int, default=3,
+                    help="FP64 steps at most after the mixed phase in the energy-refined mixed SCF")
+    ap.add_argument("--mixed-d-tol", type=float, default=1e-4,
+                    help="orbital gradient that ends the mixed phase of the mixed SCFs")
     ap.add_argument("--xc", default="PBE0")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scf", action="store_true", help="skip the two full SCF solves (FP64 and mixed)")
@@ -140,11 +144,11 @@ def main() -> int:
         t64 = time.perf_counter() - t
         note(f"FP64 SCF: {r64.n_iter} iterations, {t64:.1f} s")
         t = time.perf_counter()
-        rmx = scf(ks, precision="mixed")
+        rmx = scf(ks, precision="mixed", mixed_d_tol=args.mixed_d_tol)
         tmx = time.perf_counter() - t
         note(f"mixed SCF: {rmx.n_iter} iterations, {tmx:.1f} s")
         t = time.perf_counter()
-        rrf = scf(ks, precision="mixed", refine=3)
+        rrf = scf(ks, precision="mixed", refine=args.refine, mixed_d_tol=args.mixed_d_tol)
         trf = time.perf_counter() - t
         note(f"mixed SCF + FP64 energy refinement: {rrf.n_iter} iterations, {trf:.1f} s")
 
