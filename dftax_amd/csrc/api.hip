@@ -108,8 +108,9 @@ hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, in
                                       int64_t bs_c, hipStream_t st);
 hipError_t launch_gemm_f64_batched(const GemmArgs& g, double* C, int batch, int64_t bs_a, int64_t bs_b,
                                    int64_t bs_c, hipStream_t st);
-hipError_t launch_gemm_w8(const GemmArgs& g, double* C, hipStream_t st);
-hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, hipStream_t st);
+hipError_t launch_gemm_w8(const GemmArgs& g, double* C, const RowDot& rd, hipStream_t st);
+hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, const RowDot& rd, hipStream_t st);
+hipError_t launch_colsum(int nrows, int ncols, const double* part, double* out, int accumulate, hipStream_t st);
 hipError_t launch_gemm_partials(int prec, const GemmArgs& g, int nsplit, int sym, void* ws, int variant,
                                 hipStream_t st);
 hipError_t launch_gemm_reduce(int prec, const GemmArgs& g, double* C, double alpha, double beta, int nsplit, int sym,
@@ -1129,13 +1130,16 @@ int tuned_product(dfks_ctx* h, int tag, int64_t d0, int64_t d1, int64_t d2, int 
 }
 
 // T = B^T C (nao*nfit x nc), FP64
-void t_product(dfks_ctx* h, const double* C, int nc, hipStream_t st) {
+// Returns whether the gamma row terms of rd were produced with T (the
+// 8-wavefront kernel computes them in its epilogue).
+bool t_product(dfks_ctx* h, const double* C, int nc, const RowDot& rd, hipStream_t st) {
   const int nao = h->nao, nfit = h->nfit;
   const double one = 1.0, zero = 0.0;
-  tuned_product(h, 0, (int64_t)nao * nfit, nc, nao, 3, 0, [&](int choice) {
+  const bool fuse = nc <= 160 && nfit >= 128;
+  const int v = tuned_product(h, 0, (int64_t)nao * nfit, nc, nao, 3, 0, [&](int choice) {
     const GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B, (int64_t)nao * nfit, false, C, nao, true};
     if (choice == 2) {
-      HIPCHK(launch_gemm_w8(g, h->d_Tx, st));
+      HIPCHK(launch_gemm_w8(g, h->d_Tx, fuse ? rd : RowDot(), st));
     } else if (choice == 1) {
       HIPCHK(launch_gemm(kF64, g, h->d_Tx, 1.0, 0.0, 1, 0, nullptr, st));
     } else {
@@ -1144,6 +1148,7 @@ void t_product(dfks_ctx* h, const double* C, int nc, hipStream_t st) {
                              nc));
     }
   }, st);
+  return fuse && v == 2;
 }
 
 // gamma, J and K of P = sum_i s_i c_i c_i^T from its factor alone:
@@ -1170,6 +1175,13 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
     const int nc = i1 - i0;
     const double sg = sgn[i0];
     const double beta = first ? 0.0 : 1.0;
+    RowDot rd;
+    rd.cf = Cf + (int64_t)i0 * nao;
+    rd.sg = h->d_sgn + i0;
+    rd.nao = nao;
+    rd.nq = nfit;
+    rd.out = h->d_gpart;
+    bool rows_done = false;
     // T[m,Q,i] = sum_l B[l,m,Q] Cf[l,i]
     if (h->recompute) {
       exchange_recompute(h, Cf + (int64_t)i0 * nao, mixed ? h->d_Cf32 + (int64_t)i0 * nao : nullptr, nc, st);
@@ -1177,16 +1189,21 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
       // split-BF16 T: the 128-row tile of k_gemm or the 8-wavefront 256-row tile, whichever is faster here
       const GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
                        h->d_Cf32 + (int64_t)i0 * nao, nao, true};
-      tuned_product(h, 4, (int64_t)nao * nfit, nc, nao, 2, 1, [&](int c) {
-        if (c == 1) HIPCHK(launch_gemm_b3w8(g, h->d_T32, st));
+      const bool fuse = nc <= 160 && nfit >= 256;
+      const int v = tuned_product(h, 4, (int64_t)nao * nfit, nc, nao, 2, 1, [&](int c) {
+        if (c == 1) HIPCHK(launch_gemm_b3w8(g, h->d_T32, fuse ? rd : RowDot(), st));
         else HIPCHK(launch_gemm_f32out(g, h->d_T32, st));
       }, st);
+      rows_done = fuse && v == 1;
     } else {
-      t_product(h, Cf + (int64_t)i0 * nao, nc, st);
+      rows_done = t_product(h, Cf + (int64_t)i0 * nao, nc, rd, st);
     }
     h->mark(kStExchangeGemm + 1, st);
-    // gamma += sum_i s_i T_i c_i (timed with the exchange product K below)
-    if (mixed)
+    // gamma += sum_i s_i T_i c_i (timed with the exchange product K below); its row terms come
+    // with T from the 8-wavefront kernels, otherwise from a pass over T
+    if (rows_done)
+      HIPCHK(launch_colsum(nao, nfit, h->d_gpart, h->d_gamma, 1, st));
+    else if (mixed)
       HIPCHK(launch_gamma_f32(nao, nfit, nc, h->d_T32, Cf + (int64_t)i0 * nao, h->d_sgn + i0, h->d_gamma, h->d_gpart,
                               1, st));
     else

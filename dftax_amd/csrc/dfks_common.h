@@ -79,6 +79,18 @@ inline hipError_t launch_k(void (*k)(P...), dim3 grid, dim3 block, hipStream_t s
   return e;
 }
 
+// Optional epilogue of the exchange product T (rows m' = m * nq + q, columns
+// i): out[m'] = sum_i sg[i] cf[i * nao + m] T[m', i], the per-row terms of the
+// fitted density gamma_q = sum_m out[m * nq + q], computed from the tile in
+// registers so T is not read again for it. Needs the whole N range in one tile
+// and nq at least the tile's rows (a tile then spans at most two m).
+struct RowDot {
+  const double* cf = nullptr;
+  const double* sg = nullptr;
+  int nao = 0, nq = 1;
+  double* out = nullptr;
+};
+
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 __host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }

@@ -333,7 +333,8 @@ template <bool AK, int NJ>
 __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K, const double* __restrict__ A,
                                                    int64_t lda, const double* __restrict__ B, int64_t ldb,
                                                    double* __restrict__ C, int64_t ldc, int64_t split_stride,
-                                                   int64_t kchunk, int sym, int ntiles, int vec_a, int vec_b) {
+                                                   int64_t kchunk, int sym, int ntiles, int vec_a, int vec_b,
+                                                   RowDot rd) {
   constexpr int BN = 32 * NJ;
   __shared__ double sA[2][kW8BM][kW8BK + 1];
   __shared__ double sB[2][BN][kW8BK + 1];
@@ -359,6 +360,17 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K,
   const bool b_in = has_b && n0 + br < N;
   const double* __restrict__ pa = AK ? A + (a_in ? m0 + ar : 0) * lda : A + (a_in ? m0 + ar : 0);
   const double* __restrict__ pb = B + (int64_t)(b_in ? n0 + br : 0) * ldb;
+  // gamma row terms: the tile's rows span at most two m (nq >= 128), so the coefficients
+  // sg[i] cf[i, m] are 2 x BN values, loaded here and stored to LDS after the main loop
+  double c_sg = 0.0, c_cf = 0.0;
+  if (rd.out && tid < 2 * BN) {
+    const int gn = n0 + tid % BN;
+    const int64_t mm = m0 / rd.nq + tid / BN;
+    if (gn < N && mm < rd.nao) {
+      c_sg = rd.sg[gn];
+      c_cf = rd.cf[(int64_t)gn * rd.nao + mm];
+    }
+  }
   // A and B are loaded one k-step ahead into registers.
   double ra[4], rb[8];
   auto load_a = [&](int64_t k0) {
@@ -444,6 +456,35 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K,
         const int gn = n0 + wn + 16 * j + li;
         if (gm < M && gn < N) Cz[gm * ldc + gn] = acc[i][j][r];
       }
+  if (rd.out) {
+    // row sums over this tile's columns: 16 lanes of a row group, then the two N wavefronts through LDS
+    __shared__ double s_row[2][kW8BM];
+    __shared__ double s_cs[2][BN];
+    if (tid < 2 * BN) s_cs[tid / BN][tid % BN] = c_sg * c_cf;
+    __syncthreads();
+    const int64_t m_first = m0 / rd.nq;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int lr = wm + 16 * i + lk + 4 * r;
+        const int64_t gm = m0 + lr;
+        const int which = gm < M ? (int)(gm / rd.nq - m_first) : 0;
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int c = wn + 16 * j + li;
+          v += acc[i][j][r] * s_cs[which][c];
+        }
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 8, 64);
+        if (li == 0) s_row[wave & 1][lr] = v;
+      }
+    __syncthreads();
+    if (tid < kW8BM && m0 + tid < M) rd.out[m0 + tid] = s_row[0][tid] + s_row[1][tid];
+  }
 }
 
 // Split-BF16 (three-term) product C = A B^T with FP32 accumulate for A
@@ -468,7 +509,7 @@ __device__ __forceinline__ void split_bf16x8(const float* x, bf16x8& h, bf16x8& 
 
 __global__ __launch_bounds__(512, 1) void k_gemm_b3w8(int64_t M, int N, int K, const float* __restrict__ A,
                                                      int64_t lda, const float* __restrict__ B, int64_t ldb,
-                                                     float* __restrict__ C, int vec_a, int vec_b) {
+                                                     float* __restrict__ C, int vec_a, int vec_b, RowDot rd) {
   __shared__ __attribute__((aligned(16))) __bf16 sAh[2][kB3BM][kB3PAD];
   __shared__ __attribute__((aligned(16))) __bf16 sAl[2][kB3BM][kB3PAD];
   __shared__ __attribute__((aligned(16))) __bf16 sBh[2][kB3BN][kB3PAD];
@@ -488,6 +529,17 @@ __global__ __launch_bounds__(512, 1) void k_gemm_b3w8(int64_t M, int N, int K, c
   const bool b_in = has_b && n0 + br < N;
   const float* __restrict__ pa = A + (a_in ? m0 + ar : 0);
   const float* __restrict__ pb = B + (int64_t)(b_in ? n0 + br : 0) * ldb;
+  // gamma row terms: the tile's rows span at most two m (nq >= 256), so the coefficients
+  // sg[i] cf[i, m] are 2 x 160 values, loaded here and stored to LDS after the main loop
+  double c_sg = 0.0, c_cf = 0.0;
+  if (rd.out && tid < 2 * kB3BN) {
+    const int gn = n0 + tid % kB3BN;
+    const int64_t mm = m0 / rd.nq + tid / kB3BN;
+    if (gn < N && mm < rd.nao) {
+      c_sg = rd.sg[gn];
+      c_cf = rd.cf[(int64_t)gn * rd.nao + mm];
+    }
+  }
   // A and B are loaded one k-step ahead into registers and split into bf16
   // hi + lo when stored to LDS. ra[4 q + c]: row ar + c, k ak + q
   float ra[16], rb[16];
@@ -599,6 +651,36 @@ __global__ __launch_bounds__(512, 1) void k_gemm_b3w8(int64_t M, int N, int K, c
         const int gn = n0 + wn + 16 * j + li;
         if (gm < M && gn < N) C[gm * N + gn] = acc[i][j][r];
       }
+  if (rd.out) {
+    // row sums of the FP32 products in FP64 (as the gamma kernel sums the stored T): 16 lanes of
+    // a row group, then the two N wavefronts through LDS
+    __shared__ double s_row[2][kB3BM];
+    __shared__ double s_cs[2][kB3BN];
+    if (tid < 2 * kB3BN) s_cs[tid / kB3BN][tid % kB3BN] = c_sg * c_cf;
+    __syncthreads();
+    const int64_t m_first = m0 / rd.nq;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int lr = wm + 16 * i + 4 * lk + r;
+        const int64_t gm = m0 + lr;
+        const int which = gm < M ? (int)(gm / rd.nq - m_first) : 0;
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+          const int c = wn + 16 * j + li;
+          v += (double)acc[i][j][r] * s_cs[which][c];
+        }
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 8, 64);
+        if (li == 0) s_row[wave & 1][lr] = v;
+      }
+    __syncthreads();
+    if (tid < kB3BM && m0 + tid < M) rd.out[m0 + tid] = s_row[0][tid] + s_row[1][tid];
+  }
 }
 
 // out[m*ldo + n] = alpha * sum_z ws[z][m][n] (+ beta * out); symmetric mode reads
@@ -662,9 +744,11 @@ static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nspl
 // k_gemm_w8 launch: ldc, split and symmetric mode as launch_gemm_t (NJ = 5
 // only without the symmetric mode); B must be K-contiguous.
 template <int NJ>
-static hipError_t launch_w8_t(const GemmArgs& g, double* C, int64_t ldc, int nsplit, int sym, hipStream_t st) {
+static hipError_t launch_w8_t(const GemmArgs& g, double* C, int64_t ldc, int nsplit, int sym, hipStream_t st,
+                              const RowDot& rd = RowDot()) {
   constexpr int BN = 32 * NJ;
   if (!g.b_kcontig || (sym && NJ != 4)) return hipErrorInvalidValue;
+  if (rd.out && (g.N > BN || nsplit != 1 || sym || rd.nq < kW8BM)) return hipErrorInvalidValue;
   const int64_t ntm = ((int64_t)g.M + kW8BM - 1) / kW8BM, ntn = (g.N + BN - 1) / BN;
   const int64_t ntl = sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
   if (ntl > 0x7fffffff || nsplit > 65535) return hipErrorInvalidValue;
@@ -678,19 +762,20 @@ static hipError_t launch_w8_t(const GemmArgs& g, double* C, int64_t ldc, int nsp
   const int64_t split_stride = (int64_t)g.M * ldc;
   if (g.a_kcontig)
     return launch_k(k_gemm_w8<true, NJ>, grid, dim3(512), st, (int64_t)g.M, g.N, g.K, A, g.lda, B, g.ldb, C, ldc,
-                    split_stride, kchunk, sym, (int)ntl, vec_a, vec_b);
+                    split_stride, kchunk, sym, (int)ntl, vec_a, vec_b, rd);
   return launch_k(k_gemm_w8<false, NJ>, grid, dim3(512), st, (int64_t)g.M, g.N, g.K, A, g.lda, B, g.ldb, C, ldc,
-                  split_stride, kchunk, sym, (int)ntl, vec_a, vec_b);
+                  split_stride, kchunk, sym, (int)ntl, vec_a, vec_b, rd);
 }
 
 // C = A B^T in FP64 by k_gemm_w8, no split (ldc = N): 128- or 160-wide N
-// tiles, whichever pads N less. B must be K-contiguous.
-hipError_t launch_gemm_w8(const GemmArgs& g, double* C, hipStream_t st) {
+// tiles, whichever pads N less. B must be K-contiguous. rd.out (optional,
+// N <= 160): the row terms of gamma.
+hipError_t launch_gemm_w8(const GemmArgs& g, double* C, const RowDot& rd, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   const int t128 = (g.N + 127) / 128, t160 = (g.N + 159) / 160;
   if (t160 < t128 && (int64_t)t160 * 160 - g.N < (int64_t)t128 * 128 - g.N)
-    return launch_w8_t<5>(g, C, g.N, 1, 0, st);
-  return launch_w8_t<4>(g, C, g.N, 1, 0, st);
+    return launch_w8_t<5>(g, C, g.N, 1, 0, st, rd);
+  return launch_w8_t<4>(g, C, g.N, 1, 0, st, rd);
 }
 
 // Split-K partial tiles of A B^T into ws (nsplit x M x N, FP64 for kF64 and
@@ -762,10 +847,12 @@ hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, in
   return launch_gemm_t<kBF16x3, float>(g, C, g.N, 1, 0, st, batch, bs_a, bs_b, bs_c);
 }
 
-// C = A B^T in FP32 by k_gemm_b3w8 (split-BF16, A M-contiguous, B K-contiguous, ldc = N).
-hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, hipStream_t st) {
+// C = A B^T in FP32 by k_gemm_b3w8 (split-BF16, A M-contiguous, B K-contiguous, ldc = N);
+// rd.out (optional, N <= 160): the row terms of gamma.
+hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, const RowDot& rd, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (g.a_kcontig || !g.b_kcontig || g.K > 0x7fffffff) return hipErrorInvalidValue;
+  if (rd.out && (g.N > kB3BN || rd.nq < kB3BM)) return hipErrorInvalidValue;
   const float* A = static_cast<const float*>(g.A);
   const float* B = static_cast<const float*>(g.B);
   // 16-byte loads of 4 consecutive rows of A; 4 floats at a time along k of B
@@ -773,7 +860,7 @@ hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, hipStream_t st) {
   const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % 4 == 0);
   const dim3 grid((unsigned)(((int64_t)g.M + kB3BM - 1) / kB3BM), (unsigned)((g.N + kB3BN - 1) / kB3BN));
   return launch_k(k_gemm_b3w8, grid, dim3(512), st, (int64_t)g.M, g.N, (int)g.K, A, g.lda, B, g.ldb, C, vec_a,
-                  vec_b);
+                  vec_b, rd);
 }
 
 // Batched FP64 variant (no split, FP64 accumulate): for b < batch,

@@ -86,13 +86,23 @@ __global__ __launch_bounds__(256) void k_gamma_part(int nao, int nq, int nf, con
   }
 }
 
+// out[q] (+)= sum_m part[m][q]: a workgroup per 16 columns, its 16 row groups
+// (m = g, g + 16, ...) summed in a fixed order through LDS.
 __global__ __launch_bounds__(256) void k_colsum(int nrows, int ncols, const double* __restrict__ part,
                                                 double* __restrict__ out, int accumulate) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ncols) return;
+  __shared__ double s_part[16][17];
+  const int c = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int q = blockIdx.x * 16 + c;
   double s = 0.0;
-  for (int m = 0; m < nrows; ++m) s += part[(int64_t)m * ncols + q];
-  out[q] = accumulate ? out[q] + s : s;
+  if (q < ncols)
+    for (int m = g; m < nrows; m += 16) s += part[(int64_t)m * ncols + q];
+  s_part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && q < ncols) {
+    double t = 0.0;
+    for (int k = 0; k < 16; ++k) t += s_part[k][c];
+    out[q] = accumulate ? out[q] + t : t;
+  }
 }
 
 template <typename TT>
@@ -102,7 +112,13 @@ static hipError_t gamma_impl(int nao, int nq, int nf, const TT* T, const double*
   const int64_t waves = (int64_t)nao * ((nq + 7) / 8);
   hipLaunchKernelGGL((k_gamma_part<TT>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, nao, nq, nf, T, Cf,
                      sgn, part);
-  hipLaunchKernelGGL(k_colsum, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, nao, nq, part, gamma,
+  hipLaunchKernelGGL(k_colsum, dim3((unsigned)((nq + 15) / 16)), dim3(256), 0, st, nao, nq, part, gamma,
+                     accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_colsum(int nrows, int ncols, const double* part, double* out, int accumulate, hipStream_t st) {
+  hipLaunchKernelGGL(k_colsum, dim3((unsigned)((ncols + 15) / 16)), dim3(256), 0, st, nrows, ncols, part, out,
                      accumulate);
   return hipGetLastError();
 }
