@@ -23,6 +23,7 @@
 #include <mutex>
 #include <tuple>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dfks.h"
@@ -625,7 +626,9 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (int64_t g = 0; g < ng; ++g)
     for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], X[3 * g + d]); hi[d] = std::max(hi[d], X[3 * g + d]); }
-  std::vector<std::pair<uint32_t, int64_t>> key((size_t)ng);
+  // points in Morton order of their 30-bit keys: a stable LSD radix sort (three
+  // 10-bit passes), so equal keys keep the input order
+  std::vector<std::pair<uint32_t, int64_t>> key((size_t)ng), tmp((size_t)ng);
   for (int64_t g = 0; g < ng; ++g) {
     uint32_t c = 0;
     for (int d = 0; d < 3; ++d) {
@@ -634,13 +637,22 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
     }
     key[g] = {c, g};
   }
-  std::sort(key.begin(), key.end());
+  for (int pass = 0; pass < 3; ++pass) {
+    const int sh = 10 * pass;
+    std::vector<int64_t> cnt(1025, 0);
+    for (int64_t g = 0; g < ng; ++g) ++cnt[((key[g].first >> sh) & 1023u) + 1];
+    for (int b = 0; b < 1024; ++b) cnt[b + 1] += cnt[b];
+    for (int64_t g = 0; g < ng; ++g) tmp[cnt[(key[g].first >> sh) & 1023u]++] = key[g];
+    key.swap(tmp);
+  }
+  std::vector<std::pair<uint32_t, int64_t>>().swap(tmp);
   const int64_t nblk = (ng + kXcG - 1) / kXcG;
   std::vector<double> rcut(nsh);
   for (int s = 0; s < nsh; ++s) rcut[s] = shell_radius(O, s);
   std::vector<std::vector<int>> bsh((size_t)nblk);
   std::vector<int> npad((size_t)nblk);
-  for (int64_t b = 0; b < nblk; ++b) {
+  // blocks are independent: host threads take interleaved blocks
+  auto scan_block = [&](int64_t b) {
     const int64_t p0 = b * kXcG, p1 = std::min(ng, p0 + kXcG);
     double c[3] = {0, 0, 0};
     for (int64_t p = p0; p < p1; ++p)
@@ -664,6 +676,15 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
       }
     }
     npad[b] = std::max(32, (nb + 31) / 32 * 32);
+  };
+  {
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::thread::hardware_concurrency(), 16,
+                                                                   (nblk + 63) / 64}));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; ++t)
+      pool.emplace_back([&, t] { for (int64_t b = t; b < nblk; b += nth) scan_block(b); });
+    for (int64_t b = 0; b < nblk; b += nth) scan_block(b);
+    for (auto& th : pool) th.join();
   }
   // bins of equal padded width, blocks ordered by (width, position)
   std::vector<int64_t> order((size_t)nblk);
@@ -687,8 +708,6 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
   std::vector<int32_t> cols((size_t)col_tot, -1);
   std::vector<int64_t> shoff((size_t)nblk + 1, 0);
   std::vector<int2> shl;
-  std::vector<std::vector<std::pair<int64_t, int64_t>>> rows((size_t)nao);  // (q offset of the row, col offset)
-  std::vector<std::vector<int32_t>> rlen((size_t)nao);
   for (const auto& bn : h->xbins) {
     for (int64_t k = 0; k < bn.nblk; ++k) {
       const int64_t ob = bn.blk0 + k;             // block in bin order
@@ -708,24 +727,35 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
         col += nf;
       }
       shoff[ob + 1] = (int64_t)shl.size();
-      const int64_t qb = bn.q_off + k * (int64_t)bn.n * bn.n, cbo = bn.col_off + k * bn.n;
-      for (int i = 0; i < col; ++i) {
-        rows[cb[i]].push_back({qb + (int64_t)i * bn.n, cbo});
-        rlen[cb[i]].push_back(bn.n);
+    }
+  }
+  // row lists of the Vxc assembly (CSR by AO row m): one entry per (block, local
+  // column i) with cols = m, in bin and block order -- a counting pass, then the fill
+  std::vector<int64_t> vrow((size_t)nao + 1, 0);
+  for (const auto& bn : h->xbins)
+    for (int64_t e = 0; e < bn.nblk * bn.n; ++e) {
+      const int32_t m = cols[bn.col_off + e];
+      if (m >= 0) ++vrow[m + 1];
+    }
+  for (int m = 0; m < nao; ++m) vrow[m + 1] += vrow[m];
+  const int64_t nent = vrow[nao];
+  std::vector<int64_t> vprow((size_t)std::max<int64_t>(nent, 1), 0), vpcol((size_t)std::max<int64_t>(nent, 1), 0);
+  std::vector<int32_t> vlen((size_t)std::max<int64_t>(nent, 1), 0);
+  {
+    std::vector<int64_t> fill(vrow.begin(), vrow.end() - 1);
+    for (const auto& bn : h->xbins)
+      for (int64_t k = 0; k < bn.nblk; ++k) {
+        const int64_t qb = bn.q_off + k * (int64_t)bn.n * bn.n, cbo = bn.col_off + k * bn.n;
+        for (int i = 0; i < bn.n; ++i) {
+          const int32_t m = cols[cbo + i];
+          if (m < 0) continue;
+          const int64_t e = fill[m]++;
+          vprow[e] = qb + (int64_t)i * bn.n;
+          vpcol[e] = cbo;
+          vlen[e] = bn.n;
+        }
       }
-    }
   }
-  std::vector<int64_t> vrow((size_t)nao + 1, 0), vprow, vpcol;
-  std::vector<int32_t> vlen;
-  for (int m = 0; m < nao; ++m) {
-    for (size_t e = 0; e < rows[m].size(); ++e) {
-      vprow.push_back(rows[m][e].first);
-      vpcol.push_back(rows[m][e].second);
-      vlen.push_back(rlen[m][e]);
-    }
-    vrow[m + 1] = (int64_t)vprow.size();
-  }
-  if (vprow.empty()) { vprow.push_back(0); vpcol.push_back(0); vlen.push_back(0); }
   auto up = [&](auto& v) {
     using T = typename std::decay_t<decltype(v)>::value_type;
     T* p = dalloc<T>(v.size());

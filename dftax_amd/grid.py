@@ -128,20 +128,29 @@ def atom_shell_blocks(Z: int, n_radial: int, n_ang: int, prune, r_max):
     return tuple(blocks)
 
 
+@lru_cache(maxsize=None)
+def _atom_template(Z: int, n_radial: int, n_ang: int, prune, r_max):
+    """One atom's points relative to its nucleus and their raw weights, shell
+    block by shell block (the same for every atom of element Z)."""
+    pts, ws = [], []
+    for r, wr, order in atom_shell_blocks(Z, n_radial, n_ang, prune, r_max):
+        ang, aw = lebedev_grid(order)
+        pts.append((r[:, None, None] * ang[None, :, :]).reshape(-1, 3))
+        ws.append((wr[:, None] * (4.0 * np.pi * aw)[None, :]).reshape(-1))
+    return np.concatenate(pts), np.concatenate(ws)
+
+
 def atom_centred_points(symbols, coords_bohr, spec: Becke):
-    """Unpartitioned grid: points (ng,3), raw weights (ng,), owner atom (ng,)."""
+    """Unpartitioned grid: points (ng,3), raw weights (ng,), owner atom (ng,),
+    atom by atom; each atom's block is its element's template shifted to it."""
     coords = np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
-    pts, ws, own = [], [], []
     r_max = None if spec.r_max is None else float(spec.r_max)
-    for a, s in enumerate(symbols):
-        for r, wr, order in atom_shell_blocks(symbol_to_Z(s), spec.n_radial,
-                                              spec.lebedev, spec.prune, r_max):
-            ang, aw = lebedev_grid(order)
-            p = coords[a][None, None, :] + r[:, None, None] * ang[None, :, :]
-            pts.append(p.reshape(-1, 3))
-            ws.append((wr[:, None] * (4.0 * np.pi * aw)[None, :]).reshape(-1))
-            own.append(np.full(p.shape[0] * p.shape[1], a, dtype=np.int32))
-    return (np.concatenate(pts), np.concatenate(ws), np.concatenate(own))
+    tmpl = [_atom_template(symbol_to_Z(s), spec.n_radial, spec.lebedev, spec.prune, r_max) for s in symbols]
+    lens = np.array([t[1].shape[0] for t in tmpl], dtype=np.int64)
+    pts = np.repeat(coords, lens, axis=0) + np.concatenate([t[0] for t in tmpl])
+    ws = np.concatenate([t[1] for t in tmpl])
+    own = np.repeat(np.arange(len(symbols), dtype=np.int32), lens)
+    return pts, ws, own
 
 
 def atom_bragg(symbols) -> np.ndarray:
