@@ -678,7 +678,9 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
     npad[b] = std::max(32, (nb + 31) / 32 * 32);
   };
   {
-    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::thread::hardware_concurrency(), 16,
+    // a few threads only: the process may run many builds at once (the serving loop) and
+    // share a CPU quota with the host runtime's own worker threads
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::thread::hardware_concurrency(), 4,
                                                                    (nblk + 63) / 64}));
     std::vector<std::thread> pool;
     for (int t = 1; t < nth; ++t)
