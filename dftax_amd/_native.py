@@ -47,7 +47,8 @@ EXPORTS = [
 ]
 
 STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
-          "scf_diis", "scf_density", "build_1e", "build_2c", "build_3c", "build_metric_B", "build_ao"]
+          "scf_diis", "scf_density", "build_1e", "build_2c", "build_3c", "build_metric_B", "build_ao",
+          "build_ao_sort", "build_ao_scan", "build_ao_tables", "build_ao_upload", "build_ao_eval"]
 
 _lib = None
 

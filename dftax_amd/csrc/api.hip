@@ -334,7 +334,7 @@ struct dfks_ctx {
   hipEvent_t scf_ev[3] = {};
   hipEvent_t ev[kNumStages + 1] = {};
   bool ev_used[kNumStages + 1] = {};
-  double build_ms[8] = {};
+  double build_ms[10] = {};
   void mark(int i, hipStream_t st) {
     if (!timing) return;
     if (!ev[i]) (void)hipEventCreate(&ev[i]);
@@ -618,6 +618,7 @@ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
 // weights, column maps, shell lists and the row lists of the Vxc assembly, and
 // evaluates the compact AO values and gradients (resident).
 void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
+  const double tp0 = now_ms();
   const int64_t ng = h->ng;
   const int nao = h->nao;
   const Basis& O = h->orb;
@@ -646,6 +647,7 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
     key.swap(tmp);
   }
   std::vector<std::pair<uint32_t, int64_t>>().swap(tmp);
+  const double tp1 = now_ms();
   const int64_t nblk = (ng + kXcG - 1) / kXcG;
   std::vector<double> rcut(nsh);
   for (int s = 0; s < nsh; ++s) rcut[s] = shell_radius(O, s);
@@ -688,6 +690,7 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
     for (int64_t b = 0; b < nblk; b += nth) scan_block(b);
     for (auto& th : pool) th.join();
   }
+  const double tp2 = now_ms();
   // bins of equal padded width, blocks ordered by (width, position)
   std::vector<int64_t> order((size_t)nblk);
   for (int64_t b = 0; b < nblk; ++b) order[b] = b;
@@ -758,6 +761,7 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
         }
       }
   }
+  const double tp3 = now_ms();
   auto up = [&](auto& v) {
     using T = typename std::decay_t<decltype(v)>::value_type;
     T* p = dalloc<T>(v.size());
@@ -781,6 +785,7 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
   h->d_xc_zw = dalloc<double>(zw_max);
   h->d_xc_q = dalloc<double>(q_tot);
   h->d_xc_ao = dalloc<double>(ao_tot);
+  const double tp4 = now_ms();
   HIPCHK(hipMemsetAsync(h->d_xc_ao, 0, sizeof(double) * ao_tot, st));
   const ShellSet S = O.view();
   for (const auto& bn : h->xbins)
@@ -791,6 +796,14 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
     HIPCHK(launch_to_f32(ao_tot, h->d_xc_ao, h->d_xc_ao32, st));
   }
   sync(st);  // the uploads read host vectors that go out of scope here
+  // sub-phases of the compact-grid build (ms): Morton sort, block shell scans, host tables,
+  // uploads and allocations, AO evaluation to the final wait
+  const double tp5 = now_ms();
+  h->build_ms[5] = tp1 - tp0;
+  h->build_ms[6] = tp2 - tp1;
+  h->build_ms[7] = tp3 - tp2;
+  h->build_ms[8] = tp4 - tp3;
+  h->build_ms[9] = tp5 - tp4;
   h->xcb = true;
 }
 
@@ -1964,7 +1977,7 @@ int dfks_stage_times(dfks_handle h, double* ms_out, int32_t n) {
         ms_out[i] = ms;
       }
     }
-    for (int i = 0; i < 5 && kNumStages + i < n; ++i) ms_out[kNumStages + i] = h->build_ms[i];
+    for (int i = 0; i < 10 && kNumStages + i < n; ++i) ms_out[kNumStages + i] = h->build_ms[i];
   });
 }
 

@@ -185,7 +185,9 @@ int dfks_set_precision(dfks_handle h, int32_t mixed);
  * exchange SYRK K = T^T T, 2 Coulomb J = B gamma, 3 XC GEMM Z = Phi P, 4 XC kernel,
  * 5 XC SYR2K, 6 Fock assembly + energies, 7 commutator + DIIS, 8 eigensolve +
  * density; then build wall times 9 one-electron, 10 (P|Q), 11 (mn|P),
- * 12 metric factor + B, 13 AO grid. On the compact XC grid stage 3 is empty,
+ * 12 metric factor + B, 13 AO grid, and the compact grid's parts of 13:
+ * 14 Morton sort, 15 block shell scans, 16 host tables, 17 allocations and
+ * uploads, 18 AO evaluation to the final wait. On the compact XC grid stage 3 is empty,
  * 4 holds the per-bin gathers, products and pointwise kernel, and 5 the Vxc
  * row assembly. */
 int dfks_set_timing(dfks_handle h, int32_t on);
