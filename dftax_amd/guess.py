@@ -9,6 +9,7 @@ eigenvectors of the core Hamiltonian. Overlaps are computed by the library.
 
 from __future__ import annotations
 
+import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -80,6 +81,19 @@ def _union(b1: BasisSet, b2: BasisSet) -> BasisSet:
                     spherical=b1.spherical, nao=b1.nao + b2.nao, nao_cart=b1.nao_cart + b2.nao_cart)
 
 
+# Projected atomic blocks by (element, minimal basis, the atom's own shells): a block
+# depends on the shells' exponents and coefficients only (both bases sit on the same
+# centre), so molecules that share elements and a basis share it; the lock makes the
+# cache safe for concurrent builds.
+_BLOCKS: dict = {}
+_BLOCKS_LOCK = threading.Lock()
+
+
+def _block_key(sym: str, min_name: str, sub: BasisSet):
+    return (sym, min_name, bool(sub.spherical), sub.l.tobytes(), sub.nprim.tobytes(),
+            np.ascontiguousarray(sub.exps).tobytes(), np.ascontiguousarray(sub.coefs).tobytes())
+
+
 def minao_density(symbols, coords, basis: BasisSet, min_name: str = "sto-3g") -> np.ndarray:
     """Block-diagonal projected minimal-basis density (not yet renormalised)."""
     P = np.zeros((basis.nao, basis.nao))
@@ -89,6 +103,13 @@ def minao_density(symbols, coords, basis: BasisSet, min_name: str = "sto-3g") ->
         if sym not in cache:
             idx = np.where(basis.atom == a)[0]
             sub = _sub_basis(basis, idx)
+            key = _block_key(sym, min_name, sub)
+            with _BLOCKS_LOCK:
+                hit = _BLOCKS.get(key)
+            if hit is not None:
+                cache[sym] = hit
+                P[slices[a], slices[a]] = hit
+                continue
             minb = build_basis([sym], np.asarray(coords)[a:a + 1], min_name, spherical=True)
             if minb.max_l >= 2 and not sub.spherical:
                 raise ValueError("minao with a cartesian basis needs an s/p minimal basis")
@@ -108,5 +129,7 @@ def minao_density(symbols, coords, basis: BasisSet, min_name: str = "sto-3g") ->
             n1 = sub.nao
             C = np.linalg.solve(S[:n1, :n1], S[:n1, n1:])
             cache[sym] = (C * np.asarray(f)[None, :]) @ C.T
+            with _BLOCKS_LOCK:
+                _BLOCKS[key] = cache[sym]
         P[slices[a], slices[a]] = cache[sym]
     return P
