@@ -198,8 +198,8 @@ void pool_setup() {
 }
 
 // Buffers of 8 MiB and more are kept by the library after they are freed. A
-// fresh device allocation of a few GB holds the calling thread for 0.1-1 s
-// (4-6 GB/s), and the device pool does not hand a freed block to a request of
+// fresh device allocation of a few GB can hold the calling thread for
+// 0.1-1 s, and the device pool does not hand a freed block to a request of
 // another size, so a molecule that follows one of a different size would pay
 // that again for its (mn|P), B and grid buffers. Sizes are rounded up to eight
 // classes per power of two (at most 12.5 % unused); a freed block keeps an
