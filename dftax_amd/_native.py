@@ -44,6 +44,7 @@ EXPORTS = [
     "dfks_energy_fock", "dfks_energy_fock_factored", "dfks_xc_eval", "dfks_scf_begin", "dfks_scf_step",
     "dfks_scf_result", "dfks_becke_weights", "dfks_overlap", "dfks_set_timing", "dfks_stage_times",
     "dfks_set_precision", "dfks_stats", "dfks_xc_eval2", "dfks_response_begin", "dfks_response_fock",
+    "dfks_device_free",
 ]
 
 STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
@@ -90,6 +91,7 @@ def lib():
     L.dfks_set_precision.argtypes = [c_void_p, c_int32]
     L.dfks_stage_times.argtypes = [c_void_p, P_f64, c_int32]
     L.dfks_stats.argtypes = [c_void_p, ctypes.POINTER(c_int64), c_int32]
+    L.dfks_device_free.argtypes = [ctypes.POINTER(c_int64)]
     _lib = L
     return L
 
@@ -141,6 +143,17 @@ def becke_weights(pts: np.ndarray, raw: np.ndarray, owner: np.ndarray, atom_xyz:
     check(lib().dfks_becke_weights(raw.shape[0], f64p(pts), f64p(raw), i32p(owner), xyz.shape[0], f64p(xyz),
                                    f64p(br), f64p(out), current_stream()))
     return out
+
+
+def device_free(device) -> int:
+    """Bytes the library can obtain on ``device``: free device memory, plus the
+    default pool's unused reserve and the large buffers the library keeps for
+    reuse (both of which the device counts as in use)."""
+    import torch
+    out = c_int64()
+    with torch.cuda.device(device):
+        check(lib().dfks_device_free(ctypes.byref(out)))
+    return int(out.value)
 
 
 def overlap(basis) -> np.ndarray:

@@ -2283,6 +2283,32 @@ int dfks_overlap(const dfks_basis_desc* basis, double* host_out, dfks_stream str
   });
 }
 
+int dfks_device_free(int64_t* out) {
+  return guard([&] {
+    if (!out) fail(1, "null argument");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    // memory the default pool reserves but has not handed out
+    uint64_t held = 0;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t res = 0, used = 0;
+      if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &res) == hipSuccess &&
+          hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess && res > used)
+        held = res - used;
+    }
+    size_t kept = 0;
+    {
+      std::lock_guard<std::mutex> lk(g_cache_mu);
+      for (const auto& f : g_big_free)
+        if (f.second.dev == dev) kept += f.second.bytes;
+    }
+    *out = (int64_t)(fr + held + kept);
+  });
+}
+
 int dfks_becke_weights(int64_t n, const double* pts, const double* raw_w, const int32_t* owner, int32_t natom,
                        const double* atom_xyz, const double* bragg, double* w_out, dfks_stream stream) {
   return guard([&] {

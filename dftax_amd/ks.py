@@ -194,7 +194,7 @@ class HipDFKS:
         desc = _native.SystemDesc(self._arrays[0].desc, self._arrays[1].desc, len(self._z), _native.f64p(self._xyz),
                                   _native.f64p(self._z), self._gw.shape[0], _native.f64p(self._gxyz),
                                   _native.f64p(self._gw), xc.xc_id, int(nelec))
-        free, _ = torch.cuda.mem_get_info(self.device)
+        free = _native.device_free(self.device)
         self.plan = memory_plan(basis.nao, aux.nao, self._gw.shape[0], nelec // 2, spec.precision, chunk,
                                 spec.ao_block, int(0.85 * free))
         opts = _native.Options(1 if spec.precision == "mixed" else 0, float(spec.metric_cutoff),

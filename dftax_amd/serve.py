@@ -33,6 +33,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from dftax_amd.basis import build_basis
+from dftax_amd import _native
 from dftax_amd.grid import Becke, atom_centred_points, becke
 
 
@@ -85,9 +86,7 @@ class Server:
         self.refine = refine if precision == "mixed" else None
         self.device_index = device
         if budget_bytes is None:
-            import torch
-            free, _ = torch.cuda.mem_get_info(torch.device("cuda", device))
-            budget_bytes = int(0.85 * free)
+            budget_bytes = int(0.85 * _native.device_free(device))
         self.budget = budget_bytes
         self._free = budget_bytes
         self._cv = threading.Condition()

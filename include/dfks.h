@@ -193,6 +193,14 @@ int dfks_set_precision(dfks_handle h, int32_t mixed);
 int dfks_set_timing(dfks_handle h, int32_t on);
 int dfks_stage_times(dfks_handle h, double* ms_out, int32_t n);
 
+/* Device memory this library can obtain on the current device, in bytes: what
+ * hipMemGetInfo reports free, plus what the default memory pool reserves but has
+ * not handed out, plus the buffers of 8 MiB and more that the library keeps after
+ * they are freed for reuse by later handles. The memory plan sizes the DF chunks
+ * from it, where the reference reads the accelerator's memory statistics
+ * (dftax/ks/energy.py:197-212). */
+int dfks_device_free(int64_t* out);
+
 /* Overlap matrix of one basis (host out, nfunc x nfunc); used by the minao guess.
  * Runs on the stream and synchronises it (only it). */
 int dfks_overlap(const dfks_basis_desc* basis, double* host_out, dfks_stream stream);
