@@ -44,7 +44,7 @@ EXPORTS = [
     "dfks_energy_fock", "dfks_energy_fock_factored", "dfks_xc_eval", "dfks_scf_begin", "dfks_scf_step",
     "dfks_scf_result", "dfks_becke_weights", "dfks_overlap", "dfks_set_timing", "dfks_stage_times",
     "dfks_set_precision", "dfks_stats", "dfks_xc_eval2", "dfks_response_begin", "dfks_response_fock",
-    "dfks_device_free",
+    "dfks_device_free", "dfks_release_memory",
 ]
 
 STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
@@ -92,6 +92,7 @@ def lib():
     L.dfks_stage_times.argtypes = [c_void_p, P_f64, c_int32]
     L.dfks_stats.argtypes = [c_void_p, ctypes.POINTER(c_int64), c_int32]
     L.dfks_device_free.argtypes = [ctypes.POINTER(c_int64)]
+    L.dfks_release_memory.argtypes = [c_void_p]
     _lib = L
     return L
 
@@ -154,6 +155,14 @@ def device_free(device) -> int:
     with torch.cuda.device(device):
         check(lib().dfks_device_free(ctypes.byref(out)))
     return int(out.value)
+
+
+def release_memory(device) -> None:
+    """Give the large buffers the library keeps for reuse, and the pool's unused
+    reserve, back to ``device`` (waits for their last users)."""
+    import torch
+    with torch.cuda.device(device):
+        check(lib().dfks_release_memory(current_stream()))
 
 
 def overlap(basis) -> np.ndarray:

@@ -2283,6 +2283,22 @@ int dfks_overlap(const dfks_basis_desc* basis, double* host_out, dfks_stream str
   });
 }
 
+int dfks_release_memory(dfks_stream stream) {
+  return guard([&] {
+    hipStream_t st = S_(stream);
+    StreamScope ss(st);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    {
+      std::lock_guard<std::mutex> lk(g_cache_mu);
+      cache_release(dev);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) HIPCHK(hipMemPoolTrimTo(pool, 0));
+  });
+}
+
 int dfks_device_free(int64_t* out) {
   return guard([&] {
     if (!out) fail(1, "null argument");

@@ -201,6 +201,13 @@ int dfks_stage_times(dfks_handle h, double* ms_out, int32_t n);
  * (dftax/ks/energy.py:197-212). */
 int dfks_device_free(int64_t* out);
 
+/* Returns the buffers kept for reuse on the current device (after their last
+ * users have finished) and the default pool's unused reserve to the device, for
+ * callers that need the memory outside this library (the device then reports it
+ * free). Frees on the stream and synchronises it (only it). No counterpart in
+ * the reference, whose arrays are owned by the JAX runtime. */
+int dfks_release_memory(dfks_stream stream);
+
 /* Overlap matrix of one basis (host out, nfunc x nfunc); used by the minao guess.
  * Runs on the stream and synchronises it (only it). */
 int dfks_overlap(const dfks_basis_desc* basis, double* host_out, dfks_stream stream);

@@ -22,7 +22,7 @@ def test_kept_buffers_are_reused_and_counted(water_dimer_tzvp):
     and energy. The memory the library reports it can obtain, which sizes the
     DF chunks and the serving budget, counts the kept buffers, so it returns to
     where it was once the handles are gone, although the device counts them as
-    in use."""
+    in use. After dfks_release_memory the device reports that memory free."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
@@ -55,3 +55,8 @@ def test_kept_buffers_are_reused_and_counted(water_dimer_tzvp):
     assert dF < 1e-12 and dE < 1e-10
     assert after >= before - SLACK
     assert after >= dev_after
+    # released: the device itself reports the memory free again
+    _native.release_memory(0)
+    dev_released, _ = torch.cuda.mem_get_info(0)
+    _say(f"after dfks_release_memory: device free {dev_released / 1e9:.3f} GB")
+    assert dev_released >= after - SLACK
