@@ -253,7 +253,7 @@ def main() -> int:
             "abs_dE_mixed_vs_fp64": abs(rmx.e_tot - r64.e_tot),
             "time_to_solution_speedup": round(t64 / tmx, 3),
             # converge in mixed precision, then FP64 Fock builds until the energy settles (at most `refine`)
-            "scf_mixed_refine": {"e_tot": rrf.e_tot, "n_iter": rrf.n_iter, "energy_converged": rrf.converged,
+            "scf_mixed_refine": {"e_tot": rrf.e_tot, "n_iter": rrf.n_iter, "energy_converged": rrf.energy_converged,
                                   "refine": args.refine, "mixed_d_tol": args.mixed_d_tol,
                                   "wall_s": round(trf, 3),
                                   "n_mixed_steps": sum(1 for x in rrf.history if x[3] == "mixed"),

@@ -28,7 +28,7 @@ class BasisDesc(ctypes.Structure):
 class SystemDesc(ctypes.Structure):
     _fields_ = [("orbital", BasisDesc), ("aux", BasisDesc), ("natom", c_int32), ("atom_xyz", P_f64),
                 ("atom_z", P_f64), ("ngrid", c_int64), ("grid_xyz", P_f64), ("grid_w", P_f64),
-                ("xc_id", c_int32), ("nelec", c_int32)]
+                ("xc_id", c_int32), ("nelec", c_int32), ("nspin", c_int32), ("spin", c_int32)]
 
 
 class Options(ctypes.Structure):
@@ -75,7 +75,7 @@ def lib():
     L.dfks_build.argtypes = [c_void_p, c_void_p]
     L.dfks_sizes.argtypes = [c_void_p, ctypes.POINTER(c_int64)]
     L.dfks_get_matrix.argtypes = [c_void_p, c_int32, c_void_p, c_void_p]
-    L.dfks_energy_fock.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.dfks_energy_fock.argtypes = [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]
     L.dfks_energy_fock_factored.argtypes = [c_void_p, c_void_p, c_void_p, P_f64, c_int32, c_void_p, c_void_p,
                                             c_void_p]
     L.dfks_xc_eval.argtypes = [c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]

@@ -204,19 +204,21 @@ void pool_setup() {
 // that again for its (mn|P), B and grid buffers. Sizes are rounded up to eight
 // classes per power of two (at most 12.5 % unused); a freed block keeps an
 // event recorded on the freeing stream, and the stream that takes it next waits
-// on that event before its first use. At most kCacheKeep bytes are kept; an
-// allocation that fails returns every kept block to the pool and tries again.
+// on that event before its first use. At most a third of the device's memory
+// (and never more than 96 GB) is kept; any allocation that fails, large or
+// small, returns every kept block of its device to the pool and tries again.
 struct BigBlock {
   size_t bytes;
   int dev;
   hipEvent_t ev;
 };
 constexpr size_t kCacheMin = size_t(8) << 20;
-constexpr size_t kCacheKeep = size_t(96) << 30;
+constexpr size_t kCacheKeepMax = size_t(96) << 30;
 std::mutex g_cache_mu;
 std::map<void*, BigBlock> g_big_live;                // handed out
 std::vector<std::pair<void*, BigBlock>> g_big_free;  // kept for reuse
-size_t g_big_free_bytes = 0;
+std::map<int, size_t> g_big_free_bytes;              // kept bytes per device
+std::map<int, size_t> g_cache_keep;                  // keep limit per device
 
 size_t size_class(size_t b) {
   const int k = 63 - __builtin_clzll(b);
@@ -224,18 +226,48 @@ size_t size_class(size_t b) {
   return (b + step - 1) / step * step;
 }
 
-// Returns every kept block of device dev to the pool (caller holds g_cache_mu).
+// Keep limit of device dev (caller holds g_cache_mu; dev is current).
+size_t cache_keep(int dev) {
+  auto it = g_cache_keep.find(dev);
+  if (it != g_cache_keep.end()) return it->second;
+  size_t fr = 0, tot = 0;
+  size_t keep = kCacheKeepMax;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess && tot > 0) keep = std::min(keep, tot / 3);
+  g_cache_keep[dev] = keep;
+  return keep;
+}
+
+// Returns every kept block of device dev to the pool. The blocks leave the
+// list under the lock; the waits for their last users and the frees happen
+// after it is dropped, so other threads' allocations do not queue behind a
+// long kernel on another stream.
 void cache_release(int dev) {
-  for (size_t i = 0; i < g_big_free.size();) {
-    auto& f = g_big_free[i];
-    if (f.second.dev != dev) { ++i; continue; }
+  std::vector<std::pair<void*, BigBlock>> out;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (size_t i = 0; i < g_big_free.size();) {
+      if (g_big_free[i].second.dev != dev) { ++i; continue; }
+      out.push_back(g_big_free[i]);
+      g_big_free_bytes[dev] -= g_big_free[i].second.bytes;
+      g_big_free[i] = g_big_free.back();
+      g_big_free.pop_back();
+    }
+  }
+  for (auto& f : out) {
     (void)hipEventSynchronize(f.second.ev);
     (void)hipEventDestroy(f.second.ev);
     (void)hipFreeAsync(f.first, g_st);
-    g_big_free_bytes -= f.second.bytes;
-    f = g_big_free.back();
-    g_big_free.pop_back();
   }
+}
+
+// After a failed allocation on device dev: return the kept blocks and the
+// pool's unused reserve, then let the caller try once more.
+void release_for_retry(int dev) {
+  (void)hipGetLastError();
+  cache_release(dev);
+  (void)hipStreamSynchronize(g_st);
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
 }
 
 void* big_alloc(size_t bytes) {
@@ -249,7 +281,7 @@ void* big_alloc(size_t bytes) {
       auto f = g_big_free[i];
       g_big_free[i] = g_big_free.back();
       g_big_free.pop_back();
-      g_big_free_bytes -= cb;
+      g_big_free_bytes[dev] -= cb;
       g_big_live[f.first] = f.second;
       HIPCHK(hipStreamWaitEvent(g_st, f.second.ev, 0));
       return f.first;
@@ -259,14 +291,7 @@ void* big_alloc(size_t bytes) {
   void* p = nullptr;
   hipError_t e = hipMallocAsync(&p, cb, g_st);
   if (e != hipSuccess) {
-    (void)hipGetLastError();
-    {
-      std::lock_guard<std::mutex> lk(g_cache_mu);
-      cache_release(dev);
-    }
-    (void)hipStreamSynchronize(g_st);
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
+    release_for_retry(dev);
     e = hipMallocAsync(&p, cb, g_st);
   }
   if (e != hipSuccess) fail(4, "hipMallocAsync of " + std::to_string(cb) + " bytes: " + hipGetErrorString(e));
@@ -288,9 +313,10 @@ bool big_free(void* p) {
   if (it == g_big_live.end()) return false;
   const BigBlock b = it->second;
   g_big_live.erase(it);
-  if (g_big_free_bytes + b.bytes <= kCacheKeep && hipEventRecord(b.ev, g_st) == hipSuccess) {
+  size_t& kept = g_big_free_bytes[b.dev];
+  if (kept + b.bytes <= cache_keep(b.dev) && hipEventRecord(b.ev, g_st) == hipSuccess) {
     g_big_free.emplace_back(p, b);
-    g_big_free_bytes += b.bytes;
+    kept += b.bytes;
   } else {
     (void)hipEventDestroy(b.ev);
     (void)hipFreeAsync(p, g_st);
@@ -305,6 +331,12 @@ T* dalloc(size_t n) {
   if (n * sizeof(T) >= kCacheMin) return static_cast<T*>(big_alloc(n * sizeof(T)));
   void* p = nullptr;
   hipError_t e = hipMallocAsync(&p, n * sizeof(T), g_st);
+  if (e != hipSuccess) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    release_for_retry(dev);
+    e = hipMallocAsync(&p, n * sizeof(T), g_st);
+  }
   if (e != hipSuccess)
     fail(4, "hipMallocAsync of " + std::to_string(n * sizeof(T)) + " bytes: " + hipGetErrorString(e));
   return static_cast<T*>(p);
@@ -422,7 +454,7 @@ double hf_coeff(int xc) {
 }  // namespace
 
 struct dfks_ctx {
-  int nao = 0, naux = 0, nfit = 0, nocc = 0, natom = 0, xc_id = 0;
+  int nao = 0, naux = 0, nfit = 0, nocc = 0, natom = 0, xc_id = 0, nspin = 1;
   int64_t ng = 0;
   double hf = 0.0, e_nn = 0.0;
   dfks_options opt{};
@@ -1974,6 +2006,12 @@ const char* dfks_last_error(void) { return g_err.c_str(); }
 int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handle* out) {
   return guard([&] {
     if (!sys || !out) fail(1, "null argument");
+    // checked before any device work, so a host without a GPU gets the same answer
+    if (sys->nspin == 2)
+      fail(7, "spin-unrestricted densities (nspin = 2) are not implemented; this build solves the restricted "
+              "closed-shell case (nspin = 1)");
+    if (sys->nspin != 1) fail(1, "nspin must be 1 (restricted) or 2 (unrestricted)");
+    if (sys->spin != 0) fail(1, "nspin = 1 needs spin = 0 (n_alpha = n_beta)");
     std::lock_guard<std::mutex> lk(g_mu);
     static bool tables = false;
     if (!tables) {
@@ -2126,6 +2164,9 @@ int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream s
   return guard([&] {
     if (!h || !dev_out) fail(1, "null argument");
     if (!h->built) fail(6, "dfks_build has not been called");
+    // the handle's buffers are last used on this stream: dfks_destroy orders
+    // their release after it
+    StreamScope ss(h->st = S_(stream));
     const size_t nn = (size_t)h->nao * h->nao;
     const double* src = nullptr;
     size_t n = 0;
@@ -2154,9 +2195,12 @@ int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream s
   });
 }
 
-int dfks_energy_fock(dfks_handle h, const double* P, double* E, double* F, dfks_stream stream) {
+int dfks_energy_fock(dfks_handle h, const double* P, int32_t nspin, double* E, double* F, dfks_stream stream) {
   return guard([&] {
     if (!h || !P || !E || !F) fail(1, "null argument");
+    if (nspin != h->nspin)
+      fail(1, "P has " + std::to_string(nspin) + " spin blocks but the handle was created with nspin = " +
+                  std::to_string(h->nspin));
     StreamScope ss(h->st = S_(stream));
     energy_fock_general(h, P, E, F, S_(stream));
   });
@@ -2289,10 +2333,7 @@ int dfks_release_memory(dfks_stream stream) {
     StreamScope ss(st);
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
-    {
-      std::lock_guard<std::mutex> lk(g_cache_mu);
-      cache_release(dev);
-    }
+    cache_release(dev);
     HIPCHK(hipStreamSynchronize(st));
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) HIPCHK(hipMemPoolTrimTo(pool, 0));

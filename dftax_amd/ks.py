@@ -193,7 +193,7 @@ class HipDFKS:
         self._gw = np.ascontiguousarray(grid_w, np.float64)
         desc = _native.SystemDesc(self._arrays[0].desc, self._arrays[1].desc, len(self._z), _native.f64p(self._xyz),
                                   _native.f64p(self._z), self._gw.shape[0], _native.f64p(self._gxyz),
-                                  _native.f64p(self._gw), xc.xc_id, int(nelec))
+                                  _native.f64p(self._gw), xc.xc_id, int(nelec), 1, 0)
         free = _native.device_free(self.device)
         self.plan = memory_plan(basis.nao, aux.nao, self._gw.shape[0], nelec // 2, spec.precision, chunk,
                                 spec.ao_block, int(0.85 * free))
@@ -246,7 +246,7 @@ class HipDFKS:
         torch = _torch()
         P = P.to(device=self.device, dtype=torch.float64).contiguous()
         F = torch.empty_like(P)
-        _native.check(self.L.dfks_energy_fock(self.handle, ctypes.c_void_p(P.data_ptr()),
+        _native.check(self.L.dfks_energy_fock(self.handle, ctypes.c_void_p(P.data_ptr()), 1,
                                               ctypes.c_void_p(self._E.data_ptr()), ctypes.c_void_p(F.data_ptr()),
                                               self._stream()))
         return self._E.clone(), F

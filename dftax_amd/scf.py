@@ -33,6 +33,10 @@ class KSResult:
     mo_coeff: object
     P: object
     ts: float = 0.0
+    # the energy test alone (|dE| < e_tol) held at the last step; with
+    # scf(refine=n) the solve stops on it while ``converged`` keeps the
+    # reference's meaning (|dE| < e_tol and |err| < d_tol, scf.py:372)
+    energy_converged: bool = False
     iter_times: list = field(default_factory=list)
     history: list = field(default_factory=list)
 
@@ -108,10 +112,11 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
 
     ``refine=n`` (mixed only, n >= 2) converges the energy only: the FP64
     steps stop at the first one (from the second on) whose energy change is
-    below ``e_tol`` -- ``converged`` -- or after n of them. The energy is
-    second order in the density error, so it lands within ~1e-9 Ha of the
+    below ``e_tol`` -- ``energy_converged`` -- or after n of them. The energy
+    is second order in the density error, so it lands within ~1e-9 Ha of the
     fully converged FP64 energy while the orbital gradient is left at the
-    size the split-BF16 Fock error imposes (~1e-4); use the default for
+    size the split-BF16 Fock error imposes (~1e-4), so ``converged`` (which
+    also needs ``|err| < d_tol``) is normally False; use the default for
     densities and orbitals converged to ``d_tol``.
     """
     torch = __import__("torch")
@@ -130,11 +135,12 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
     _native.check(L.dfks_scf_begin(h, ctypes.c_void_p(P0.data_ptr()), float(lindep_thresh), int(diis_space),
                                    float(level_shift), stream))
     out = (ctypes.c_double * 3)()
-    converged = False
+    converged = e_conv = False
+    stop = False
     it = 0
     e = float("nan")
     times, hist = [], []
-    while it < max_iter and not converged:
+    while it < max_iter and not stop:
         t0 = time.perf_counter()
         _native.check(L.dfks_scf_step(h, out, stream))
         times.append(time.perf_counter() - t0)
@@ -148,14 +154,13 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
                 mixed = False
         else:
             n_fp64 = sum(1 for h in hist if h[3] == "fp64")
+            converged = abs(de) < e_tol and derr < d_tol
             if refine is not None:
-                if n_fp64 >= 2 and abs(de) < e_tol:
-                    converged = True
-                if converged or n_fp64 >= int(refine):
-                    it += 1
-                    break
+                e_conv = n_fp64 >= 2 and abs(de) < e_tol
+                stop = converged or e_conv or n_fp64 >= int(refine)
             else:
-                converged = abs(de) < e_tol and derr < d_tol
+                e_conv = abs(de) < e_tol
+                stop = converged
         it += 1
     n = ks.nao
     P = torch.empty((n, n), dtype=torch.float64, device=ks.device)
@@ -167,8 +172,12 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
                                     ctypes.c_void_p(eps.data_ptr()), ctypes.byref(nmo), stream))
     torch.cuda.current_stream(ks.device).synchronize()
     res = KSResult(e_tot=float(e), e_elec=float(e) - ks.e_nn, converged=converged, n_iter=it, nocc=ks.nocc,
-                   mo_energy=eps[None], mo_coeff=C.T[None], P=P[None], iter_times=times, history=hist)
-    if not converged:
+                   mo_energy=eps[None], mo_coeff=C.T[None], P=P[None], energy_converged=e_conv or converged,
+                   iter_times=times, history=hist)
+    if refine is not None and not res.energy_converged:
+        warnings.warn(f"SCF energy did NOT converge in {it} iterations ({refine} FP64 refinement steps, "
+                      f"e_tol={e_tol}); the returned energy is unreliable.", stacklevel=2)
+    elif refine is None and not converged:
         warnings.warn(f"SCF did NOT converge in {it} iterations (e_tol={e_tol}, d_tol={d_tol}); the returned "
                       f"energy is unreliable. Increase max_iter or try level_shift>0.", stacklevel=2)
     return res

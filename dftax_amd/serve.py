@@ -48,6 +48,7 @@ class Request:
     e_tot: float = float("nan")
     n_iter: int = 0
     converged: bool = False
+    energy_converged: bool = False
     error: str = ""
     extra: dict = field(default_factory=dict)
 
@@ -123,6 +124,7 @@ class Server:
                 t1 = time.perf_counter()
                 r = scf(ks, e_tol=self.e_tol, d_tol=self.d_tol, precision=self.precision, refine=self.refine)
                 req.e_tot, req.n_iter, req.converged = r.e_tot, r.n_iter, r.converged
+                req.energy_converged = r.energy_converged
                 # the library's build phases (host timers, always kept; read with timing on for the call)
                 hnd = ks.coulomb.handle
                 _native.check(ks.coulomb.L.dfks_set_timing(hnd, 1))
@@ -201,6 +203,7 @@ def latency_stats(reqs: list[Request], alone: dict | None = None) -> dict:
     wait = np.array([r.start - r.arrival for r in ok])
     out = {
         "n": len(reqs), "ok": len(ok), "converged": sum(r.converged for r in ok),
+        "energy_converged": sum(r.energy_converged for r in ok),
         "mean_build_s": float(np.nanmean(build)) if ok else None,
         "mean_scf_s": float(np.nanmean(scf_t)) if ok else None,
         "mean_queue_wait_s": float(wait.mean()) if ok else None,

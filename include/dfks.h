@@ -61,7 +61,11 @@ typedef struct {
   const double* grid_xyz;  /* [3*ngrid] Bohr */
   const double* grid_w;    /* [ngrid] quadrature weights */
   int32_t xc_id;           /* 1 PBE, 2 PBE0, 3 B3LYP */
-  int32_t nelec;           /* closed shell: nocc = nelec / 2 */
+  int32_t nelec;           /* electrons; nspin = 1: nocc = nelec / 2 */
+  int32_t nspin;           /* spin blocks of P and F (the leading axis of dftax's P[nspin, nao, nao],
+                              dftax/ks/terms.py:520-527): 1 restricted closed shell; 2 spin-unrestricted,
+                              not implemented -- dfks_create returns status 7 */
+  int32_t spin;            /* 2S = n_alpha - n_beta (dftax Molecule.spin); must be 0 when nspin = 1 */
 } dfks_system_desc;
 
 typedef struct {
@@ -132,8 +136,10 @@ int dfks_stats(dfks_handle h, int64_t* out, int32_t n);
 int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream stream);
 
 /* E(P) and F = sym(dE/dP) = hcore + J - a_x/2 K + Vxc for a symmetric density P
- * (any P: K uses a signed eigen-factorisation of P). E: device double[8]. */
-int dfks_energy_fock(dfks_handle h, const double* P, double* E, double* F, dfks_stream stream);
+ * (any P: K uses a signed eigen-factorisation of P). P and F hold nspin
+ * consecutive nao x nao blocks (dftax's P[nspin, nao, nao]); nspin must equal
+ * the handle's (status 1 otherwise). E: device double[8]. */
+int dfks_energy_fock(dfks_handle h, const double* P, int32_t nspin, double* E, double* F, dfks_stream stream);
 
 /* Same for P = sum_i s_i c_i c_i^T given its factor: Cf column-major
  * [nf][nao] (column i contiguous), signs s_i in {+1,-1} (host array). */

@@ -47,3 +47,42 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_native, "LIB_PATH", tmp_path / "libdfks.so")
     with pytest.raises(_native.DfksError):
         _native.lib()
+
+
+def _water_desc(nspin, spin):
+    """A system descriptor for water/def2-SVP (host arrays only)."""
+    import numpy as np
+    from dftax_amd import _native
+    from dftax_amd.basis import build_basis
+    from dftax_amd.system import WATER, Molecule
+    mol = Molecule.from_xyz(WATER, "def2-svp", spherical=True)
+    orb = _native.BasisArrays(build_basis(mol.symbols, mol.coords, "def2-svp", spherical=True))
+    aux = _native.BasisArrays(build_basis(mol.symbols, mol.coords, "even-tempered-jk", spherical=True))
+    xyz = np.ascontiguousarray(mol.coords, np.float64)
+    z = np.ascontiguousarray(mol.atom_charges(), np.float64)
+    g = np.zeros((1, 3))
+    w = np.ones(1)
+    keep = (orb, aux, xyz, z, g, w)
+    desc = _native.SystemDesc(orb.desc, aux.desc, len(z), _native.f64p(xyz), _native.f64p(z), 1,
+                              _native.f64p(g), _native.f64p(w), 2, 10, nspin, spin)
+    return desc, keep
+
+
+def test_nspin_in_the_descriptor():
+    """P[nspin, nao, nao] (dftax/ks/terms.py:520-527): the descriptor carries
+    nspin; nspin = 2 is refused with its own status before any device work,
+    and nspin = 1 needs spin = 0."""
+    from dftax_amd import _native
+    L = _native.lib()
+    h = ctypes.c_void_p()
+    desc, _keep = _water_desc(2, 0)
+    assert L.dfks_create(ctypes.byref(desc), None, ctypes.byref(h)) == 7
+    assert b"nspin = 2" in L.dfks_last_error()
+    desc, _keep = _water_desc(1, 2)
+    assert L.dfks_create(ctypes.byref(desc), None, ctypes.byref(h)) == 1
+    assert b"spin = 0" in L.dfks_last_error()
+    desc, _keep = _water_desc(3, 0)
+    assert L.dfks_create(ctypes.byref(desc), None, ctypes.byref(h)) == 1
+    assert not h.value
+    # dfks_energy_fock takes the spin-block count of P explicitly
+    assert L.dfks_energy_fock(None, None, 1, None, None, None) == 1

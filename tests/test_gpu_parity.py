@@ -385,7 +385,7 @@ def test_recompute_mixed(dimer_recompute, torch_cuda):
     rr = scf(r, precision="mixed", refine=3)
     nfp = sum(1 for h in rr.history if h[3] == "fp64")
     _say(f"mixed + {nfp} FP64 steps {rr.e_tot:.12f} ({rr.n_iter} it), |dE| {abs(rr.e_tot - rm.e_tot):.1e}")
-    assert rr.converged and 2 <= nfp <= 3 and abs(rr.e_tot - rm.e_tot) < 1e-6
+    assert rr.energy_converged and 2 <= nfp <= 3 and abs(rr.e_tot - rm.e_tot) < 1e-6
 
 
 # ---------------------------------------------------------------------------

@@ -83,6 +83,6 @@ def test_same_energy_alone_and_in_the_mix():
     reqs = [Request(rid=i, mol=m, arrival=0.0) for i, m in enumerate(mols)]
     srv.run(reqs)
     for a, r in zip(alone, reqs):
-        assert not r.error and r.converged
+        assert not r.error and r.energy_converged
         print(f"alone {a:.10f} mix {r.e_tot:.10f}", flush=True)
         assert abs(a - r.e_tot) < 1e-8
