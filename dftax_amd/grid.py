@@ -86,46 +86,47 @@ def becke_radial(n: int, scale: float):
     return r, w
 
 
+def _region_orders(n_ang: int) -> tuple:
+    """Lebedev orders of the five NWChem regions of r / R_bragg for a
+    bonding-region order n_ang: 50 and 86 inside, the order one rung below
+    n_ang on both sides of the bonding region (50 / 74 / 50 for n_ang = 50)."""
+    if n_ang == 50:
+        return (50, 74, 74, 74, 50)
+    if n_ang not in LEBEDEV_LADDER[4:]:
+        raise ValueError(f"lebedev={n_ang} is not a Lebedev order")
+    below = LEBEDEV_LADDER[LEBEDEV_LADDER.index(n_ang) - 1]
+    return (50, 86, below, n_ang, below)
+
+
 def nwchem_orders(Z: int, rads: np.ndarray, n_ang: int) -> np.ndarray:
-    """Angular order per radial shell by the NWChem region rule."""
+    """Angular order per radial shell by the NWChem region rule (grid.py:116-144):
+    the region of a shell is the number of bounds that r / R_bragg exceeds."""
     if n_ang < 50:
         return np.full(rads.shape[0], n_ang, dtype=np.int64)
-    ladder = np.asarray(LEBEDEV_LADDER[3:])
-    if n_ang == 50:
-        leb_l = np.array([1, 2, 2, 2, 1])
-    else:
-        hit = np.where(ladder == n_ang)[0]
-        if hit.size == 0:
-            raise ValueError(f"lebedev={n_ang} is not a Lebedev order")
-        idx = int(hit[0])
-        leb_l = np.array([1, 3, idx - 1, idx, idx - 1])
-    row = 0 if Z <= 2 else (1 if Z <= 10 else 2)
-    place = (rads[:, None] / bragg_radius(Z) > _NWCHEM_ALPHAS[row]).sum(axis=1)
-    return ladder[leb_l[place]]
+    bounds = _NWCHEM_ALPHAS[0 if Z <= 2 else (1 if Z <= 10 else 2)]
+    region = np.searchsorted(bounds, rads / bragg_radius(Z), side="left")
+    return np.asarray(_region_orders(n_ang), dtype=np.int64)[region]
 
 
 @lru_cache(maxsize=None)
 def atom_shell_blocks(Z: int, n_radial: int, n_ang: int, prune, r_max):
+    """Per-element radial shells as runs of equal angular order:
+    ((radii, radial weights, order), ...) in increasing radius."""
     scale = bragg_radius(Z) if Z == 1 else 0.5 * bragg_radius(Z)
     r, wr = becke_radial(n_radial, scale)
     if r_max is not None:
-        keep = r <= r_max
-        r, wr = r[keep], wr[keep]
-    if prune is None:
-        orders = np.full(r.shape[0], n_ang, dtype=np.int64)
-    elif prune == "nwchem":
+        inside = r <= r_max
+        r, wr = r[inside], wr[inside]
+    if prune == "nwchem":
         orders = nwchem_orders(Z, r, n_ang)
+    elif prune is None:
+        orders = np.full(r.shape[0], n_ang, dtype=np.int64)
     else:
         raise ValueError(f"unknown prune scheme {prune!r}")
-    blocks = []
-    i = 0
-    while i < r.shape[0]:
-        j = i
-        while j < r.shape[0] and orders[j] == orders[i]:
-            j += 1
-        blocks.append((r[i:j].copy(), wr[i:j].copy(), int(orders[i])))
-        i = j
-    return tuple(blocks)
+    cuts = np.flatnonzero(np.diff(orders)) + 1
+    return tuple((rr.copy(), ww.copy(), int(oo[0]))
+                 for rr, ww, oo in zip(np.split(r, cuts), np.split(wr, cuts), np.split(orders, cuts))
+                 if rr.size)
 
 
 @lru_cache(maxsize=None)

@@ -483,3 +483,98 @@ void oracle_int1e(int ns, const int *l, const int *prim0, const int *nprim, cons
         (void)one;
     }
 }
+
+/* Four-centre (ab|cd) in the cartesian span, for orbital shells up to l = 2:
+ *   (ab|cd) = K_ab K_cd 2 pi^2.5 / (p q sqrt(p+q))
+ *             sum_{tuv} E^{ab}_{tuv} sum_{tau nu phi} (-1)^{tau+nu+phi} E^{cd}_{tau nu phi}
+ *             R_{t+tau, u+nu, v+phi}(pq/(p+q), P - Q)
+ * (the exact-ERI path of dftax/integrals/eri4c.py in McMurchie-Davidson form,
+ * with the same E and R tables as the 3-centre engine). Only the test pin of
+ * the exact path uses it: out[((a*n + b)*n + c)*n + d], all n^4 entries. */
+void oracle_int4c_cart(int ns, const int *l, const int *prim0, const int *nprim, const int *ao0,
+                       const double *centre, const double *exps, const double *coefs, int n, double *out) {
+    if (!boys_ready) boys_build();
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int ia = 0; ia < ns; ++ia)
+        for (int ib = 0; ib < ns; ++ib) {
+            double *R = (double *)malloc(sizeof(double) * MTMAX * MTMAX * MTMAX);
+            double *Eab = (double *)malloc(sizeof(double) * 3 * 3 * 3 * MTMAX);
+            double *Ecd = (double *)malloc(sizeof(double) * 3 * 3 * 3 * MTMAX);
+            for (int ic = 0; ic < ns; ++ic)
+                for (int id = 0; id < ns; ++id) {
+                    int la = l[ia], lb = l[ib], lc = l[ic], ld = l[id];
+                    int mt = la + lb + lc + ld + 1;
+                    int ca[6][3], cb[6][3], cc[6][3], cd[6][3];
+                    cart_comps(la, ca); cart_comps(lb, cb); cart_comps(lc, cc); cart_comps(ld, cd);
+                    const double *A = centre + 3 * ia, *B = centre + 3 * ib, *C = centre + 3 * ic,
+                                 *D = centre + 3 * id;
+                    double AB[3], CD[3], ab2 = 0, cd2 = 0;
+                    for (int x = 0; x < 3; ++x) {
+                        AB[x] = A[x] - B[x]; CD[x] = C[x] - D[x];
+                        ab2 += AB[x] * AB[x]; cd2 += CD[x] * CD[x];
+                    }
+                    int eab = (la + 1) * (lb + 1) * mt, ecd = (lc + 1) * (ld + 1) * mt;
+                    for (int pa = 0; pa < nprim[ia]; ++pa)
+                        for (int pb = 0; pb < nprim[ib]; ++pb) {
+                            double al = exps[prim0[ia] + pa], be = exps[prim0[ib] + pb];
+                            double p = al + be;
+                            double cab = coefs[prim0[ia] + pa] * coefs[prim0[ib] + pb] * exp(-al * be / p * ab2);
+                            double P[3];
+                            for (int x = 0; x < 3; ++x) {
+                                P[x] = (al * A[x] + be * B[x]) / p;
+                                e_table(la, lb, al, be, AB[x], mt, Eab + x * eab);
+                            }
+                            for (int pc = 0; pc < nprim[ic]; ++pc)
+                                for (int pd = 0; pd < nprim[id]; ++pd) {
+                                    double ga = exps[prim0[ic] + pc], de = exps[prim0[id] + pd];
+                                    double q = ga + de;
+                                    double ccd = coefs[prim0[ic] + pc] * coefs[prim0[id] + pd] *
+                                                 exp(-ga * de / q * cd2);
+                                    double Q[3], PQ[3];
+                                    for (int x = 0; x < 3; ++x) {
+                                        Q[x] = (ga * C[x] + de * D[x]) / q;
+                                        PQ[x] = P[x] - Q[x];
+                                        e_table(lc, ld, ga, de, CD[x], mt, Ecd + x * ecd);
+                                    }
+                                    r_table(p * q / (p + q), PQ, mt, R);
+                                    double pref = cab * ccd * 2.0 * pow(M_PI, 2.5) / (p * q * sqrt(p + q));
+#define EAB(x, i, j, t) Eab[(x) * eab + ((i) * (lb + 1) + (j)) * mt + (t)]
+#define ECD(x, i, j, t) Ecd[(x) * ecd + ((i) * (ld + 1) + (j)) * mt + (t)]
+                                    for (int a = 0; a < ncart(la); ++a)
+                                        for (int b = 0; b < ncart(lb); ++b)
+                                            for (int c = 0; c < ncart(lc); ++c)
+                                                for (int d = 0; d < ncart(ld); ++d) {
+                                                    double acc = 0.0;
+                                                    int ax = ca[a][0] , ay = ca[a][1], az = ca[a][2];
+                                                    int bx = cb[b][0], by = cb[b][1], bz = cb[b][2];
+                                                    int cx = cc[c][0], cy = cc[c][1], cz = cc[c][2];
+                                                    int dx = cd[d][0], dy = cd[d][1], dz = cd[d][2];
+                                                    for (int t = 0; t <= ax + bx; ++t)
+                                                        for (int u = 0; u <= ay + by; ++u)
+                                                            for (int v = 0; v <= az + bz; ++v) {
+                                                                double e1 = EAB(0, ax, bx, t) * EAB(1, ay, by, u) *
+                                                                            EAB(2, az, bz, v);
+                                                                if (e1 == 0.0) continue;
+                                                                for (int tt = 0; tt <= cx + dx; ++tt)
+                                                                    for (int uu = 0; uu <= cy + dy; ++uu)
+                                                                        for (int vv = 0; vv <= cz + dz; ++vv) {
+                                                                            double sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
+                                                                            acc += e1 * sg * ECD(0, cx, dx, tt) *
+                                                                                   ECD(1, cy, dy, uu) *
+                                                                                   ECD(2, cz, dz, vv) *
+                                                                                   R[((t + tt) * mt + (u + uu)) * mt +
+                                                                                     (v + vv)];
+                                                                        }
+                                                            }
+                                                    int m = ao0[ia] + a, nn = ao0[ib] + b, k = ao0[ic] + c,
+                                                        s = ao0[id] + d;
+                                                    out[(((size_t)m * n + nn) * n + k) * n + s] += pref * acc;
+                                                }
+#undef EAB
+#undef ECD
+                                }
+                        }
+                }
+            free(R); free(Eab); free(Ecd);
+        }
+}

@@ -120,6 +120,21 @@ def int3c(basis, aux):
     return out
 
 
+def int4c_cart(basis):
+    """Exact four-centre (ab|cd), cartesian span, orbital l <= 2 (n^4 FP64):
+    the reference's exact-ERI path (dftax/integrals/eri4c.py), used only for
+    the exact-Coulomb known-answer pin."""
+    if basis.spherical and int(np.max(basis.l)) >= 2:
+        raise ValueError("int4c_cart needs a cartesian span")
+    if int(np.max(basis.l)) > 2:
+        raise ValueError("int4c_cart covers l <= 2")
+    args, keep = _basis_args(basis)
+    n = basis.nao
+    out = np.zeros((n, n, n, n))
+    lib().oracle_int4c_cart(*args[:-2], ctypes.c_int(n), _p(out))
+    return out
+
+
 def nuclear_repulsion(xyz, z):
     e = 0.0
     for i in range(len(z)):
@@ -167,11 +182,79 @@ def becke_partition(points, coords, bragg, chunk=4096):
     return out
 
 
+# Bragg-Slater radii in Bohr (H..Ne), the constants of dftax/grid/becke.py:23-37.
+ORACLE_BRAGG_BOHR = {1: 0.66140414, 2: 2.64561657, 3: 2.74010288, 4: 1.98421243, 5: 1.60626721,
+                     6: 1.32280829, 7: 1.22832198, 8: 1.13383567, 9: 0.94486306, 10: 2.83458919}
+_Z = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Ne": 10}
+# Lebedev orders in increasing size and the NWChem region bounds on r / R_bragg
+# for the rows H-He, Li-Ne, Na and up (dftax/grid/grid.py:104-116).
+_LEB_ORDERS = (6, 14, 26, 38, 50, 74, 86, 110, 146, 170, 194, 230, 266, 302, 350, 434, 590, 770)
+_REGION_BOUNDS = ((0.25, 0.5, 1.0, 4.5), (0.1667, 0.5, 0.9, 3.5), (0.1, 0.4, 0.8, 2.5))
+
+
+def lebedev_table(n):
+    """Unit-sphere Lebedev points (n, 3) and weights summing to 1 (data file)."""
+    path = _HERE.parent / "dftax_amd" / "data" / "lebedev.npz"
+    with np.load(path) as z:
+        t = np.asarray(z[f"n{n}"], dtype=np.float64)
+    return t[:, :3], t[:, 3]
+
+
+def region_orders(n_ang):
+    """Angular orders of the five NWChem regions (grid.py:119-144): 50 in the
+    core, 86 next, then one Lebedev step below n_ang, n_ang in the bonding
+    region and one step below it in the tail; (50, 74, 74, 74, 50) for
+    n_ang = 50; no pruning below 50."""
+    if n_ang < 50:
+        return (n_ang,) * 5
+    if n_ang == 50:
+        return (50, 74, 74, 74, 50)
+    below = _LEB_ORDERS[_LEB_ORDERS.index(n_ang) - 1]
+    return (50, 86, below, n_ang, below)
+
+
+def oracle_grid_points(symbols, coords, n_radial=75, n_ang=302, prune="nwchem", r_max=45.0):
+    """Unpartitioned atom-centred grid written shell by shell (grid.py:206-255,
+    becke.py:50-65): per atom, second-kind Chebyshev nodes x_i = cos(i pi/(n+1))
+    mapped to r = R (1+x)/(1-x) (R = Bragg radius for H, half of it otherwise)
+    with radial weight pi/(n+1) sin^2-form r^2 dr/dx; shells beyond r_max
+    dropped; each shell carries the Lebedev grid of its region's order, points
+    A + r u_k with raw weight w_r 4 pi w_k, angular index fastest.
+    Returns (points, raw weights, owner atom)."""
+    pts, ws, own = [], [], []
+    coords = np.asarray(coords, dtype=np.float64).reshape(-1, 3)
+    for a, sym in enumerate(symbols):
+        Z = _Z[sym]
+        Rb = ORACLE_BRAGG_BOHR[Z]
+        R = Rb if Z == 1 else 0.5 * Rb
+        i = np.arange(1, n_radial + 1)
+        th = i * np.pi / (n_radial + 1)
+        x = np.cos(th)
+        r = R * (1.0 + x) / (1.0 - x)
+        wr = (np.pi / (n_radial + 1)) * np.sin(th) * r ** 2 * (2.0 * R / (1.0 - x) ** 2)
+        bounds = _REGION_BOUNDS[0 if Z <= 2 else (1 if Z <= 10 else 2)]
+        orders = region_orders(n_ang)
+        for k in range(n_radial):
+            if r_max is not None and not r[k] <= r_max:
+                continue
+            if prune is None:
+                order = n_ang
+            else:
+                region = sum(1 for b in bounds if r[k] / Rb > b)
+                order = orders[region]
+            u, wu = lebedev_table(order)
+            pts.append(coords[a][None, :] + r[k] * u)
+            ws.append(wr[k] * 4.0 * np.pi * wu)
+            own.append(np.full(order, a, dtype=np.int32))
+    return np.concatenate(pts), np.concatenate(ws), np.concatenate(own)
+
+
 def becke_grid(symbols, coords, spec):
-    """Full molecular grid (grid.py:206-255 + cutoff of energy.py:247-254)."""
-    from dftax_amd.grid import atom_centred_points, atom_bragg
-    pts, raw, owner = atom_centred_points(symbols, coords, spec)
-    bragg = atom_bragg(symbols)
+    """Full molecular grid (grid.py:206-255 + cutoff of energy.py:247-254),
+    with the oracle's own point layout."""
+    pts, raw, owner = oracle_grid_points(symbols, coords, spec.n_radial, spec.lebedev, spec.prune,
+                                         None if spec.r_max is None else float(spec.r_max))
+    bragg = np.array([ORACLE_BRAGG_BOHR[_Z[s]] for s in symbols])
     P = becke_partition(pts, coords, bragg)
     w = raw * P[np.arange(len(owner)), owner] / P.sum(axis=1)
     if spec.cutoff is not None:
@@ -231,15 +314,26 @@ def _torch():
     return torch
 
 
+# Functional constants (dftax/energy/xc.py), module-level so the pin tests can
+# perturb one at a time and show that a check notices.
+PBE_X = {"kappa": 0.804, "mu": 0.2195149727645171}                      # xc.py:94-98
+PBE_C = {"beta": 0.06672455060314922, "gamma": (1.0 - math.log(2.0)) / (math.pi ** 2),
+         "fz20": 1.709920934161365617563962776245}                        # xc.py:148-150
+B88_BETA = 0.0042                                                          # xc.py:255-294
+VWN_RPA_P = (0.0310907, 13.0720, 42.7198, -0.409286)                       # xc.py:388-391 (A, b, c, x0)
+VWN5_P = (0.0310907, 3.72744, 12.9352, -0.10498)                           # xc.py:330-333
+LYP = {"a": 0.04918, "b": 0.132, "c": 0.2533, "d": 0.349}                 # xc.py:427-519
+MIX = {"PBE0_x": 0.75, "B3LYP": (0.08, 0.72, 0.19, 0.81)}                  # xc.py:527-601
+
+
 def lda_x(rho):
-    t = _torch()
     kF = (3 * math.pi ** 2 * rho + 1e-30) ** (1 / 3)
     return -3 * kF / (4 * math.pi)
 
 
 def pbe_x(rho, grad):
     t = _torch()
-    kappa, mu = 0.804, 0.2195149727645171
+    kappa, mu = PBE_X["kappa"], PBE_X["mu"]
     gn = t.linalg.norm(grad + 1e-30, dim=-1)
     kF = (3 * math.pi ** 2 * rho + 1e-30) ** (1 / 3)
     s = gn / t.clamp(2 * kF * rho, min=1e-30)
@@ -255,11 +349,20 @@ _PW = np.array([
 ])
 
 
+def pw92_c(rho):
+    """PW92 correlation per electron of the unpolarised gas (xc.py:165-177, zeta = 0)."""
+    t = _torch()
+    r = t.clamp(rho, min=1e-20)
+    rs = (3 / (4 * math.pi * r) + 1e-20) ** (1 / 3)
+    A, a1, b1, b2, b3, b4 = _PW[0]
+    rq = t.sqrt(rs)
+    denom = 2.0 * A * (b1 * rq + b2 * rs + b3 * rs * rq + b4 * rs ** 2)
+    return -2.0 * A * (1.0 + a1 * rs) * t.log1p(1 / t.clamp(denom, min=1e-20))
+
+
 def pbe_c(rho, grad):
     t = _torch()
-    gamma = (1.0 - math.log(2.0)) / (math.pi ** 2)
-    beta = 0.06672455060314922
-    fz20 = 1.709920934161365617563962776245
+    gamma, beta, fz20 = PBE_C["gamma"], PBE_C["beta"], PBE_C["fz20"]
     rho_tot = t.clamp(rho, min=1e-20)
     rs = (3 / (4 * math.pi * rho_tot) + 1e-20) ** (1 / 3)
     zeta, phi = 0.0, 1.0
@@ -285,7 +388,7 @@ def pbe_c(rho, grad):
 
 def b88_x(rho, grad):
     t = _torch()
-    beta = 0.0042
+    beta = B88_BETA
     gn = t.linalg.norm(grad + 1e-30, dim=-1)
     r = t.clamp(rho / 2, min=1e-20)
     r13 = r ** (1 / 3)
@@ -311,12 +414,21 @@ def vwn_rpa_c(rho):
     t = _torch()
     r = t.clamp(rho, min=1e-20)
     rs = (3.0 / (4.0 * math.pi * r)) ** (1 / 3)
-    return _vwn_branch(t.sqrt(rs), 0.0310907, 13.0720, 42.7198, -0.409286)
+    return _vwn_branch(t.sqrt(rs), *VWN_RPA_P)
+
+
+def vwn5_c(rho):
+    """VWN5 correlation (xc.py:323-373, unpolarised); the LDA functional of the
+    exact-Coulomb known-answer pin, not a product functional."""
+    t = _torch()
+    r = t.clamp(rho, min=1e-20)
+    rs = (3.0 / (4.0 * math.pi * r)) ** (1 / 3)
+    return _vwn_branch(t.sqrt(rs), *VWN5_P)
 
 
 def lyp_c(rho, grad):
     t = _torch()
-    a, b, c, d = 0.04918, 0.132, 0.2533, 0.349
+    a, b, c, d = LYP["a"], LYP["b"], LYP["c"], LYP["d"]
     C_F = 0.3 * (3.0 * math.pi ** 2) ** (2 / 3)
     gam = t.sum(grad ** 2, dim=-1)
     r = t.clamp(rho, min=1e-20)
@@ -330,15 +442,20 @@ def lyp_c(rho, grad):
 
 def eps_xc(name, rho, grad):
     if name == "PBE0":
-        return 0.75 * pbe_x(rho, grad) + pbe_c(rho, grad)
+        return MIX["PBE0_x"] * pbe_x(rho, grad) + pbe_c(rho, grad)
     if name == "PBE":
         return pbe_x(rho, grad) + pbe_c(rho, grad)
     if name == "B3LYP":
-        return 0.08 * lda_x(rho) + 0.72 * b88_x(rho, grad) + 0.19 * vwn_rpa_c(rho) + 0.81 * lyp_c(rho, grad)
+        s, b, v, l = MIX["B3LYP"]
+        return s * lda_x(rho) + b * b88_x(rho, grad) + v * vwn_rpa_c(rho) + l * lyp_c(rho, grad)
+    if name == "LDA":
+        return lda_x(rho) + vwn5_c(rho)
     raise ValueError(name)
 
 
-HF_COEFF = {"PBE0": 0.25, "B3LYP": 0.20, "PBE": 0.0}
+HF_COEFF = {"PBE0": 0.25, "B3LYP": 0.20, "PBE": 0.0, "LDA": 0.0}
+# closed-shell exchange energy -1/4 a_x sum P o K(P) (terms.py:500-509)
+EXCHANGE_PREFACTOR = -0.25
 
 
 def xc_energy_t(name, rho, grad, w, thresh=1e-10):
@@ -358,7 +475,7 @@ def xc_energy_t(name, rho, grad, w, thresh=1e-10):
 class OracleKS:
     """All FP64 inputs of one molecule, built by the oracle's own routines."""
 
-    def __init__(self, mol, xc, auxbasis="even-tempered-jk", grid=None, int3c_tensor=True):
+    def __init__(self, mol, xc, auxbasis="even-tempered-jk", grid=None, int3c_tensor=True, exact=False):
         from dftax_amd.basis import build_basis
         from dftax_amd.grid import becke as becke_spec
         self.mol = mol
@@ -372,7 +489,10 @@ class OracleKS:
         self.e_nn = nuclear_repulsion(mol.coords, z)
         self.int2c = int2c(self.aux)
         self.Vp, self.L = metric_pinv(self.int2c)
-        self.int3c = int3c(self.basis, self.aux) if int3c_tensor else None
+        self.int3c = int3c(self.basis, self.aux) if int3c_tensor and not exact else None
+        # exact Coulomb and exchange from four-centre integrals (the reference's
+        # coulomb=exact() path, terms.py exact term) instead of density fitting
+        self.eri = int4c_cart(self.basis) if exact else None
         spec = grid if grid is not None else becke_spec()
         self.grid_xyz, self.grid_w = becke_grid(mol.symbols, mol.coords, spec)
         self.ao, self.dao = eval_ao(self.basis, self.grid_xyz)
@@ -382,18 +502,24 @@ class OracleKS:
     def energy_t(self, P):
         """Total energy as a torch function of P (nao, nao): terms.py:592-613 + 791-828."""
         t = _torch()
-        I3 = t.from_numpy(self.int3c)
+        I3 = t.from_numpy(self.int3c) if self.int3c is not None else None
         Vp = t.from_numpy(self.Vp)
         ao = t.from_numpy(self.ao)
         dao = t.from_numpy(self.dao)
         w = t.from_numpy(self.grid_w)
         e1 = t.sum(P * t.from_numpy(self.hcore))
-        gamma = t.einsum("mnP,mn->P", I3, P)
-        eJ = 0.5 * t.dot(gamma, Vp @ gamma)
         eK = t.zeros((), dtype=t.float64)
-        if self.hf != 0.0:
-            Kmat = t.einsum("mlP,PQ,nsQ,ls->mn", I3, Vp, I3, P)
-            eK = -0.25 * self.hf * t.sum(P * Kmat)
+        if self.eri is not None:
+            G = t.from_numpy(self.eri)
+            eJ = 0.5 * t.einsum("mn,mnls,ls->", P, G, P)
+            if self.hf != 0.0:
+                eK = EXCHANGE_PREFACTOR * self.hf * t.einsum("mn,mlns,ls->", P, G, P)
+        else:
+            gamma = t.einsum("mnP,mn->P", I3, P)
+            eJ = 0.5 * t.dot(gamma, Vp @ gamma)
+            if self.hf != 0.0:
+                Kmat = t.einsum("mlP,PQ,nsQ,ls->mn", I3, Vp, I3, P)
+                eK = EXCHANGE_PREFACTOR * self.hf * t.sum(P * Kmat)
         rho = t.einsum("gm,mn,gn->g", ao, P, ao)
         grad = 2.0 * t.einsum("gm,mn,gnx->gx", ao, P, dao)
         exc = xc_energy_t(self.xc, rho, grad, w)
@@ -422,19 +548,26 @@ class OracleKS:
     def energy_fock(self, P):
         """Analytic form: F = hcore + J - 1/2 a_x K + Vxc (checked against autograd)."""
         t = _torch()
-        B = np.einsum("mnP,PQ->mnQ", self.int3c, self.L, optimize=True)
         nao = P.shape[0]
-        Bm = B.reshape(nao * nao, -1)
-        g = Bm.T @ P.reshape(-1)
-        eJ = 0.5 * float(g @ g)
-        J = (Bm @ g).reshape(nao, nao)
         eK = 0.0
         K = np.zeros_like(P)
-        if self.hf != 0.0:
-            for q in range(B.shape[2]):
-                Bq = B[:, :, q]
-                K += Bq @ P @ Bq
-            eK = -0.25 * self.hf * float(np.sum(P * K))
+        if self.eri is not None:
+            J = np.einsum("mnls,ls->mn", self.eri, P)
+            eJ = 0.5 * float(np.sum(P * J))
+            if self.hf != 0.0:
+                K = np.einsum("mlns,ls->mn", self.eri, P)
+                eK = EXCHANGE_PREFACTOR * self.hf * float(np.sum(P * K))
+        else:
+            B = np.einsum("mnP,PQ->mnQ", self.int3c, self.L, optimize=True)
+            Bm = B.reshape(nao * nao, -1)
+            g = Bm.T @ P.reshape(-1)
+            eJ = 0.5 * float(g @ g)
+            J = (Bm @ g).reshape(nao, nao)
+            if self.hf != 0.0:
+                for q in range(B.shape[2]):
+                    Bq = B[:, :, q]
+                    K += Bq @ P @ Bq
+                eK = EXCHANGE_PREFACTOR * self.hf * float(np.sum(P * K))
         # XC via pointwise autograd of the functional
         rho = np.einsum("gm,mn,gn->g", self.ao, P, self.ao, optimize=True)
         grad = 2.0 * np.einsum("gm,mn,gnx->gx", self.ao, P, self.dao, optimize=True)
@@ -442,14 +575,15 @@ class OracleKS:
         gt = t.tensor(grad, requires_grad=True)
         wt = t.from_numpy(self.grid_w)
         exc_t = xc_energy_t(self.xc, rt, gt, wt)
-        vr, vg = t.autograd.grad(exc_t, (rt, gt))
-        vr, vg = vr.numpy(), vg.numpy()
+        vr, vg = t.autograd.grad(exc_t, (rt, gt), allow_unused=True)
+        vr = vr.numpy()
+        vg = np.zeros_like(grad) if vg is None else vg.numpy()      # LDA: no gradient dependence
         W = 0.5 * vr[:, None] * self.ao + np.einsum("gx,gnx->gn", vg, self.dao)
         Vxc = self.ao.T @ W
         Vxc = Vxc + Vxc.T
         e1 = float(np.sum(P * self.hcore))
         e = e1 + eJ + eK + float(exc_t) + self.e_nn
-        F = self.hcore + J - 0.5 * self.hf * K + Vxc
+        F = self.hcore + J + 2.0 * EXCHANGE_PREFACTOR * self.hf * K + Vxc
         return e, F, dict(e1=e1, eJ=eJ, eK=eK, exc=float(exc_t), J=J, K=K, Vxc=Vxc)
 
 
