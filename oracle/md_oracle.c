@@ -578,3 +578,27 @@ void oracle_int4c_cart(int ns, const int *l, const int *prim0, const int *nprim,
             free(R); free(Eab); free(Ecd);
         }
 }
+
+/* Selected shell triples (ia, ib | ic) of (mn|P) in the final spans, packed
+ * block after block as [a][b][c] (for checks of a tensor too large to build
+ * whole). trip[3 t + 0..2] = ia, ib, ic; out receives sum_t na nb nc values. */
+void oracle_int3c_triples(int ns, const int *l, const int *prim0, const int *nprim, const int *ao0,
+                          const double *centre, const double *exps, const double *coefs, int sph, int nao,
+                          int nsx, const int *lx, const int *prim0x, const int *nprimx, const int *ao0x,
+                          const double *centrex, const double *expsx, const double *coefsx, int sphx, int naux,
+                          const double *c2s, const int *c2soff, int ntrip, const int *trip, const long *off,
+                          double *out) {
+    (void)nao; (void)naux;
+    set_c2s(c2s, c2soff);
+    if (!boys_ready) boys_build();
+    basis_t bo = {ns, l, prim0, nprim, ao0, centre, exps, coefs, sph};
+    basis_t bx = {nsx, lx, prim0x, nprimx, ao0x, centrex, expsx, coefsx, sphx};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < ntrip; ++t) {
+        int ia = trip[3 * t], ib = trip[3 * t + 1], ic = trip[3 * t + 2];
+        double *cblk = (double *)malloc(sizeof(double) * 28 * 28 * 28);
+        eri3c_block(&bo, ia, ib, &bx, ic, cblk);
+        c2s_apply3(&bo, l[ia], &bo, l[ib], &bx, lx[ic], cblk, out + off[t]);
+        free(cblk);
+    }
+}

@@ -120,6 +120,30 @@ def int3c(basis, aux):
     return out
 
 
+def int3c_triples(basis, aux, triples):
+    """Blocks (mn|P) of the given shell triples (ia, ib, ic), each [a][b][c] in
+    the final spans: a sample of a tensor too large to build whole."""
+    from dftax_amd.basis import ncart, nsph
+    a1, k1 = _basis_args(basis)
+    a2, k2 = _basis_args(aux)
+    c2s, off = _c2s_pack()
+    tri = np.ascontiguousarray(np.asarray(triples, dtype=np.int32).reshape(-1, 3))
+
+    def nf(b, l):
+        return nsph(l) if (b.spherical and l >= 2) else ncart(l)
+    sizes = [nf(basis, basis.l[i]) * nf(basis, basis.l[j]) * nf(aux, aux.l[k]) for i, j, k in tri]
+    offs = np.zeros(len(sizes), dtype=np.int64)
+    offs[1:] = np.cumsum(sizes)[:-1]
+    out = np.zeros(int(np.sum(sizes)))
+    lib().oracle_int3c_triples(*a1, *a2, _p(c2s), _p(off), ctypes.c_int(len(tri)), _p(tri),
+                               _p(offs), _p(out))
+    blocks = []
+    for (i, j, k), o, n in zip(tri, offs, sizes):
+        shape = (nf(basis, basis.l[i]), nf(basis, basis.l[j]), nf(aux, aux.l[k]))
+        blocks.append(out[o:o + n].reshape(shape))
+    return blocks
+
+
 def int4c_cart(basis):
     """Exact four-centre (ab|cd), cartesian span, orbital l <= 2 (n^4 FP64):
     the reference's exact-ERI path (dftax/integrals/eri4c.py), used only for
@@ -591,51 +615,33 @@ class OracleKS:
 # Guess and SCF (dftax/ks/guess.py, dftax/ks/scf.py)
 # ---------------------------------------------------------------------------
 
-_MADELUNG = ((1, 0), (2, 0), (2, 1), (3, 0), (3, 1))
-
-
-def _ground_state_config(Z):
-    left = Z
-    out = []
-    for n, l in _MADELUNG:
-        c = min(left, 2 * (2 * l + 1))
-        if c > 0:
-            out.append((n, l, c))
-        left -= c
-        if left == 0:
-            break
-    return out
+# sto-3g shells of each element in build order (core first) and the electrons each
+# holds in the neutral ground state, 1s2 2s2 2p^k spherically averaged
+# (guess.py:190-214, 250-278): the minao occupations written out for H, C, N, O.
+_MINAO_SHELL_ELECTRONS = {"H": ((0, 1.0),), "C": ((0, 2.0), (0, 2.0), (1, 2.0)),
+                          "N": ((0, 2.0), (0, 2.0), (1, 3.0)), "O": ((0, 2.0), (0, 2.0), (1, 4.0))}
 
 
 def minao_density(mol, basis):
-    """Projected sto-3g atomic densities (guess.py:296-337, 529-545)."""
+    """minao guess (guess.py:281-296, 602-616) restated for the pins: per atom,
+    the sto-3g occupations f from the table above (count / (2l+1) per m), the
+    cross overlap S_AM and the atom's own overlap S_AA from the oracle's
+    one-electron integrals, and P_A = X diag(f) X^T with X = S_AA^-1 S_AM
+    through a Cholesky factor of S_AA. Blocks on the diagonal; not rescaled."""
     from dftax_amd.basis import build_basis
-    from dftax_amd.system import symbol_to_Z
-    nao = basis.nao
-    P = np.zeros((nao, nao))
-    slices = basis.atom_ao_slices(mol.natm)
-    cache = {}
+    P = np.zeros((basis.nao, basis.nao))
     for a, sym in enumerate(mol.symbols):
-        sl = slices[a]
-        if sym not in cache:
-            idx = np.where(basis.atom == a)[0]
-            sub = _sub_basis(basis, idx)
-            minb = build_basis([sym], mol.coords[a:a + 1], "sto-3g", spherical=True)
-            per_l = {}
-            for _, l, c in _ground_state_config(symbol_to_Z(sym)):
-                per_l.setdefault(l, []).append(float(c))
-            f = []
-            for ish in range(minb.nshell):
-                l = int(minb.l[ish])
-                lst = per_l.get(l, [])
-                c = lst.pop(0) if lst else 0.0
-                f.extend([c / (2 * l + 1)] * (2 * l + 1))
-            f = np.array(f)
-            S_aa = _overlap_pair(sub, sub)
-            S_am = _overlap_pair(sub, minb)
-            C = np.linalg.solve(S_aa, S_am)
-            cache[sym] = (C * f[None, :]) @ C.T
-        P[sl, sl] = cache[sym]
+        idx = np.where(basis.atom == a)[0]
+        sub = _sub_basis(basis, idx)
+        lo = int(basis.ao0[idx[0]])
+        sl = slice(lo, lo + sub.nao)
+        minb = build_basis([sym], mol.coords[a:a + 1], "sto-3g", spherical=True)
+        shells = _MINAO_SHELL_ELECTRONS[sym]
+        assert [l for l, _ in shells] == [int(x) for x in minb.l], (sym, minb.l)
+        f = np.concatenate([np.full(2 * l + 1, n / (2 * l + 1)) for l, n in shells])
+        Lc = np.linalg.cholesky(_overlap_pair(sub, sub))
+        X = np.linalg.solve(Lc.T, np.linalg.solve(Lc, _overlap_pair(sub, minb)))
+        P[sl, sl] = X @ np.diag(f) @ X.T
     return P
 
 

@@ -369,3 +369,53 @@ def test_product_grid_layout_equals_oracle(spec):
     assert np.abs(p1 - p2).max() < 1e-13
     assert np.abs(w1 / w2 - 1).max() < 1e-14
     assert np.array_equal(o1, o2)
+
+
+# ---------------------------------------------------------------------------
+# minao guess
+# ---------------------------------------------------------------------------
+
+def test_minao_in_the_minimal_basis_is_the_occupation_table(monkeypatch):
+    """With sto-3g as the molecular basis the projection is the identity, so
+    each atom's block is diag(f): 2, 2, 4/3 x 3 for O and 1 for H."""
+    from dftax_amd.basis import build_basis
+    from dftax_amd.system import WATER
+    m = Molecule.from_xyz(WATER, "sto-3g", spherical=True)
+    b = build_basis(m.symbols, m.coords, "sto-3g", spherical=True)
+    P = ref.minao_density(m, b)
+    assert np.abs(P - np.diag([2, 2, 4 / 3, 4 / 3, 4 / 3, 1, 1])).max() < 1e-12
+    monkeypatch.setitem(ref._MINAO_SHELL_ELECTRONS, "O", ((0, 2.0), (0, 2.0), (1, 3.0)))
+    assert np.abs(ref.minao_density(m, b) - np.diag([2, 2, 4 / 3, 4 / 3, 4 / 3, 1, 1])).max() > 0.1
+
+
+def test_minao_projected_electron_count():
+    """In a larger basis the projection keeps nearly all electrons: Tr(P S)
+    within 1 % of N before the rescaling (guess.py:602-616)."""
+    from dftax_amd.basis import build_basis
+    from dftax_amd.system import serving_stream
+    m = serving_stream(1, seed=2, max_heavy=5)[0]
+    b = build_basis(m.symbols, m.coords, m.basis, spherical=True)
+    P = ref.minao_density(m, b)
+    S, _, _ = ref.int1e(b, m.coords, m.atom_charges())
+    assert abs(np.sum(P * S) - m.nelectron) < 0.01 * m.nelectron
+    assert np.abs(P - P.T).max() < 1e-14
+
+
+@pytest.mark.gpu
+def test_product_minao_equals_oracle():
+    """The product's guess (overlaps from the library) against the oracle's
+    restatement, for def2-SVP/def2-TZVP water and a C/N/O/H serving molecule."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dftax_amd.basis import build_basis
+    from dftax_amd.guess import minao_density
+    from dftax_amd.system import WATER, serving_stream
+    mols = [Molecule.from_xyz(WATER, "def2-svp", spherical=True), Molecule.from_xyz(WATER, "def2-tzvp", spherical=True),
+            serving_stream(1, seed=2, max_heavy=5)[0]]
+    for m in mols:
+        b = build_basis(m.symbols, m.coords, m.basis, spherical=m.spherical)
+        P = minao_density(m.symbols, m.coords, b, "sto-3g")
+        P_ref = ref.minao_density(m, b)
+        print(f"minao {m.basis} {len(m.symbols)} atoms: max |dP| {np.abs(P - P_ref).max():.2e}", flush=True)
+        assert np.abs(P - P_ref).max() < 1e-10
