@@ -6,6 +6,7 @@ and XC terms, DIIS and the Fock eigensolve through libdfks.so, a C-ABI
 library of hand-written gfx950 kernels.
 """
 
+from dftax_amd.forces import forces
 from dftax_amd.grid import Becke, becke
 from dftax_amd.guess import core, minao
 from dftax_amd.ks import KS, DFSpec, HipDFKS, HipSpec, df, hip
@@ -15,4 +16,4 @@ from dftax_amd.system import WATER, Molecule, serving_stream, tree_molecule, wat
 from dftax_amd.xc import B3LYP, PBE, PBE0
 
 __all__ = ["KS", "df", "hip", "becke", "Becke", "scf", "newton", "KSResult", "core", "minao", "Molecule", "PBE", "PBE0",
-           "B3LYP", "DFSpec", "HipSpec", "HipDFKS", "water_cluster", "tree_molecule", "serving_stream", "WATER"]
+           "B3LYP", "DFSpec", "HipSpec", "HipDFKS", "water_cluster", "tree_molecule", "serving_stream", "WATER", "forces"]

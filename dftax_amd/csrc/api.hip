@@ -91,6 +91,24 @@ hipError_t launch_gamma_f32(int nao, int nq, int nf, const float* T, const doubl
 hipError_t launch_jvec_f32(int64_t nrows, int nq, const float* B, const double* g, double* J, hipStream_t st);
 hipError_t launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st);
 hipError_t launch_to_f64(int64_t n, const float* x, double* y, hipStream_t st);
+hipError_t launch_eri_grad(const int2* pairs, int npair, int nket, const ShellSet& A, const ShellSet& C,
+                           const int32_t* atom_a, const int32_t* atom_c, int b_unit, const double* boys,
+                           const double* G, int64_t nao_g, double* grad, hipStream_t st);
+hipError_t launch_int1e_grad(const ShellSet& A, const int32_t* atom_a, int natm, const double* atm,
+                             const double* boys, const double* P, const double* W, int64_t nao, double* grad,
+                             hipStream_t st);
+hipError_t launch_ao_hess(int64_t npts, const double* pts, const ShellSet& S, int64_t nao, double* ao,
+                          hipStream_t st);
+hipError_t launch_xcg_rho(int64_t npts, int64_t nao, const double* ao, const double* Z, double* rho, double* grad,
+                          double* sigma, hipStream_t st);
+hipError_t launch_xcg_gdot(int64_t npts, int64_t nao, const double* ao, const double* grad, double* Gd,
+                           hipStream_t st);
+hipError_t launch_xcg_q(int64_t npts, int64_t nao, const double* ao, const double* Z, const double* X,
+                        const double* rho, const double* grad, const double* xc, const double* w,
+                        const int32_t* owner, double thresh, double* q, double* gout, double* fm, hipStream_t st);
+hipError_t launch_becke_grad(int64_t n, const double* pts, const double* raw, const int32_t* owner, const double* f,
+                             int natm, const double* atm, const double* invR, const double* aAB, double* gout,
+                             hipStream_t st);
 enum GemmPrec { kF64 = 0, kBF16x3 = 1 };
 struct GemmArgs {
   int M, N;
@@ -1067,7 +1085,7 @@ void build_all(dfks_ctx* h, hipStream_t st) {
     RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, h->nfit, (int64_t)nao * nao, naux,
                            &one, d_Lt, h->nfit, d3, naux, &zero, h->d_B, h->nfit));
     sync(st);
-    dfree(d_Lt);
+    h->d_Lt = d_Lt;   // kept: the nuclear gradient maps fitted quantities back to (P|Q) (nfit naux words)
     if (h->opt.keep_int3c) h->d_int3c = d3;
     else dfree(d3);
   } else {
@@ -1246,9 +1264,10 @@ void coulomb_recompute(dfks_ctx* h, hipStream_t st) {
 //   0  FP64 exchange T = B^T C:      0 rocBLAS, 1 k_gemm, 2 k_gemm_w8
 //   1  FP64 XC Z_b = Phi_b Pg_b:     0 rocBLAS, 1 k_gemm (batched)
 //   2  FP64 XC Phi_b^T W_b:          0 rocBLAS, 1 k_gemm (batched)
-//   3  FP64 exchange K partials:     0 k_gemm, 1 k_gemm_w8   (base 1)
+//   3  FP64 exchange K partials:     0 k_gemm, 1 k_gemm_w8, 2 k_gemm_w8 split by XCD (base 1)
 //   4  split-BF16 exchange T:        0 k_gemm, 1 k_gemm_b3w8 (base 1)
 //   5  SP2 product partials X X:     0 k_gemm, 1 k_gemm_w8   (base 1)
+//   6  split-BF16 exchange K:        0 k_gemm, 1 k_gemm split by XCD (base 1)
 // The first call of a shape on a device times every variant with HIP events
 // and keeps the fastest; the output of that call is recomputed by the kept
 // variant when it is not the one that ran last, so every call of a shape
@@ -1393,16 +1412,22 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
       const int64_t kd = (int64_t)nfit * nc;
       const int ntn = (nao + 127) / 128;
       const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, kd);
+      // split-K layouts: the tile-major split (nsplit by the tile count) or the split by
+      // XCD (16 splits, each XCD's tiles on the same k-window of T, read once per XCD)
+      const int nsx = 16;
       if (mixed) {
         GemmArgs g{nao, nao, kd, h->d_T32, kd, true, h->d_T32, kd, true};
-        HIPCHK(launch_gemm(kBF16x3, g, h->d_K, sg, beta, nsplit, 1, h->d_gws, st));
-      } else if (lib_k) {
-        // split-K partial tiles by the faster of the two library tiles for this shape, then the reduction
-        const GemmArgs g{nao, nao, kd, h->d_Tx, kd, true, h->d_Tx, kd, true};
-        tuned_product(h, 3, nao, kd, nsplit, 2, 1, [&](int c) {
-          HIPCHK(launch_gemm_partials(kF64, g, nsplit, 1, h->d_gws, c, st));
+        const int v = tuned_product(h, 6, nao, kd, nsplit, 2, 1, [&](int c) {
+          HIPCHK(launch_gemm_partials(kBF16x3, g, c == 1 ? nsx : nsplit, c == 1 ? 3 : 1, h->d_gws, 0, st));
         }, st);
-        HIPCHK(launch_gemm_reduce(kF64, g, h->d_K, sg, beta, nsplit, 1, h->d_gws, st));
+        HIPCHK(launch_gemm_reduce(kBF16x3, g, h->d_K, sg, beta, v == 1 ? nsx : nsplit, 1, h->d_gws, st));
+      } else if (lib_k) {
+        // split-K partial tiles by the fastest of the library tiles and layouts for this shape, then the reduction
+        const GemmArgs g{nao, nao, kd, h->d_Tx, kd, true, h->d_Tx, kd, true};
+        const int v = tuned_product(h, 3, nao, kd, nsplit, 3, 1, [&](int c) {
+          HIPCHK(launch_gemm_partials(kF64, g, c == 2 ? nsx : nsplit, c == 2 ? 3 : 1, h->d_gws, c == 2 ? 1 : c, st));
+        }, st);
+        HIPCHK(launch_gemm_reduce(kF64, g, h->d_K, sg, beta, v == 2 ? nsx : nsplit, 1, h->d_gws, st));
       } else {
         RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_none, nao, nao, kd, &sg,
                                h->d_Tx, kd, h->d_Tx, kd, &beta, h->d_K, nao));
@@ -1755,6 +1780,257 @@ void response_fock(dfks_ctx* h, const double* U, const double* V, int r, double*
   }
   HIPCHK(launch_fock(nao, nullptr, h->d_J, h->hf != 0.0 ? h->d_K : nullptr, -0.5 * h->hf,
                      ng > 0 ? h->d_Vxc : nullptr, R, st));
+}
+
+// ---------------------------------------------------------------------------
+// Nuclear gradient dE/dR at a frozen density (dftax/ks/forces.py:128-270): the
+// energy is differentiated with the basis and auxiliary centres and the Becke
+// grid following their atoms, and P = 2 Z (Z^T S Z)^-1 Z^T held at the
+// converged orbitals, so the density's response to S(R) is the energy-weighted
+// term -Tr(W dS), W = 1/2 P F P. Parts, each with its share of that term
+// (W_k = 1/2 P F_k P): [0] nuclear repulsion, [1] one-electron (F = hcore),
+// [2] Coulomb (F = J), [3] exchange (F = -a_x/2 K), [4] XC (F = Vxc).
+// Density-fitting weights, with d = V+ gamma, (ij|Q) of the orbital factor Cf
+// (P = Cf Cf^T) and D^P_ij = sum_Q V+_PQ (ij|Q):
+//   3-centre  Gamma_P = d_P P - a_x/2 Cf D^P Cf^T,
+//   2-centre  Gamma2  = -1/2 d d^T + a_x/4 sum_ij D_ij D_ij^T,
+// the metric's derivative taken as d(V+) = -V+ dV V+ (terms.py:195-204).
+// ---------------------------------------------------------------------------
+const double kOne = 1.0, kZero = 0.0;   // rocBLAS scalars (host pointers)
+
+__global__ __launch_bounds__(256) void k_bcast_scale(int64_t nrep, int64_t n, const double* __restrict__ d,
+                                                     const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nrep * n) return;
+  y[i] = d[i / n] * x[i % n];
+}
+
+// atom of each shell, by its centre
+std::vector<int32_t> shell_atoms(const Basis& B, const std::vector<double>& atm4, int natom) {
+  std::vector<int32_t> out(B.n(), -1);
+  for (int s = 0; s < B.n(); ++s) {
+    for (int a = 0; a < natom; ++a) {
+      double d2 = 0.0;
+      for (int x = 0; x < 3; ++x) {
+        const double d = B.centre[3 * s + x] - atm4[4 * a + x];
+        d2 += d * d;
+      }
+      if (d2 < 1e-20) { out[s] = a; break; }
+    }
+    if (out[s] < 0) fail(1, "a basis shell is not centred on an atom; forces need atom-centred shells");
+  }
+  return out;
+}
+
+void forces_run(dfks_ctx* h, const double* Co, int nocc, int64_t ng, const double* gxyz, const double* graw,
+                const int32_t* gown, const double* bragg, double* out, hipStream_t st) {
+  if (!h->built) fail(6, "dfks_build has not been called");
+  if (h->recompute || !h->d_B) fail(1, "forces need the materialised fitted tensor (df(chunk=None))");
+  if (nocc != h->nocc) fail(1, "nocc does not match the handle");
+  if (nocc > h->nf_chunk) fail(1, "too many occupied orbitals for the exchange intermediate");
+  RBCHK(rocblas_set_stream(h->rb, st));
+  const bool mixed_was = h->mixed_now;
+  h->mixed_now = false;
+  struct Restore {
+    dfks_ctx* h;
+    bool m;
+    ~Restore() { h->mixed_now = m; }
+  } restore{h, mixed_was};
+  const int nao = h->nao, naux = h->naux, nfit = h->nfit, natom = h->natom;
+  const int64_t nn = (int64_t)nao * nao, na3 = 3 * (int64_t)natom;
+  const double ax = h->hf;
+  std::vector<void*> owned;
+  struct Free {
+    std::vector<void*>& p;
+    ~Free() { for (void* q : p) dfree(q); }
+  } fr{owned};
+  auto alloc = [&](size_t n) { double* p = dalloc<double>(n); owned.push_back(p); return p; };
+  auto alloc_i = [&](size_t n) { int32_t* p = dalloc<int32_t>(n); owned.push_back(p); return p; };
+  auto up = [&](auto* dst, const auto* src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(*src), hipMemcpyHostToDevice, st));
+  };
+
+  // density, Fock parts and the energy-weighted densities
+  double* Cf = alloc((size_t)nao * nocc);
+  HIPCHK(launch_axpy((int64_t)nao * nocc, std::sqrt(2.0), Co, Cf, 0, st));
+  double* P = alloc(nn);
+  gemm(h, false, true, nao, nao, nocc, 1.0, Cf, nao, Cf, nao, 0.0, P, nao);
+  double* F = alloc(nn);
+  double* E = alloc(8);
+  energy_fock_factored(h, P, Cf, std::vector<double>(nocc, 1.0), nocc, E, F, st);
+  double* Fk = alloc(nn);
+  double* tmp = alloc(nn);
+  double* zero = alloc(nn);
+  HIPCHK(hipMemsetAsync(zero, 0, sizeof(double) * nn, st));
+  double* Wk[4];
+  for (int k = 0; k < 4; ++k) {
+    Wk[k] = alloc(nn);
+    // the term's Fock contribution, assembled as the Fock build does (K symmetrised,
+    // Vxc = Vh + Vh^T from the half potential the XC pass leaves)
+    if (k == 0) HIPCHK(launch_fock(nao, h->d_H, zero, nullptr, 0.0, nullptr, Fk, st));
+    else if (k == 1) HIPCHK(launch_fock(nao, nullptr, h->d_J, nullptr, 0.0, nullptr, Fk, st));
+    else if (k == 2) HIPCHK(launch_fock(nao, nullptr, zero, ax != 0.0 ? h->d_K : nullptr, -0.5 * ax, nullptr, Fk, st));
+    else HIPCHK(launch_fock(nao, nullptr, zero, nullptr, 0.0, h->ng > 0 ? h->d_Vxc : nullptr, Fk, st));
+    gemm(h, false, false, nao, nao, nao, 1.0, Fk, nao, P, nao, 0.0, tmp, nao);
+    gemm(h, false, false, nao, nao, nao, 0.5, P, nao, tmp, nao, 0.0, Wk[k], nao);
+  }
+  double* g = alloc(4 * na3);   // parts 1..4 on the device
+  HIPCHK(hipMemsetAsync(g, 0, sizeof(double) * 4 * na3, st));
+  double* gp[4] = {g, g + na3, g + 2 * na3, g + 3 * na3};
+
+  // atoms of the shells, device copies
+  const std::vector<int32_t> at_o = shell_atoms(h->orb, h->atm_host, natom);
+  const std::vector<int32_t> at_x = shell_atoms(h->aux, h->atm_host, natom);
+  int32_t* d_at_o = alloc_i(at_o.size());
+  int32_t* d_at_x = alloc_i(at_x.size());
+  up(d_at_o, at_o.data(), at_o.size());
+  up(d_at_x, at_x.data(), at_x.size());
+  const ShellSet O = h->orb.view(), X = h->aux.view();
+
+  // one-electron and the S-derivative shares of every part
+  HIPCHK(launch_int1e_grad(O, d_at_o, natom, h->d_atm, h->d_boys, P, Wk[0], nao, gp[0], st));
+  for (int k = 1; k < 4; ++k)
+    HIPCHK(launch_int1e_grad(O, d_at_o, natom, h->d_atm, h->d_boys, zero, Wk[k], nao, gp[k], st));
+
+  // density fitting: d = L gamma', (ij|Q) of Cf, D = L (ij|Q)
+  double* dvec = alloc(naux);
+  RBCHK(rocblas_dgemv(h->rb, rocblas_operation_transpose, nfit, naux, &kOne, h->d_Lt, nfit, h->d_gamma, 1, &kZero,
+                      dvec, 1));
+  const int64_t no2 = (int64_t)nocc * nocc;
+  double* Aq = alloc((size_t)nfit * no2);   // [Q][i][j]
+  RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_none, rocblas_operation_none, nocc, nocc, nao, &kOne,
+                                      h->d_Tx, (int64_t)nfit * nocc, nocc, Cf, nao, 0, &kZero, Aq, nocc, no2, nfit));
+  double* D = alloc((size_t)naux * no2);    // [P][i][j]
+  gemm(h, false, false, no2, naux, nfit, 1.0, Aq, no2, h->d_Lt, nfit, 0.0, D, no2);
+  // pairs (ia >= ib) of orbital shells, and the auxiliary shells with a unit partner
+  std::vector<int2> pr;
+  for (int a = 0; a < h->orb.n(); ++a)
+    for (int b = 0; b <= a; ++b) pr.push_back(make_int2(a, b));
+  std::vector<int2> apr;
+  for (int c = 0; c < h->aux.n(); ++c) apr.push_back(make_int2(c, c));
+  int2* d_pr = reinterpret_cast<int2*>(alloc_i(2 * pr.size()));
+  int2* d_apr = reinterpret_cast<int2*>(alloc_i(2 * apr.size()));
+  up(d_pr, pr.data(), pr.size());
+  up(d_apr, apr.data(), apr.size());
+  double* G2 = alloc((size_t)naux * naux);
+  double* G3 = alloc((size_t)naux * nn);
+  // Coulomb: Gamma_P = d_P P, Gamma2 = -1/2 d d^T
+  {
+    const int64_t tot = (int64_t)naux * nn;
+    hipLaunchKernelGGL(k_bcast_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, (int64_t)naux, nn, dvec,
+                       P, G3);
+    HIPCHK(hipGetLastError());
+    gemm(h, false, true, naux, naux, 1, -0.5, dvec, naux, dvec, naux, 0.0, G2, naux);
+    HIPCHK(launch_eri_grad(d_pr, (int)pr.size(), h->aux.n(), O, X, d_at_o, d_at_x, 0, h->d_boys, G3, nao, gp[1], st));
+    HIPCHK(launch_eri_grad(d_apr, (int)apr.size(), h->aux.n(), X, X, d_at_x, d_at_x, 1, h->d_boys, G2, naux, gp[1],
+                           st));
+  }
+  // exchange: Gamma_P = -a_x/2 Cf D^P Cf^T, Gamma2 = a_x/4 D D^T
+  if (ax != 0.0) {
+    double* Xp = alloc((size_t)naux * nao * nocc);
+    RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_none, rocblas_operation_none, nao, nocc, nocc, &kOne,
+                                        Cf, nao, 0, D, nocc, no2, &kZero, Xp, nao, (int64_t)nao * nocc, naux));
+    const double sk = -0.5 * ax;
+    RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_none, rocblas_operation_transpose, nao, nao, nocc,
+                                        &sk, Xp, nao, (int64_t)nao * nocc, Cf, nao, 0, &kZero, G3, nao, nn, naux));
+    gemm(h, true, false, naux, naux, no2, 0.25 * ax, D, no2, D, no2, 0.0, G2, naux);
+    HIPCHK(launch_eri_grad(d_pr, (int)pr.size(), h->aux.n(), O, X, d_at_o, d_at_x, 0, h->d_boys, G3, nao, gp[2], st));
+    HIPCHK(launch_eri_grad(d_apr, (int)apr.size(), h->aux.n(), X, X, d_at_x, d_at_x, 1, h->d_boys, G2, naux, gp[2],
+                           st));
+  }
+
+  // XC on the caller's atom-centred grid (no weight cutoff), in chunks of points
+  if (ng > 0 && h->xc_id > 0) {
+    std::vector<double> invR((size_t)natom * natom), aAB((size_t)natom * natom), atm3(3 * (size_t)natom);
+    for (int a = 0; a < natom; ++a) {
+      for (int x = 0; x < 3; ++x) atm3[3 * a + x] = h->atm_host[4 * a + x];
+      for (int b = 0; b < natom; ++b) {
+        double d2 = 0.0;
+        for (int x = 0; x < 3; ++x) {
+          const double d = h->atm_host[4 * a + x] - h->atm_host[4 * b + x];
+          d2 += d * d;
+        }
+        invR[(size_t)a * natom + b] = 1.0 / std::sqrt(d2 + (a == b ? 1.0 : 0.0));
+        const double chi = bragg[a] / bragg[b], u = (chi - 1.0) / (chi + 1.0);
+        aAB[(size_t)a * natom + b] = std::min(0.45, std::max(-0.45, u / (u * u - 1.0)));
+      }
+    }
+    for (int64_t i = 0; i < ng; ++i)
+      if (gown[i] < 0 || gown[i] >= natom) fail(1, "grid owner atom index out of range");
+    double* d_pts = alloc(3 * (size_t)ng);
+    double* d_raw = alloc(ng);
+    double* d_w = alloc(ng);
+    int32_t* d_own = alloc_i(ng);
+    double* d_atm3 = alloc(atm3.size());
+    double* d_invR = alloc(invR.size());
+    double* d_aAB = alloc(aAB.size());
+    up(d_pts, gxyz, 3 * (size_t)ng);
+    up(d_raw, graw, ng);
+    up(d_own, gown, ng);
+    up(d_atm3, atm3.data(), atm3.size());
+    up(d_invR, invR.data(), invR.size());
+    up(d_aAB, aAB.data(), aAB.size());
+    HIPCHK(launch_becke(ng, d_pts, d_raw, d_own, natom, d_atm3, d_invR, d_aAB, d_w, st));
+    const int64_t chunk = std::max<int64_t>(256, std::min<int64_t>(ng, (int64_t)(1.5e9 / (8.0 * 16.0 * nao))));
+    double* ao = alloc(10 * (size_t)chunk * nao);
+    double* Z = alloc((size_t)chunk * nao);
+    double* Xg = alloc((size_t)chunk * nao);
+    double* q = alloc(3 * (size_t)chunk * nao);
+    double* rho = alloc(chunk);
+    double* grad = alloc(3 * (size_t)chunk);
+    double* sigma = alloc(chunk);
+    double* xco = alloc(3 * (size_t)chunk);
+    double* fm = alloc(chunk);
+    double* Sm = alloc(3 * (size_t)nao);
+    HIPCHK(hipMemsetAsync(Sm, 0, sizeof(double) * 3 * nao, st));
+    for (int64_t g0 = 0; g0 < ng; g0 += chunk) {
+      const int64_t nc = std::min(chunk, ng - g0);
+      HIPCHK(launch_ao_hess(nc, d_pts + 3 * g0, O, nao, ao, st));
+      // Z = Phi P and X = (grad rho . grad Phi) P, row-major nc x nao
+      gemm(h, false, false, nao, nc, nao, 1.0, P, nao, ao, nao, 0.0, Z, nao);
+      HIPCHK(launch_xcg_rho(nc, nao, ao, Z, rho, grad, sigma, st));
+      HIPCHK(launch_xc_pointwise(nc, h->xc_id, rho, sigma, xco, st));
+      HIPCHK(launch_xcg_gdot(nc, nao, ao, grad, q, st));
+      gemm(h, false, false, nao, nc, nao, 1.0, P, nao, q, nao, 0.0, Xg, nao);
+      HIPCHK(launch_xcg_q(nc, nao, ao, Z, Xg, rho, grad, xco, d_w + g0, d_own + g0, h->opt.rho_thresh, q, gp[3], fm,
+                          st));
+      for (int j = 0; j < 3; ++j)
+        RBCHK(rocblas_dgemv(h->rb, rocblas_operation_none, nao, nc, &kOne, q + (int64_t)j * nc * nao, nao, d_w + g0, 1,
+                            &kOne, Sm + (int64_t)j * nao, 1));
+      HIPCHK(launch_becke_grad(nc, d_pts + 3 * g0, d_raw + g0, d_own + g0, fm, natom, d_atm3, d_invR, d_aAB, gp[3],
+                               st));
+    }
+    // basis part: -2 sum over the functions of each atom
+    std::vector<double> hS(3 * (size_t)nao), hg(na3);
+    HIPCHK(hipMemcpyAsync(hS.data(), Sm, sizeof(double) * 3 * nao, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hg.data(), gp[3], sizeof(double) * na3, hipMemcpyDeviceToHost, st));
+    sync(st);
+    for (int s = 0; s < h->orb.n(); ++s) {
+      const int a = at_o[s];
+      for (int f = 0; f < nfun(h->orb.l[s], h->orb.spherical); ++f)
+        for (int j = 0; j < 3; ++j) hg[3 * a + j] -= 2.0 * hS[(size_t)j * nao + h->orb.ao0[s] + f];
+    }
+    up(gp[3], hg.data(), na3);
+  }
+  // out: [0] nuclear repulsion (host), [1..4] from the device
+  std::vector<double> hall(4 * na3);
+  HIPCHK(hipMemcpyAsync(hall.data(), g, sizeof(double) * 4 * na3, hipMemcpyDeviceToHost, st));
+  sync(st);
+  for (int a = 0; a < natom; ++a)
+    for (int x = 0; x < 3; ++x) {
+      double v = 0.0;
+      for (int b = 0; b < natom; ++b) {
+        if (b == a) continue;
+        double d2 = 0.0, dx = h->atm_host[4 * a + x] - h->atm_host[4 * b + x];
+        for (int y = 0; y < 3; ++y) {
+          const double d = h->atm_host[4 * a + y] - h->atm_host[4 * b + y];
+          d2 += d * d;
+        }
+        v -= h->atm_host[4 * a + 3] * h->atm_host[4 * b + 3] * dx / (d2 * std::sqrt(d2));
+      }
+      out[3 * a + x] = v;
+    }
+  for (int64_t i = 0; i < 4 * na3; ++i) out[na3 + i] = hall[i];
 }
 
 void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level_shift, hipStream_t st) {
@@ -2406,6 +2682,17 @@ int dfks_becke_weights(int64_t n, const double* pts, const double* raw_w, const 
     HIPCHK(launch_becke(n, d_pts, d_raw, d_own, natom, d_atm, d_invR, d_a, d_w, st));
     HIPCHK(hipMemcpyAsync(w_out, d_w, n * sizeof(double), hipMemcpyDeviceToHost, st));
     sync(st);
+  });
+}
+
+int dfks_forces(dfks_handle h, const double* C_occ, int32_t nocc, int64_t ngrid, const double* grid_xyz,
+                const double* grid_raw_w, const int32_t* grid_owner, const double* bragg, double* grad_parts,
+                dfks_stream stream) {
+  return guard([&] {
+    if (!h || !C_occ || !grad_parts || !bragg) fail(1, "null argument");
+    if (ngrid < 0 || (ngrid > 0 && (!grid_xyz || !grid_raw_w || !grid_owner))) fail(1, "bad grid arguments");
+    StreamScope ss(h->st = S_(stream));
+    forces_run(h, C_occ, nocc, ngrid, grid_xyz, grid_raw_w, grid_owner, bragg, grad_parts, S_(stream));
   });
 }
 

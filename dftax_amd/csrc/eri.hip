@@ -531,4 +531,349 @@ hipError_t launch_int1e(const ShellSet& A, int natm, const double* atm, const do
   return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// Nuclear-coordinate derivatives of the integrals, contracted on the fly with
+// the densities they meet in dE/dR (dftax/ks/forces.py:245-270, where the
+// energy is differentiated with the basis centres following their atoms).
+//
+// A cartesian Gaussian's centre derivative raises and lowers its exponent,
+// d/dA_x x_A^i e^{-a x_A^2} = 2a x_A^{i+1} e^{..} - i x_A^{i-1} e^{..}, so in
+// the Hermite picture the x-axis table of the bra pair is replaced by
+//   DA^{ij}_t = 2a E^{i+1,j}_t - i E^{i-1,j}_t   (derivative in A)
+//   DB^{ij}_t = 2b E^{i,j+1}_t - j E^{i,j-1}_t   (derivative in B)
+// and the R table needs one more order. The auxiliary centre follows from
+// translational invariance, d/dC = -(d/dA + d/dB). The spherical density
+// blocks are taken to the cartesian components first, so each lane weights
+// its cartesian derivative integrals directly; the wavefront's six sums go to
+// the atoms with one atomic add each.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+constexpr int kMaxLG = kMaxL3 + 1;                                   // (ff|g) plus one derivative order
+constexpr int kNRG = (kMaxLG + 1) * (kMaxLG + 2) * (kMaxLG + 3) / 6;  // 364
+
+// Spherical block of a symmetric density weight -> cartesian components:
+// G_cart[a][b][c] = sum_{m n p} T[a][m] T[b][n] T[c][p] G[m][n][p] with the sparse
+// cart->sph lists (the transpose of the integral transform). Layout of the
+// spherical block: element (m, n, p) at G[p * sp + m * sm + n * sn].
+__device__ void weight_to_cart(int la, int lb, int lc, int sphA, int sphB, int sphC, const double* __restrict__ G,
+                               int64_t sp, int64_t sm, int64_t sn, int64_t m0, int64_t n0, int64_t p0,
+                               double* s_out /*[na][nb][nc]*/) {
+  const int lane = threadIdx.x;
+  const int na = ncart(la), nb = ncart(lb), nc = ncart(lc);
+  const int fa = sphA ? nsph(la) : na, fb = sphB ? nsph(lb) : nb, fc = sphC ? nsph(lc) : nc;
+  for (int o = lane; o < na * nb * nc; o += kWave) s_out[o] = 0.0;
+  __syncthreads();
+  // one lane per spherical element scatters into the cartesian block (LDS atomics)
+  for (int o = lane; o < fa * fb * fc; o += kWave) {
+    const int C = o % fc, B = (o / fc) % fb, A = o / (fc * fb);
+    const double g = G[(p0 + C) * sp + (m0 + A) * sm + (n0 + B) * sn];
+    const int nA = sphA ? g_c2s.n[la][A] : 1, nB = sphB ? g_c2s.n[lb][B] : 1, nC = sphC ? g_c2s.n[lc][C] : 1;
+    for (int x = 0; x < nA; ++x) {
+      const int ai = sphA ? g_c2s.cart[la][A][x] : A;
+      const double wa = g * (sphA ? g_c2s.coef[la][A][x] : 1.0);
+      for (int y = 0; y < nB; ++y) {
+        const int bi = sphB ? g_c2s.cart[lb][B][y] : B;
+        const double wb = wa * (sphB ? g_c2s.coef[lb][B][y] : 1.0);
+        for (int z = 0; z < nC; ++z) {
+          const int ci = sphC ? g_c2s.cart[lc][C][z] : C;
+          atomicAdd(&s_out[(ai * nb + bi) * nc + ci], wb * (sphC ? g_c2s.coef[lc][C][z] : 1.0));
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// sum over the 3-/2-centre derivative integrals weighted by Gamma:
+//   3c: tasks (bra pair ia >= ib, aux shell ic); Gamma[P][m][n] (naux x nao x nao),
+//       factor 2 for ia != ib (both bra orders);
+//   2c (b_unit): bra = aux shell p with a unit s partner, ket = aux shell q, all
+//       ordered (p, q); Gamma2[p][q] (naux x naux).
+// grad[3 * atom + x] receives dE/dR.
+__global__ __launch_bounds__(64) void k_eri_grad(const int2* __restrict__ pairs, int npair, int nket, ShellSet A,
+                                                 ShellSet C, const int32_t* __restrict__ atom_a,
+                                                 const int32_t* __restrict__ atom_c, int b_unit,
+                                                 const double* __restrict__ boys, const double* __restrict__ G,
+                                                 int64_t nao_g, double* __restrict__ grad, int64_t blk0) {
+  __shared__ double s_E[3][5 * 5 * 10];
+  __shared__ double s_Ec[5 * (kMaxLG + 1)];
+  __shared__ double s_F[kMaxLG + 1];
+  __shared__ double s_R[2][kNRG];
+  __shared__ double s_G[kMaxCartBlk];
+  const int lane = threadIdx.x;
+  const int64_t task = blk0 + blockIdx.x;
+  const int k = (int)(task / nket), ic = (int)(task % nket);
+  if (k >= npair) return;
+  const int2 pr = pairs[k];
+  const int ia = pr.x, ib = b_unit ? -1 : pr.y;
+  const int la = A.l[ia], lb = b_unit ? 0 : A.l[ib], lc = C.l[ic];
+  const int L = la + lb + lc + 1;           // one order above the integrals
+  const int mt = L + 1, lab = la + lb, mtab = lab + 3, nbb = lb + 2;
+  const int na = ncart(la), nb = ncart(lb), nc = ncart(lc);
+  const int sphA = A.spherical && la >= 2, sphB = !b_unit && A.spherical && lb >= 2, sphC = C.spherical && lc >= 2;
+  if (b_unit)   // Gamma2[p][q]: rows p (bra), columns q (ket)
+    weight_to_cart(la, 0, lc, sphA, 0, sphC, G, 1, nao_g, 0, A.ao0[ia], 0, C.ao0[ic], s_G);
+  else          // Gamma[P][m][n]
+    weight_to_cart(la, lb, lc, sphA, sphB, sphC, G, nao_g * nao_g, nao_g, 1, A.ao0[ia], A.ao0[ib], C.ao0[ic], s_G);
+
+  const double Ax = A.centre[3 * ia], Ay = A.centre[3 * ia + 1], Az = A.centre[3 * ia + 2];
+  double Bx = Ax, By = Ay, Bz = Az;
+  int npb = 1, p0b = 0;
+  if (!b_unit) {
+    Bx = A.centre[3 * ib]; By = A.centre[3 * ib + 1]; Bz = A.centre[3 * ib + 2];
+    npb = A.nprim[ib]; p0b = A.prim0[ib];
+  }
+  const double Cx = C.centre[3 * ic], Cy = C.centre[3 * ic + 1], Cz = C.centre[3 * ic + 2];
+  const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz, ab2 = ABx * ABx + ABy * ABy + ABz * ABz;
+  double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
+  for (int pa = 0; pa < A.nprim[ia]; ++pa)
+    for (int pb = 0; pb < npb; ++pb) {
+      const double al = A.exps[A.prim0[ia] + pa];
+      const double be = b_unit ? 0.0 : A.exps[p0b + pb];
+      const double cab = A.coefs[A.prim0[ia] + pa] * (b_unit ? 1.0 : A.coefs[p0b + pb]);
+      const double p = al + be;
+      const double Px = (al * Ax + be * Bx) / p, Py = (al * Ay + be * By) / p, Pz = (al * Az + be * Bz) / p;
+      const double Kab = exp(-al * be / p * ab2);
+      __syncthreads();
+      if (lane < 3) {
+        const double xab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
+        e_table_1d(la + 1, lb + 1, al, be, xab, mtab, s_E[lane]);
+      }
+      for (int pc = 0; pc < C.nprim[ic]; ++pc) {
+        const double ga = C.exps[C.prim0[ic] + pc];
+        const double cc = C.coefs[C.prim0[ic] + pc];
+        const double rho = p * ga / (p + ga);
+        const double pref = cab * cc * Kab * 2.0 * 17.493418327624862 / (p * ga * sqrt(p + ga));  // 2 pi^2.5
+        const double pcx = Px - Cx, pcy = Py - Cy, pcz = Pz - Cz;
+        __syncthreads();
+        if (lane == 0) {
+          boys_array_rt(L, rho * (pcx * pcx + pcy * pcy + pcz * pcz), s_F, boys);
+          const double i2g = 0.5 / ga;
+          for (int q = 0; q < (lc + 1) * mt; ++q) s_Ec[q] = 0.0;
+          s_Ec[0] = 1.0;
+          for (int kk = 0; kk < lc; ++kk)
+            for (int t = 0; t < mt; ++t) {
+              double v = 0.0;
+              if (t > 0) v += i2g * s_Ec[kk * mt + t - 1];
+              if (t + 1 < mt) v += (t + 1) * s_Ec[kk * mt + t + 1];
+              s_Ec[(kk + 1) * mt + t] = v;
+            }
+        }
+        __syncthreads();
+        const double* R = r_ladder(L, rho, pcx, pcy, pcz, s_F, s_R[0], s_R[1]);
+        const double sgn = (lc & 1) ? -pref : pref;
+        for (int o = lane; o < na * nb * nc; o += kWave) {
+          const double w = s_G[o];
+          if (w == 0.0) continue;
+          const int ci = o % nc, bi = (o / nc) % nb, ai = o / (nc * nb);
+          int ae[3], be3[3], ce[3];
+          for (int x = 0; x < 3; ++x) {
+            ae[x] = cexp(la, ai, x);
+            be3[x] = b_unit ? 0 : cexp(lb, bi, x);
+            ce[x] = cexp(lc, ci, x);
+          }
+          // per axis: value E, A-derivative DA and B-derivative DB tables (t <= i + j + 1)
+          double Ev[3][8], DA[3][8], DB[3][8];
+          for (int x = 0; x < 3; ++x) {
+            const double* E = s_E[x];
+            const int i = ae[x], j = be3[x];
+            for (int t = 0; t <= i + j + 1; ++t) {
+              const double e0 = (t <= i + j) ? E[(i * nbb + j) * mtab + t] : 0.0;
+              double da = 2.0 * al * E[((i + 1) * nbb + j) * mtab + t];
+              if (i > 0 && t <= i + j - 1) da -= i * E[((i - 1) * nbb + j) * mtab + t];
+              double db = 2.0 * be * E[(i * nbb + j + 1) * mtab + t];
+              if (j > 0 && t <= i + j - 1) db -= j * E[(i * nbb + j - 1) * mtab + t];
+              Ev[x][t] = e0;
+              DA[x][t] = da;
+              DB[x][t] = db;
+            }
+          }
+          const double* Ecx = s_Ec + ce[0] * mt;
+          const double* Ecy = s_Ec + ce[1] * mt;
+          const double* Ecz = s_Ec + ce[2] * mt;
+          double sa[3] = {0, 0, 0}, sb[3] = {0, 0, 0};
+          for (int t = 0; t <= ae[0] + be3[0] + 1; ++t)
+            for (int u = 0; u <= ae[1] + be3[1] + 1; ++u)
+              for (int v = 0; v <= ae[2] + be3[2] + 1; ++v) {
+                double inner = 0.0;
+                for (int ta = ce[0] & 1; ta <= ce[0]; ta += 2)
+                  for (int nu = ce[1] & 1; nu <= ce[1]; nu += 2)
+                    for (int ph = ce[2] & 1; ph <= ce[2]; ph += 2)
+                      inner += Ecx[ta] * Ecy[nu] * Ecz[ph] * R[ridx(t + ta, u + nu, v + ph)];
+                sa[0] += DA[0][t] * Ev[1][u] * Ev[2][v] * inner;
+                sa[1] += Ev[0][t] * DA[1][u] * Ev[2][v] * inner;
+                sa[2] += Ev[0][t] * Ev[1][u] * DA[2][v] * inner;
+                sb[0] += DB[0][t] * Ev[1][u] * Ev[2][v] * inner;
+                sb[1] += Ev[0][t] * DB[1][u] * Ev[2][v] * inner;
+                sb[2] += Ev[0][t] * Ev[1][u] * DB[2][v] * inner;
+              }
+          for (int x = 0; x < 3; ++x) {
+            gA[x] += w * sgn * sa[x];
+            gB[x] += w * sgn * sb[x];
+          }
+        }
+      }
+    }
+  const double f = (!b_unit && ia != ib) ? 2.0 : 1.0;
+  double tot[6];
+  for (int x = 0; x < 3; ++x) {
+    tot[x] = wave_sum(gA[x]) * f;
+    tot[3 + x] = wave_sum(gB[x]) * f;
+  }
+  if (lane == 0) {
+    const int at_a = atom_a[ia], at_c = atom_c[ic];
+    for (int x = 0; x < 3; ++x) {
+      atomicAdd(&grad[3 * at_a + x], tot[x]);
+      if (!b_unit) atomicAdd(&grad[3 * atom_a[ib] + x], tot[3 + x]);
+      atomicAdd(&grad[3 * at_c + x], -(tot[x] + tot[3 + x]));
+    }
+  }
+}
+
+// One-electron part of dE/dR over every ordered shell pair (a, b), derivative in
+// the bra centre only (the ket-centre share equals it by symmetry, hence the 2):
+//   2 sum_{mn in (a,b)} [ P_mn (dT_mn + sum_C dV^C_mn) - W_mn dS_mn ]  -> atom of a,
+//   and the Hellmann-Feynman term of each nucleus C, d/dC V^C = -(d/dA + d/dB) V^C:
+//   -2 sum_{mn} P_mn dV^C_mn -> atom C.
+__global__ __launch_bounds__(64) void k_int1e_grad(ShellSet A, const int32_t* __restrict__ atom_a, int natm,
+                                                   const double* __restrict__ atm, const double* __restrict__ boys,
+                                                   const double* __restrict__ P, const double* __restrict__ W,
+                                                   int64_t nao, double* __restrict__ grad) {
+  __shared__ double s_E[3][5 * 6 * 11];
+  __shared__ double s_F[12];
+  __shared__ double s_R[2][120];
+  __shared__ double s_P[100], s_W[100];
+  const int lane = threadIdx.x;
+  const int ia = blockIdx.x / A.n, ib = blockIdx.x % A.n;
+  const int la = A.l[ia], lb = A.l[ib];
+  const int na = ncart(la), nb = ncart(lb), nout = na * nb;
+  const int L = la + lb + 1;
+  const int mt = la + lb + 4;   // E up to (la+1, lb+2)
+  const int nbb = lb + 3;
+  const int sphA = A.spherical && la >= 2, sphB = A.spherical && lb >= 2;
+  weight_to_cart(la, lb, 0, sphA, sphB, 0, P, 0, nao, 1, A.ao0[ia], A.ao0[ib], 0, s_P);
+  weight_to_cart(la, lb, 0, sphA, sphB, 0, W, 0, nao, 1, A.ao0[ia], A.ao0[ib], 0, s_W);
+  const double Ax = A.centre[3 * ia], Ay = A.centre[3 * ia + 1], Az = A.centre[3 * ia + 2];
+  const double Bx = A.centre[3 * ib], By = A.centre[3 * ib + 1], Bz = A.centre[3 * ib + 2];
+  const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz, ab2 = ABx * ABx + ABy * ABy + ABz * ABz;
+  double gA[3] = {0, 0, 0};   // this lane's share of the S and T terms
+  double gV[3] = {0, 0, 0};   // the wavefront's nuclear-attraction sums (the same on every lane)
+  for (int pa = 0; pa < A.nprim[ia]; ++pa)
+    for (int pb = 0; pb < A.nprim[ib]; ++pb) {
+      const double al = A.exps[A.prim0[ia] + pa], be = A.exps[A.prim0[ib] + pb];
+      const double cab = A.coefs[A.prim0[ia] + pa] * A.coefs[A.prim0[ib] + pb];
+      const double p = al + be;
+      const double Px = (al * Ax + be * Bx) / p, Py = (al * Ay + be * By) / p, Pz = (al * Az + be * Bz) / p;
+      const double Kab = exp(-al * be / p * ab2);
+      __syncthreads();
+      if (lane < 3) {
+        const double xab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
+        e_table_1d(la + 1, lb + 2, al, be, xab, mt, s_E[lane]);
+      }
+      __syncthreads();
+      const double s0 = pow(kPi / p, 1.5) * Kab * cab;
+      // overlap and kinetic: per-axis factors s_ij = E^{ij}_0, t_ij from E^{i,j+2}, E^{ij}, E^{i,j-2};
+      // their bra derivatives 2a f_{i+1,j} - i f_{i-1,j}
+      for (int o = lane; o < nout; o += kWave) {
+        const int ai = o / nb, bi = o % nb;
+        double s1[3], t1[3], ds[3], dt[3];
+        for (int x = 0; x < 3; ++x) {
+          const int i = cexp(la, ai, x), j = cexp(lb, bi, x);
+          const double* E = s_E[x];
+          auto sv = [&](int ii, int jj) { return E[(ii * nbb + jj) * mt]; };
+          auto tv = [&](int ii, int jj) {
+            double r = -2.0 * be * be * sv(ii, jj + 2) + be * (2 * jj + 1) * sv(ii, jj);
+            if (jj >= 2) r -= 0.5 * jj * (jj - 1) * sv(ii, jj - 2);
+            return r;
+          };
+          s1[x] = sv(i, j);
+          t1[x] = tv(i, j);
+          ds[x] = 2.0 * al * sv(i + 1, j) - (i > 0 ? i * sv(i - 1, j) : 0.0);
+          dt[x] = 2.0 * al * tv(i + 1, j) - (i > 0 ? i * tv(i - 1, j) : 0.0);
+        }
+        const double wP = s_P[o], wW = s_W[o];
+        for (int x = 0; x < 3; ++x) {
+          const int y = (x + 1) % 3, z = (x + 2) % 3;
+          const double dS = s0 * ds[x] * s1[y] * s1[z];
+          const double dT = s0 * (dt[x] * s1[y] * s1[z] + ds[x] * t1[y] * s1[z] + ds[x] * s1[y] * t1[z]);
+          gA[x] += wP * dT - wW * dS;
+        }
+      }
+      // nuclear attraction, nucleus by nucleus
+      for (int c = 0; c < natm; ++c) {
+        const double pcx = Px - atm[4 * c], pcy = Py - atm[4 * c + 1], pcz = Pz - atm[4 * c + 2];
+        __syncthreads();
+        if (lane == 0) boys_array_rt(L, p * (pcx * pcx + pcy * pcy + pcz * pcz), s_F, boys);
+        __syncthreads();
+        const double* R = r_ladder(L, p, pcx, pcy, pcz, s_F, s_R[0], s_R[1]);
+        const double pref = -atm[4 * c + 3] * cab * Kab * 2.0 * kPi / p;
+        double gc[3] = {0, 0, 0};
+        for (int o = lane; o < nout; o += kWave) {
+          const double wP = s_P[o];
+          if (wP == 0.0) continue;
+          const int ai = o / nb, bi = o % nb;
+          double Ev[3][8], DA[3][8];
+          int ij[3];
+          for (int x = 0; x < 3; ++x) {
+            const int i = cexp(la, ai, x), j = cexp(lb, bi, x);
+            const double* E = s_E[x];
+            ij[x] = i + j;
+            for (int t = 0; t <= i + j + 1; ++t) {
+              Ev[x][t] = t <= i + j ? E[(i * nbb + j) * mt + t] : 0.0;
+              double da = 2.0 * al * E[((i + 1) * nbb + j) * mt + t];
+              if (i > 0 && t <= i + j - 1) da -= i * E[((i - 1) * nbb + j) * mt + t];
+              DA[x][t] = da;
+            }
+          }
+          double d3[3] = {0, 0, 0};
+          for (int t = 0; t <= ij[0] + 1; ++t)
+            for (int u = 0; u <= ij[1] + 1; ++u)
+              for (int v = 0; v <= ij[2] + 1; ++v) {
+                const double r = R[ridx(t, u, v)];
+                d3[0] += DA[0][t] * Ev[1][u] * Ev[2][v] * r;
+                d3[1] += Ev[0][t] * DA[1][u] * Ev[2][v] * r;
+                d3[2] += Ev[0][t] * Ev[1][u] * DA[2][v] * r;
+              }
+          for (int x = 0; x < 3; ++x) gc[x] += wP * pref * d3[x];
+        }
+        for (int x = 0; x < 3; ++x) {
+          const double v = wave_sum(gc[x]);
+          gV[x] += v;
+          if (lane == 0) atomicAdd(&grad[3 * c + x], -2.0 * v);
+        }
+        __syncthreads();
+      }
+    }
+  for (int x = 0; x < 3; ++x) {
+    const double v = wave_sum(gA[x]) + gV[x];
+    if (lane == 0) atomicAdd(&grad[3 * atom_a[ia] + x], 2.0 * v);
+  }
+}
+
+hipError_t launch_eri_grad(const int2* pairs, int npair, int nket, const ShellSet& A, const ShellSet& C,
+                           const int32_t* atom_a, const int32_t* atom_c, int b_unit, const double* boys,
+                           const double* G, int64_t nao_g, double* grad, hipStream_t st) {
+  const int64_t ntask = (int64_t)npair * nket;
+  const int64_t chunk = 1 << 30;
+  for (int64_t b0 = 0; b0 < ntask; b0 += chunk) {
+    const int64_t nb = std::min(chunk, ntask - b0);
+    hipLaunchKernelGGL(k_eri_grad, dim3((unsigned)nb), dim3(kWave), 0, st, pairs, npair, nket, A, C, atom_a, atom_c,
+                       b_unit, boys, G, nao_g, grad, b0);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_int1e_grad(const ShellSet& A, const int32_t* atom_a, int natm, const double* atm,
+                             const double* boys, const double* P, const double* W, int64_t nao, double* grad,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_int1e_grad, dim3((unsigned)(A.n * A.n)), dim3(kWave), 0, st, A, atom_a, natm, atm, boys, P, W,
+                     nao, grad);
+  return hipGetLastError();
+}
+
 }  // namespace dfks

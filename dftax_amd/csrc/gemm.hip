@@ -146,15 +146,20 @@ __global__ __launch_bounds__(256, (gemm_blocks_per_cu<PREC, BKC, NJ>())) void k_
   constexpr int VEC = (128 * BK) / 256;  // values per thread per operand tile
   constexpr int BN = 32 * NJ;
   static_assert(NJ == 4 || NJ == 5, "128- or 160-wide tiles");
-  const int tile = xcd_tile(blockIdx.x, ntiles);
+  // split-by-XCD mode (sym bit 1): block x runs on XCD x % 8, which takes the
+  // splits z = x % 8 (mod 8) for every tile, so the tiles on one XCD walk the same
+  // k-window and share it through that XCD's L2 (the operand is read about once)
+  const bool xs = (sym & 2) != 0;
+  const int tile = xs ? (int)(blockIdx.x >> 3) : xcd_tile(blockIdx.x, ntiles);
+  const int split = xs ? (int)((blockIdx.x & 7) + 8 * blockIdx.z) : (int)blockIdx.z;
   if (tile >= ntiles) return;
   const int ntn = (N + BN - 1) / BN;
   int tm, tn;
-  tile_of(tile, ntn, sym != 0, tm, tn);
+  tile_of(tile, ntn, (sym & 1) != 0, tm, tn);
   const int m0 = tm * kBM, n0 = tn * BN;
-  const int64_t k_begin = (int64_t)blockIdx.z * kchunk;
+  const int64_t k_begin = (int64_t)split * kchunk;
   const int64_t k_end = k_begin + kchunk < K ? k_begin + kchunk : K;
-  Out* Cz = C + (int64_t)blockIdx.z * split_stride;
+  Out* Cz = C + (int64_t)split * split_stride;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -338,15 +343,19 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K,
   constexpr int BN = 32 * NJ;
   __shared__ double sA[2][kW8BM][kW8BK + 1];
   __shared__ double sB[2][BN][kW8BK + 1];
-  if ((int)blockIdx.x >= ntiles) return;
+  // split-by-XCD mode (sym bit 1) as in k_gemm
+  const bool xs = (sym & 2) != 0;
+  const int tile = xs ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+  const int split = xs ? (int)((blockIdx.x & 7) + 8 * blockIdx.z) : (int)blockIdx.z;
+  if (tile >= ntiles) return;
   const int ntn = (N + BN - 1) / BN;
   int tm, tn;
-  tile_of(blockIdx.x, ntn, sym != 0, tm, tn);
+  tile_of(tile, ntn, (sym & 1) != 0, tm, tn);
   const int64_t m0 = (int64_t)tm * kW8BM;
   const int n0 = tn * BN;
-  const int64_t k_begin = (int64_t)blockIdx.z * kchunk;
+  const int64_t k_begin = (int64_t)split * kchunk;
   const int64_t k_end = k_begin + kchunk < K ? k_begin + kchunk : K;
-  double* __restrict__ Cz = C + (int64_t)blockIdx.z * split_stride;
+  double* __restrict__ Cz = C + (int64_t)split * split_stride;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * (16 * NJ);
   const int li = lane & 15, lk = lane >> 4;
@@ -717,14 +726,16 @@ static hipError_t launch_gemm_t(const GemmArgs& g, Out* C, int64_t ldc, int nspl
   using In = typename GemmTraits<PREC>::In;
   constexpr int BK = GemmTraits<PREC>::BK;
   constexpr int BN = 32 * NJ;
-  if (sym && NJ != 4) return hipErrorInvalidValue;
+  if ((sym & 1) && NJ != 4) return hipErrorInvalidValue;
+  const bool xs = (sym & 2) != 0;
+  if (xs && (nsplit % 8 != 0 || batch != 1)) return hipErrorInvalidValue;
   const int ntm = (g.M + kBM - 1) / kBM, ntn = (g.N + BN - 1) / BN;
-  const int ntiles = sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
+  const int ntiles = (sym & 1) ? ntn * (ntn + 1) / 2 : ntm * ntn;
   int64_t kchunk = (g.K + nsplit - 1) / nsplit;
   kchunk = (kchunk + BK - 1) / BK * BK;
   const int64_t split_stride = (int64_t)g.M * ldc;
   const int nblk = (ntiles + 7) / 8 * 8;
-  dim3 grid(nblk, batch, nsplit);
+  const dim3 grid = xs ? dim3(8 * ntiles, 1, nsplit / 8) : dim3(nblk, batch, nsplit);
   const In* A = static_cast<const In*>(g.A);
   const In* B = static_cast<const In*>(g.B);
   // 16-byte vector loads need a 16-byte aligned base and row pitch (K-contiguous operands)
@@ -747,18 +758,20 @@ template <int NJ>
 static hipError_t launch_w8_t(const GemmArgs& g, double* C, int64_t ldc, int nsplit, int sym, hipStream_t st,
                               const RowDot& rd = RowDot()) {
   constexpr int BN = 32 * NJ;
-  if (!g.b_kcontig || (sym && NJ != 4)) return hipErrorInvalidValue;
+  if (!g.b_kcontig || ((sym & 1) && NJ != 4)) return hipErrorInvalidValue;
   if (rd.out && (g.N > BN || nsplit != 1 || sym || rd.nq < kW8BM)) return hipErrorInvalidValue;
+  const bool xs = (sym & 2) != 0;
+  if (xs && nsplit % 8 != 0) return hipErrorInvalidValue;
   const int64_t ntm = ((int64_t)g.M + kW8BM - 1) / kW8BM, ntn = (g.N + BN - 1) / BN;
-  const int64_t ntl = sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
-  if (ntl > 0x7fffffff || nsplit > 65535) return hipErrorInvalidValue;
+  const int64_t ntl = (sym & 1) ? ntn * (ntn + 1) / 2 : ntm * ntn;
+  if (8 * ntl > 0x7fffffff || nsplit > 65535) return hipErrorInvalidValue;
   int64_t kchunk = (g.K + nsplit - 1) / nsplit;
   kchunk = (kchunk + kW8BK - 1) / kW8BK * kW8BK;
   const double* A = static_cast<const double*>(g.A);
   const double* B = static_cast<const double*>(g.B);
   const int vec_a = ((uintptr_t)A % 16 == 0) && (g.lda % 2 == 0);
   const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % 2 == 0);
-  const dim3 grid((unsigned)ntl, 1, (unsigned)nsplit);
+  const dim3 grid = xs ? dim3((unsigned)(8 * ntl), 1, (unsigned)(nsplit / 8)) : dim3((unsigned)ntl, 1, (unsigned)nsplit);
   const int64_t split_stride = (int64_t)g.M * ldc;
   if (g.a_kcontig)
     return launch_k(k_gemm_w8<true, NJ>, grid, dim3(512), st, (int64_t)g.M, g.N, g.K, A, g.lda, B, g.ldb, C, ldc,
@@ -801,9 +814,9 @@ hipError_t launch_gemm_reduce(int prec, const GemmArgs& g, double* C, double alp
   const unsigned nb = (unsigned)((stride + 255) / 256);
   if (prec == kF64)
     return launch_k(k_gemm_reduce<double, double>, dim3(nb), dim3(256), st, g.M, g.N, nsplit,
-                    static_cast<const double*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
+                    static_cast<const double*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym & 1);
   return launch_k(k_gemm_reduce<float, double>, dim3(nb), dim3(256), st, g.M, g.N, nsplit,
-                  static_cast<const float*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym);
+                  static_cast<const float*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta, sym & 1);
 }
 
 // C = alpha * A B^T (+ beta C), C row-major M x N (ldc = N), FP64 output.
@@ -814,6 +827,7 @@ hipError_t launch_gemm(int prec, const GemmArgs& g, double* C, double alpha, dou
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (nsplit < 1) nsplit = 1;
   const bool direct = nsplit == 1 && !sym && alpha == 1.0 && beta == 0.0 && prec == kF64;
+  if ((sym & 2) && nsplit % 8 != 0) return hipErrorInvalidValue;
   if (direct) {
     const int t128 = (g.N + 127) / 128, t160 = (g.N + 159) / 160;
     if (t160 < t128 && (int64_t)t160 * 160 - g.N < (int64_t)t128 * 128 - g.N)

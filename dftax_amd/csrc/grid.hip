@@ -319,4 +319,329 @@ hipError_t launch_ao(int64_t ng, const double* pts, const ShellSet& S, int64_t n
   return hipSuccess;
 }
 
+
+// ---------------------------------------------------------------------------
+// Nuclear gradient of E_xc on the moving Becke grid (dftax/ks/forces.py:245-270:
+// the basis centres and the grid points follow their atoms, the Becke weights
+// are differentiated). With Z = Phi P, X = (grad rho . grad Phi) P and, per point g
+// and function m,
+//   q_{g m j} = v_rho d_j phi Z + 2 v_sigma ( (H_j . grad rho) Z + d_j phi X ),
+// the basis part is -2 sum_{m on A} sum_g w_g q_{g m j}, the point part (points
+// owned by A move with it) is 2 sum_{g owned by A} w_g sum_m q_{g m j}, and the
+// weight part is sum_g f_g dw_g/dR.
+// ---------------------------------------------------------------------------
+
+// AO values, gradients and Hessians: planes [10][npts][nao] in the order
+// value, x, y, z, xx, xy, xz, yy, yz, zz.
+template <int L>
+__device__ __forceinline__ void ao_shell_hess(double dx, double dy, double dz, double R, double dR, double ddR,
+                                               int sph, double* __restrict__ r0, int64_t plane) {
+  constexpr int NC = ncart(L);
+  double o[10][NC];
+  const double d[3] = {dx, dy, dz};
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    int l[3] = {cart_exp<L>(c, 0), cart_exp<L>(c, 1), cart_exp<L>(c, 2)};
+    // per axis: x^l, l x^(l-1), l (l-1) x^(l-2)
+    double p0[3], p1[3], p2[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double a = 1.0, b = 0.0, e = 0.0;   // value, first, second derivative of x^l
+      for (int k = 0; k < l[x]; ++k) {
+        e = e * d[x] + 2.0 * b;
+        b = b * d[x] + a;
+        a *= d[x];
+      }
+      p0[x] = a; p1[x] = b; p2[x] = e;
+    }
+    const double A = p0[0] * p0[1] * p0[2];
+    const double Ag[3] = {p1[0] * p0[1] * p0[2], p0[0] * p1[1] * p0[2], p0[0] * p0[1] * p1[2]};
+    o[0][c] = A * R;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) o[1 + x][c] = Ag[x] * R + A * dR * d[x];
+    int h = 4;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = i; j < 3; ++j) {
+        double Aij;
+        if (i == j) Aij = (i == 0 ? p2[0] * p0[1] * p0[2] : (i == 1 ? p0[0] * p2[1] * p0[2] : p0[0] * p0[1] * p2[2]));
+        else {
+          double f[3] = {p0[0], p0[1], p0[2]};
+          f[i] = p1[i];
+          f[j] = p1[j];
+          Aij = f[0] * f[1] * f[2];
+        }
+        o[h][c] = Aij * R + (Ag[i] * d[j] + Ag[j] * d[i]) * dR + A * (ddR * d[i] * d[j] + (i == j ? dR : 0.0));
+        ++h;
+      }
+  }
+  if (sph && L >= 2) {
+    constexpr int NS = nsph(L);
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const double w = c2s_coef<L>(c, s);
+          if (w != 0.0) acc += w * o[k][c];
+        }
+        r0[k * plane + s] = acc;
+      }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int k = 0; k < 10; ++k) r0[k * plane + c] = o[k][c];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ao_hess(int64_t npts, const double* __restrict__ pts, ShellSet S, int64_t nao,
+                                                  double* __restrict__ ao) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (gid >= npts * S.n) return;
+  const int64_t g = gid / S.n;
+  const int ish = (int)(gid % S.n);
+  const double dx = pts[3 * g] - S.centre[3 * ish];
+  const double dy = pts[3 * g + 1] - S.centre[3 * ish + 1];
+  const double dz = pts[3 * g + 2] - S.centre[3 * ish + 2];
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double R = 0.0, dR = 0.0, ddR = 0.0;
+  for (int k = S.prim0[ish]; k < S.prim0[ish] + S.nprim[ish]; ++k) {
+    const double a = S.exps[k];
+    const double e = S.coefs[k] * exp(-a * r2);
+    R += e;
+    dR += a * e;
+    ddR += a * a * e;
+  }
+  dR *= -2.0;    // (1/r) dR/dr
+  ddR *= 4.0;    // its derivative over r, per unit x_i x_j
+  const int64_t plane = npts * nao;
+  double* r0 = ao + g * nao + S.ao0[ish];
+  switch (S.l[ish]) {
+    case 0: ao_shell_hess<0>(dx, dy, dz, R, dR, ddR, S.spherical, r0, plane); break;
+    case 1: ao_shell_hess<1>(dx, dy, dz, R, dR, ddR, S.spherical, r0, plane); break;
+    case 2: ao_shell_hess<2>(dx, dy, dz, R, dR, ddR, S.spherical, r0, plane); break;
+    default: ao_shell_hess<3>(dx, dy, dz, R, dR, ddR, S.spherical, r0, plane); break;
+  }
+}
+
+// rho, grad rho and sigma at each point from the AO planes and Z = Phi P
+// (one wavefront per point, lanes over the functions).
+__global__ __launch_bounds__(64) void k_xcg_rho(int64_t npts, int64_t nao, const double* __restrict__ ao,
+                                                const double* __restrict__ Z, double* __restrict__ rho,
+                                                double* __restrict__ grad, double* __restrict__ sigma) {
+  const int64_t g = blockIdx.x;
+  if (g >= npts) return;
+  const int64_t plane = npts * nao;
+  double a[4] = {0, 0, 0, 0};
+  for (int64_t m = threadIdx.x; m < nao; m += 64) {
+    const double z = Z[g * nao + m];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] += ao[k * plane + g * nao + m] * z;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    for (int off = 32; off > 0; off >>= 1) a[k] += __shfl_xor(a[k], off, 64);
+  if (threadIdx.x == 0) {
+    rho[g] = a[0];
+    const double gx = 2.0 * a[1], gy = 2.0 * a[2], gz = 2.0 * a[3];
+    grad[3 * g] = gx; grad[3 * g + 1] = gy; grad[3 * g + 2] = gz;
+    sigma[g] = gx * gx + gy * gy + gz * gz;
+  }
+}
+
+// Gd[g][m] = grad rho_g . grad phi_m(g)
+__global__ __launch_bounds__(256) void k_xcg_gdot(int64_t npts, int64_t nao, const double* __restrict__ ao,
+                                                   const double* __restrict__ grad, double* __restrict__ Gd) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= npts * nao) return;
+  const int64_t g = i / nao, plane = npts * nao;
+  Gd[i] = grad[3 * g] * ao[plane + i] + grad[3 * g + 1] * ao[2 * plane + i] + grad[3 * g + 2] * ao[3 * plane + i];
+}
+
+// q_{g m j} (above) into q[3][npts][nao]; the point part 2 w_g sum_m q to the
+// point's owner atom. xc[3 g ..] = {f, v_rho, v_sigma}; points with
+// rho <= thresh contribute nothing (the energy's density mask).
+__global__ __launch_bounds__(64) void k_xcg_q(int64_t npts, int64_t nao, const double* __restrict__ ao,
+                                              const double* __restrict__ Z, const double* __restrict__ X,
+                                              const double* __restrict__ rho, const double* __restrict__ grad,
+                                              const double* __restrict__ xc, const double* __restrict__ w,
+                                              const int32_t* __restrict__ owner, double thresh, double* __restrict__ q,
+                                              double* __restrict__ gout, double* __restrict__ fm) {
+  const int64_t g = blockIdx.x;
+  if (g >= npts) return;
+  const int64_t plane = npts * nao;
+  const bool on = rho[g] > thresh;
+  if (threadIdx.x == 0) fm[g] = on ? xc[3 * g] : 0.0;   // energy density f of the masked functional
+  const double vr = on ? xc[3 * g + 1] : 0.0, vs = on ? xc[3 * g + 2] : 0.0;
+  const double gr[3] = {grad[3 * g], grad[3 * g + 1], grad[3 * g + 2]};
+  double acc[3] = {0, 0, 0};
+  for (int64_t m = threadIdx.x; m < nao; m += 64) {
+    const int64_t i = g * nao + m;
+    const double z = Z[i], x = X[i];
+    const double d[3] = {ao[plane + i], ao[2 * plane + i], ao[3 * plane + i]};
+    const double hxx = ao[4 * plane + i], hxy = ao[5 * plane + i], hxz = ao[6 * plane + i];
+    const double hyy = ao[7 * plane + i], hyz = ao[8 * plane + i], hzz = ao[9 * plane + i];
+    const double hg[3] = {hxx * gr[0] + hxy * gr[1] + hxz * gr[2], hxy * gr[0] + hyy * gr[1] + hyz * gr[2],
+                          hxz * gr[0] + hyz * gr[1] + hzz * gr[2]};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double v = vr * d[j] * z + 2.0 * vs * (hg[j] * z + d[j] * x);
+      q[j * plane + i] = v;
+      acc[j] += v;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+  if (threadIdx.x == 0 && on) {
+    const int a = owner[g];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) atomicAdd(&gout[3 * a + j], 2.0 * w[g] * acc[j]);
+  }
+}
+
+// Becke weight derivatives: sum_g f_g dw_g/dR_D (becke.py:68-108 differentiated),
+// with w = w_raw P_O / sum_A P_A, P_A = prod_{B != A} s(nu_AB), and the point
+// moving with its owner O:
+//   dw/dR_D = w sum_A (delta_AO - P_A / Z) sum_{B != A} (s'/s)(nu_AB) (1 - 2 a_AB mu_AB) dmu_AB/dR_D,
+//   dmu_AB/dR_A = -u_A / R_AB - mu_AB (R_A - R_B) / R_AB^2,
+//   dmu_AB/dR_B =  u_B / R_AB + mu_AB (R_A - R_B) / R_AB^2,
+//   dmu_AB/dr   = (u_A - u_B) / R_AB  (to the owner), u_X = (r - R_X)/|r - R_X|.
+// One thread per point; per-atom sums in LDS, added to gout once per workgroup.
+__device__ __forceinline__ void becke_s(double nu, double& s, double& ds) {
+  const double f1 = 1.5 * nu - 0.5 * nu * nu * nu;
+  const double f2 = 1.5 * f1 - 0.5 * f1 * f1 * f1;
+  const double f3 = 1.5 * f2 - 0.5 * f2 * f2 * f2;
+  s = 0.5 * (1.0 - f3);
+  ds = -0.5 * (1.5 - 1.5 * f2 * f2) * (1.5 - 1.5 * f1 * f1) * (1.5 - 1.5 * nu * nu);
+}
+
+__global__ __launch_bounds__(256) void k_becke_grad(int64_t n, const double* __restrict__ pts,
+                                                     const double* __restrict__ raw, const int32_t* __restrict__ owner,
+                                                     const double* __restrict__ f, int natm,
+                                                     const double* __restrict__ atm /*[3 natm]*/,
+                                                     const double* __restrict__ invR, const double* __restrict__ aAB,
+                                                     double* __restrict__ gout) {
+  extern __shared__ double s_g[];   // 3 natm accumulators
+  for (int i = threadIdx.x; i < 3 * natm; i += blockDim.x) s_g[i] = 0.0;
+  __syncthreads();
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g < n && f[g] != 0.0) {
+    const double px = pts[3 * g], py = pts[3 * g + 1], pz = pts[3 * g + 2];
+    const int own = owner[g];
+    auto dist = [&](int a, double* u) {
+      const double dx = px - atm[3 * a], dy = py - atm[3 * a + 1], dz = pz - atm[3 * a + 2];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz + 1e-30);
+      u[0] = dx / r; u[1] = dy / r; u[2] = dz / r;
+      return r;
+    };
+    // pass 1: Z = sum_A P_A and P_owner
+    double Zs = 0.0, Po = 0.0, u[3];
+    for (int a = 0; a < natm; ++a) {
+      const double ra = dist(a, u);
+      double prod = 1.0;
+      for (int b = 0; b < natm; ++b) {
+        if (b == a) continue;
+        const double rb = dist(b, u);
+        const double mu = (ra - rb) * invR[a * natm + b];
+        double s, ds;
+        becke_s(mu + aAB[a * natm + b] * (1.0 - mu * mu), s, ds);
+        prod *= s;
+      }
+      Zs += prod;
+      if (a == own) Po = prod;
+    }
+    if (Zs > 0.0) {
+      const double wg = raw[g] * Po / Zs;
+      const double fw = f[g] * wg;
+      // pass 2: the derivative terms
+      for (int a = 0; a < natm; ++a) {
+        double ua[3];
+        const double ra = dist(a, ua);
+        double prod = 1.0;
+        for (int b = 0; b < natm; ++b) {
+          if (b == a) continue;
+          double ub[3];
+          const double rb = dist(b, ub);
+          const double mu = (ra - rb) * invR[a * natm + b];
+          double s, ds;
+          becke_s(mu + aAB[a * natm + b] * (1.0 - mu * mu), s, ds);
+          prod *= s;
+        }
+        const double Ka = fw * ((a == own ? 1.0 : 0.0) - prod / Zs);
+        if (Ka == 0.0) continue;
+        for (int b = 0; b < natm; ++b) {
+          if (b == a) continue;
+          double ub[3];
+          const double rb = dist(b, ub);
+          const double iR = invR[a * natm + b];
+          const double mu = (ra - rb) * iR;
+          const double ab = aAB[a * natm + b];
+          double s, ds;
+          becke_s(mu + ab * (1.0 - mu * mu), s, ds);
+          if (s < 1e-300) continue;
+          const double c = Ka * ds / s * (1.0 - 2.0 * ab * mu);
+          for (int x = 0; x < 3; ++x) {
+            const double Rab = atm[3 * a + x] - atm[3 * b + x];
+            const double da = -ua[x] * iR - mu * Rab * iR * iR;
+            const double db = ub[x] * iR + mu * Rab * iR * iR;
+            const double dr = (ua[x] - ub[x]) * iR;
+            atomicAdd(&s_g[3 * a + x], c * da);
+            atomicAdd(&s_g[3 * b + x], c * db);
+            atomicAdd(&s_g[3 * own + x], c * dr);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * natm; i += blockDim.x)
+    if (s_g[i] != 0.0) atomicAdd(&gout[i], s_g[i]);
+}
+
+hipError_t launch_ao_hess(int64_t npts, const double* pts, const ShellSet& S, int64_t nao, double* ao,
+                          hipStream_t st) {
+  const int64_t total = npts * S.n;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ao_hess, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, npts, pts, S, nao, ao);
+  return hipGetLastError();
+}
+
+hipError_t launch_xcg_rho(int64_t npts, int64_t nao, const double* ao, const double* Z, double* rho, double* grad,
+                          double* sigma, hipStream_t st) {
+  if (npts == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xcg_rho, dim3((unsigned)npts), dim3(64), 0, st, npts, nao, ao, Z, rho, grad, sigma);
+  return hipGetLastError();
+}
+
+hipError_t launch_xcg_gdot(int64_t npts, int64_t nao, const double* ao, const double* grad, double* Gd,
+                           hipStream_t st) {
+  const int64_t total = npts * nao;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xcg_gdot, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, npts, nao, ao, grad, Gd);
+  return hipGetLastError();
+}
+
+hipError_t launch_xcg_q(int64_t npts, int64_t nao, const double* ao, const double* Z, const double* X,
+                        const double* rho, const double* grad, const double* xc, const double* w,
+                        const int32_t* owner, double thresh, double* q, double* gout, double* fm, hipStream_t st) {
+  if (npts == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xcg_q, dim3((unsigned)npts), dim3(64), 0, st, npts, nao, ao, Z, X, rho, grad, xc, w, owner,
+                     thresh, q, gout, fm);
+  return hipGetLastError();
+}
+
+hipError_t launch_becke_grad(int64_t n, const double* pts, const double* raw, const int32_t* owner, const double* f,
+                             int natm, const double* atm, const double* invR, const double* aAB, double* gout,
+                             hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_becke_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 3 * natm * sizeof(double), st, n,
+                     pts, raw, owner, f, natm, atm, invR, aAB, gout);
+  return hipGetLastError();
+}
+
 }  // namespace dfks

@@ -317,6 +317,7 @@ class KS:
         self.basis = build_basis(system.symbols, system.coords, system.basis, spherical=system.spherical)
         self.aux_basis = build_basis(system.symbols, system.coords, spec.auxbasis, spherical=True)
         gspec = grid if grid is not None else becke()
+        self.grid_spec = gspec
         self.grid_xyz, self.grid_w = build_grid(system.symbols, system.coords, gspec)
         self.coulomb = HipDFKS(self.basis, self.aux_basis, system, self.grid_xyz, self.grid_w, self.xc, nelec,
                                spec.engine, device=device, chunk=spec.chunk, screen=spec.screen)

@@ -25,6 +25,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import sys
 
 import threading
 import time
@@ -143,8 +144,9 @@ class Server:
         return req
 
     # -- the loop ------------------------------------------------------------
-    def run(self, requests: list[Request]) -> list[Request]:
-        """Serve requests at their ``arrival`` offsets (seconds from now)."""
+    def run(self, requests: list[Request], progress=None) -> list[Request]:
+        """Serve requests at their ``arrival`` offsets (seconds from now);
+        ``progress(msg)`` is called every 64 completed requests."""
         for r in requests:
             r.footprint = self.estimate(r)
             if r.footprint > self.budget:
@@ -166,6 +168,8 @@ class Server:
                 free_slots.append(slot)
                 with lock:
                     done.append(req)
+                    if progress is not None and len(done) % 64 == 0:
+                        progress(f"served {len(done)} of {len(requests)} requests")
                 self._cv.notify_all()
 
         i = 0
@@ -221,6 +225,65 @@ def latency_stats(reqs: list[Request], alone: dict | None = None) -> dict:
     return out
 
 
+def run_stream(n: int = 64, rates=(1.0, 2.0), *, seed: int = 0, slots: int = 16, xc: str = "B3LYP",
+               precision: str = "mixed", isolated: int = 0, refine: int | None = 3, progress=None) -> dict:
+    """The serving workload end to end: ``n`` molecules of the seeded stream,
+    the isolated-solve time of the first ``isolated`` of them (0: all; after an
+    untimed pass over the same requests, so the autotuner has seen their
+    shapes), then Poisson arrivals at each rate x the isolated throughput.
+    Returns the isolated times and, per rate, ``latency_stats`` with the
+    per-request slowdown over the isolated sample. ``progress(msg)`` is called
+    as the phases advance."""
+    import torch
+    from dftax_amd.system import serving_stream
+    say = progress or (lambda msg: None)
+    mols = serving_stream(n, seed=seed)
+    srv = Server(xc, precision=precision, slots=slots, refine=refine or None)
+    n_iso = len(mols) if isolated <= 0 else min(isolated, len(mols))
+    warm = []
+    for k in range(n_iso):
+        t = time.perf_counter()
+        srv.solve(Request(rid=-1, mol=mols[k]))
+        warm.append(time.perf_counter() - t)
+    say(f"untimed first pass over {n_iso} requests: {sum(warm):.1f} s")
+    alone, iso_req = {}, []
+    for k in range(n_iso):
+        t = time.perf_counter()
+        r = srv.solve(Request(rid=k, mol=mols[k]))
+        alone[k] = time.perf_counter() - t
+        iso_req.append(r)
+        if r.error:
+            raise RuntimeError(r.error)
+    iso = list(alone.values())
+    tau = float(np.mean(iso))
+    say(f"isolated solves: {n_iso}, mean {tau:.3f} s")
+    out = {"isolated_solve_s_mean": tau, "isolated_solve_s_p99": float(np.percentile(iso, 99)),
+           "isolated_build_s_mean": float(np.mean([r.extra["build_s"] for r in iso_req])),
+           "isolated_scf_s_mean": float(np.mean([r.extra["scf_s"] for r in iso_req])),
+           "isolated_requests": n_iso,
+           # per request, in request order: the untimed first solve, the isolated solve and its build part
+           "isolated_first_pass_s": [round(x, 4) for x in warm],
+           "isolated_s": [round(alone[k], 4) for k in range(n_iso)],
+           "isolated_build_s": [round(r.extra["build_s"], 4) for r in iso_req],
+           "isolated_build_lib_ms": [r.extra["build_lib_ms"] for r in iso_req]}
+    rng = np.random.default_rng(seed + 1)
+    for rate in rates:
+        lam = rate / tau
+        gaps = rng.exponential(1.0 / lam, size=len(mols))
+        arr = np.cumsum(gaps) - gaps[0]
+        reqs = [Request(rid=k, mol=m, arrival=float(a)) for k, (m, a) in enumerate(zip(mols, arr))]
+        srv.run(reqs, progress=progress)
+        st = latency_stats(reqs, alone)
+        st["p99_over_isolated"] = st["p99_s"] / tau if st["p99_s"] else None
+        st["p99_over_isolated_p99"] = st["p99_s"] / out["isolated_solve_s_p99"] if st["p99_s"] else None
+        out[f"rate_{rate:g}x"] = st
+        say(f"rate {rate:g}x: {st['solves_per_min']:.0f} solves/min, p99 {st['p99_s']:.2f} s")
+    out.update({"n_requests": len(mols), "xc": xc, "basis": "def2-svp / even-tempered-jk",
+                "precision": precision, "refine_fp64_steps": srv.refine, "slots": slots,
+                "budget_GB": srv.budget / 1e9, "device": torch.cuda.get_device_name(0)})
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=64)
@@ -237,53 +300,9 @@ def main() -> int:
     args = ap.parse_args()
     # one hardware queue per worker stream (HIP's default is 4), set before the runtime starts
     os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(max(4, args.slots), 16)))
-    import torch
-    from dftax_amd.system import serving_stream
-    mols = serving_stream(args.n, seed=args.seed)
-    srv = Server(args.xc, precision=args.precision, slots=args.slots, refine=args.refine or None)
-    # isolated-solve time of every request, one at a time, after an untimed pass over the
-    # same requests: the first product of each new shape is timed by the library's
-    # autotuner (every variant runs once), which the served requests never pay again
-    n_iso = len(mols) if args.isolated <= 0 else min(args.isolated, len(mols))
-    warm = []
-    for k in range(n_iso):
-        t = time.perf_counter()
-        srv.solve(Request(rid=-1, mol=mols[k]))
-        warm.append(time.perf_counter() - t)
-    alone, iso_req = {}, []
-    for k in range(n_iso):
-        t = time.perf_counter()
-        r = srv.solve(Request(rid=k, mol=mols[k]))
-        alone[k] = time.perf_counter() - t
-        iso_req.append(r)
-        if r.error:
-            raise RuntimeError(r.error)
-    iso = list(alone.values())
-    tau = float(np.mean(iso))
-    out = {"isolated_solve_s_mean": tau, "isolated_solve_s_p99": float(np.percentile(iso, 99)),
-           "isolated_build_s_mean": float(np.mean([r.extra["build_s"] for r in iso_req])),
-           "isolated_scf_s_mean": float(np.mean([r.extra["scf_s"] for r in iso_req])),
-           "isolated_requests": n_iso,
-           # per request, in request order: the untimed first solve, the isolated solve and its build part
-           "isolated_first_pass_s": [round(x, 4) for x in warm],
-           "isolated_s": [round(alone[k], 4) for k in range(n_iso)],
-           "isolated_build_s": [round(r.extra["build_s"], 4) for r in iso_req],
-           "isolated_build_lib_ms": [r.extra["build_lib_ms"] for r in iso_req]}
-    rng = np.random.default_rng(args.seed + 1)
-    for rate in args.rate:
-        lam = rate / tau
-        gaps = rng.exponential(1.0 / lam, size=len(mols))
-        arr = np.cumsum(gaps) - gaps[0]
-        reqs = [Request(rid=k, mol=m, arrival=float(a)) for k, (m, a) in enumerate(zip(mols, arr))]
-        srv.run(reqs)
-        st = latency_stats(reqs, alone)
-        st["p99_over_isolated"] = st["p99_s"] / tau if st["p99_s"] else None
-        st["p99_over_isolated_p99"] = st["p99_s"] / out["isolated_solve_s_p99"] if st["p99_s"] else None
-        out[f"rate_{rate:g}x"] = st
-    out.update({"n_requests": len(mols), "xc": args.xc, "basis": "def2-svp / even-tempered-jk",
-                "precision": args.precision, "refine_fp64_steps": srv.refine, "slots": args.slots,
-                "budget_GB": srv.budget / 1e9,
-                "device": torch.cuda.get_device_name(0)})
+    out = run_stream(args.n, args.rate, seed=args.seed, slots=args.slots, xc=args.xc, precision=args.precision,
+                     isolated=args.isolated, refine=args.refine,
+                     progress=lambda msg: print(f"serve: {msg}", file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
     return 0
 

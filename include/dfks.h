@@ -23,6 +23,8 @@
  *   dfks_get_matrix              KS.S / KS.hcore and the integral tensors (energy.py:369-436)
  *   dfks_overlap                 overlap_matrix / cross_overlap_matrix as used by the minao
  *                                guess (dftax/ks/guess.py:319-337)
+ *   dfks_forces                  forces(mol, xc, result) = -jax.grad of the frozen-density
+ *                                energy (dftax/ks/forces.py:128-270)
  */
 #ifndef DFKS_H
 #define DFKS_H
@@ -222,6 +224,24 @@ int dfks_overlap(const dfks_basis_desc* basis, double* host_out, dfks_stream str
  * Runs on the stream and synchronises it (only it). */
 int dfks_becke_weights(int64_t n, const double* pts, const double* raw_w, const int32_t* owner, int32_t natom,
                        const double* atom_xyz, const double* bragg, double* w_out, dfks_stream stream);
+
+/* Nuclear gradient dE/dR (Ha/Bohr) at a frozen converged density, the quantity
+ * dftax's forces() differentiates (dftax/ks/forces.py:128-270: the basis and
+ * auxiliary centres and the Becke grid follow their atoms, P = 2 Z (Z^T S Z)^-1 Z^T
+ * with Z the occupied orbitals held fixed). C_occ: device, column-major
+ * nao x nocc occupied orbitals of the converged SCF (nocc = the handle's).
+ * The grid is the unpruned-by-weight atom-centred grid of the energy's
+ * Becke spec (host arrays: points [3 ngrid] Bohr, raw radial x angular
+ * weights [ngrid], owner atom [ngrid]) and the atoms' Bragg radii [natom];
+ * the library forms the Becke weights and their derivatives itself.
+ * grad_parts: host [5][natom][3], the terms of dE/dR: nuclear repulsion,
+ * one-electron, Coulomb, exchange, exchange-correlation, each with its share
+ * of the density's response to the overlap (-Tr(W_k dS), W_k = 1/2 P F_k P);
+ * their sum is dE/dR and forces are its negative. Needs the materialised
+ * fitted tensor (options.aux_chunk = 0). Synchronises the stream. */
+int dfks_forces(dfks_handle h, const double* C_occ, int32_t nocc, int64_t ngrid, const double* grid_xyz,
+                const double* grid_raw_w, const int32_t* grid_owner, const double* bragg, double* grad_parts,
+                dfks_stream stream);
 
 #ifdef __cplusplus
 }

@@ -482,10 +482,12 @@ HF_COEFF = {"PBE0": 0.25, "B3LYP": 0.20, "PBE": 0.0, "LDA": 0.0}
 EXCHANGE_PREFACTOR = -0.25
 
 
-def xc_energy_t(name, rho, grad, w, thresh=1e-10):
-    """energy/grid.py:10-44: nan-safe masked quadrature."""
+def xc_energy_t(name, rho, grad, w, thresh=1e-10, mask=None):
+    """energy/grid.py:10-44: nan-safe masked quadrature (``mask``: the points
+    that count, rho > thresh unless given)."""
     t = _torch()
-    mask = rho > thresh
+    if mask is None:
+        mask = rho > thresh
     safe_rho = t.where(mask, rho, t.ones_like(rho))
     safe_grad = t.where(mask[:, None], grad, t.zeros_like(grad))
     eps = eps_xc(name, safe_rho, safe_grad)
@@ -546,7 +548,7 @@ class OracleKS:
                 eK = EXCHANGE_PREFACTOR * self.hf * t.sum(P * Kmat)
         rho = t.einsum("gm,mn,gn->g", ao, P, ao)
         grad = 2.0 * t.einsum("gm,mn,gnx->gx", ao, P, dao)
-        exc = xc_energy_t(self.xc, rho, grad, w)
+        exc = xc_energy_t(self.xc, rho, grad, w, mask=getattr(self, "xc_mask", None))
         return e1 + eJ + eK + exc + self.e_nn, dict(e1=e1, eJ=eJ, eK=eK, exc=exc)
 
     def energy_fock_autograd(self, P):
@@ -792,3 +794,62 @@ def cpu_iteration_estimate(nao, nfit, nocc, ngrid, *, q_slice=96, g_slice=40000,
     np.linalg.eigh(F)
     out["eigh"] = time.perf_counter() - t0
     return sum(out.values()), out
+
+
+# ---------------------------------------------------------------------------
+# Nuclear forces by central differences of E(R) at the frozen density
+# (dftax/ks/forces.py:128-270)
+# ---------------------------------------------------------------------------
+
+def frozen_density_energy(mol, xc, Z, coords, grid, auxbasis="even-tempered-jk", parts_out=None, xc_mask=None):
+    """E at nuclear coordinates ``coords`` with the occupied coefficients Z
+    frozen: the basis and auxiliary centres follow their atoms, the Becke grid
+    is rebuilt (no weight cutoff, as forces.py:245-268 rebuild it), and
+    P = 2 Z (Z^T S(R) Z)^-1 Z^T (forces.py:80-91, 265-268). ``xc_mask`` fixes
+    which grid points the XC quadrature counts (by index; default rho > 1e-10
+    at these coordinates): the mask is a step function of the geometry, so its
+    derivative is zero wherever it is defined and the analytic gradient holds
+    it fixed; a difference quotient has to as well, or a point whose density
+    crosses 1e-10 between the two geometries adds a jump of up to ~1e-9 Ha."""
+    from dataclasses import replace
+    from dftax_amd.system import Molecule
+    t = _torch()
+    m = Molecule(mol.symbols, coords, mol.basis, spherical=mol.spherical)
+    oks = OracleKS(m, xc, auxbasis=auxbasis, grid=replace(grid, cutoff=None))
+    oks.xc_mask = None if xc_mask is None else _torch().as_tensor(xc_mask)
+    M = Z.T @ oks.S @ Z
+    P = 2.0 * Z @ np.linalg.solve(M, Z.T)
+    e, parts = oks.energy_t(t.tensor(P))
+    if parts_out is not None:
+        parts_out.update({k: float(v) for k, v in parts.items()}, enn=oks.e_nn)
+        rho = np.einsum("gm,mn,gn->g", oks.ao, P, oks.ao, optimize=True)
+        parts_out["mask"] = rho > 1e-10
+    return float(e)
+
+
+FORCE_PARTS = ("enn", "e1", "eJ", "eK", "exc")
+
+
+def fd_forces(mol, xc, Z, grid, h=1e-4, auxbasis="even-tempered-jk", atoms=None, parts=False):
+    """F = -dE/dR by central differences of frozen_density_energy, (natom, 3)
+    (rows of atoms not in ``atoms`` are left at zero). With ``parts`` also the
+    same differences of each energy term (FORCE_PARTS), shape (5, natom, 3):
+    each term's share of the total, density response included."""
+    F = np.zeros((len(mol.symbols), 3))
+    Fp = np.zeros((len(FORCE_PARTS), len(mol.symbols), 3))
+    p0 = {}
+    frozen_density_energy(mol, xc, Z, mol.coords, grid, auxbasis, p0)
+    mask = p0["mask"]           # the XC mask of the reference geometry, held for every displacement
+    for a in (range(len(mol.symbols)) if atoms is None else atoms):
+        for x in range(3):
+            cp = mol.coords.copy()
+            cm = mol.coords.copy()
+            cp[a, x] += h
+            cm[a, x] -= h
+            pp, pm = {}, {}
+            ep = frozen_density_energy(mol, xc, Z, cp, grid, auxbasis, pp, mask)
+            em = frozen_density_energy(mol, xc, Z, cm, grid, auxbasis, pm, mask)
+            F[a, x] = -(ep - em) / (2 * h)
+            for k, name in enumerate(FORCE_PARTS):
+                Fp[k, a, x] = -(pp[name] - pm[name]) / (2 * h)
+    return (F, Fp) if parts else F

@@ -8,9 +8,8 @@
   pairs against the oracle's (mn|P) V+ (P|kl), the metric against the oracle,
   and the FP64 / mixed SCFs: idempotency, electron count, commutator below
   d_tol, mixed refined within 1e-6 Ha of FP64.
-- (H2O)_64 def2-TZVP PBE0, auxiliary slabs recomputed every iteration with
-  Schwarz screening 1e-12: overlap, core Hamiltonian and the (P|Q) metric
-  against the oracle in full, and the same SCF properties.
+- (H2O)_64: tests/test_gpu_configs_h2o64.py (a module of its own, so the
+  handles here are gone before it allocates).
 - The serving stream (def2-SVP B3LYP molecules of 2-24 heavy atoms, default
   grid): every energy the same alone and in the mix.
 Every system is built from the same seeded generators as the benchmark.
@@ -35,6 +34,17 @@ def torch_cuda():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch
+
+
+def _release(ks):
+    """Destroy the handle and return the library's kept buffers and torch's cache,
+    so the next system of the session starts from an empty device."""
+    import torch
+    from dftax_amd import _native as N
+    ks.coulomb.close()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    N.check(N.lib().dfks_release_memory(N.current_stream()))
 
 
 def _p0(ks):
@@ -72,7 +82,8 @@ def h2o8(torch_cuda):
     _say("case: (H2O)_8 def2-TZVP PBE0, default grid, against the oracle's golden vectors")
     assert np.abs(mol.coords - g["coords"]).max() < 1e-12   # the generator is the one the vectors came from
     ks = KS(mol, "PBE0", coulomb=df(engine=hip(keep_int3c=True)))
-    return ks, g
+    yield ks, g
+    _release(ks)
 
 
 def test_h2o8_integrals(h2o8, torch_cuda):
@@ -121,7 +132,8 @@ def h2o32(torch_cuda):
     _say("case: (H2O)_32 def2-TZVP PBE0, B resident, FP64 and mixed")
     mol = water_cluster((4, 4, 2), seed=0, basis="def2-tzvp")
     ks = KS(mol, "PBE0", coulomb=df(chunk=None, engine=hip(precision="mixed")))
-    return ks, {}
+    yield ks, {}
+    _release(ks)
 
 
 def _shell_of(basis):
@@ -196,78 +208,29 @@ def test_h2o32_mixed_scf(h2o32):
 
 
 # ---------------------------------------------------------------------------
-# (H2O)_64 def2-TZVP PBE0: auxiliary slabs recomputed every iteration
-# ---------------------------------------------------------------------------
-
-@pytest.fixture(scope="module")
-def h2o64(torch_cuda):
-    from dftax_amd import KS, df, hip
-    from dftax_amd.system import water_cluster
-    _say("case: (H2O)_64 def2-TZVP PBE0, recomputed slabs, Schwarz 1e-12")
-    mol = water_cluster((4, 4, 4), seed=0, basis="def2-tzvp")
-    ks = KS(mol, "PBE0", coulomb=df(screen=1e-12, engine=hip(precision="mixed")))
-    return ks, {}
-
-
-def test_h2o64_recompute_build(h2o64):
-    from dftax_amd import _native as N
-    from oracle import ref
-    ks, _ = h2o64
-    st = ks.coulomb.stats()
-    nao, naux = ks.nao, ks.coulomb.naux
-    _say(f"(H2O)_64 plan: {st}")
-    assert st["aux_slabs"] > 1 and st["pairs_kept"] < st["pairs_total"]
-    S_ref, T_ref, V_ref = ref.int1e(ks.basis, ks.mol.coords, ks.mol.atom_charges())
-    dS = np.abs(ks.S.cpu().numpy() - S_ref).max()
-    dH = np.abs(ks.hcore.cpu().numpy() - (T_ref + V_ref)).max()
-    dM = np.abs(ks.coulomb.matrix(N.DFKS_INT2C, (naux, naux)).cpu().numpy() - ref.int2c(ks.aux_basis)).max()
-    _say(f"(H2O)_64 S max |d| {dS:.2e}, hcore {dH:.2e}, (P|Q) {dM:.2e}")
-    assert dS < 1e-10 and dH < 1e-10 and dM < 1e-10
-
-
-def test_h2o64_fp64_scf(h2o64):
-    from dftax_amd import scf
-    ks, res = h2o64
-    r = scf(ks, precision="fp64")
-    _scf_properties(ks, r, "(H2O)_64 FP64 SCF")
-    res["fp64"] = r.e_tot
-
-
-def test_h2o64_mixed_refined(h2o64):
-    """Mixed phase ended at an orbital gradient of 1e-3, then FP64 energy
-    refinement: within the 1e-6 Ha budget of the FP64 SCF."""
-    from dftax_amd import scf
-    ks, res = h2o64
-    e64 = res.get("fp64")
-    if e64 is None:
-        pytest.skip("needs the FP64 SCF of the test before")
-    rr = scf(ks, precision="mixed", refine=8, mixed_d_tol=1e-3)
-    nmix = sum(1 for h in rr.history if h[3] == "mixed")
-    _say(f"(H2O)_64 mixed ({nmix} steps) + FP64 refinement: {rr.e_tot:.10f} ({rr.n_iter} it), "
-         f"|dE| {abs(rr.e_tot - e64):.1e}")
-    assert nmix >= 3 and rr.energy_converged and abs(rr.e_tot - e64) < 1e-6
-
-
-# ---------------------------------------------------------------------------
 # Serving stream: the same energy alone and in the mix
 # ---------------------------------------------------------------------------
 
 def test_serving_stream_alone_equals_mix(torch_cuda):
-    """12 molecules of the serving stream (seed 0: 2-24 heavy atoms, def2-SVP,
+    """8 molecules of the serving stream (seed 0: 2-24 heavy atoms, def2-SVP,
     B3LYP, default grid, mixed precision with FP64 energy refinement), solved
     one at a time and then arriving together on 8 slots."""
+    import time
     from dftax_amd.serve import Request, Server, latency_stats
     from dftax_amd.system import serving_stream
-    mols = serving_stream(12, seed=0)
+    mols = serving_stream(8, seed=0)
     srv = Server("B3LYP", precision="mixed", slots=8)
     alone = []
     for i, m in enumerate(mols):
+        t = time.perf_counter()
         alone.append(srv.solve(Request(rid=i, mol=m)))
+        _say(f"alone {i}: {len(m.symbols)} atoms, {alone[-1].extra.get('nao')} functions, "
+             f"{time.perf_counter() - t:.2f} s")
         assert not alone[-1].error, alone[-1].error
     reqs = [Request(rid=i, mol=m, arrival=0.0) for i, m in enumerate(mols)]
     srv.run(reqs)
     st = latency_stats(reqs)
-    _say(f"serving 12 molecules ({min(len(m.symbols) for m in mols)}-{max(len(m.symbols) for m in mols)} atoms): "
+    _say(f"serving {len(mols)} molecules ({min(len(m.symbols) for m in mols)}-{max(len(m.symbols) for m in mols)} atoms): "
          f"{st['solves_per_min']:.0f} solves/min in the mix")
     for a, r in zip(alone, reqs):
         assert not r.error, r.error

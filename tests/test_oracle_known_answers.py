@@ -419,3 +419,27 @@ def test_product_minao_equals_oracle():
         P_ref = ref.minao_density(m, b)
         print(f"minao {m.basis} {len(m.symbols)} atoms: max |dP| {np.abs(P - P_ref).max():.2e}", flush=True)
         assert np.abs(P - P_ref).max() < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# Golden forces (tests/golden/make_forces.py)
+# ---------------------------------------------------------------------------
+
+def test_golden_forces_are_consistent():
+    """The oracle's frozen-density differences: the terms add up to the total,
+    the total sums to zero over the atoms (the energy is invariant under a
+    rigid translation of nuclei, basis and grid), and the nuclear term equals
+    its closed form."""
+    from pathlib import Path
+    g = np.load(Path(__file__).resolve().parent / "golden" / "forces_oracle.npz")
+    for case, zs in (("water_svp_pbe0", [8, 1, 1]), ("dimer_tzvp_b3lyp", [8, 1, 1, 8, 1, 1])):
+        F, Fp, R = g[f"{case}_forces"], g[f"{case}_parts"], g[f"{case}_coords"]
+        assert np.abs(Fp.sum(0) - F).max() < 1e-9
+        assert np.abs(F.sum(0)).max() < 1e-7
+        Fnn = np.zeros_like(F)
+        for a in range(len(zs)):
+            for b in range(len(zs)):
+                if a != b:
+                    d = R[a] - R[b]
+                    Fnn[a] += zs[a] * zs[b] * d / np.linalg.norm(d) ** 3
+        assert np.abs(Fp[0] - Fnn).max() < 1e-8
