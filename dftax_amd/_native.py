@@ -44,7 +44,7 @@ EXPORTS = [
     "dfks_energy_fock", "dfks_energy_fock_factored", "dfks_xc_eval", "dfks_scf_begin", "dfks_scf_step",
     "dfks_scf_result", "dfks_becke_weights", "dfks_overlap", "dfks_set_timing", "dfks_stage_times",
     "dfks_set_precision", "dfks_stats", "dfks_xc_eval2", "dfks_response_begin", "dfks_response_fock",
-    "dfks_device_free", "dfks_release_memory", "dfks_forces",
+    "dfks_device_free", "dfks_release_memory", "dfks_becke_weights_noalloc", "dfks_forces",
 ]
 
 STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
@@ -94,6 +94,8 @@ def lib():
     L.dfks_device_free.argtypes = [ctypes.POINTER(c_int64)]
     L.dfks_release_memory.argtypes = [c_void_p]
     L.dfks_forces.argtypes = [c_void_p, c_void_p, c_int32, c_int64, P_f64, P_f64, P_i32, P_f64, P_f64, c_void_p]
+    L.dfks_becke_weights_noalloc.argtypes = [c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
+                                             c_void_p, c_void_p]
     _lib = L
     return L
 

@@ -2275,6 +2275,59 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // extern "C"
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Allocation-free Becke weights: the pair tables of the partition live in a
+// file-scope device array instead of buffers the call allocates.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxAtoms = 1024;
+// 1/R_AB and the size-adjustment a_AB, natom^2 each (16 MiB at the maximum)
+constexpr size_t kScratchDoubles = 2 * (size_t)kNoAllocMaxAtoms * kNoAllocMaxAtoms;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_addr;  // device -> address of that device's copy of scratch
+
+// The device address of the current device's scratch array, looked up once per
+// device with that device current (each device holds its own copy).
+double* scratch_address() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_addr.find(dev);
+  if (it != g_scratch_addr.end()) return it->second;
+  double* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void**>(&ptr), HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) fail(2, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_addr[dev] = ptr;
+  return ptr;
+}
+
+// Pair tables of the Becke partition (becke.py:88-95), one thread per (A, B):
+// invR = 1 / R_AB (1 on the diagonal), aAB = clip(u / (u^2 - 1), -0.45, 0.45)
+// with u = (chi - 1) / (chi + 1), chi = R_bragg(A) / R_bragg(B).
+// Contraction is off so every product and sum rounds as in dfks_becke_weights'
+// host loop (x86-64 without FMA): the tables, and so the weights, come out bit
+// for bit the same. A fused d*d + d2 moves 1/R_AB by an ulp for some pairs, and
+// where a cell function s = (1 - f3) / 2 is tiny its cancellation turns that ulp
+// into a relative change of ~1e-8 in the weight.
+__global__ __launch_bounds__(256) void k_becke_pairs(int natm, const double* __restrict__ atm,
+                                                     const double* __restrict__ bragg, double* __restrict__ invR,
+                                                     double* __restrict__ aAB) {
+#pragma clang fp contract(off)
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)natm * natm) return;
+  const int a = (int)(i / natm), b = (int)(i % natm);
+  double d2 = 0.0;
+  for (int x = 0; x < 3; ++x) {
+    const double d = atm[3 * a + x] - atm[3 * b + x];
+    d2 += d * d;
+  }
+  invR[i] = 1.0 / sqrt(d2 + (a == b ? 1.0 : 0.0));
+  const double chi = bragg[a] / bragg[b];
+  const double u = (chi - 1.0) / (chi + 1.0);
+  aAB[i] = fmin(0.45, fmax(-0.45, u / (u * u - 1.0)));
+}
+
 extern "C" {
 
 const char* dfks_last_error(void) { return g_err.c_str(); }
@@ -2682,6 +2735,27 @@ int dfks_becke_weights(int64_t n, const double* pts, const double* raw_w, const 
     HIPCHK(launch_becke(n, d_pts, d_raw, d_own, natom, d_atm, d_invR, d_a, d_w, st));
     HIPCHK(hipMemcpyAsync(w_out, d_w, n * sizeof(double), hipMemcpyDeviceToHost, st));
     sync(st);
+  });
+}
+
+int dfks_becke_weights_noalloc(int64_t n, const double* pts, const double* raw_w, const int32_t* owner,
+                               int32_t natom, const double* atom_xyz, const double* bragg, double* w_out,
+                               dfks_stream stream) {
+  return guard([&] {
+    if (n < 0 || natom <= 0) fail(1, "bad sizes");
+    if (natom > kNoAllocMaxAtoms)
+      fail(1, "dfks_becke_weights_noalloc takes at most " + std::to_string(kNoAllocMaxAtoms) + " atoms (got " +
+                  std::to_string(natom) + "); use dfks_becke_weights");
+    if (n == 0) return;
+    if (!pts || !raw_w || !owner || !atom_xyz || !bragg || !w_out) fail(1, "null argument");
+    hipStream_t st = S_(stream);
+    double* invR = scratch_address();
+    double* aAB = invR + (size_t)natom * natom;
+    const int64_t np = (int64_t)natom * natom;
+    hipLaunchKernelGGL(k_becke_pairs, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, natom, atom_xyz, bragg,
+                       invR, aAB);
+    HIPCHK(hipGetLastError());
+    HIPCHK(launch_becke(n, pts, raw_w, owner, natom, atom_xyz, invR, aAB, w_out, st));
   });
 }
 

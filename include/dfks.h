@@ -243,6 +243,19 @@ int dfks_forces(dfks_handle h, const double* C_occ, int32_t nocc, int64_t ngrid,
                 const double* grid_raw_w, const int32_t* grid_owner, const double* bragg, double* grad_parts,
                 dfks_stream stream);
 
+/* Becke partition weights of device-resident points, with no device memory
+ * allocated by the call. No counterpart in the reference: this is
+ * dfks_becke_weights for callers that cannot allocate (code under HIP graph
+ * capture, serving loops that set up grids on the device). All pointers are
+ * device memory: pts [3n], raw_w [n], owner [n] (an owner outside 0..natom-1
+ * gets weight 0), atom_xyz [3 natom], bragg [natom], w_out [n]. The
+ * partition's pair tables go to a fixed device array of the library, 16 MiB
+ * per device, so natom is at most 1024 (status 1 above). Enqueues on the
+ * stream and returns without waiting. Calls that overlap on one device share
+ * that array and race: order them on one stream, or synchronise in between. */
+int dfks_becke_weights_noalloc(int64_t n, const double* pts, const double* raw_w, const int32_t* owner, int32_t natom,
+                               const double* atom_xyz, const double* bragg, double* w_out, dfks_stream stream);
+
 #ifdef __cplusplus
 }
 #endif
