@@ -84,6 +84,9 @@ hipError_t launch_diis_solve(int m, int count, const double* gram, double* coef,
 hipError_t launch_diis_extrap(int m, int64_t n, const double* coef, const double* Fs, double* out, hipStream_t st);
 hipError_t launch_gram_update(int m, int slot, int count, const double* dots, double* gram, hipStream_t st);
 hipError_t launch_scaled_cols(int n, int k0, const double* U, const double* w, double* X, hipStream_t st);
+hipError_t launch_xcf(int nb, int n, const float* ao, const int32_t* cols, const float* P, int64_t nao,
+                      const double* w, int xc_id, double thresh, double* exc, double* ws, float* q, hipStream_t st);
+int64_t xcf_ws_doubles(int nb, int n);
 hipError_t launch_xc_f32(int64_t ng, int64_t nao, const float* ao, const double* w, float* zw, int xc_id,
                          double thresh, double* exc, hipStream_t st);
 hipError_t launch_gamma_f32(int nao, int nq, int nf, const float* T, const double* Cf, const double* sgn,
@@ -1465,12 +1468,17 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
     const int64_t nb = bn.nblk, rows = nb * kXcG;
     const int32_t* cols = h->d_xc_cols + bn.col_off;
     if (mixed) {
-      // FP32 operands and FP32 MFMA products (rocBLAS strided batched): on these short
-      // per-block products (K = n or 128) they beat the split-BF16 kernel and are exact to FP32
+      // options.gemm = rocblas: FP32 operands and FP32 MFMA products (rocBLAS strided batched)
       const float onef = 1.0f, zerof = 0.0f;
       float* q = reinterpret_cast<float*>(h->d_xc_q) + bn.q_off;
       const float* ao = h->d_xc_ao32 + bn.ao_off;
       float* zw = reinterpret_cast<float*>(h->d_xc_zw);
+      if (h->opt.gemm != 1 && n <= kXfMaxN && xcf_ws_doubles((int)nb, n) <= h->xc_zw) {
+        // fused split-BF16 kernels: P gathered while staging, Z reduced to rho in LDS, W formed while staging
+        HIPCHK(launch_xcf((int)nb, n, ao, cols, h->d_P32, nao, h->d_xc_w + bn.pt0, h->xc_id, thresh,
+                          h->d_xc_exc + bn.pt0, h->d_xc_zw, q, st));
+        continue;
+      }
       HIPCHK((launch_gather_sym<float, float>(nb, n, cols, h->d_P32, nao, q, st)));
       RBCHK(rocblas_sgemm_strided_batched(h->rb, rocblas_operation_transpose, rocblas_operation_none, n, kXcG, n,
                                           &onef, q, n, (rocblas_stride)n * n, ao, n, (rocblas_stride)kXcG * n,

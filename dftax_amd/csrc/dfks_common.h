@@ -36,6 +36,8 @@ constexpr int kSp2Groups = 512;       // workgroups of the SP2 update (partial t
 constexpr int kSpPart = kSpHist + kSp2MaxIter;  // their partial sums: 2 kSp2Groups doubles
 constexpr int kSp2State = kSpPart + 2 * kSp2Groups;
 constexpr int kSp2Block = 4;          // purification iterations per graph launch (divides kSp2MaxIter)
+// widest compact-grid block (functions) the fused mixed-precision XC kernels take
+constexpr int kXfMaxN = 2048;
 constexpr int kPanel = 32;            // pivoted Cholesky panel width
 
 // Graph recording without stream capture. While a thread has a GraphRec

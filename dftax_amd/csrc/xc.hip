@@ -661,4 +661,361 @@ hipError_t launch_fxc(int64_t ng, int64_t nao, const double* ao, double* zw, con
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Fused compact-grid XC pass, mixed precision (split-BF16 MFMA, FP32
+// accumulate; reductions and the functional in FP64).
+//
+// A bin holds nb blocks of kXfG = 128 grid points; block b touches the n
+// functions cols[b][0..n) (n a multiple of 32, -1 marks padding) and its AO
+// planes are ao[k][b 128 + g][i] (k = value, x, y, z; plane = nb 128 n floats).
+//   k_xcf_rho: one workgroup per block. Z = Phi_b Pg_b in 128 x 128 tiles,
+//     Pg_b[c][i] = P[cols[c]][cols[i]] gathered while staging; each Z tile is
+//     reduced at once to the block's rho and grad rho (LDS), so Z is never
+//     stored; then the functional per point, writing the W coefficients
+//     {w f_rho / 2, 2 w f_sigma grad rho} and w f.
+//   k_xcf_v: one workgroup per (block, 128 x 128 tile of the n x n block):
+//     Vh_b[i][j] = sum_g Phi[g][i] W[g][j], W[g][j] = a_g phi_gj + b_g . grad phi_gj
+//     formed while staging, written as the row assembly reads it.
+// Operands are split into bf16 hi + lo when staged to LDS; every k-step is
+// three v_mfma_f32_16x16x32_bf16 products (hi*hi + hi*lo + lo*hi).
+// ---------------------------------------------------------------------------
+typedef float xflt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 xbf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 xbf16x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kXfG = 128;      // points per block
+constexpr int kXfPad = 40;     // bf16 per LDS row: 32 k values + 8 (16-byte reads stay conflict-free)
+
+// Split 8 floats into bf16 hi and lo parts and store them as 16-byte vectors.
+__device__ __forceinline__ void xf_split8(const float* x, __bf16* hi, __bf16* lo) {
+  xbf16x8 h, l;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const __bf16 xh = (__bf16)x[j];
+    h[j] = xh;
+    l[j] = (__bf16)(x[j] - (float)xh);
+  }
+  *reinterpret_cast<xbf16x8*>(hi) = h;
+  *reinterpret_cast<xbf16x8*>(lo) = l;
+}
+
+// One k-step of the 2 x 2 wavefront arrangement over a 128 x 128 tile: wave
+// (wm, wn) owns 64 x 64 = 4 x 4 MFMA blocks; A rows and B rows are LDS rows.
+__device__ __forceinline__ void xf_mfma_step(const __bf16 (*sAh)[kXfPad], const __bf16 (*sAl)[kXfPad],
+                                             const __bf16 (*sBh)[kXfPad], const __bf16 (*sBl)[kXfPad], int wm,
+                                             int wn, int li, int lk, xflt4 (&acc)[4][4]) {
+  xbf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    ah[i] = *reinterpret_cast<const xbf16x8*>(&sAh[wm + 16 * i + li][8 * lk]);
+    al[i] = *reinterpret_cast<const xbf16x8*>(&sAl[wm + 16 * i + li][8 * lk]);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    bh[j] = *reinterpret_cast<const xbf16x8*>(&sBh[wn + 16 * j + li][8 * lk]);
+    bl[j] = *reinterpret_cast<const xbf16x8*>(&sBl[wn + 16 * j + li][8 * lk]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+// Z tile rows staged through LDS for the rho reduction: 64 rows of 128 floats
+// (+4 so consecutive rows start in different banks), reusing the operand tiles' LDS.
+constexpr int kXfZPad = 132;
+static_assert(64 * kXfZPad * 4 <= 4 * 128 * kXfPad * 2, "Z half tile fits the staging LDS");
+
+// Partial rho sums of one (block, 128-column Z tile, K chunk): rho = sum_c phi_gc Z_gc
+// and Z = Phi Pg are linear in both, so the tiles and K chunks of a block run as
+// separate workgroups and their sums add up (k_xcf_pt, in a fixed order).
+// part[((b nt + t) nks + ks)][4][128]: sum_c phi Z, sum_c d_x phi Z, d_y, d_z.
+__global__ __launch_bounds__(256, 2) void k_xcf_z(int nb, int n, int nt, int nks, int kchunk,
+                                                  const float* __restrict__ ao, int64_t plane,
+                                                  const int32_t* __restrict__ cols, const float* __restrict__ P,
+                                                  int64_t nao, double* __restrict__ part) {
+  // staging tiles (A: points x 32 functions, B: 128 Z columns x 32 functions, bf16 hi/lo),
+  // reused after the K loop for half of the Z tile in FP32
+  __shared__ __attribute__((aligned(16))) __bf16 s_stage[4][128][kXfPad];
+  __shared__ int32_t s_col[kXfMaxN];
+  __shared__ double s_r[4][kXfG];
+  auto sAh = s_stage[0];
+  auto sAl = s_stage[1];
+  auto sBh = s_stage[2];
+  auto sBl = s_stage[3];
+  float (*sZ)[kXfZPad] = reinterpret_cast<float (*)[kXfZPad]>(&s_stage[0][0][0]);
+  const int per_blk = nt * nks, ntot = nb * per_blk;
+  // workgroup x runs on XCD x % 8: each XCD takes a contiguous range of ids, so the
+  // workgroups of one block share its AO rows and P's rows through one L2
+  const int per = (ntot + 7) / 8;
+  const int id = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (id >= ntot) return;
+  const int64_t blk = id / per_blk;
+  const int t = (id % per_blk) / nks, ks = id % nks;
+  const int c0 = 128 * t, kb = ks * kchunk, ke = min(n, kb + kchunk);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t row0 = blk * kXfG;        // first AO row of the block
+  for (int i = tid; i < n; i += 256) s_col[i] = cols[blk * n + i];
+  for (int i = tid; i < 4 * kXfG; i += 256) (&s_r[0][0])[i] = 0.0;
+  __syncthreads();
+  // staging: row sr of the A tile (point), 16 consecutive k from sk (two threads per row);
+  // row br of the B tile (Z column), 16 k from bk: a wavefront's lanes take 64
+  // consecutive columns of one row of the symmetric P, Pg[c][k] = P[cols[k]][cols[c]]
+  const int sr = tid >> 1, sk = (tid & 1) * 16;
+  const int br = tid & 127, bk = (tid >> 7) * 16;
+  const float* arow = ao + (row0 + sr) * n;
+  // the block's rows of the four planes (uniform), read with 32-bit offsets
+  const float* q0 = ao + row0 * n;
+  const float* q1 = q0 + plane;
+  const float* q2 = q0 + 2 * plane;
+  const float* q3 = q0 + 3 * plane;
+  xflt4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = xflt4{0.f, 0.f, 0.f, 0.f};
+  const int32_t pc = c0 + br < n ? s_col[c0 + br] : -1;
+  float a[16], b[16];
+  auto load = [&](int k0) {
+    const float4* src = reinterpret_cast<const float4*>(arow + k0 + sk);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float4 x = src[v];
+      a[4 * v] = x.x; a[4 * v + 1] = x.y; a[4 * v + 2] = x.z; a[4 * v + 3] = x.w;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int32_t kc = s_col[k0 + bk + j];
+      b[j] = (pc >= 0 && kc >= 0) ? P[(int64_t)kc * nao + pc] : 0.f;
+    }
+  };
+  load(kb);
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+    __syncthreads();
+    xf_split8(a, &sAh[sr][sk], &sAl[sr][sk]);
+    xf_split8(a + 8, &sAh[sr][sk + 8], &sAl[sr][sk + 8]);
+    xf_split8(b, &sBh[br][bk], &sBl[br][bk]);
+    xf_split8(b + 8, &sBh[br][bk + 8], &sBl[br][bk + 8]);
+    __syncthreads();
+    if (k0 + 32 < ke) load(k0 + 32);     // next k-step's operands while this one computes
+    xf_mfma_step(sAh, sAl, sBh, sBl, wm, wn, li, lk, acc);
+  }
+  // acc[i][j][r] = Z(point wm + 16 i + 4 lk + r, column c0 + wn + 16 j + li). Each half of
+  // the tile's points goes to LDS, then four threads per point reduce it against the four
+  // AO planes (float4 reads along the row), one writer per point, in a fixed order.
+#pragma unroll 1
+  for (int hf = 0; hf < 2; ++hf) {
+    __syncthreads();
+    if (wm == 64 * hf) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sZ[16 * i + 4 * lk + r][wn + 16 * j + li] = acc[i][j][r];
+    }
+    __syncthreads();
+    const int g = 64 * hf + (tid >> 2), pq = tid & 3;
+    const int cc = c0 + 32 * pq;               // 32 columns: all in range or all out (n % 32 == 0)
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (cc < n) {
+      const int o = g * n + cc;
+      const float4* z4 = reinterpret_cast<const float4*>(&sZ[tid >> 2][32 * pq]);
+      const float4* v0 = reinterpret_cast<const float4*>(q0 + o);
+      const float4* v1 = reinterpret_cast<const float4*>(q1 + o);
+      const float4* v2 = reinterpret_cast<const float4*>(q2 + o);
+      const float4* v3 = reinterpret_cast<const float4*>(q3 + o);
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const float4 z = z4[v], x0 = v0[v], x1 = v1[v], x2 = v2[v], x3 = v3[v];
+        s0 += (double)z.x * x0.x + (double)z.y * x0.y + (double)z.z * x0.z + (double)z.w * x0.w;
+        s1 += (double)z.x * x1.x + (double)z.y * x1.y + (double)z.z * x1.z + (double)z.w * x1.w;
+        s2 += (double)z.x * x2.x + (double)z.y * x2.y + (double)z.z * x2.z + (double)z.w * x2.w;
+        s3 += (double)z.x * x3.x + (double)z.y * x3.y + (double)z.z * x3.z + (double)z.w * x3.w;
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < 4; off <<= 1) {
+      s0 += __shfl_xor(s0, off, 64);
+      s1 += __shfl_xor(s1, off, 64);
+      s2 += __shfl_xor(s2, off, 64);
+      s3 += __shfl_xor(s3, off, 64);
+    }
+    if (pq == 0) {
+      s_r[0][g] = s0;
+      s_r[1][g] = s1;
+      s_r[2][g] = s2;
+      s_r[3][g] = s3;
+    }
+  }
+  __syncthreads();
+  double* out = part + (int64_t)id * 4 * kXfG;
+  for (int i = tid; i < 4 * kXfG; i += 256) out[i] = (&s_r[0][0])[i];
+}
+
+// Per point: the partial sums of its block in (tile, K chunk) order, then the
+// functional: exc = w f, coef = {w f_rho / 2, 2 w f_sigma grad rho}.
+__global__ __launch_bounds__(128) void k_xcf_pt(int nb, int nparts, const double* __restrict__ part,
+                                                const double* __restrict__ wts, int xc_id, double thresh,
+                                                double* __restrict__ exc, float* __restrict__ coef) {
+  const int64_t gg = blockIdx.x * (int64_t)kXfG + threadIdx.x;   // one workgroup per block
+  if (blockIdx.x >= (unsigned)nb) return;
+  const double* src = part + (int64_t)blockIdx.x * nparts * 4 * kXfG + threadIdx.x;
+  double r = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int pp = 0; pp < nparts; ++pp) {
+    const double* s = src + (int64_t)pp * 4 * kXfG;
+    r += s[0];
+    gx += s[kXfG];
+    gy += s[2 * kXfG];
+    gz += s[3 * kXfG];
+  }
+  gx *= 2.0; gy *= 2.0; gz *= 2.0;
+  double a = 0.0, bx = 0.0, by = 0.0, bz = 0.0, e = 0.0;
+  if (r > thresh) {
+    const double w = wts[gg];
+    D rho{r, 1.0, 0.0};
+    D sig{gx * gx + gy * gy + gz * gz, 0.0, 1.0};
+    D f = eps_xc(xc_id, rho, sig) * rho;
+    e = w * f.v;
+    a = 0.5 * w * f.r;
+    const double tt = 2.0 * w * f.s;
+    bx = tt * gx; by = tt * gy; bz = tt * gz;
+  }
+  exc[gg] = e;
+  reinterpret_cast<float4*>(coef)[gg] = make_float4((float)a, (float)bx, (float)by, (float)bz);
+}
+
+__global__ __launch_bounds__(256, 2) void k_xcf_v(int nb, int n, const float* __restrict__ ao, int64_t plane,
+                                                  const float* __restrict__ coef, float* __restrict__ q) {
+  __shared__ __attribute__((aligned(16))) __bf16 sAh[128][kXfPad];
+  __shared__ __attribute__((aligned(16))) __bf16 sAl[128][kXfPad];
+  __shared__ __attribute__((aligned(16))) __bf16 sBh[128][kXfPad];
+  __shared__ __attribute__((aligned(16))) __bf16 sBl[128][kXfPad];
+  const int nt = (n + 127) / 128, per_blk = nt * nt;
+  const int ntot = nb * per_blk;
+  // workgroup x runs on XCD x % 8: each XCD takes a contiguous range of (block, tile)
+  // ids, so the tiles of one block read its AO rows through one L2
+  const int per = (ntot + 7) / 8;
+  const int id = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (id >= ntot) return;
+  const int64_t blk = id / per_blk;
+  const int t = id % per_blk, m0 = (t / nt) * 128, n0 = (t % nt) * 128;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t row0 = blk * kXfG;
+  // staging: 4 consecutive functions (one float4) x 4 points per thread; 32 threads
+  // span the tile's 128 functions, 8 groups its 32 points
+  const int cg = 4 * (tid & 31), pg = 4 * (tid >> 5);
+  const bool a_in = m0 + cg < n, b_in = n0 + cg < n;   // 4-column groups lie wholly inside or outside (n % 32 == 0)
+  const float* p0 = ao + row0 * n;
+  const float* p1 = p0 + plane;
+  const float* p2 = p0 + 2 * plane;
+  const float* p3 = p0 + 3 * plane;
+  const float4* cb = reinterpret_cast<const float4*>(coef) + row0;
+  xflt4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = xflt4{0.f, 0.f, 0.f, 0.f};
+  float4 a[4], b[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int g = k0 + pg + p;
+      const int o = g * n;
+      a[p] = a_in ? *reinterpret_cast<const float4*>(p0 + o + m0 + cg) : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (b_in) {
+        const float4 cf = cb[g];
+        const int oc = o + n0 + cg;
+        const float4 x0 = *reinterpret_cast<const float4*>(p0 + oc);
+        const float4 x1 = *reinterpret_cast<const float4*>(p1 + oc);
+        const float4 x2 = *reinterpret_cast<const float4*>(p2 + oc);
+        const float4 x3 = *reinterpret_cast<const float4*>(p3 + oc);
+        b[p] = make_float4(cf.x * x0.x + cf.y * x1.x + cf.z * x2.x + cf.w * x3.x,
+                           cf.x * x0.y + cf.y * x1.y + cf.z * x2.y + cf.w * x3.y,
+                           cf.x * x0.z + cf.y * x1.z + cf.z * x2.z + cf.w * x3.z,
+                           cf.x * x0.w + cf.y * x1.w + cf.z * x2.w + cf.w * x3.w);
+      } else {
+        b[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+  };
+  // LDS row = function, 4 points (k) per 8-byte store, hi and lo
+  auto store = [&](const float4 (&x)[4], __bf16 (*hi)[kXfPad], __bf16 (*lo)[kXfPad]) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      xbf16x4 h, l;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const float v = e == 0 ? x[p].x : e == 1 ? x[p].y : e == 2 ? x[p].z : x[p].w;
+        const __bf16 vh = (__bf16)v;
+        h[p] = vh;
+        l[p] = (__bf16)(v - (float)vh);
+      }
+      *reinterpret_cast<xbf16x4*>(&hi[cg + e][pg]) = h;
+      *reinterpret_cast<xbf16x4*>(&lo[cg + e][pg]) = l;
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < kXfG; k0 += 32) {
+    __syncthreads();
+    store(a, sAh, sAl);
+    store(b, sBh, sBl);
+    __syncthreads();
+    if (k0 + 32 < kXfG) load(k0 + 32);
+    xf_mfma_step(sAh, sAl, sBh, sBl, wm, wn, li, lk, acc);
+  }
+  float* qb = q + blk * (int64_t)n * n;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm + 16 * i + 4 * lk + r;
+        const int gn = n0 + wn + 16 * j + li;
+        if (gm < n && gn < n) qb[(int64_t)gm * n + gn] = acc[i][j][r];
+      }
+}
+
+// The fused pass over one bin (nb blocks of width n <= kXfMaxN, a multiple of
+// 32). ws: the partial sums and the W coefficients, nb (512 nt nks + 256) doubles
+// (at most nb 128 n for n >= 32); q: nb n^2 floats.
+constexpr int kXfKChunk = 384;   // functions of K per k_xcf_z workgroup
+
+int64_t xcf_ws_doubles(int nb, int n) {
+  const int nt = (n + 127) / 128, nks = (n + kXfKChunk - 1) / kXfKChunk;
+  return (int64_t)nb * (4 * kXfG * nt * nks + 2 * kXfG);
+}
+
+hipError_t launch_xcf(int nb, int n, const float* ao, const int32_t* cols, const float* P, int64_t nao,
+                      const double* w, int xc_id, double thresh, double* exc, double* ws, float* q, hipStream_t st) {
+  if (nb == 0) return hipSuccess;
+  if (n % 32 != 0 || n > kXfMaxN || (uintptr_t)ao % 16 != 0 || (uintptr_t)ws % 16 != 0)
+    return hipErrorInvalidValue;
+  const int64_t plane = (int64_t)nb * kXfG * n;
+  const int nt = (n + 127) / 128, nks = (n + kXfKChunk - 1) / kXfKChunk;
+  const int64_t nz = (int64_t)nb * nt * nks;
+  double* part = ws;
+  float* coef = reinterpret_cast<float*>(ws + nz * 4 * kXfG);
+  // 8 ceil(total / 8) workgroups: the XCD mappings produce every id < total only then
+  hipLaunchKernelGGL(k_xcf_z, dim3((unsigned)(8 * ((nz + 7) / 8))), dim3(256), 0, st, nb, n, nt, nks, kXfKChunk, ao,
+                     plane, cols, P, nao, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_xcf_pt, dim3((unsigned)nb), dim3(kXfG), 0, st, nb, nt * nks, part, w, xc_id, thresh, exc,
+                     coef);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t ntot = (int64_t)nb * nt * nt;
+  hipLaunchKernelGGL(k_xcf_v, dim3((unsigned)(8 * ((ntot + 7) / 8))), dim3(256), 0, st, nb, n, ao, plane, coef, q);
+  return hipGetLastError();
+}
 }  // namespace dfks
