@@ -138,6 +138,7 @@ hipError_t launch_gemm_partials(int prec, const GemmArgs& g, int nsplit, int sym
 hipError_t launch_gemm_reduce(int prec, const GemmArgs& g, double* C, double alpha, double beta, int nsplit, int sym,
                               const void* ws, hipStream_t st);
 hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s);
+hipError_t launch_sp2_lanczos(int n, const double* F, double* lz, double* st, hipStream_t s);
 hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, hipStream_t s);
 hipError_t launch_sp2_begin(int n, const double* X, int nocc, double* st, hipStream_t s);
 hipError_t launch_sp2_update(int n, int nsplit, const double* ws, double* X, int nocc, double* st, hipStream_t s);
@@ -526,6 +527,9 @@ struct dfks_ctx {
   double *d_coef = nullptr, *d_dots = nullptr, *d_Fx = nullptr, *d_C = nullptr, *d_eps = nullptr, *d_Ws = nullptr;
   // purification: state, Gershgorin bounds, residual diagonal, occupied factor, pivots
   double *d_sp2 = nullptr, *d_lohi = nullptr, *d_dvec = nullptr, *d_L = nullptr;
+  double* d_lz = nullptr;   // Lanczos vectors and coefficients of the SP2 spectral bounds
+  hipGraphExec_t lz_exec = nullptr;  // the Lanczos bounds as one graph (per nmo)
+  int lz_exec_n = -1;
   int32_t* d_piv = nullptr;
   // compact XC grid: bins of blocks of equal padded width (see plan_xc_blocks)
   struct XcBin {
@@ -583,7 +587,7 @@ struct dfks_ctx {
     void* ptrs[] = {d_boys, d_atm, d_S, d_T, d_V, d_H, d_int2c, d_int3c, d_B, d_gxyz, d_gw, d_ao, d_zw, d_exc,
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
                     d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws,
-                    d_sp2, d_lohi, d_dvec, d_L, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
+                    d_sp2, d_lz, d_lohi, d_dvec, d_L, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
                     d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf, d_xc_pts, d_xc_w, d_xc_exc, d_xc_ao,
                     d_xc_zw, d_xc_q, d_xc_ao32, d_xc_cols, d_xc_vlen, d_xc_shl, d_xc_shoff, d_xc_vrow,
                     d_xc_vprow, d_xc_vpcol};
@@ -596,6 +600,7 @@ struct dfks_ctx {
     for (auto& e : scf_ev)
       if (e) (void)hipEventDestroy(e);
     if (sp2_exec) (void)hipGraphExecDestroy(sp2_exec);
+    if (lz_exec) (void)hipGraphExecDestroy(lz_exec);
     rb_release(rb);
   }
 };
@@ -1132,6 +1137,7 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   // purification state: fixed size, allocated once, because the purification
   // graph keeps its address (as it keeps those of d_W1 and d_gws)
   h->d_sp2 = dalloc<double>(kSp2State);
+  h->d_lz = dalloc<double>(3 * (size_t)nao + kLzParts + 2 * kLzSteps);
   h->d_gpart = dalloc<double>((size_t)nao * h->nfit);
   if (h->opt.precision == 1) {
     if (!h->recompute) {
@@ -2155,10 +2161,38 @@ void sp2_run_blocks(dfks_ctx* h, int nblocks, hipStream_t st) {
   }
 }
 
+// Lanczos spectral bounds of F' = d_Ws into the purification state (after the
+// Gershgorin bounds), replayed as one graph per handle and nmo (49 small kernels).
+void sp2_bounds(dfks_ctx* h, hipStream_t st) {
+  const int n = h->nmo;
+  if (n > 4 * kLzParts) return;   // beyond the partial-sum buffer: Gershgorin bounds only
+  if (h->lz_exec_n != n) {
+    if (h->lz_exec) (void)hipGraphExecDestroy(h->lz_exec);
+    h->lz_exec = nullptr;
+    h->lz_exec_n = n;
+    hipGraph_t gr = nullptr;
+    if (hipGraphCreate(&gr, 0) == hipSuccess && gr) {
+      GraphRec rec;
+      rec.g = gr;
+      g_rec = &rec;
+      const hipError_t e1 = launch_sp2_lanczos(n, h->d_Ws, h->d_lz, h->d_sp2, st);
+      g_rec = nullptr;
+      if (e1 == hipSuccess && rec.err == hipSuccess &&
+          hipGraphInstantiate(&h->lz_exec, gr, nullptr, nullptr, 0) != hipSuccess)
+        h->lz_exec = nullptr;
+      (void)hipGraphDestroy(gr);
+      (void)hipGetLastError();
+    }
+  }
+  if (h->lz_exec) HIPCHK(hipGraphLaunch(h->lz_exec, st));
+  else HIPCHK(launch_sp2_lanczos(n, h->d_Ws, h->d_lz, h->d_sp2, st));
+}
+
 bool purify_density(dfks_ctx* h, hipStream_t st) {
   const int n = h->nmo, nocc = h->nocc, nao = h->nao;
   double* X = h->d_W1;
   HIPCHK(launch_gersh(n, h->d_Ws, h->d_lohi, h->d_sp2, st));
+  sp2_bounds(h, st);
   HIPCHK(launch_sp2_init(n, h->d_Ws, h->d_sp2, X, st));
   HIPCHK(launch_sp2_begin(n, X, nocc, h->d_sp2, st));
   int done = 0;

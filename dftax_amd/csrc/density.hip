@@ -10,7 +10,8 @@
 //   X_{k+1} = X_k^2          if |tr X_k^2 - nocc| < |2 tr X_k - tr X_k^2 - nocc|
 //           = 2 X_k - X_k^2  otherwise
 //
-// with Gershgorin bounds e_min, e_max. Both maps keep the spectrum in [0, 1],
+// with e_min, e_max from 24 Lanczos steps (Ritz values widened by their
+// residuals, inside the Gershgorin bounds). Both maps keep the spectrum in [0, 1],
 // so e_k = tr X_k - tr X_k^2 = sum lambda (1 - lambda) >= 0 bounds every
 // eigenvalue's distance from {0, 1}. Once e_k < 1e-9 two forced steps follow
 // (X^2, then 2X - X^2), which square the remaining deviations on both sides,
@@ -275,6 +276,190 @@ __global__ __launch_bounds__(1024) void k_pchol_panel(int n, int k0, int b, cons
     const double r = block_sum_1024(a, s_red);
     if (tid == 0) st[kSpResid] = r;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Spectral bounds for X_0 by Lanczos. Gershgorin intervals of F' overestimate
+// its spectrum several-fold (for (H2O)_32 def2-TZVP: [-371, 400] Ha against
+// [-19.3, 43.7]), and SP2 needs about 2 log2(range / gap) iterations, so kLzSteps
+// Lanczos steps from a fixed pseudo-random start give the extreme Ritz values
+// theta and their residuals rho = beta_k |y_k| (an eigenvalue lies within rho
+// of theta); the bounds become theta_min - m, theta_max + m with m = max(rho,
+// 1e-3 of the Ritz range), intersected with Gershgorin's. Lanczos finds the
+// extreme eigenvalues first; should a bound still fall inside the spectrum,
+// the eigenvalues beyond it map slightly outside [0, 1] and the iteration
+// still takes them to 0 or 1, and the step's idempotency and trace checks
+// stand behind it as before.
+// Work vectors: lz[0..n) v, lz[n..2n) previous v, lz[2n..3n) w, then
+// kLzParts partial dot products, then alpha_k, beta_k (2 kLzSteps).
+// ---------------------------------------------------------------------------
+
+// v_0: a fixed +-(1 + small) pattern, normalised; previous v = 0; coefficients cleared.
+__global__ __launch_bounds__(1024) void k_lz_start(int n, double* __restrict__ lz) {
+  __shared__ double s_red[16];
+  double* v = lz;
+  double* vp = lz + n;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t hsh = (uint32_t)i * 2654435761u;
+    const double x = ((hsh >> 15) & 1u ? 1.0 : -1.0) * (1.0 + (double)((hsh >> 20) & 255u) / 1024.0);
+    v[i] = x;
+    vp[i] = 0.0;
+    a += x * x;
+  }
+  const double t = block_sum_1024(a, s_red);
+  __shared__ double s_inv;
+  if (threadIdx.x == 0) s_inv = 1.0 / sqrt(t);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] *= s_inv;
+  double* ab = lz + 3 * (int64_t)n + kLzParts;
+  for (int i = threadIdx.x; i < 2 * kLzSteps; i += blockDim.x) ab[i] = 0.0;
+}
+
+// w = F v (one wavefront per row of the symmetric F), and per workgroup the partial v . w
+__global__ __launch_bounds__(256) void k_lz_matvec(int n, const double* __restrict__ F, double* __restrict__ lz) {
+  __shared__ double s_p[4];
+  const double* v = lz;
+  double* w = lz + 2 * (int64_t)n;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * 4 + wv;
+  double acc = 0.0;
+  if (i < n) {
+    const double* row = F + (int64_t)i * n;
+    for (int j = lane; j < n; j += 64) acc += row[j] * v[j];
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) {
+    if (i < n) w[i] = acc;
+    s_p[wv] = i < n ? acc * v[i] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) lz[3 * (int64_t)n + blockIdx.x] = s_p[0] + s_p[1] + s_p[2] + s_p[3];
+}
+
+// alpha = v . w (the partials in order); w -= alpha v + beta_{k-1} v_prev; beta = |w|;
+// v_prev = v; v = w / beta (a zero beta ends the recurrence: v stays 0).
+__global__ __launch_bounds__(1024) void k_lz_update(int n, int nparts, int k, double* __restrict__ lz) {
+  __shared__ double s_red[16];
+  __shared__ double s_alpha, s_beta;
+  double* v = lz;
+  double* vp = lz + n;
+  double* w = lz + 2 * (int64_t)n;
+  const double* part = lz + 3 * (int64_t)n;
+  double* ab = lz + 3 * (int64_t)n + kLzParts;
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    for (int i = 0; i < nparts; ++i) a += part[i];
+    s_alpha = a;
+  }
+  __syncthreads();
+  const double alpha = s_alpha, bprev = k > 0 ? ab[2 * (k - 1) + 1] : 0.0;
+  double q = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double x = w[i] - alpha * v[i] - bprev * vp[i];
+    w[i] = x;
+    q += x * x;
+  }
+  const double t = block_sum_1024(q, s_red);
+  if (threadIdx.x == 0) {
+    const double b = sqrt(t);
+    s_beta = b > 1e-12 * (fabs(alpha) + 1.0) ? b : 0.0;
+    ab[2 * k] = alpha;
+    ab[2 * k + 1] = s_beta;
+  }
+  __syncthreads();
+  const double beta = s_beta;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double vi = v[i];
+    vp[i] = vi;
+    v[i] = beta > 0.0 ? w[i] / beta : 0.0;
+  }
+}
+
+// Eigenvalues of the Lanczos tridiagonal below x (Sturm count).
+__device__ int lz_count_below(const double* ab, int m, double x) {
+  int c = 0;
+  double d = 1.0;
+  for (int i = 0; i < m; ++i) {
+    const double b2 = i > 0 ? ab[2 * (i - 1) + 1] * ab[2 * (i - 1) + 1] : 0.0;
+    d = (ab[2 * i] - x) - (i > 0 ? b2 / d : 0.0);
+    if (d == 0.0) d = -1e-300;
+    if (d < 0.0) ++c;
+  }
+  return c;
+}
+
+// Residual beta_m |y_m| of the Ritz value theta (y the normalised eigenvector of T).
+__device__ double lz_residual(const double* ab, int m, double theta) {
+  double y0 = 1.0, y1 = 0.0, nrm = 1.0;
+  double last = 1.0;
+  for (int i = 0; i + 1 < m; ++i) {
+    const double b = ab[2 * i + 1];
+    const double bp = i > 0 ? ab[2 * (i - 1) + 1] : 0.0;
+    const double y2 = ((theta - ab[2 * i]) * y0 - bp * y1) / b;
+    y1 = y0;
+    y0 = y2;
+    nrm += y2 * y2;
+    last = y2;
+    if (nrm > 1e200) {   // rescale
+      const double sc = 1.0 / sqrt(nrm);
+      y0 *= sc; y1 *= sc; last *= sc; nrm = 1.0;
+    }
+  }
+  return fabs(ab[2 * (m - 1) + 1] * last) / sqrt(nrm);
+}
+
+// One thread: the extreme Ritz values by bisection, their residuals, and the
+// bounds in st (intersected with the Gershgorin bounds already there).
+__global__ void k_lz_bounds(int n, const double* __restrict__ lz, double* __restrict__ st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double* ab = lz + 3 * (int64_t)n + kLzParts;
+  int m = 0;
+  while (m < kLzSteps) {
+    ++m;
+    if (ab[2 * (m - 1) + 1] == 0.0) break;
+  }
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? fabs(ab[2 * (i - 1) + 1]) : 0.0) + (i + 1 < m ? fabs(ab[2 * i + 1]) : 0.0);
+    lo = fmin(lo, ab[2 * i] - r);
+    hi = fmax(hi, ab[2 * i] + r);
+  }
+  // smallest and largest eigenvalue of T by bisection
+  double a = lo, b = hi;
+  for (int it = 0; it < 200 && b - a > 1e-13 * fmax(1.0, fabs(a) + fabs(b)); ++it) {
+    const double mid = 0.5 * (a + b);
+    (lz_count_below(ab, m, mid) >= 1 ? b : a) = mid;
+  }
+  const double tmin = 0.5 * (a + b);
+  a = lo; b = hi;
+  for (int it = 0; it < 200 && b - a > 1e-13 * fmax(1.0, fabs(a) + fabs(b)); ++it) {
+    const double mid = 0.5 * (a + b);
+    (lz_count_below(ab, m, mid) >= m ? b : a) = mid;
+  }
+  const double tmax = 0.5 * (a + b);
+  const bool exact = ab[2 * (m - 1) + 1] == 0.0;   // invariant subspace: the Ritz values are eigenvalues
+  const double rmin = exact ? 0.0 : lz_residual(ab, m, tmin);
+  const double rmax = exact ? 0.0 : lz_residual(ab, m, tmax);
+  const double pad = 1e-3 * (tmax - tmin);
+  const double emin = tmin - fmax(rmin, pad), emax = tmax + fmax(rmax, pad);
+  if (!(emin < emax) || !isfinite(emin) || !isfinite(emax)) return;   // keep Gershgorin's
+  st[kSpEmin] = fmax(st[kSpEmin], emin);
+  st[kSpEmax] = fmin(st[kSpEmax], emax);
+}
+
+// The Lanczos bounds after launch_gersh: kLzSteps steps, kernels through launch_k
+// (so that they can be recorded as a graph). lz: 3 n + kLzParts + 2 kLzSteps doubles.
+hipError_t launch_sp2_lanczos(int n, const double* F, double* lz, double* st, hipStream_t s) {
+  const int nparts = (n + 3) / 4;
+  if (nparts > kLzParts) return hipErrorInvalidValue;
+  hipError_t e = launch_k(k_lz_start, dim3(1), dim3(1024), s, n, lz);
+  for (int k = 0; k < kLzSteps && e == hipSuccess; ++k) {
+    e = launch_k(k_lz_matvec, dim3((unsigned)nparts), dim3(256), s, n, F, lz);
+    if (e == hipSuccess) e = launch_k(k_lz_update, dim3(1), dim3(1024), s, n, nparts, k, lz);
+  }
+  if (e == hipSuccess) e = launch_k(k_lz_bounds, dim3(1), dim3(64), s, n, (const double*)lz, st);
+  return e;
 }
 
 hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s) {

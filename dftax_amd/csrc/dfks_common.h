@@ -39,6 +39,8 @@ constexpr int kSp2Block = 4;          // purification iterations per graph launc
 // widest compact-grid block (functions) the fused mixed-precision XC kernels take
 constexpr int kXfMaxN = 2048;
 constexpr int kPanel = 32;            // pivoted Cholesky panel width
+constexpr int kLzSteps = 24;          // Lanczos steps for the SP2 spectral bounds
+constexpr int kLzParts = 4096;        // partial dot products of its matrix-vector kernel (n <= 4 kLzParts)
 
 // Graph recording without stream capture. While a thread has a GraphRec
 // installed in g_rec, launch_k appends each kernel as a node of g_rec->g that
