@@ -45,6 +45,7 @@ EXPORTS = [
     "dfks_scf_result", "dfks_becke_weights", "dfks_overlap", "dfks_set_timing", "dfks_stage_times",
     "dfks_set_precision", "dfks_stats", "dfks_xc_eval2", "dfks_response_begin", "dfks_response_fock",
     "dfks_device_free", "dfks_release_memory", "dfks_becke_weights_noalloc", "dfks_forces",
+    "dfks_xc_eval_pol",
 ]
 
 STAGES = ["exchange_gemm", "exchange_syrk", "coulomb_j", "xc_gemm", "xc_kernel", "xc_syr2k", "fock_assembly",
@@ -93,6 +94,7 @@ def lib():
     L.dfks_stats.argtypes = [c_void_p, ctypes.POINTER(c_int64), c_int32]
     L.dfks_device_free.argtypes = [ctypes.POINTER(c_int64)]
     L.dfks_release_memory.argtypes = [c_void_p]
+    L.dfks_xc_eval_pol.argtypes = [c_int32, c_int64, c_void_p, c_void_p, c_void_p]
     L.dfks_forces.argtypes = [c_void_p, c_void_p, c_int32, c_int64, P_f64, P_f64, P_i32, P_f64, P_f64, c_void_p]
     L.dfks_becke_weights_noalloc.argtypes = [c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
                                              c_void_p, c_void_p]

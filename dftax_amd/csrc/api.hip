@@ -62,6 +62,9 @@ hipError_t launch_xc(int64_t ng, int64_t nao, const double* ao, const double* w,
                      double* exc, double* rho_out, hipStream_t st);
 hipError_t launch_xc_pointwise(int64_t n, int xc_id, const double* rho, const double* sigma, double* out,
                                hipStream_t st);
+hipError_t launch_xcs(int64_t ng, int64_t nao, const double* ao, const double* w, double* zwa, double* zwb,
+                      int xc_id, double thresh, double* exc, hipStream_t st);
+hipError_t launch_xc_pointwise_pol(int64_t n, int xc_id, const double* in5, double* out6, hipStream_t st);
 hipError_t launch_xc_pointwise2(int64_t n, int xc_id, const double* rho, const double* sigma, double* out,
                                 hipStream_t st);
 hipError_t launch_fxc_coef(int64_t ng, const double* rho0, const double* w, int xc_id, double thresh, double* rec,
@@ -540,7 +543,12 @@ struct dfks_ctx {
   };
   bool xcb = false;
   std::vector<XcBin> xbins;
-  int64_t xc_npts = 0, xc_zw = 0, xc_nblk = 0;
+  int64_t xc_npts = 0, xc_zw = 0, xc_nblk = 0, xc_qtot = 0;
+  // spin-polarised (nspin = 2) Fock builds: beta-channel XC buffers, the total density,
+  // J of the total density, K per channel, gamma of the total density
+  int nocc_b = 0, spin = 0;
+  double *d_xc_zw2 = nullptr, *d_xc_q2 = nullptr, *d_Vxc2 = nullptr, *d_uP = nullptr, *d_uJ = nullptr;
+  double *d_uK = nullptr, *d_ug = nullptr;
   double *d_xc_pts = nullptr, *d_xc_w = nullptr, *d_xc_exc = nullptr, *d_xc_ao = nullptr, *d_xc_zw = nullptr;
   double* d_xc_q = nullptr;
   float* d_xc_ao32 = nullptr;
@@ -590,7 +598,7 @@ struct dfks_ctx {
                     d_sp2, d_lz, d_lohi, d_dvec, d_L, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
                     d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf, d_xc_pts, d_xc_w, d_xc_exc, d_xc_ao,
                     d_xc_zw, d_xc_q, d_xc_ao32, d_xc_cols, d_xc_vlen, d_xc_shl, d_xc_shoff, d_xc_vrow,
-                    d_xc_vprow, d_xc_vpcol};
+                    d_xc_vprow, d_xc_vpcol, d_xc_zw2, d_xc_q2, d_Vxc2, d_uP, d_uJ, d_uK, d_ug};
     for (void* p : ptrs) dfree(p);
     void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, t32_alias ? nullptr : d_T32, d_Cf32, d_P32, d_ws32,
                       d_eps32, d_offd32, d_slab32, d_Lt32, d_Tslab32, d_aob32, d_zwb32};
@@ -941,6 +949,7 @@ void plan_xc_blocks(dfks_ctx* h, hipStream_t st) {
   h->xc_npts = npts;
   h->xc_nblk = nblk;
   h->xc_zw = zw_max;
+  h->xc_qtot = q_tot;
   h->d_xc_exc = dalloc<double>(npts);
   h->d_xc_zw = dalloc<double>(zw_max);
   h->d_xc_q = dalloc<double>(q_tot);
@@ -1546,6 +1555,48 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
   h->mark(kStXcSyr2k + 1, st);
 }
 
+// Spin-polarised XC on the compact grid (FP64): per bin both channels' Z_s =
+// Phi_b Pg_s (rocBLAS batched), the spin pointwise kernel (W_a, W_b over Z_a,
+// Z_b), Phi_b^T W_s per channel, then the row assembly of each channel's half
+// potential: Vh_a in d_Vxc, Vh_b in d_Vxc2; E_xc into E + DFKS_E_XC.
+void xc_compact_uks(dfks_ctx* h, const double* Pa, const double* Pb, double* E, hipStream_t st) {
+  const int nao = h->nao;
+  const double one = 1.0, zero = 0.0;
+  if (!h->d_xc_zw2) h->d_xc_zw2 = dalloc<double>(h->xc_zw);
+  if (!h->d_xc_q2) h->d_xc_q2 = dalloc<double>(h->xc_qtot);
+  if (!h->d_Vxc2) h->d_Vxc2 = dalloc<double>((size_t)nao * nao);
+  for (const auto& bn : h->xbins) {
+    const int n = bn.n;
+    const int64_t nb = bn.nblk, rows = nb * kXcG;
+    const int32_t* cols = h->d_xc_cols + bn.col_off;
+    const double* ao = h->d_xc_ao + bn.ao_off;
+    double* q[2] = {h->d_xc_q + bn.q_off, h->d_xc_q2 + bn.q_off};
+    double* zw[2] = {h->d_xc_zw, h->d_xc_zw2};
+    const double* Ps[2] = {Pa, Pb};
+    for (int sp = 0; sp < 2; ++sp) {
+      HIPCHK((launch_gather_sym<double, double>(nb, n, cols, Ps[sp], nao, q[sp], st)));
+      RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_transpose, rocblas_operation_none, n, kXcG, n,
+                                          &one, q[sp], n, (rocblas_stride)n * n, ao, n, (rocblas_stride)kXcG * n,
+                                          &zero, zw[sp], n, (rocblas_stride)kXcG * n, (int)nb));
+    }
+    HIPCHK(launch_xcs(rows, n, ao, h->d_xc_w + bn.pt0, zw[0], zw[1], h->xc_id, h->opt.rho_thresh,
+                      h->d_xc_exc + bn.pt0, st));
+    for (int sp = 0; sp < 2; ++sp)
+      RBCHK(rocblas_dgemm_strided_batched(h->rb, rocblas_operation_none, rocblas_operation_transpose, n, n, kXcG,
+                                          &one, zw[sp], n, (rocblas_stride)kXcG * n, ao, n,
+                                          (rocblas_stride)kXcG * n, &zero, q[sp], n, (rocblas_stride)n * n,
+                                          (int)nb));
+  }
+  h->mark(kStXcKernel + 1, st);
+  const int nslice = std::max(1, std::min(32, (32768 + nao - 1) / nao));
+  HIPCHK(launch_vassemble<double>(nao, h->d_xc_vrow, h->d_xc_vprow, h->d_xc_vpcol, h->d_xc_vlen, h->d_xc_q,
+                                  h->d_xc_cols, h->d_Vxc, h->d_gws, nslice, st));
+  HIPCHK(launch_vassemble<double>(nao, h->d_xc_vrow, h->d_xc_vprow, h->d_xc_vpcol, h->d_xc_vlen, h->d_xc_q2,
+                                  h->d_xc_cols, h->d_Vxc2, h->d_gws, nslice, st));
+  HIPCHK(launch_dot(h->xc_npts, h->d_xc_exc, nullptr, h->d_part, 1.0, E + DFKS_E_XC, 0, st));
+  h->mark(kStXcSyr2k + 1, st);
+}
+
 // XC energy (E + DFKS_E_XC) and the half potential Vh (d_Vxc, Vxc = Vh + Vh^T) at P.
 void xc_potential(dfks_ctx* h, const double* P, double* E, hipStream_t st) {
   const int nao = h->nao;
@@ -1682,6 +1733,56 @@ int factor_density(dfks_ctx* h, const double* P, std::vector<double>& sg, hipStr
   HIPCHK(launch_factor_cols(nao, nf, h->d_W1, h->d_evals, h->d_sel, h->d_Cf, h->d_sgn, st));
   sync(st);
   return nf;
+}
+
+// Spin-polarised energy and Fock matrices of P = [P_a; P_b] (2 nao^2): the
+// one-electron and Coulomb terms of P_a + P_b, exchange -1/2 a_x sum_s P_s . K(P_s)
+// (terms.py:500-509), the spin-polarised XC (terms.py:838-874), and
+// F_s = H + J(P_a + P_b) - a_x K(P_s) + Vxc_s into F = [F_a; F_b] (FP64).
+void energy_fock_uks(dfks_ctx* h, const double* P, double* E, double* F, hipStream_t st) {
+  if (!h->built) fail(6, "dfks_build has not been called");
+  if (h->mixed_now) fail(1, "nspin = 2 Fock builds run in FP64: call dfks_set_precision(h, 0)");
+  if (h->ng > 0 && !h->xcb)
+    fail(1, "nspin = 2 needs the compact XC grid (options.xc_blocks = 1 with options.ao_block = 0)");
+  RBCHK(rocblas_set_stream(h->rb, st));
+  const int nao = h->nao, nfit = h->nfit;
+  const int64_t nn = (int64_t)nao * nao;
+  if (!h->d_uP) {
+    h->d_uP = dalloc<double>(nn);
+    h->d_uJ = dalloc<double>(nn);
+    h->d_uK = dalloc<double>(2 * nn);
+    h->d_ug = dalloc<double>(nfit);
+  }
+  h->mark(0, st);
+  HIPCHK(hipMemsetAsync(E, 0, 8 * sizeof(double), st));
+  HIPCHK(launch_axpy(nn, 1.0, P, h->d_uP, 0, st));
+  HIPCHK(launch_axpy(nn, 1.0, P + nn, h->d_uP, 1, st));
+  HIPCHK(launch_dot(nn, h->d_uP, h->d_H, h->d_part, 1.0, E + DFKS_E_ONE, 0, st));
+  HIPCHK(hipMemsetAsync(h->d_uJ, 0, sizeof(double) * nn, st));
+  HIPCHK(hipMemsetAsync(h->d_ug, 0, sizeof(double) * nfit, st));
+  HIPCHK(hipMemsetAsync(h->d_uK, 0, sizeof(double) * 2 * nn, st));
+  for (int sp = 0; sp < 2; ++sp) {
+    const double* Ps = P + sp * nn;
+    std::vector<double> sg;
+    const int nf = factor_density(h, Ps, sg, st);
+    if (nf == 0) continue;                       // an empty channel: no gamma, J or K
+    jk_factored(h, h->d_Cf, sg, nf, st);
+    HIPCHK(launch_axpy(nfit, 1.0, h->d_gamma, h->d_ug, 1, st));
+    HIPCHK(launch_axpy(nn, 1.0, h->d_J, h->d_uJ, 1, st));
+    if (h->hf != 0.0) {
+      HIPCHK(launch_axpy(nn, 1.0, h->d_K, h->d_uK + sp * nn, 0, st));
+      HIPCHK(launch_dot(nn, Ps, h->d_K, h->d_part, -0.5 * h->hf, E + DFKS_E_K, 1, st));
+    }
+  }
+  // E_J = 1/2 |gamma_a + gamma_b|^2
+  HIPCHK(launch_dot(nfit, h->d_ug, h->d_ug, h->d_part, 0.5, E + DFKS_E_J, 0, st));
+  if (h->ng > 0) xc_compact_uks(h, P, P + nn, E, st);
+  for (int sp = 0; sp < 2; ++sp)
+    HIPCHK(launch_fock(nao, h->d_H, h->d_uJ, h->hf != 0.0 ? h->d_uK + sp * nn : nullptr, -h->hf,
+                       h->ng > 0 ? (sp == 0 ? h->d_Vxc : h->d_Vxc2) : nullptr, F + sp * nn, st));
+  HIPCHK(hipMemcpyAsync(E + DFKS_E_NN, &h->e_nn, sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(launch_dot(5, E + 1, nullptr, h->d_part, 1.0, E + DFKS_E_TOT, 0, st));
+  h->mark(kStFock + 1, st);
 }
 
 void energy_fock_general(dfks_ctx* h, const double* P, double* E, double* F, hipStream_t st) {
@@ -2370,6 +2471,14 @@ __global__ __launch_bounds__(256) void k_becke_pairs(int natm, const double* __r
   aAB[i] = fmin(0.45, fmax(-0.45, u / (u * u - 1.0)));
 }
 
+// Entry points of the restricted closed-shell path: an nspin = 2 handle gets
+// status 7 (its SCF loop runs on the host over dfks_energy_fock).
+void closed_shell_only(const dfks_ctx* h, const char* what) {
+  if (h && h->nspin == 2)
+    fail(7, std::string(what) + " is implemented for nspin = 1; an nspin = 2 handle supports dfks_energy_fock "
+                                "(FP64) and the matrix and setup calls");
+}
+
 extern "C" {
 
 const char* dfks_last_error(void) { return g_err.c_str(); }
@@ -2378,11 +2487,10 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
   return guard([&] {
     if (!sys || !out) fail(1, "null argument");
     // checked before any device work, so a host without a GPU gets the same answer
-    if (sys->nspin == 2)
-      fail(7, "spin-unrestricted densities (nspin = 2) are not implemented; this build solves the restricted "
-              "closed-shell case (nspin = 1)");
-    if (sys->nspin != 1) fail(1, "nspin must be 1 (restricted) or 2 (unrestricted)");
-    if (sys->spin != 0) fail(1, "nspin = 1 needs spin = 0 (n_alpha = n_beta)");
+    if (sys->nspin != 1 && sys->nspin != 2) fail(1, "nspin must be 1 (restricted) or 2 (unrestricted)");
+    if (sys->nspin == 1 && sys->spin != 0) fail(1, "nspin = 1 needs spin = 0 (n_alpha = n_beta)");
+    if (sys->nspin == 2 && (sys->spin < 0 || sys->nelec < sys->spin || (sys->nelec + sys->spin) % 2 != 0))
+      fail(1, "nspin = 2 needs 0 <= spin <= nelec with nelec + spin even (n_alpha - n_beta = spin)");
     std::lock_guard<std::mutex> lk(g_mu);
     static bool tables = false;
     if (!tables) {
@@ -2405,8 +2513,16 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
 
       h->xc_id = sys->xc_id;
       h->hf = hf_coeff(sys->xc_id);
-      if (sys->nelec <= 0 || sys->nelec % 2) fail(1, "closed shell needs an even, positive electron count");
-      h->nocc = sys->nelec / 2;
+      h->nspin = sys->nspin;
+      if (h->nspin == 1) {
+        if (sys->nelec <= 0 || sys->nelec % 2) fail(1, "closed shell needs an even, positive electron count");
+        h->nocc = sys->nelec / 2;
+      } else {
+        if (sys->nelec <= 0) fail(1, "needs a positive electron count");
+        h->spin = sys->spin;
+        h->nocc = (sys->nelec + sys->spin) / 2;      // alpha (the larger channel)
+        h->nocc_b = (sys->nelec - sys->spin) / 2;
+      }
       h->orb.load(sys->orbital);
       h->aux.load(sys->aux);
       if (sys->ngrid < 0) fail(1, "negative grid size");
@@ -2448,6 +2564,7 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
 
 int dfks_set_precision(dfks_handle h, int32_t mixed) {
   return guard([&] {
+    if (mixed) closed_shell_only(h, "dfks_set_precision(h, 1)");
     if (!h) fail(1, "null handle");
     if (mixed && h->opt.precision != 1) fail(1, "mixed steps need options.precision = 1 at create");
     const bool was = h->mixed_now;
@@ -2573,13 +2690,15 @@ int dfks_energy_fock(dfks_handle h, const double* P, int32_t nspin, double* E, d
       fail(1, "P has " + std::to_string(nspin) + " spin blocks but the handle was created with nspin = " +
                   std::to_string(h->nspin));
     StreamScope ss(h->st = S_(stream));
-    energy_fock_general(h, P, E, F, S_(stream));
+    if (h->nspin == 2) energy_fock_uks(h, P, E, F, S_(stream));
+    else energy_fock_general(h, P, E, F, S_(stream));
   });
 }
 
 int dfks_energy_fock_factored(dfks_handle h, const double* P, const double* Cf, const double* signs_host, int32_t nf,
                               double* E, double* F, dfks_stream stream) {
   return guard([&] {
+    closed_shell_only(h, "dfks_energy_fock_factored");
     if (!h || !P || !Cf || !signs_host || !E || !F) fail(1, "null argument");
     if (nf <= 0) fail(1, "nf must be positive");
     std::vector<double> sg(signs_host, signs_host + nf);
@@ -2597,6 +2716,14 @@ int dfks_xc_eval(int32_t xc_id, int64_t n, const double* rho, const double* sigm
   });
 }
 
+int dfks_xc_eval_pol(int32_t xc_id, int64_t n, const double* in5, double* out6, dfks_stream stream) {
+  return guard([&] {
+    hf_coeff(xc_id);
+    if (n < 0 || (n > 0 && (!in5 || !out6))) fail(1, "bad arguments");
+    HIPCHK(launch_xc_pointwise_pol(n, xc_id, in5, out6, S_(stream)));
+  });
+}
+
 int dfks_xc_eval2(int32_t xc_id, int64_t n, const double* rho, const double* sigma, double* out6,
                   dfks_stream stream) {
   return guard([&] {
@@ -2607,6 +2734,7 @@ int dfks_xc_eval2(int32_t xc_id, int64_t n, const double* rho, const double* sig
 
 int dfks_response_begin(dfks_handle h, const double* P, dfks_stream stream) {
   return guard([&] {
+    closed_shell_only(h, "dfks_response_begin");
     if (!h || !P) fail(1, "null argument");
     StreamScope ss(h->st = S_(stream));
     response_begin(h, P, S_(stream));
@@ -2615,6 +2743,7 @@ int dfks_response_begin(dfks_handle h, const double* P, dfks_stream stream) {
 
 int dfks_response_fock(dfks_handle h, const double* U, const double* V, int32_t r, double* R, dfks_stream stream) {
   return guard([&] {
+    closed_shell_only(h, "dfks_response_fock");
     if (!h || !U || !V || !R) fail(1, "null argument");
     StreamScope ss(h->st = S_(stream));
     response_fock(h, U, V, r, R, S_(stream));
@@ -2624,6 +2753,7 @@ int dfks_response_fock(dfks_handle h, const double* U, const double* V, int32_t 
 int dfks_scf_begin(dfks_handle h, const double* P0, double lindep, int32_t diis_space, double level_shift,
                    dfks_stream stream) {
   return guard([&] {
+    closed_shell_only(h, "dfks_scf_begin");
     if (!h || !P0) fail(1, "null argument");
     StreamScope ss(h->st = S_(stream));
     scf_begin(h, P0, lindep, diis_space, level_shift, S_(stream));
@@ -2632,6 +2762,7 @@ int dfks_scf_begin(dfks_handle h, const double* P0, double lindep, int32_t diis_
 
 int dfks_scf_step(dfks_handle h, double* out3, dfks_stream stream) {
   return guard([&] {
+    closed_shell_only(h, "dfks_scf_step");
     if (!h || !out3) fail(1, "null argument");
     StreamScope ss(h->st = S_(stream));
     scf_step(h, out3, S_(stream));
@@ -2640,6 +2771,7 @@ int dfks_scf_step(dfks_handle h, double* out3, dfks_stream stream) {
 
 int dfks_scf_result(dfks_handle h, double* P, double* C, double* eps, int32_t* nmo, dfks_stream stream) {
   return guard([&] {
+    closed_shell_only(h, "dfks_scf_result");
     if (!h) fail(1, "null handle");
     if (!h->d_X) fail(6, "dfks_scf_begin has not been called");
     hipStream_t st = S_(stream);
@@ -2805,6 +2937,7 @@ int dfks_forces(dfks_handle h, const double* C_occ, int32_t nocc, int64_t ngrid,
                 const double* grid_raw_w, const int32_t* grid_owner, const double* bragg, double* grad_parts,
                 dfks_stream stream) {
   return guard([&] {
+    closed_shell_only(h, "dfks_forces");
     if (!h || !C_occ || !grad_parts || !bragg) fail(1, "null argument");
     if (ngrid < 0 || (ngrid > 0 && (!grid_xyz || !grid_raw_w || !grid_owner))) fail(1, "bad grid arguments");
     StreamScope ss(h->st = S_(stream));

@@ -209,6 +209,197 @@ __device__ N eps_xc(int id, N rho, N sig) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Spin-polarised functionals (dftax/energy/xc.py spin branches), evaluated on
+// a dual number over the five arguments (rho_a, rho_b, sigma_aa, sigma_ab,
+// sigma_bb), so one pass gives f and its five first partials. Each returns
+// the energy per electron of the total density.
+// ---------------------------------------------------------------------------
+struct D5 {
+  double v;
+  double d[5];
+};
+__device__ __forceinline__ double val(const D5& a) { return a.v; }
+__device__ __forceinline__ D5 d5_map(const D5& a, double f, double df) {
+  D5 r;
+  r.v = f;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.d[k] = df * a.d[k];
+  return r;
+}
+__device__ __forceinline__ D5 operator+(const D5& a, const D5& b) {
+  D5 r; r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+__device__ __forceinline__ D5 operator-(const D5& a, const D5& b) {
+  D5 r; r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+__device__ __forceinline__ D5 operator-(const D5& a) { return d5_map(a, -a.v, -1.0); }
+__device__ __forceinline__ D5 operator*(const D5& a, const D5& b) {
+  D5 r; r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+__device__ __forceinline__ D5 operator/(const D5& a, const D5& b) {
+  const double iv = 1.0 / b.v, q = a.v * iv;
+  D5 r; r.v = q;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.d[k] = (a.d[k] - q * b.d[k]) * iv;
+  return r;
+}
+__device__ __forceinline__ D5 operator+(const D5& a, double c) { D5 r = a; r.v += c; return r; }
+__device__ __forceinline__ D5 operator+(double c, const D5& a) { return a + c; }
+__device__ __forceinline__ D5 operator-(const D5& a, double c) { D5 r = a; r.v -= c; return r; }
+__device__ __forceinline__ D5 operator-(double c, const D5& a) { return d5_map(a, c - a.v, -1.0); }
+__device__ __forceinline__ D5 operator*(const D5& a, double c) { return d5_map(a, a.v * c, c); }
+__device__ __forceinline__ D5 operator*(double c, const D5& a) { return d5_map(a, a.v * c, c); }
+__device__ __forceinline__ D5 operator/(const D5& a, double c) { return d5_map(a, a.v / c, 1.0 / c); }
+__device__ __forceinline__ D5 operator/(double c, const D5& a) {
+  const double q = c / a.v;
+  return d5_map(a, q, -q / a.v);
+}
+__device__ __forceinline__ D5 dsqrt(const D5& a) { const double f = sqrt(a.v); return d5_map(a, f, 0.5 / f); }
+__device__ __forceinline__ D5 dexp(const D5& a) { const double f = exp(a.v); return d5_map(a, f, f); }
+__device__ __forceinline__ D5 dlog(const D5& a) { return d5_map(a, log(a.v), 1.0 / a.v); }
+__device__ __forceinline__ D5 dlog1p(const D5& a) { return d5_map(a, log1p(a.v), 1.0 / (1.0 + a.v)); }
+__device__ __forceinline__ D5 dexpm1(const D5& a) { return d5_map(a, expm1(a.v), exp(a.v)); }
+__device__ __forceinline__ D5 datan(const D5& a) { return d5_map(a, atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+__device__ __forceinline__ D5 dasinh(const D5& a) { return d5_map(a, asinh(a.v), 1.0 / sqrt(1.0 + a.v * a.v)); }
+__device__ __forceinline__ D5 dpow(const D5& a, double p) {
+  const double f = pow(a.v, p);
+  return d5_map(a, f, p * f / a.v);
+}
+// jnp.clip(x, lo, hi): derivative passes inside
+template <class N>
+__device__ __forceinline__ N dclip2(const N& a, double lo, double hi) {
+  return val(a) < lo ? N{} + lo : (val(a) > hi ? N{} + hi : a);
+}
+
+// Spin scaling E_x[ra, rb] = (E_x[2 ra] + E_x[2 rb]) / 2 per electron (xc.py:76-83, 116-134):
+// the unpolarised forms at (2 r_s, 4 sigma_ss), mixed by the channel populations.
+template <class N>
+__device__ N lda_x_pol(N ra, N rb) {
+  N n = dclip(ra + rb, 1e-30);
+  return (ra / n) * lda_x(2.0 * ra) + (rb / n) * lda_x(2.0 * rb);
+}
+
+template <class N>
+__device__ N pbe_x_pol(N ra, N rb, N saa, N sbb) {
+  N n = dclip(ra + rb, 1e-30);
+  return (ra / n) * pbe_x(2.0 * ra, 4.0 * saa) + (rb / n) * pbe_x(2.0 * rb, 4.0 * sbb);
+}
+
+// B88 per channel (xc.py:271-292); b88_x(2 r_s, 4 sigma_ss) is the channel's form
+template <class N>
+__device__ N b88_x_pol(N ra, N rb, N saa, N sbb) {
+  N n = dclip(ra + rb, 1e-20);
+  return (ra / n) * b88_x(2.0 * ra, 4.0 * saa) + (rb / n) * b88_x(2.0 * rb, 4.0 * sbb);
+}
+
+// PW92 with its spin interpolation (xc.py:165-177); rows para, ferro, -alpha_c
+template <class N>
+__device__ N pw92_pol(N rs, N zeta) {
+  const double P[3][6] = {{0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294},
+                          {0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517},
+                          {0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671}};
+  const double fz20 = 1.709920934161365617563962776245;
+  N rq = dsqrt(rs);
+  N g[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double A = P[r][0], a1 = P[r][1], b1 = P[r][2], b2 = P[r][3], b3 = P[r][4], b4 = P[r][5];
+    N denom = 2.0 * A * (b1 * rq + b2 * rs + b3 * rs * rq + b4 * rs * rs);
+    g[r] = -2.0 * A * (1.0 + a1 * rs) * dlog1p(1.0 / dclip(denom, 1e-20));
+  }
+  N fz = (dpow(1.0 + zeta, 4.0 / 3.0) + dpow(1.0 - zeta, 4.0 / 3.0) - 2.0) / (2.5198420997897464 - 2.0);
+  N z2 = zeta * zeta;
+  return g[0] + z2 * z2 * fz * (g[1] - g[0] + g[2] / fz20) - fz * g[2] / fz20;
+}
+
+// PBE correlation of a polarised density (xc.py:179-237): stot = |grad ra + grad rb|^2
+template <class N>
+__device__ N pbe_c_pol(N ra, N rb, N stot) {
+  const double gamma = (1.0 - 0.69314718055994530942) / kPi2;
+  const double beta = 0.06672455060314922;
+  N a = dclip(ra, 1e-20), b = dclip(rb, 1e-20);
+  N n = a + b;
+  N zeta = dclip2((a - b) / n, -1.0 + 1e-7, 1.0 - 1e-7);
+  N rs = dpow(3.0 / (4.0 * kPi * n) + 1e-20, 1.0 / 3.0);
+  N phi = dclip((dpow(1.0 + zeta, 2.0 / 3.0) + dpow(1.0 - zeta, 2.0 / 3.0)) * 0.5, 1e-20);
+  N kf = dpow(3.0 * kPi2 * n + 1e-20, 1.0 / 3.0);
+  N ks = dsqrt(4.0 * kf / kPi);
+  N gn = dsqrt(stot + 1e-30);
+  N t = gn / dclip(2.0 * phi * ks * n, 1e-20);
+  N eps_pw = pw92_pol(rs, zeta);
+  N gphi3 = gamma * phi * phi * phi;
+  N Ac = beta / dclip(gamma * dexpm1(-eps_pw / gphi3), 1e-20);
+  N t2 = t * t;
+  N f1 = t2 + Ac * t2 * t2;
+  N frac = (beta * f1) / dclip(gamma * (1.0 + Ac * f1), 1e-20);
+  return eps_pw + gphi3 * dlog1p(frac);
+}
+
+template <class N>
+__device__ N vwn_branch(N x, double A, double b, double c, double x0) {
+  N Xx = x * x + b * x + c;
+  const double Xx0 = x0 * x0 + b * x0 + c;
+  const double Q = sqrt(4.0 * c - b * b);
+  N at = datan(Q / dclip(2.0 * x + b, 1e-20));
+  N xm = x - x0;
+  return A * (dlog(x * x / dclip(Xx, 1e-20)) + (2.0 * b / Q) * at -
+              (b * x0 / Xx0) * (dlog(xm * xm / dclip(Xx, 1e-20)) + (2.0 * (b + 2.0 * x0) / Q) * at));
+}
+
+// VWN RPA: para-to-ferro interpolation without spin stiffness (xc.py:403-417)
+template <class N>
+__device__ N vwn_rpa_c_pol(N ra, N rb) {
+  N n = dclip(ra + rb, 1e-20);
+  N zeta = dclip2((ra - rb) / n, -1.0 + 1e-7, 1.0 - 1e-7);
+  N x = dsqrt(dpow(3.0 / (4.0 * kPi * n), 1.0 / 3.0));
+  N ep = vwn_branch(x, 0.0310907, 13.0720, 42.7198, -0.409286);
+  N ef = vwn_branch(x, 0.01554535, 20.1231, 101.578, -0.743294);
+  N fz = (dpow(1.0 + zeta, 4.0 / 3.0) + dpow(1.0 - zeta, 4.0 / 3.0) - 2.0) / (2.5198420997897464 - 2.0);
+  return ep + (ef - ep) * fz;
+}
+
+// Open-shell LYP, Miehlich et al. 1989 (xc.py:489-519), per electron
+template <class N>
+__device__ N lyp_c_pol(N ra, N rb, N gaa, N gab, N gbb) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double C_F = 0.3 * 9.5707800006273;  // (3 pi^2)^(2/3)
+  N ca = dclip(ra, 1e-20), cb = dclip(rb, 1e-20);
+  N rho = ca + cb;
+  N gt = gaa + 2.0 * gab + gbb;
+  N r13 = dpow(rho, -1.0 / 3.0);
+  N g = 1.0 / (1.0 + d * r13);
+  N om = dexp(-c * r13) * dpow(rho, -11.0 / 3.0) * g;
+  N dl = c * r13 + d * r13 * g;
+  N e1 = -a * 4.0 * ca * cb / rho * g;
+  N br = ca * cb * (12.699208415745595 * C_F * (dpow(ca, 8.0 / 3.0) + dpow(cb, 8.0 / 3.0))
+                    + (47.0 / 18.0 - 7.0 / 18.0 * dl) * gt - (2.5 - dl / 18.0) * (gaa + gbb)
+                    - (dl - 11.0) / 9.0 * (ca / rho * gaa + cb / rho * gbb))
+         - (2.0 / 3.0) * rho * rho * gt + ((2.0 / 3.0) * rho * rho - ca * ca) * gbb
+         + ((2.0 / 3.0) * rho * rho - cb * cb) * gaa;
+  return (e1 - a * b * om * br) / rho;
+}
+
+template <class N>
+__device__ N eps_xc_pol(int id, N ra, N rb, N saa, N sab, N sbb) {
+  switch (id) {
+    case kXcPBE: return pbe_x_pol(ra, rb, saa, sbb) + pbe_c_pol(ra, rb, saa + 2.0 * sab + sbb);
+    case kXcPBE0: return 0.75 * pbe_x_pol(ra, rb, saa, sbb) + pbe_c_pol(ra, rb, saa + 2.0 * sab + sbb);
+    default:
+      return 0.08 * lda_x_pol(ra, rb) + 0.72 * b88_x_pol(ra, rb, saa, sbb) + 0.19 * vwn_rpa_c_pol(ra, rb) +
+             0.81 * lyp_c_pol(ra, rb, saa, sab, sbb);
+  }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -1018,4 +1209,97 @@ hipError_t launch_xcf(int nb, int n, const float* ao, const int32_t* cols, const
   hipLaunchKernelGGL(k_xcf_v, dim3((unsigned)(8 * ((ntot + 7) / 8))), dim3(256), 0, st, nb, n, ao, plane, coef, q);
   return hipGetLastError();
 }
+// Spin-polarised pointwise pass, one wavefront per grid point: on entry zwa, zwb
+// hold Z_s = Phi P_s (ng x nao); on exit the W_s of the half potential,
+//   W_a = w (f_ra / 2 phi + (2 f_saa grad ra + f_sab grad rb) . grad phi),
+//   W_b = w (f_rb / 2 phi + (2 f_sbb grad rb + f_sab grad ra) . grad phi),
+// with f = eps_xc_pol * (ra + rb) masked as terms.py:838-874 (points with
+// ra + rb <= thresh count nothing; a channel at or below thresh enters the
+// functional as thresh with zero gradient).
+__global__ __launch_bounds__(256) void k_xcs(int64_t ng, int64_t nao, const double* __restrict__ ao,
+                                             const double* __restrict__ wts, double* __restrict__ zwa,
+                                             double* __restrict__ zwb, int xc_id, double thresh,
+                                             double* __restrict__ exc) {
+  const int lane = threadIdx.x & 63;
+  const int64_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ng) return;
+  const int64_t plane = ng * nao;
+  const double* p0 = ao + g * nao;
+  const double* p1 = p0 + plane;
+  const double* p2 = p0 + 2 * plane;
+  const double* p3 = p0 + 3 * plane;
+  double* za = zwa + g * nao;
+  double* zb = zwb + g * nao;
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t n = lane; n < nao; n += 64) {
+    const double a = za[n], b = zb[n], v0 = p0[n], v1 = p1[n], v2 = p2[n], v3 = p3[n];
+    s[0] += a * v0; s[1] += a * v1; s[2] += a * v2; s[3] += a * v3;
+    s[4] += b * v0; s[5] += b * v1; s[6] += b * v2; s[7] += b * v3;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s[k] = wave_sum(s[k]);
+  const double ra = s[0], rb = s[4];
+  const double ga[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]}, gb[3] = {2.0 * s[5], 2.0 * s[6], 2.0 * s[7]};
+  double aa = 0.0, ab = 0.0, ba[3] = {0, 0, 0}, bb[3] = {0, 0, 0}, e = 0.0;
+  if (ra + rb > thresh) {
+    const bool ta = ra > thresh, tb = rb > thresh;
+    D5 Ra{}, Rb{}, Saa{}, Sab{}, Sbb{}, Rt{};
+    Ra.v = ta ? ra : 1e-10; Ra.d[0] = ta ? 1.0 : 0.0;
+    Rb.v = tb ? rb : 1e-10; Rb.d[1] = tb ? 1.0 : 0.0;
+    Saa.v = ta ? ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2] : 0.0; Saa.d[2] = ta ? 1.0 : 0.0;
+    Sab.v = ta && tb ? ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2] : 0.0; Sab.d[3] = ta && tb ? 1.0 : 0.0;
+    Sbb.v = tb ? gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2] : 0.0; Sbb.d[4] = tb ? 1.0 : 0.0;
+    Rt.v = ra + rb; Rt.d[0] = 1.0; Rt.d[1] = 1.0;      // the unclamped total density
+    const D5 f = eps_xc_pol(xc_id, Ra, Rb, Saa, Sab, Sbb) * Rt;
+    const double w = wts[g];
+    e = w * f.v;
+    aa = 0.5 * w * f.d[0];
+    ab = 0.5 * w * f.d[1];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      ba[x] = w * (2.0 * f.d[2] * ga[x] + f.d[3] * gb[x]);
+      bb[x] = w * (2.0 * f.d[4] * gb[x] + f.d[3] * ga[x]);
+    }
+  }
+  if (lane == 0) exc[g] = e;
+  for (int64_t n = lane; n < nao; n += 64) {
+    const double v0 = p0[n], v1 = p1[n], v2 = p2[n], v3 = p3[n];
+    za[n] = aa * v0 + ba[0] * v1 + ba[1] * v2 + ba[2] * v3;
+    zb[n] = ab * v0 + bb[0] * v1 + bb[1] * v2 + bb[2] * v3;
+  }
+}
+
+hipError_t launch_xcs(int64_t ng, int64_t nao, const double* ao, const double* w, double* zwa, double* zwb,
+                      int xc_id, double thresh, double* exc, hipStream_t st) {
+  if (ng == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xcs, dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, st, ng, nao, ao, w, zwa, zwb, xc_id,
+                     thresh, exc);
+  return hipGetLastError();
+}
+
+// Pointwise spin-polarised functional (parity tests): out[6 i ..] = {f, f_ra, f_rb,
+// f_saa, f_sab, f_sbb} of f = eps_xc_pol (ra, rb, saa, sab, sbb) * (ra + rb), no masking.
+__global__ void k_xc_pointwise_pol(int64_t n, int xc_id, const double* __restrict__ in5,
+                                   double* __restrict__ out6) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  D5 x[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    x[k] = D5{};
+    x[k].v = in5[5 * i + k];
+    x[k].d[k] = 1.0;
+  }
+  const D5 f = eps_xc_pol(xc_id, x[0], x[1], x[2], x[3], x[4]) * (x[0] + x[1]);
+  out6[6 * i] = f.v;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) out6[6 * i + 1 + k] = f.d[k];
+}
+
+hipError_t launch_xc_pointwise_pol(int64_t n, int xc_id, const double* in5, double* out6, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xc_pointwise_pol, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, n, xc_id, in5, out6);
+  return hipGetLastError();
+}
+
 }  // namespace dfks

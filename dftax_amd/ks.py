@@ -182,7 +182,8 @@ class HipDFKS:
     """Native DF Coulomb/exchange + grid XC term (one libdfks handle)."""
 
     def __init__(self, basis: BasisSet, aux: BasisSet, mol: Molecule, grid_xyz, grid_w, xc: XCFunctional,
-                 nelec: int, spec: HipSpec, device=None, chunk="auto", screen: float | None = None):
+                 nelec: int, spec: HipSpec, device=None, chunk="auto", screen: float | None = None,
+                 nspin: int = 1, spin: int = 0):
         torch = _torch()
         self.L = _native.lib()
         self.device = torch.device("cuda") if device is None else torch.device(device)
@@ -193,7 +194,8 @@ class HipDFKS:
         self._gw = np.ascontiguousarray(grid_w, np.float64)
         desc = _native.SystemDesc(self._arrays[0].desc, self._arrays[1].desc, len(self._z), _native.f64p(self._xyz),
                                   _native.f64p(self._z), self._gw.shape[0], _native.f64p(self._gxyz),
-                                  _native.f64p(self._gw), xc.xc_id, int(nelec), 1, 0)
+                                  _native.f64p(self._gw), xc.xc_id, int(nelec), int(nspin), int(spin))
+        self.nspin = int(nspin)
         free = _native.device_free(self.device)
         self.plan = memory_plan(basis.nao, aux.nao, self._gw.shape[0], nelec // 2, spec.precision, chunk,
                                 spec.ao_block, int(0.85 * free))
@@ -242,11 +244,13 @@ class HipDFKS:
         _native.check(self.L.dfks_set_precision(self.handle, int(bool(mixed))))
 
     def energy_fock(self, P):
-        """(E components [8], F) for a device density P (nao x nao)."""
+        """(E components [8], F) for a device density P: nao x nao (nspin = 1), or
+        2 x nao x nao (alpha, beta) for an nspin = 2 handle, F of the same shape."""
         torch = _torch()
         P = P.to(device=self.device, dtype=torch.float64).contiguous()
+        nsp = int(P.shape[0]) if P.dim() == 3 else 1
         F = torch.empty_like(P)
-        _native.check(self.L.dfks_energy_fock(self.handle, ctypes.c_void_p(P.data_ptr()), 1,
+        _native.check(self.L.dfks_energy_fock(self.handle, ctypes.c_void_p(P.data_ptr()), nsp,
                                               ctypes.c_void_p(self._E.data_ptr()), ctypes.c_void_p(F.data_ptr()),
                                               self._stream()))
         return self._E.clone(), F
@@ -290,21 +294,28 @@ class HipDFKS:
 
     def energy(self, P, S=None, nocc=None):
         """dftax's CoulombTerm contract: the J + K + XC energy of a spin-stacked P."""
-        E, _ = self.energy_fock(P[0] if P.dim() == 3 else P)
+        E, _ = self.energy_fock(P[0] if P.dim() == 3 and self.nspin == 1 else P)
         return float(E[2] + E[3] + E[4])
 
 
 class KS:
-    """Closed-shell KS energy functional on the MI355X engine (energy.py:439-723)."""
+    """KS energy functional on the MI355X engine (energy.py:439-723).
+
+    ``spin=None`` follows the molecule: restricted (one doubly occupied channel)
+    when ``system.spin == 0``, else spin-unrestricted with two channels;
+    ``spin=<int>`` always builds the two channels (n_alpha - n_beta = spin), as
+    dftax's KS does (energy.py, the spin dispatch of test_uks.py)."""
 
     def __init__(self, system: Molecule, xc, *, grid: Becke | None = None, coulomb: DFSpec | None = None,
                  spin: int | None = None, device=None):
         torch = _torch()
-        if spin not in (None, 0) or system.spin != 0:
-            raise NotImplementedError("the HIP engine implements the closed-shell (nspin=1) case")
         nelec = system.nelectron
-        if nelec % 2:
+        nspin = 1 if spin is None and system.spin == 0 else 2
+        spin_v = system.spin if spin is None else int(spin)
+        if nspin == 1 and nelec % 2:
             raise ValueError(f"nelec={nelec} is odd but the system has spin=0")
+        if nspin == 2 and (spin_v < 0 or spin_v > nelec or (nelec + spin_v) % 2):
+            raise ValueError(f"inconsistent (nelec={nelec}, spin={spin_v})")
         self.xc = resolve(xc)
         spec = coulomb if coulomb is not None else df()
         if not isinstance(spec, DFSpec):
@@ -313,14 +324,17 @@ class KS:
         self.symbols = tuple(system.symbols)
         self.atom_coords = np.asarray(system.coords)
         self.nelec = nelec
-        self.nocc = (nelec // 2,)
+        self.nspin = nspin
+        self.spin = spin_v
+        self.nocc = (nelec // 2,) if nspin == 1 else ((nelec + spin_v) // 2, (nelec - spin_v) // 2)
         self.basis = build_basis(system.symbols, system.coords, system.basis, spherical=system.spherical)
         self.aux_basis = build_basis(system.symbols, system.coords, spec.auxbasis, spherical=True)
         gspec = grid if grid is not None else becke()
         self.grid_spec = gspec
         self.grid_xyz, self.grid_w = build_grid(system.symbols, system.coords, gspec)
         self.coulomb = HipDFKS(self.basis, self.aux_basis, system, self.grid_xyz, self.grid_w, self.xc, nelec,
-                               spec.engine, device=device, chunk=spec.chunk, screen=spec.screen)
+                               spec.engine, device=device, chunk=spec.chunk, screen=spec.screen, nspin=nspin,
+                               spin=spin_v)
         self.xc_term = None  # XC is evaluated inside the native term
         n = self.coulomb.nao
         self.S = self.coulomb.matrix(_native.DFKS_S, (n, n))
@@ -333,15 +347,21 @@ class KS:
     def nao(self) -> int:
         return self.coulomb.nao
 
+    def _stacked(self, P):
+        if self.nspin == 2:
+            if P.dim() != 3 or P.shape[0] != 2:
+                raise ValueError(f"an open-shell KS takes P of shape (2, nao, nao), got {tuple(P.shape)}")
+            return P
+        return P[0] if P.dim() == 3 else P
+
     def energy_fock(self, P):
-        """(E_total, F) at a density P (device tensor nao x nao or 1 x nao x nao)."""
-        P2 = P[0] if P.dim() == 3 else P
-        E, F = self.coulomb.energy_fock(P2)
+        """(E_total, F) at a density P: nao x nao or 1 x nao x nao (restricted),
+        2 x nao x nao (alpha, beta; F likewise per channel)."""
+        E, F = self.coulomb.energy_fock(self._stacked(P))
         return float(E[0]), F
 
     def electronic(self, P) -> float:
-        P2 = P[0] if P.dim() == 3 else P
-        E, _ = self.coulomb.energy_fock(P2)
+        E, _ = self.coulomb.energy_fock(self._stacked(P))
         return float(E[1] + E[2] + E[3] + E[4])
 
     def total(self, P) -> float:

@@ -42,7 +42,10 @@ class KSResult:
 
 
 def initial_density(ks, guess):
-    """Device P0 for a guess spec or an explicit array (guess.py:602-683)."""
+    """Device P0 for a guess spec or an explicit array (guess.py:602-683): nao x nao
+    for a restricted KS, 2 x nao x nao (alpha, beta) for an open-shell one."""
+    if getattr(ks, "nspin", 1) == 2:
+        return _initial_density_uks(ks, guess)
     torch = __import__("torch")
     n = ks.nao
     if guess is None or isinstance(guess, MinAOSpec):
@@ -59,6 +62,105 @@ def initial_density(ks, guess):
     if tuple(P0.shape) != (n, n):
         raise ValueError(f"guess density has shape {tuple(P0.shape)}, expected {(n, n)}")
     return P0.contiguous()
+
+
+def _orthonormalizer(ks, thresh=1e-7):
+    torch = __import__("torch")
+    s, U = torch.linalg.eigh(ks.S)
+    keep = s > thresh
+    return U[:, keep] / torch.sqrt(s[keep])[None, :]
+
+
+def _initial_density_uks(ks, guess):
+    """Spin-stacked P0 (guess.py:603-626): the minao density renormalised to the
+    electron count and split by the channel occupations, the aufbau fill of the
+    hcore eigenvectors per channel, or an explicit (2, nao, nao) array."""
+    torch = __import__("torch")
+    n = ks.nao
+    na, nb = ks.nocc
+    if guess is None or isinstance(guess, MinAOSpec):
+        name = guess.basis if isinstance(guess, MinAOSpec) else "sto-3g"
+        P = torch.tensor(minao_density(ks.symbols, ks.atom_coords, ks.basis, name), dtype=torch.float64,
+                         device=ks.device)
+        P = P * (ks.nelec / float(torch.sum(P * ks.S)))
+        return torch.stack([(na / ks.nelec) * P, (nb / ks.nelec) * P]).contiguous()
+    if isinstance(guess, CoreSpec):
+        X = _orthonormalizer(ks)
+        _, Cp = torch.linalg.eigh(X.T @ ks.hcore @ X)
+        C = X @ Cp
+        return torch.stack([C[:, :na] @ C[:, :na].T, C[:, :nb] @ C[:, :nb].T]).contiguous()
+    P0 = torch.as_tensor(guess, dtype=torch.float64, device=ks.device)
+    if tuple(P0.shape) != (2, n, n):
+        raise ValueError(f"an open-shell guess density has shape (2, {n}, {n}), got {tuple(P0.shape)}")
+    return P0.contiguous()
+
+
+def _diis_coefficients(errs, count: int, m: int):
+    """Pulay coefficients of the first ``count`` of m stored error vectors (rows of
+    ``errs``), the restated system of scf.py:252-276 (a 1e-10 ridge on the valid
+    diagonal, identity rows for the empty slots); solved on the host."""
+    B = (errs @ errs.T).cpu().numpy()
+    valid = np.arange(m) < count
+    B = np.where(valid[:, None] & valid[None, :], B, np.eye(m)) + 1e-10 * np.diag(valid.astype(float))
+    A = np.zeros((m + 1, m + 1))
+    A[:m, :m] = B
+    A[:m, m] = A[m, :m] = np.where(valid, -1.0, 0.0)
+    rhs = np.zeros(m + 1)
+    rhs[m] = -1.0
+    return np.linalg.solve(A, rhs)[:m]
+
+
+def _scf_uks(ks, *, max_iter, e_tol, d_tol, diis_space, lindep_thresh, level_shift, guess, verbose):
+    """scf.py:279-391 with two spin channels, on the device: the channel-stacked
+    Fock and error matrices share one DIIS, each channel takes the aufbau fill of
+    its own nocc[s] lowest orbitals of X^T F_s X; the Fock builds are the
+    library's (dfks_energy_fock with nspin = 2)."""
+    torch = __import__("torch")
+    S = ks.S
+    X = _orthonormalizer(ks, lindep_thresh)
+    n, nmo = X.shape
+    nocc = ks.nocc
+    m = int(diis_space)
+    P = _initial_density_uks(ks, guess)
+    e_prev, F = ks.energy_fock(P)
+    dF = torch.zeros((m, 2 * n, n), dtype=torch.float64, device=ks.device)
+    dE = torch.zeros((m, 2 * nmo, nmo), dtype=torch.float64, device=ks.device)
+    converged = False
+    it = 0
+    times, hist = [], []
+    eps = C = None
+    while it < max_iter and not converged:
+        t0 = time.perf_counter()
+        FPS = F @ P @ S
+        err = X.T @ (FPS - FPS.transpose(1, 2)) @ X            # S P F = (F P S)^T
+        derr = float(torch.linalg.norm(err))
+        slot = it % m
+        dF[slot] = F.reshape(2 * n, n)
+        dE[slot] = err.reshape(2 * nmo, nmo)
+        c = torch.tensor(_diis_coefficients(dE.reshape(m, -1), min(it + 1, m), m), dtype=torch.float64,
+                         device=ks.device)
+        Fx = torch.einsum("i,iab->ab", c, dF).reshape(2, n, n)
+        if level_shift != 0.0:
+            Fx = Fx + level_shift * (S[None] - S[None] @ P @ S[None])
+        eps, Cp = torch.linalg.eigh(X.T @ Fx @ X)
+        C = X @ Cp                                              # (2, nao, nmo)
+        P = torch.stack([C[s, :, :nocc[s]] @ C[s, :, :nocc[s]].T for s in range(2)]).contiguous()
+        e, F = ks.energy_fock(P)
+        de = e - e_prev
+        converged = abs(de) < e_tol and derr < d_tol
+        times.append(time.perf_counter() - t0)
+        hist.append((e, de, derr, "fp64"))
+        if verbose:
+            print(f"  uks {it}: E={e:.10f} dE={de:+.2e} |[F,P]|={derr:.2e}")
+        e_prev = e
+        it += 1
+    res = KSResult(e_tot=float(e_prev), e_elec=float(e_prev) - ks.e_nn, converged=converged, n_iter=it,
+                   nocc=tuple(nocc), mo_energy=eps, mo_coeff=C, P=P, energy_converged=converged,
+                   iter_times=times, history=hist)
+    if not converged:
+        warnings.warn(f"SCF did NOT converge in {it} iterations (e_tol={e_tol}, d_tol={d_tol}); the returned "
+                      f"energy is unreliable. Increase max_iter or try level_shift>0.", stacklevel=3)
+    return res
 
 
 def _core_density(ks):
@@ -119,6 +221,11 @@ def scf(ks, *, max_iter: int = 128, e_tol: float = 1e-8, d_tol: float = 1e-6, di
     also needs ``|err| < d_tol``) is normally False; use the default for
     densities and orbitals converged to ``d_tol``.
     """
+    if getattr(ks, "nspin", 1) == 2:
+        if precision not in (None, "fp64") or refine is not None:
+            raise ValueError("an open-shell (nspin = 2) SCF runs in FP64")
+        return _scf_uks(ks, max_iter=max_iter, e_tol=e_tol, d_tol=d_tol, diis_space=diis_space,
+                        lindep_thresh=lindep_thresh, level_shift=level_shift, guess=guess, verbose=verbose)
     torch = __import__("torch")
     L = ks.coulomb.L
     h = ks.coulomb.handle

@@ -63,10 +63,13 @@ typedef struct {
   const double* grid_xyz;  /* [3*ngrid] Bohr */
   const double* grid_w;    /* [ngrid] quadrature weights */
   int32_t xc_id;           /* 1 PBE, 2 PBE0, 3 B3LYP */
-  int32_t nelec;           /* electrons; nspin = 1: nocc = nelec / 2 */
+  int32_t nelec;           /* electrons; nspin = 1: nocc = nelec / 2; nspin = 2: n_alpha, n_beta =
+                              (nelec +- spin) / 2 */
   int32_t nspin;           /* spin blocks of P and F (the leading axis of dftax's P[nspin, nao, nao],
-                              dftax/ks/terms.py:520-527): 1 restricted closed shell; 2 spin-unrestricted,
-                              not implemented -- dfks_create returns status 7 */
+                              dftax/ks/terms.py:520-527): 1 restricted closed shell; 2 spin-unrestricted
+                              (alpha, beta): dfks_energy_fock in FP64 on the compact XC grid; the
+                              closed-shell-only calls (device SCF, response, forces, factored Fock,
+                              mixed precision) return status 7 for such a handle */
   int32_t spin;            /* 2S = n_alpha - n_beta (dftax Molecule.spin); must be 0 when nspin = 1 */
 } dfks_system_desc;
 
@@ -140,7 +143,11 @@ int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream s
 /* E(P) and F = sym(dE/dP) = hcore + J - a_x/2 K + Vxc for a symmetric density P
  * (any P: K uses a signed eigen-factorisation of P). P and F hold nspin
  * consecutive nao x nao blocks (dftax's P[nspin, nao, nao]); nspin must equal
- * the handle's (status 1 otherwise). E: device double[8]. */
+ * the handle's (status 1 otherwise). E: device double[8].
+ * nspin = 2 (replaces CoulombTerm.energy + GridXC.energy of terms.py:520-527,
+ * 838-874 for P[2, nao, nao]): J of P_a + P_b, exchange -1/2 a_x sum_s P_s . K(P_s)
+ * (terms.py:500-509), the spin-polarised functional, and F_s = sym(dE/dP_s) =
+ * hcore + J - a_x K(P_s) + Vxc_s. */
 int dfks_energy_fock(dfks_handle h, const double* P, int32_t nspin, double* E, double* F, dfks_stream stream);
 
 /* Same for P = sum_i s_i c_i c_i^T given its factor: Cf column-major
@@ -151,6 +158,12 @@ int dfks_energy_fock_factored(dfks_handle h, const double* P, const double* Cf, 
 /* Pointwise functional f = eps_xc * rho and its derivatives f_rho, f_sigma. */
 int dfks_xc_eval(int32_t xc_id, int64_t n, const double* rho, const double* sigma, double* out3,
                  dfks_stream stream);
+
+/* Spin-polarised pointwise functional (dftax/energy/xc.py spin branches):
+ * in5[5 i ..] = {rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb}, out6[6 i ..] =
+ * {f, f_rho_a, f_rho_b, f_sigma_aa, f_sigma_ab, f_sigma_bb} of
+ * f = eps_xc(rho_a, rho_b, ...) (rho_a + rho_b), unmasked. Device pointers. */
+int dfks_xc_eval_pol(int32_t xc_id, int64_t n, const double* in5, double* out6, dfks_stream stream);
 
 /* Pointwise second derivatives: out6[6 i ..] = {f, f_rho, f_sigma, f_rho rho,
  * f_rho sigma, f_sigma sigma} (the XC kernel of the response below). */

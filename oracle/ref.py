@@ -495,6 +495,158 @@ def xc_energy_t(name, rho, grad, w, thresh=1e-10, mask=None):
 
 
 # ---------------------------------------------------------------------------
+# Spin-polarised functionals (the spin branches of dftax/energy/xc.py): each
+# returns the energy per electron of the total density from the spin densities
+# ra, rb and their gradients ga, gb (ng x 3)
+# ---------------------------------------------------------------------------
+
+VWN_RPA_F = (0.01554535, 20.1231, 101.578, -0.743294)                    # xc.py:392-395 (A, b, c, x0)
+
+
+def _spin_fractions(ra, rb, floor):
+    t = _torch()
+    n = t.clamp(ra + rb, min=floor)
+    return ra / n, rb / n
+
+
+def lda_x_pol(ra, rb):
+    """Spin scaling E_x[ra, rb] = (E_x[2 ra] + E_x[2 rb]) / 2 (xc.py:76-83)."""
+    fa, fb = _spin_fractions(ra, rb, 1e-30)
+    return fa * lda_x(2 * ra) + fb * lda_x(2 * rb)
+
+
+def pbe_x_pol(ra, rb, ga, gb):
+    """xc.py:116-134: the unpolarised form at (2 r_s, 2 grad r_s), mixed by population."""
+    fa, fb = _spin_fractions(ra, rb, 1e-30)
+    return fa * pbe_x(2 * ra, 2 * ga) + fb * pbe_x(2 * rb, 2 * gb)
+
+
+def b88_per_spin(rs, gn):
+    """One spin channel of B88 (xc.py:271-280): rs the channel density, gn |grad rs|."""
+    t = _torch()
+    beta = B88_BETA
+    r = t.clamp(rs, min=1e-20)
+    r13 = r ** (1 / 3)
+    C = 0.75 * (6 / math.pi) ** (1 / 3)
+    x = gn / t.clamp(r ** (4 / 3), min=1e-20)
+    denom = 1.0 + 6 * beta * x * t.asinh(x)
+    return -C * r13 - beta * r13 * x ** 2 / t.clamp(denom, min=1e-20)
+
+
+def b88_x_pol(ra, rb, ga, gb):
+    """xc.py:283-292."""
+    t = _torch()
+    gna = t.linalg.norm(ga + 1e-30, dim=-1)
+    gnb = t.linalg.norm(gb + 1e-30, dim=-1)
+    fa, fb = _spin_fractions(ra, rb, 1e-20)
+    return fa * b88_per_spin(ra, gna) + fb * b88_per_spin(rb, gnb)
+
+
+def pw92_c_pol(rs, zeta):
+    """PW92 with the spin interpolation (xc.py:165-177): rows para, ferro, -alpha_c."""
+    t = _torch()
+    g = []
+    rq = t.sqrt(rs)
+    for row in range(3):
+        A, a1, b1, b2, b3, b4 = _PW[row]
+        denom = 2.0 * A * (b1 * rq + b2 * rs + b3 * rs * rq + b4 * rs ** 2)
+        g.append(-2.0 * A * (1.0 + a1 * rs) * t.log1p(1 / t.clamp(denom, min=1e-20)))
+    fz20 = PBE_C["fz20"]
+    f_z = ((1.0 + zeta) ** (4 / 3) + (1.0 - zeta) ** (4 / 3) - 2.0) / (2.0 ** (4 / 3) - 2.0)
+    return g[0] + zeta ** 4 * f_z * (g[1] - g[0] + g[2] / fz20) - f_z * g[2] / fz20
+
+
+def pbe_c_pol(ra, rb, ga, gb):
+    """PBE correlation of a spin-polarised density (xc.py:179-237, spin branch)."""
+    t = _torch()
+    gamma, beta = PBE_C["gamma"], PBE_C["beta"]
+    ra = t.clamp(ra, min=1e-20)
+    rb = t.clamp(rb, min=1e-20)
+    n = ra + rb
+    zeta = t.clamp((ra - rb) / n, -1.0 + 1e-7, 1.0 - 1e-7)
+    rs = (3 / (4 * math.pi * n) + 1e-20) ** (1 / 3)
+    phi = t.clamp(((1.0 + zeta) ** (2 / 3) + (1.0 - zeta) ** (2 / 3)) / 2, min=1e-20)
+    g = ga + gb
+    kf = (3 * math.pi ** 2 * n + 1e-20) ** (1 / 3)
+    ks = t.sqrt(4 * kf / math.pi)
+    gnorm = t.sqrt(t.sum(g ** 2, dim=-1) + 1e-30)
+    tt = gnorm / t.clamp(2 * phi * ks * n, min=1e-20)
+    eps_pw = pw92_c_pol(rs, zeta)
+    gphi3 = gamma * phi ** 3
+    Acoef = beta / t.clamp(gamma * t.expm1(-eps_pw / gphi3), min=1e-20)
+    f1 = tt ** 2 + Acoef * tt ** 4
+    frac = (beta * f1) / t.clamp(gamma * (1.0 + Acoef * f1), min=1e-20)
+    return eps_pw + gphi3 * t.log1p(frac)
+
+
+def vwn_rpa_c_pol(ra, rb):
+    """VWN RPA: para-to-ferro interpolation without spin stiffness (xc.py:403-417)."""
+    t = _torch()
+    n = t.clamp(ra + rb, min=1e-20)
+    zeta = t.clamp((ra - rb) / n, -1.0 + 1e-7, 1.0 - 1e-7)
+    x = t.sqrt((3.0 / (4.0 * math.pi * n)) ** (1 / 3))
+    ep = _vwn_branch(x, *VWN_RPA_P)
+    ef = _vwn_branch(x, *VWN_RPA_F)
+    f_z = ((1 + zeta) ** (4 / 3) + (1 - zeta) ** (4 / 3) - 2.0) / (2.0 ** (4 / 3) - 2.0)
+    return ep + (ef - ep) * f_z
+
+
+def lyp_c_pol(ra, rb, ga, gb):
+    """Open-shell LYP (Miehlich et al. 1989) per electron (xc.py:489-519)."""
+    t = _torch()
+    a, b, c, d = LYP["a"], LYP["b"], LYP["c"], LYP["d"]
+    C_F = 0.3 * (3.0 * math.pi ** 2) ** (2 / 3)
+    gaa = t.sum(ga * ga, dim=-1)
+    gbb = t.sum(gb * gb, dim=-1)
+    gab = t.sum(ga * gb, dim=-1)
+    ra = t.clamp(ra, min=1e-20)
+    rb = t.clamp(rb, min=1e-20)
+    rho = ra + rb
+    g_tot = gaa + 2.0 * gab + gbb
+    r13 = rho ** (-1 / 3)
+    g = 1.0 / (1.0 + d * r13)
+    omega = t.exp(-c * r13) * rho ** (-11 / 3) * g
+    delta = c * r13 + d * r13 * g
+    e1 = -a * 4.0 * ra * rb / rho * g
+    e2 = -a * b * omega * (
+        ra * rb * (2.0 ** (11 / 3) * C_F * (ra ** (8 / 3) + rb ** (8 / 3))
+                   + (47.0 / 18.0 - 7.0 * delta / 18.0) * g_tot
+                   - (5.0 / 2.0 - delta / 18.0) * (gaa + gbb)
+                   - (delta - 11.0) / 9.0 * (ra / rho * gaa + rb / rho * gbb))
+        - 2.0 / 3.0 * rho ** 2 * g_tot
+        + (2.0 / 3.0 * rho ** 2 - ra ** 2) * gbb
+        + (2.0 / 3.0 * rho ** 2 - rb ** 2) * gaa)
+    return (e1 + e2) / rho
+
+
+def eps_xc_pol(name, ra, rb, ga, gb):
+    if name == "PBE0":
+        return MIX["PBE0_x"] * pbe_x_pol(ra, rb, ga, gb) + pbe_c_pol(ra, rb, ga, gb)
+    if name == "PBE":
+        return pbe_x_pol(ra, rb, ga, gb) + pbe_c_pol(ra, rb, ga, gb)
+    if name == "B3LYP":
+        s, b, v, l = MIX["B3LYP"]
+        return (s * lda_x_pol(ra, rb) + b * b88_x_pol(ra, rb, ga, gb) + v * vwn_rpa_c_pol(ra, rb)
+                + l * lyp_c_pol(ra, rb, ga, gb))
+    raise ValueError(name)
+
+
+def xc_energy_pol_t(name, ra, rb, ga, gb, w, thresh=1e-10):
+    """terms.py:838-874: the total density's mask, and each channel below the
+    threshold clamped to it with its gradient zeroed."""
+    t = _torch()
+    mask = ra + rb > thresh
+    ta = ra > thresh
+    tb = rb > thresh
+    ras = t.where(ta, ra, t.full_like(ra, 1e-10))
+    rbs = t.where(tb, rb, t.full_like(rb, 1e-10))
+    gas = t.where(ta[:, None], ga, t.zeros_like(ga))
+    gbs = t.where(tb[:, None], gb, t.zeros_like(gb))
+    eps = eps_xc_pol(name, ras, rbs, gas, gbs)
+    return t.sum(t.where(mask, w * eps * (ra + rb), t.zeros_like(ra)))
+
+
+# ---------------------------------------------------------------------------
 # The DF-KS energy functional E(P) and its Fock matrix
 # ---------------------------------------------------------------------------
 
@@ -523,6 +675,8 @@ class OracleKS:
         self.grid_xyz, self.grid_w = becke_grid(mol.symbols, mol.coords, spec)
         self.ao, self.dao = eval_ao(self.basis, self.grid_xyz)
         self.nocc = mol.nelectron // 2
+        # spin-polarised channels (alpha, beta) of an open shell, KS(spin=...)
+        self.nocc_ab = ((mol.nelectron + mol.spin) // 2, (mol.nelectron - mol.spin) // 2)
 
     # -- energy pieces --------------------------------------------------------
     def energy_t(self, P):
@@ -550,6 +704,48 @@ class OracleKS:
         grad = 2.0 * t.einsum("gm,mn,gnx->gx", ao, P, dao)
         exc = xc_energy_t(self.xc, rho, grad, w, mask=getattr(self, "xc_mask", None))
         return e1 + eJ + eK + exc + self.e_nn, dict(e1=e1, eJ=eJ, eK=eK, exc=exc)
+
+    def _exchange_matrix_t(self, P):
+        """K(P)_mn = sum (ml|ns) P_ls, exact or density fitted, as a torch function."""
+        t = _torch()
+        if self.eri is not None:
+            return t.einsum("mlns,ls->mn", t.from_numpy(self.eri), P)
+        I3 = t.from_numpy(self.int3c)
+        return t.einsum("mlP,PQ,nsQ,ls->mn", I3, t.from_numpy(self.Vp), I3, P)
+
+    def energy_uks_t(self, Ps):
+        """Total energy of the spin-stacked P (2, nao, nao): one-electron and
+        Coulomb terms of the total density, exchange -1/2 a_x sum_s P_s . K(P_s)
+        (terms.py:500-509), the spin-polarised XC (terms.py:838-874)."""
+        t = _torch()
+        Pa, Pb = Ps[0], Ps[1]
+        P = Pa + Pb
+        e1 = t.sum(P * t.from_numpy(self.hcore))
+        if self.eri is not None:
+            eJ = 0.5 * t.einsum("mn,mnls,ls->", P, t.from_numpy(self.eri), P)
+        else:
+            gamma = t.einsum("mnP,mn->P", t.from_numpy(self.int3c), P)
+            eJ = 0.5 * t.dot(gamma, t.from_numpy(self.Vp) @ gamma)
+        eK = t.zeros((), dtype=t.float64)
+        if self.hf != 0.0:
+            eK = -0.5 * self.hf * (t.sum(Pa * self._exchange_matrix_t(Pa)) + t.sum(Pb * self._exchange_matrix_t(Pb)))
+        ao = t.from_numpy(self.ao)
+        dao = t.from_numpy(self.dao)
+        ra = t.einsum("gm,mn,gn->g", ao, Pa, ao)
+        rb = t.einsum("gm,mn,gn->g", ao, Pb, ao)
+        ga = 2.0 * t.einsum("gm,mn,gnx->gx", ao, Pa, dao)
+        gb = 2.0 * t.einsum("gm,mn,gnx->gx", ao, Pb, dao)
+        exc = xc_energy_pol_t(self.xc, ra, rb, ga, gb, t.from_numpy(self.grid_w))
+        return e1 + eJ + eK + exc + self.e_nn, dict(e1=e1, eJ=eJ, eK=eK, exc=exc)
+
+    def energy_fock_uks(self, Ps):
+        """E and the per-spin Fock matrices F_s = sym(dE/dP_s) (autograd), (2, nao, nao)."""
+        t = _torch()
+        Pt = t.tensor(np.asarray(Ps), dtype=t.float64, requires_grad=True)
+        e, parts = self.energy_uks_t(Pt)
+        (g,) = t.autograd.grad(e, Pt)
+        g = g.numpy()
+        return float(e), 0.5 * (g + g.transpose(0, 2, 1)), {k: float(v) for k, v in parts.items()}
 
     def energy_fock_autograd(self, P):
         t = _torch()
@@ -743,6 +939,51 @@ def scf(oks, max_iter=128, e_tol=1e-8, d_tol=1e-6, m=8, P0=None, fock=None, verb
         e_prev = e
         it += 1
     return dict(e_tot=e_prev, P=P, C=C, eps=eps, converged=conv, n_iter=it, hist=hist)
+
+
+def scf_uks(oks, max_iter=128, e_tol=1e-8, d_tol=1e-6, m=8, P0=None, verbose=False):
+    """scf.py:279-391 with nspin = 2: DIIS on the channel-stacked Fock and error
+    matrices, aufbau per channel; the minao guess split by the channel
+    occupations (guess.py:603-619)."""
+    S = oks.S
+    X = canonical_orthonormalizer(S)
+    nao, nmo = X.shape
+    na, nb = oks.nocc_ab
+    N = oks.mol.nelectron
+    if P0 is None:
+        P = minao_density(oks.mol, oks.basis)
+        P = P * (N / np.sum(P * S))
+        P0 = np.stack([(na / N) * P, (nb / N) * P])
+    Ps = np.asarray(P0)
+    e_prev, F, _ = oks.energy_fock_uks(Ps)
+    dF = np.zeros((m, 2 * nao, nao))
+    dE = np.zeros((m, 2 * nmo, nmo))
+    conv = False
+    it = 0
+    hist = []
+    C = eps = None
+    while it < max_iter and not conv:
+        err = np.stack([X.T @ (F[s] @ Ps[s] @ S - S @ Ps[s] @ F[s]) @ X for s in range(2)])
+        derr = float(np.linalg.norm(err))
+        slot = it % m
+        dF[slot] = F.reshape(2 * nao, nao)
+        dE[slot] = err.reshape(2 * nmo, nmo)
+        Fx = diis_extrapolate(dF, dE, min(it + 1, m), m).reshape(2, nao, nao)
+        eps, C = [], []
+        for s_, n in zip(range(2), (na, nb)):
+            e_s, Cp = np.linalg.eigh(X.T @ Fx[s_] @ X)
+            eps.append(e_s)
+            C.append(X @ Cp)
+        Ps = np.stack([C[0][:, :na] @ C[0][:, :na].T, C[1][:, :nb] @ C[1][:, :nb].T])
+        e, F, _ = oks.energy_fock_uks(Ps)
+        de = e - e_prev
+        conv = abs(de) < e_tol and derr < d_tol
+        hist.append((e, de, derr))
+        if verbose:
+            print(f"  uks {it}: E={e:.10f} dE={de:+.2e} |[F,P]|={derr:.2e}")
+        e_prev = e
+        it += 1
+    return dict(e_tot=e_prev, P=Ps, C=C, eps=eps, converged=conv, n_iter=it, hist=hist)
 
 
 # ---------------------------------------------------------------------------

@@ -70,14 +70,16 @@ def _water_desc(nspin, spin):
 
 def test_nspin_in_the_descriptor():
     """P[nspin, nao, nao] (dftax/ks/terms.py:520-527): the descriptor carries
-    nspin; nspin = 2 is refused with its own status before any device work,
-    and nspin = 1 needs spin = 0."""
+    nspin and spin = n_alpha - n_beta; inconsistent pairs are refused before
+    any device work: nspin = 1 needs spin = 0, nspin = 2 needs nelec + spin
+    even and 0 <= spin <= nelec (water has 10 electrons)."""
     from dftax_amd import _native
     L = _native.lib()
     h = ctypes.c_void_p()
-    desc, _keep = _water_desc(2, 0)
-    assert L.dfks_create(ctypes.byref(desc), None, ctypes.byref(h)) == 7
-    assert b"nspin = 2" in L.dfks_last_error()
+    for spin in (1, -2, 12):
+        desc, _keep = _water_desc(2, spin)
+        assert L.dfks_create(ctypes.byref(desc), None, ctypes.byref(h)) == 1
+        assert b"nspin = 2 needs" in L.dfks_last_error()
     desc, _keep = _water_desc(1, 2)
     assert L.dfks_create(ctypes.byref(desc), None, ctypes.byref(h)) == 1
     assert b"spin = 0" in L.dfks_last_error()

@@ -443,3 +443,140 @@ def test_golden_forces_are_consistent():
                     d = R[a] - R[b]
                     Fnn[a] += zs[a] * zs[b] * d / np.linalg.norm(d) ** 3
         assert np.abs(Fp[0] - Fnn).max() < 1e-8
+
+
+# ---------------------------------------------------------------------------
+# Spin-polarised functionals (the open-shell path): limits and the independent
+# open-shell LYP
+# ---------------------------------------------------------------------------
+
+def _spin_points(n=40, seed=11):
+    torch = _torch()
+    rng = np.random.default_rng(seed)
+    ra = torch.tensor(10.0 ** rng.uniform(-3, 1.5, n))
+    rb = torch.tensor(10.0 ** rng.uniform(-3, 1.5, n))
+    ga = torch.tensor(rng.normal(size=(n, 3))) * ra[:, None] ** (4 / 3)
+    gb = torch.tensor(rng.normal(size=(n, 3))) * rb[:, None] ** (4 / 3)
+    return ra, rb, ga, gb
+
+
+def test_polarised_functionals_reduce_to_closed_shell():
+    """rho_a = rho_b = rho/2, grad rho_s = grad rho/2: every spin-polarised form
+    equals the restricted one (structure of the spin branches, xc.py)."""
+    torch = _torch()
+    ra, _, ga, _ = _spin_points()
+    rho, g = 2 * ra, 2 * ga
+    for name in ("PBE", "PBE0", "B3LYP"):
+        a = ref.eps_xc(name, rho, g)
+        b = ref.eps_xc_pol(name, ra, ra, ga, ga)
+        assert float((a - b).abs().max()) <= 1e-13 * float(a.abs().max()), name
+
+
+def test_lyp_open_shell_matches_independent_form(monkeypatch):
+    """The oracle's open-shell LYP against the form written out above at
+    rho_a != rho_b (Miehlich et al. 1989)."""
+    torch = _torch()
+    ra, rb, ga, gb = _spin_points()
+
+    def check():
+        e_or = (ref.lyp_c_pol(ra, rb, ga, gb) * (ra + rb)).numpy()
+        gaa = torch.sum(ga * ga, -1).numpy()
+        gbb = torch.sum(gb * gb, -1).numpy()
+        gab = torch.sum(ga * gb, -1).numpy()
+        e_in = _lyp_open_shell(ra.numpy(), rb.numpy(), gaa, gbb, gab)
+        assert np.all(np.abs(e_or - e_in) <= 1e-11 * np.abs(e_in)), np.abs(e_or - e_in).max()
+    check()
+    for key in ("a", "b", "c", "d"):
+        monkeypatch.setitem(ref.LYP, key, ref.LYP[key] * (1 + 1e-6))
+        with pytest.raises(AssertionError):
+            check()
+        monkeypatch.setitem(ref.LYP, key, ref.LYP[key] / (1 + 1e-6))
+
+
+def _slope_ln_rs(fn, r1=1e-10, r2=1e-11):
+    return (fn(r1) - fn(r2)) / (math.log(r1) - math.log(r2))
+
+
+def test_polarised_correlation_high_density_limits(monkeypatch):
+    """RPA high-density limits: the fully polarised gas has eps_c -> (c0/2) ln rs
+    with c0 = (1 - ln 2)/pi^2 (PW92's and VWN's ferromagnetic A), and the spin
+    stiffness alpha_c(rs) = d^2 eps_c / d zeta^2 at zeta = 0 goes as
+    -(1/(6 pi^2)) ln rs (PW92's third row)."""
+    torch = _torch()
+    c0 = (1 - math.log(2)) / math.pi ** 2
+
+    def pw(rs, zeta):
+        return float(ref.pw92_c_pol(torch.tensor([rs], dtype=torch.float64),
+                                    torch.tensor([zeta], dtype=torch.float64))[0])
+
+    def stiffness(rs, z=1e-3):
+        return 2.0 * (pw(rs, z) - pw(rs, 0.0)) / z ** 2
+
+    def vwn_ferro(rs):
+        rho = _rho_of_rs(rs)
+        return float(ref.vwn_rpa_c_pol(torch.tensor([rho]), torch.tensor([0.0]))[0])
+
+    def check():
+        assert abs(_slope_ln_rs(lambda r: pw(r, 1.0)) - c0 / 2) < 1e-6 * c0
+        assert abs(_slope_ln_rs(stiffness, 1e-9, 1e-10) + 1 / (6 * math.pi ** 2)) < 2e-5 / (6 * math.pi ** 2)
+        assert abs(_slope_ln_rs(vwn_ferro) - c0 / 2) < 1e-6 * c0
+    check()
+    for row in (1, 2):
+        bad = ref._PW.copy()
+        bad[row, 0] *= 1 + 1e-4
+        monkeypatch.setattr(ref, "_PW", bad)
+        with pytest.raises(AssertionError):
+            check()
+        monkeypatch.setattr(ref, "_PW", ref._PW)
+    monkeypatch.setattr(ref, "VWN_RPA_F", (0.01554535 * (1 + 1e-4), 20.1231, 101.578, -0.743294))
+    with pytest.raises(AssertionError):
+        check()
+
+
+def test_polarised_pbe_correlation_gradient_expansion(monkeypatch):
+    """H(rs, zeta, t) -> beta phi^3 t^2 as t -> 0 (PBE 1996), with
+    phi = ((1 + zeta)^(2/3) + (1 - zeta)^(2/3)) / 2 and t = |grad rho| / (2 phi k_s rho)."""
+    torch = _torch()
+
+    def check():
+        for ra_, rb_ in ((0.3, 0.1), (2.0, 0.05), (1e-2, 8e-3)):
+            rho = ra_ + rb_
+            zeta = (ra_ - rb_) / rho
+            phi = ((1 + zeta) ** (2 / 3) + (1 - zeta) ** (2 / 3)) / 2
+            ks = math.sqrt(4 * (3 * math.pi ** 2 * rho) ** (1 / 3) / math.pi)
+            t = 1e-4
+            gtot = t * 2 * phi * ks * rho
+            ga = torch.tensor([[0.0, 0.0, gtot * ra_ / rho]], dtype=torch.float64)
+            gb = torch.tensor([[0.0, 0.0, gtot * rb_ / rho]], dtype=torch.float64)
+            a, b = torch.tensor([ra_]), torch.tensor([rb_])
+            z = torch.zeros((1, 3), dtype=torch.float64)
+            h = float(ref.pbe_c_pol(a, b, ga, gb)[0] - ref.pbe_c_pol(a, b, z, z)[0])
+            assert abs(h / t ** 2 - 0.066725 * phi ** 3) < 1e-5 * phi ** 3, (ra_, rb_, h / t ** 2)
+    check()
+    monkeypatch.setitem(ref.PBE_C, "beta", 0.06672455060314922 * (1 + 1e-3))
+    with pytest.raises(AssertionError):
+        check()
+
+
+def test_lda_exchange_full_polarisation():
+    """eps_x of a fully polarised gas is 2^(1/3) times the unpolarised one at the
+    same density (exchange spin scaling)."""
+    torch = _torch()
+    rho = torch.tensor(10.0 ** np.linspace(-3, 2, 12))
+    a = ref.lda_x_pol(rho, torch.zeros_like(rho))
+    b = 2 ** (1 / 3) * ref.lda_x(rho)
+    assert float((a - b).abs().max()) <= 1e-14 * float(b.abs().max())
+
+
+def test_uks_energy_and_fock_reduce_to_rks():
+    """P_a = P_b = P/2: the oracle's spin-stacked energy and both channels' Fock
+    matrices equal the restricted ones (terms.py:500-509, 838-874)."""
+    from dftax_amd.grid import becke
+    mol = Molecule.from_xyz("O 0 0 0; H 0.757 0.587 0; H -0.757 0.587 0", "sto-3g")
+    oks = ref.OracleKS(mol, "B3LYP", grid=becke(30, 110))
+    P = ref.minao_density(mol, oks.basis)
+    P *= mol.nelectron / np.sum(P * oks.S)
+    e_r, F_r, _ = oks.energy_fock_autograd(P)
+    e_u, F_u, _ = oks.energy_fock_uks(np.stack([P / 2, P / 2]))
+    assert abs(e_u - e_r) < 1e-10
+    assert np.abs(F_u - F_r[None]).max() < 1e-10
