@@ -29,7 +29,7 @@ CFLAGS = [
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
 LDFLAGS = ["-shared", f"--offload-arch={ARCH}", "-L/opt/rocm/lib", "-lrocblas", "-lrocsolver",
-           "-Wl,-rpath,/opt/rocm/lib"]
+           "-Wl,-rpath,/opt/rocm/lib", "-Wl,--no-undefined"]
 
 
 def sources() -> list[Path]:

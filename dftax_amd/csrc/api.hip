@@ -87,9 +87,10 @@ hipError_t launch_diis_solve(int m, int count, const double* gram, double* coef,
 hipError_t launch_diis_extrap(int m, int64_t n, const double* coef, const double* Fs, double* out, hipStream_t st);
 hipError_t launch_gram_update(int m, int slot, int count, const double* dots, double* gram, hipStream_t st);
 hipError_t launch_scaled_cols(int n, int k0, const double* U, const double* w, double* X, hipStream_t st);
-hipError_t launch_xcf(int nb, int n, const float* ao, const int32_t* cols, const float* P, int64_t nao,
-                      const double* w, int xc_id, double thresh, double* exc, double* ws, float* q, hipStream_t st);
-int64_t xcf_ws_doubles(int nb, int n);
+hipError_t launch_xcf_all(const XcfBin* tab, int nbins, int ztot, int ptot, int vtot, const float* ao,
+                          const int32_t* cols, const float* P, int64_t nao, const double* w, int xc_id, double thresh,
+                          double* exc, double* part, float* coef, float* q, hipStream_t st);
+void xcf_bin_geometry(int n, int nb, int& nt, int& nks, int64_t& part_doubles, int64_t& coef_floats);
 hipError_t launch_xc_f32(int64_t ng, int64_t nao, const float* ao, const double* w, float* zw, int xc_id,
                          double thresh, double* exc, hipStream_t st);
 hipError_t launch_gamma_f32(int nao, int nq, int nf, const float* T, const double* Cf, const double* sgn,
@@ -544,6 +545,15 @@ struct dfks_ctx {
   bool xcb = false;
   std::vector<XcBin> xbins;
   int64_t xc_npts = 0, xc_zw = 0, xc_nblk = 0, xc_qtot = 0;
+  // the fused mixed-precision XC pass over all eligible bins at once: the bin table
+  // (host copy kept for the upload), its device copy, the workspace, launch sizes
+  std::vector<XcfBin> xcf_tab;
+  std::vector<char> xcf_fused;  // per bin of xbins: taken by the fused launch
+  XcfBin* d_xcf_tab = nullptr;
+  double* d_xcf_ws = nullptr;
+  int64_t xcf_part = 0;
+  int xcf_ztot = 0, xcf_ptot = 0, xcf_vtot = 0;
+  bool xcf_ready = false;
   // spin-polarised (nspin = 2) Fock builds: beta-channel XC buffers, the total density,
   // J of the total density, K per channel, gamma of the total density
   int nocc_b = 0, spin = 0;
@@ -598,7 +608,8 @@ struct dfks_ctx {
                     d_sp2, d_lz, d_lohi, d_dvec, d_L, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
                     d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf, d_xc_pts, d_xc_w, d_xc_exc, d_xc_ao,
                     d_xc_zw, d_xc_q, d_xc_ao32, d_xc_cols, d_xc_vlen, d_xc_shl, d_xc_shoff, d_xc_vrow,
-                    d_xc_vprow, d_xc_vpcol, d_xc_zw2, d_xc_q2, d_Vxc2, d_uP, d_uJ, d_uK, d_ug};
+                    d_xc_vprow, d_xc_vpcol, d_xc_zw2, d_xc_q2, d_Vxc2, d_uP, d_uJ, d_uK, d_ug, d_xcf_tab,
+                    d_xcf_ws};
     for (void* p : ptrs) dfree(p);
     void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, t32_alias ? nullptr : d_T32, d_Cf32, d_P32, d_ws32,
                       d_eps32, d_offd32, d_slab32, d_Lt32, d_Tslab32, d_aob32, d_zwb32};
@@ -1471,6 +1482,43 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
 //   kXcResponse: Z1 = Phi dP, the response kernel with the records in d_fxc (FP64)
 enum XcMode { kXcEnergy = 0, kXcRho = 1, kXcResponse = 2 };
 
+// The table of the fused mixed XC launch (once per handle: the bins are fixed at
+// build): every bin up to kXfMaxN wide, its offsets and first workgroup ids; the
+// workspace holds all bins' partial sums, then their W coefficients.
+void xcf_prepare(dfks_ctx* h, hipStream_t st) {
+  if (h->xcf_ready) return;
+  h->xcf_tab.clear();
+  h->xcf_fused.assign(h->xbins.size(), 0);
+  int64_t part = 0, coef = 0, wz = 0, wp = 0, wv = 0;
+  for (size_t i = 0; i < h->xbins.size(); ++i) {
+    const auto& bn = h->xbins[i];
+    if (bn.n > kXfMaxN || bn.n % 32 != 0) continue;
+    int nt = 0, nks = 0;
+    int64_t pd = 0, cf = 0;
+    xcf_bin_geometry(bn.n, (int)bn.nblk, nt, nks, pd, cf);
+    XcfBin e{bn.n, (int32_t)bn.nblk, nt, nks, bn.ao_off, bn.col_off, bn.q_off, bn.pt0, part, coef,
+             (int32_t)wz, (int32_t)wp, (int32_t)wv, 0};
+    h->xcf_tab.push_back(e);
+    h->xcf_fused[i] = 1;
+    part += pd;
+    coef += cf;
+    wz += bn.nblk * nt * nks;
+    wp += bn.nblk;
+    wv += bn.nblk * nt * nt;
+  }
+  if (wz > 0x7fffffff / 2 || wv > 0x7fffffff / 2) fail(1, "compact XC grid too large for the fused launch");
+  h->xcf_ztot = (int)wz;
+  h->xcf_ptot = (int)wp;
+  h->xcf_vtot = (int)wv;
+  h->xcf_part = part;
+  h->d_xcf_ws = dalloc<double>((size_t)(part + (coef + 1) / 2 + 2));
+  h->d_xcf_tab = dalloc<XcfBin>(std::max<size_t>(1, h->xcf_tab.size()));
+  if (!h->xcf_tab.empty())
+    HIPCHK(hipMemcpyAsync(h->d_xcf_tab, h->xcf_tab.data(), sizeof(XcfBin) * h->xcf_tab.size(),
+                          hipMemcpyHostToDevice, st));
+  h->xcf_ready = true;
+}
+
 void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t st) {
   const int nao = h->nao;
   const bool mixed = h->mixed_now && mode == kXcEnergy;
@@ -1478,22 +1526,28 @@ void xc_compact(dfks_ctx* h, const double* P, double* E, int mode, hipStream_t s
   const double thresh = h->opt.rho_thresh;
   // mixed: the bins gather from an FP32 copy of P (the same rounding as converting each gathered element)
   if (mixed) HIPCHK(launch_to_f32((int64_t)nao * nao, P, h->d_P32, st));
-  for (const auto& bn : h->xbins) {
+  // mixed, library kernels: every bin up to kXfMaxN wide in one launch of each fused kernel
+  const bool fused = mixed && h->opt.gemm != 1;
+  if (fused) {
+    xcf_prepare(h, st);
+    HIPCHK(launch_xcf_all(h->d_xcf_tab, (int)h->xcf_tab.size(), h->xcf_ztot, h->xcf_ptot, h->xcf_vtot,
+                          h->d_xc_ao32, h->d_xc_cols, h->d_P32, nao, h->d_xc_w, h->xc_id, thresh, h->d_xc_exc,
+                          h->d_xcf_ws, reinterpret_cast<float*>(h->d_xcf_ws + h->xcf_part),
+                          reinterpret_cast<float*>(h->d_xc_q), st));
+  }
+  for (size_t bi = 0; bi < h->xbins.size(); ++bi) {
+    if (fused && h->xcf_fused[bi]) continue;
+    const auto& bn = h->xbins[bi];
     const int n = bn.n;
     const int64_t nb = bn.nblk, rows = nb * kXcG;
     const int32_t* cols = h->d_xc_cols + bn.col_off;
     if (mixed) {
-      // options.gemm = rocblas: FP32 operands and FP32 MFMA products (rocBLAS strided batched)
+      // options.gemm = rocblas (or a bin wider than kXfMaxN): FP32 operands and FP32 MFMA products
+      // (rocBLAS strided batched)
       const float onef = 1.0f, zerof = 0.0f;
       float* q = reinterpret_cast<float*>(h->d_xc_q) + bn.q_off;
       const float* ao = h->d_xc_ao32 + bn.ao_off;
       float* zw = reinterpret_cast<float*>(h->d_xc_zw);
-      if (h->opt.gemm != 1 && n <= kXfMaxN && xcf_ws_doubles((int)nb, n) <= h->xc_zw) {
-        // fused split-BF16 kernels: P gathered while staging, Z reduced to rho in LDS, W formed while staging
-        HIPCHK(launch_xcf((int)nb, n, ao, cols, h->d_P32, nao, h->d_xc_w + bn.pt0, h->xc_id, thresh,
-                          h->d_xc_exc + bn.pt0, h->d_xc_zw, q, st));
-        continue;
-      }
       HIPCHK((launch_gather_sym<float, float>(nb, n, cols, h->d_P32, nao, q, st)));
       RBCHK(rocblas_sgemm_strided_batched(h->rb, rocblas_operation_transpose, rocblas_operation_none, n, kXcG, n,
                                           &onef, q, n, (rocblas_stride)n * n, ao, n, (rocblas_stride)kXcG * n,

@@ -38,6 +38,16 @@ constexpr int kSp2State = kSpPart + 2 * kSp2Groups;
 constexpr int kSp2Block = 4;          // purification iterations per graph launch (divides kSp2MaxIter)
 // widest compact-grid block (functions) the fused mixed-precision XC kernels take
 constexpr int kXfMaxN = 2048;
+// One bin of the compact XC grid inside a single fused mixed-precision launch: its
+// width n (functions per block), blocks, Z-column tiles and K chunks, the offsets of
+// its AO rows (floats), column lists (ints), block products (floats), points, partial
+// sums (doubles) and W coefficients (floats), and the first workgroup id of its
+// blocks in each of the three kernels (Z tiles, per-point pass, Vh tiles).
+struct XcfBin {
+  int32_t n, nb, nt, nks;
+  int64_t ao_off, col_off, q_off, pt0, part_off, coef_off;
+  int32_t wz0, wp0, wv0, pad;
+};
 constexpr int kPanel = 32;            // pivoted Cholesky panel width
 constexpr int kLzSteps = 24;          // Lanczos steps for the SP2 spectral bounds
 constexpr int kLzParts = 4096;        // partial dot products of its matrix-vector kernel (n <= 4 kLzParts)

@@ -925,10 +925,10 @@ static_assert(64 * kXfZPad * 4 <= 4 * 128 * kXfPad * 2, "Z half tile fits the st
 // and Z = Phi Pg are linear in both, so the tiles and K chunks of a block run as
 // separate workgroups and their sums add up (k_xcf_pt, in a fixed order).
 // part[((b nt + t) nks + ks)][4][128]: sum_c phi Z, sum_c d_x phi Z, d_y, d_z.
-__global__ __launch_bounds__(256, 2) void k_xcf_z(int nb, int n, int nt, int nks, int kchunk,
-                                                  const float* __restrict__ ao, int64_t plane,
-                                                  const int32_t* __restrict__ cols, const float* __restrict__ P,
-                                                  int64_t nao, double* __restrict__ part) {
+__global__ __launch_bounds__(256, 2) void k_xcf_z(const XcfBin* __restrict__ tab, int nbins, int ztot, int kchunk,
+                                                  const float* __restrict__ ao_base,
+                                                  const int32_t* __restrict__ cols_base, const float* __restrict__ P,
+                                                  int64_t nao, double* __restrict__ part_base) {
   // staging tiles (A: points x 32 functions, B: 128 Z columns x 32 functions, bf16 hi/lo),
   // reused after the K loop for half of the Z tile in FP32
   __shared__ __attribute__((aligned(16))) __bf16 s_stage[4][128][kXfPad];
@@ -939,12 +939,19 @@ __global__ __launch_bounds__(256, 2) void k_xcf_z(int nb, int n, int nt, int nks
   auto sBh = s_stage[2];
   auto sBl = s_stage[3];
   float (*sZ)[kXfZPad] = reinterpret_cast<float (*)[kXfZPad]>(&s_stage[0][0][0]);
-  const int per_blk = nt * nks, ntot = nb * per_blk;
   // workgroup x runs on XCD x % 8: each XCD takes a contiguous range of ids, so the
   // workgroups of one block share its AO rows and P's rows through one L2
-  const int per = (ntot + 7) / 8;
-  const int id = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-  if (id >= ntot) return;
+  const int per = (ztot + 7) / 8;
+  const int gid = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (gid >= ztot) return;
+  int bi = 0;
+  while (bi + 1 < nbins && tab[bi + 1].wz0 <= gid) ++bi;
+  const int n = tab[bi].n, nt = tab[bi].nt, nks = tab[bi].nks;
+  const int64_t plane = (int64_t)tab[bi].nb * kXfG * n;
+  const float* ao = ao_base + tab[bi].ao_off;
+  const int32_t* cols = cols_base + tab[bi].col_off;
+  double* part = part_base + tab[bi].part_off;
+  const int id = gid - tab[bi].wz0, per_blk = nt * nks;
   const int64_t blk = id / per_blk;
   const int t = (id % per_blk) / nks, ks = id % nks;
   const int c0 = 128 * t, kb = ks * kchunk, ke = min(n, kb + kchunk);
@@ -1052,12 +1059,19 @@ __global__ __launch_bounds__(256, 2) void k_xcf_z(int nb, int n, int nt, int nks
 
 // Per point: the partial sums of its block in (tile, K chunk) order, then the
 // functional: exc = w f, coef = {w f_rho / 2, 2 w f_sigma grad rho}.
-__global__ __launch_bounds__(128) void k_xcf_pt(int nb, int nparts, const double* __restrict__ part,
-                                                const double* __restrict__ wts, int xc_id, double thresh,
-                                                double* __restrict__ exc, float* __restrict__ coef) {
-  const int64_t gg = blockIdx.x * (int64_t)kXfG + threadIdx.x;   // one workgroup per block
-  if (blockIdx.x >= (unsigned)nb) return;
-  const double* src = part + (int64_t)blockIdx.x * nparts * 4 * kXfG + threadIdx.x;
+__global__ __launch_bounds__(128) void k_xcf_pt(const XcfBin* __restrict__ tab, int nbins, int ptot,
+                                                const double* __restrict__ part_base,
+                                                const double* __restrict__ w_base, int xc_id, double thresh,
+                                                double* __restrict__ exc_base, float* __restrict__ coef_base) {
+  if (blockIdx.x >= (unsigned)ptot) return;       // one workgroup per block
+  int bi = 0;
+  while (bi + 1 < nbins && tab[bi + 1].wp0 <= (int)blockIdx.x) ++bi;
+  const int lb = (int)blockIdx.x - tab[bi].wp0, nparts = tab[bi].nt * tab[bi].nks;
+  const int64_t gg = lb * (int64_t)kXfG + threadIdx.x;
+  const double* wts = w_base + tab[bi].pt0;
+  double* exc = exc_base + tab[bi].pt0;
+  float* coef = coef_base + tab[bi].coef_off;
+  const double* src = part_base + tab[bi].part_off + (int64_t)lb * nparts * 4 * kXfG + threadIdx.x;
   double r = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
   for (int pp = 0; pp < nparts; ++pp) {
     const double* s = src + (int64_t)pp * 4 * kXfG;
@@ -1082,19 +1096,26 @@ __global__ __launch_bounds__(128) void k_xcf_pt(int nb, int nparts, const double
   reinterpret_cast<float4*>(coef)[gg] = make_float4((float)a, (float)bx, (float)by, (float)bz);
 }
 
-__global__ __launch_bounds__(256, 2) void k_xcf_v(int nb, int n, const float* __restrict__ ao, int64_t plane,
-                                                  const float* __restrict__ coef, float* __restrict__ q) {
+__global__ __launch_bounds__(256, 2) void k_xcf_v(const XcfBin* __restrict__ tab, int nbins, int vtot,
+                                                  const float* __restrict__ ao_base,
+                                                  const float* __restrict__ coef_base, float* __restrict__ q_base) {
   __shared__ __attribute__((aligned(16))) __bf16 sAh[128][kXfPad];
   __shared__ __attribute__((aligned(16))) __bf16 sAl[128][kXfPad];
   __shared__ __attribute__((aligned(16))) __bf16 sBh[128][kXfPad];
   __shared__ __attribute__((aligned(16))) __bf16 sBl[128][kXfPad];
-  const int nt = (n + 127) / 128, per_blk = nt * nt;
-  const int ntot = nb * per_blk;
   // workgroup x runs on XCD x % 8: each XCD takes a contiguous range of (block, tile)
   // ids, so the tiles of one block read its AO rows through one L2
-  const int per = (ntot + 7) / 8;
-  const int id = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-  if (id >= ntot) return;
+  const int per = (vtot + 7) / 8;
+  const int gid = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (gid >= vtot) return;
+  int bi = 0;
+  while (bi + 1 < nbins && tab[bi + 1].wv0 <= gid) ++bi;
+  const int n = tab[bi].n, nt = tab[bi].nt, per_blk = nt * nt;
+  const int64_t plane = (int64_t)tab[bi].nb * kXfG * n;
+  const float* ao = ao_base + tab[bi].ao_off;
+  const float* coef = coef_base + tab[bi].coef_off;
+  float* q = q_base + tab[bi].q_off;
+  const int id = gid - tab[bi].wv0;
   const int64_t blk = id / per_blk;
   const int t = id % per_blk, m0 = (t / nt) * 128, n0 = (t % nt) * 128;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1176,39 +1197,40 @@ __global__ __launch_bounds__(256, 2) void k_xcf_v(int nb, int n, const float* __
       }
 }
 
-// The fused pass over one bin (nb blocks of width n <= kXfMaxN, a multiple of
-// 32). ws: the partial sums and the W coefficients, nb (512 nt nks + 256) doubles
-// (at most nb 128 n for n >= 32); q: nb n^2 floats.
+// The fused pass over every bin of a table in one launch per kernel: bins of
+// width n <= kXfMaxN (a multiple of 32); tab (device) as XcfBin above, with ztot,
+// ptot, vtot the three kernels' workgroup counts; part the partial sums, coef the W
+// coefficients (4 floats per point), q the block products (floats).
 constexpr int kXfKChunk = 384;   // functions of K per k_xcf_z workgroup
 
-int64_t xcf_ws_doubles(int nb, int n) {
-  const int nt = (n + 127) / 128, nks = (n + kXfKChunk - 1) / kXfKChunk;
-  return (int64_t)nb * (4 * kXfG * nt * nks + 2 * kXfG);
-}
-
-hipError_t launch_xcf(int nb, int n, const float* ao, const int32_t* cols, const float* P, int64_t nao,
-                      const double* w, int xc_id, double thresh, double* exc, double* ws, float* q, hipStream_t st) {
-  if (nb == 0) return hipSuccess;
-  if (n % 32 != 0 || n > kXfMaxN || (uintptr_t)ao % 16 != 0 || (uintptr_t)ws % 16 != 0)
-    return hipErrorInvalidValue;
-  const int64_t plane = (int64_t)nb * kXfG * n;
-  const int nt = (n + 127) / 128, nks = (n + kXfKChunk - 1) / kXfKChunk;
-  const int64_t nz = (int64_t)nb * nt * nks;
-  double* part = ws;
-  float* coef = reinterpret_cast<float*>(ws + nz * 4 * kXfG);
+hipError_t launch_xcf_all(const XcfBin* tab, int nbins, int ztot, int ptot, int vtot, const float* ao,
+                          const int32_t* cols, const float* P, int64_t nao, const double* w, int xc_id, double thresh,
+                          double* exc, double* part, float* coef, float* q, hipStream_t st) {
+  if (nbins == 0) return hipSuccess;
+  if ((uintptr_t)ao % 16 != 0 || (uintptr_t)coef % 16 != 0) return hipErrorInvalidValue;
   // 8 ceil(total / 8) workgroups: the XCD mappings produce every id < total only then
-  hipLaunchKernelGGL(k_xcf_z, dim3((unsigned)(8 * ((nz + 7) / 8))), dim3(256), 0, st, nb, n, nt, nks, kXfKChunk, ao,
-                     plane, cols, P, nao, part);
+  hipLaunchKernelGGL(k_xcf_z, dim3((unsigned)(8 * (((int64_t)ztot + 7) / 8))), dim3(256), 0, st, tab, nbins, ztot,
+                     kXfKChunk, ao, cols, P, nao, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_xcf_pt, dim3((unsigned)nb), dim3(kXfG), 0, st, nb, nt * nks, part, w, xc_id, thresh, exc,
-                     coef);
+  hipLaunchKernelGGL(k_xcf_pt, dim3((unsigned)ptot), dim3(kXfG), 0, st, tab, nbins, ptot, (const double*)part, w,
+                     xc_id, thresh, exc, coef);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int64_t ntot = (int64_t)nb * nt * nt;
-  hipLaunchKernelGGL(k_xcf_v, dim3((unsigned)(8 * ((ntot + 7) / 8))), dim3(256), 0, st, nb, n, ao, plane, coef, q);
+  hipLaunchKernelGGL(k_xcf_v, dim3((unsigned)(8 * (((int64_t)vtot + 7) / 8))), dim3(256), 0, st, tab, nbins, vtot,
+                     ao, (const float*)coef, q);
   return hipGetLastError();
 }
+
+// Workgroup counts and workspace of a bin in the table: K chunks per Z tile, then
+// the partial sums (doubles) and W coefficients (floats) it needs.
+void xcf_bin_geometry(int n, int nb, int& nt, int& nks, int64_t& part_doubles, int64_t& coef_floats) {
+  nt = (n + 127) / 128;
+  nks = (n + kXfKChunk - 1) / kXfKChunk;
+  part_doubles = (int64_t)nb * nt * nks * 4 * kXfG;
+  coef_floats = (int64_t)nb * kXfG * 4;
+}
+
 // Spin-polarised pointwise pass, one wavefront per grid point: on entry zwa, zwb
 // hold Z_s = Phi P_s (ng x nao); on exit the W_s of the half potential,
 //   W_a = w (f_ra / 2 phi + (2 f_saa grad ra + f_sab grad rb) . grad phi),
