@@ -56,6 +56,18 @@ def note(msg: str) -> None:
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+def _cpu_model() -> str:
+    """The host CPU's model name (Linux /proc/cpuinfo), for the CPU baseline's record."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,9 +253,12 @@ def main() -> int:
         threads = torch.get_num_threads()
         sec, parts = cpu_iteration_estimate(nao, nfit, nocc, ng)
         cpu = {"value": round(1.0 / sec, 6), "unit": "SCF iter/s", "cores": threads, "kind": "port",
-               "sample": ("CPU FP64 restatement (numpy/BLAS) of one iteration: exchange/J on 96 of "
-                          f"{nfit} fitted aux functions, XC GEMMs on 40000 of {ng} grid points, full "
-                          f"{nao}x{nao} eigh; each stage scaled linearly to the full size"),
+               "cpu_model": _cpu_model(),
+               "sample": ("CPU FP64 restatement (numpy/BLAS, torch for the functional) of one iteration on "
+                          f"synthetic operands of the workload's shapes: exchange/J on 96 of {nfit} fitted aux "
+                          f"functions; XC on 40000 of {ng} grid points (Z = Phi P, rho and grad rho, PBE0 "
+                          "with its first derivatives, W, V = Phi^T W); each slice scaled linearly to the "
+                          f"full size; full-size DIIS/transform GEMMs and {nao}x{nao} eigh"),
                "stages_s": {k: round(v, 3) for k, v in parts.items()}}
 
     mixed = {

@@ -996,15 +996,25 @@ def cpu_iteration_estimate(nao, nfit, nocc, ngrid, *, q_slice=96, g_slice=40000,
     The iteration is timed stage by stage on synthetic operands of the
     workload's shapes, each stage on a bounded slice scaled linearly to the
     full size: exchange T = B^T C and K = T^T T on ``q_slice`` fitted
-    auxiliary functions, J on the same slice, the XC GEMMs Z = Phi P and
-    V = Phi^T W on ``g_slice`` grid points, and the full eigensolve of an
-    nao x nao Fock matrix (numpy/LAPACK, all host threads).
+    auxiliary functions, J on the same slice, the XC work on ``g_slice`` grid
+    points (the GEMMs Z = Phi P and V = Phi^T W, rho and grad rho from Z and the
+    AO gradient planes, the functional with its first derivatives -- this
+    restatement's own ``eps_xc`` under autograd -- and W from the AO planes),
+    then the full-size density-step linear algebra: the DIIS commutator and
+    orthonormal-basis transforms (six nao^3 GEMMs) and the eigensolve of an
+    nao x nao Fock matrix (numpy/LAPACK, torch for the functional, all host
+    threads).
     """
     import time
+    t = _torch()
     rng = np.random.default_rng(seed)
     q = min(q_slice, nfit)
     g = min(g_slice, ngrid)
     out = {}
+    # untimed warm-up of the functional path (the first autograd pass builds torch's kernels)
+    w0 = t.tensor(np.full(16, 0.1), requires_grad=True)
+    g0 = t.tensor(np.full((16, 3), 0.1), requires_grad=True)
+    t.autograd.grad(t.sum(eps_xc("PBE0", w0, g0) * w0), (w0, g0))
     B = rng.standard_normal((nao, nao, q))
     C = rng.standard_normal((nao, nocc))
     P = rng.standard_normal((nao, nao))
@@ -1023,17 +1033,35 @@ def cpu_iteration_estimate(nao, nfit, nocc, ngrid, *, q_slice=96, g_slice=40000,
     out["coulomb_j"] = (t3 - t2) * nfit / q
     del B, T, Tm, K, J
     ao = rng.standard_normal((g, nao))
+    dao = rng.standard_normal((3, g, nao))
     t0 = time.perf_counter()
     Z = ao @ P
     t1 = time.perf_counter()
-    V = ao.T @ Z
+    rho = np.abs(np.einsum("gm,gm->g", ao, Z)) + 1e-3
+    grad = 2.0 * np.einsum("xgm,gm->gx", dao, Z)
     t2 = time.perf_counter()
-    out["xc_gemms"] = (t2 - t0) * ngrid / g
-    del ao, Z, V
+    rt = t.tensor(rho, requires_grad=True)
+    gt = t.tensor(grad, requires_grad=True)
+    e = t.sum(eps_xc("PBE0", rt, gt) * rt)
+    vr, vg = t.autograd.grad(e, (rt, gt))
+    vr, vg = vr.numpy(), vg.numpy()
+    t3 = time.perf_counter()
+    W = 0.5 * vr[:, None] * ao + np.einsum("gx,xgm->gm", vg, dao)
+    V = ao.T @ W
+    t4 = time.perf_counter()
+    out["xc_gemms"] = ((t1 - t0) + (t4 - t3)) * ngrid / g
+    out["xc_rho_grad"] = (t2 - t1) * ngrid / g
+    out["xc_functional"] = (t3 - t2) * ngrid / g
+    del ao, dao, Z, W, V
     F = P
     t0 = time.perf_counter()
-    np.linalg.eigh(F)
-    out["eigh"] = time.perf_counter() - t0
+    for _ in range(6):
+        F = F @ P / nao
+    t1 = time.perf_counter()
+    np.linalg.eigh(P)
+    t2 = time.perf_counter()
+    out["scf_linalg"] = t1 - t0
+    out["eigh"] = t2 - t1
     return sum(out.values()), out
 
 
