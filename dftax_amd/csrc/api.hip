@@ -135,7 +135,7 @@ hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, in
 hipError_t launch_gemm_f64_batched(const GemmArgs& g, double* C, int batch, int64_t bs_a, int64_t bs_b,
                                    int64_t bs_c, hipStream_t st);
 hipError_t launch_gemm_w8(const GemmArgs& g, double* C, const RowDot& rd, hipStream_t st);
-hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, const RowDot& rd, hipStream_t st);
+hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, const RowDot& rd, hipStream_t st, int depth);
 hipError_t launch_colsum(int nrows, int ncols, const double* part, double* out, int accumulate, hipStream_t st);
 hipError_t launch_gemm_partials(int prec, const GemmArgs& g, int nsplit, int sym, void* ws, int variant,
                                 hipStream_t st);
@@ -1413,15 +1413,16 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
     if (h->recompute) {
       exchange_recompute(h, Cf + (int64_t)i0 * nao, mixed ? h->d_Cf32 + (int64_t)i0 * nao : nullptr, nc, st);
     } else if (mixed) {
-      // split-BF16 T: the 128-row tile of k_gemm or the 8-wavefront 256-row tile, whichever is faster here
+      // split-BF16 T: the 128-row tile of k_gemm or the 8-wavefront 256-row tile with one or two k-steps
+      // of operands in flight, whichever is fastest here
       const GemmArgs g{(int)((int64_t)nao * nfit), nc, nao, h->d_B32, (int64_t)nao * nfit, false,
                        h->d_Cf32 + (int64_t)i0 * nao, nao, true};
       const bool fuse = nc <= 160 && nfit >= 256;
-      const int v = tuned_product(h, 4, (int64_t)nao * nfit, nc, nao, 2, 1, [&](int c) {
-        if (c == 1) HIPCHK(launch_gemm_b3w8(g, h->d_T32, fuse ? rd : RowDot(), st));
+      const int v = tuned_product(h, 4, (int64_t)nao * nfit, nc, nao, 3, 1, [&](int c) {
+        if (c >= 1) HIPCHK(launch_gemm_b3w8(g, h->d_T32, fuse ? rd : RowDot(), st, c));
         else HIPCHK(launch_gemm_f32out(g, h->d_T32, st));
       }, st);
-      rows_done = fuse && v == 1;
+      rows_done = fuse && v >= 1;
     } else {
       rows_done = t_product(h, Cf + (int64_t)i0 * nao, nc, rd, st);
     }
@@ -2556,8 +2557,9 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
       if (opt) h->opt = *opt;
       else h->opt = dfks_options{0, 1e-7, 1e-10, 0, 0, 0, 0, 0.0, 0, 1};
       if (h->opt.precision != 0 && h->opt.precision != 1) fail(1, "precision must be 0 (FP64) or 1 (mixed)");
-      if (h->opt.gemm < 0 || h->opt.gemm > 3)
-        fail(1, "gemm must be 0 (auto), 1 (rocBLAS), 2 (library MFMA) or 3 (library 8-wave exchange tile)");
+      if (h->opt.gemm < 0 || h->opt.gemm > 4)
+        fail(1, "gemm must be 0 (auto), 1 (rocBLAS), 2 (library MFMA), 3 (library 8-wave exchange tile) or 4 "
+                "(3 with the further variants)");
       if (h->opt.density != 0 && h->opt.density != 1) fail(1, "density must be 0 (purification) or 1 (eigensolver)");
       if (h->opt.aux_chunk < 0 || h->opt.ao_block < 0 || h->opt.screen < 0.0 || h->opt.screen >= 1.0)
         fail(1, "aux_chunk and ao_block must be >= 0 and screen in [0, 1)");

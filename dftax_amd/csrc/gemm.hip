@@ -502,9 +502,19 @@ __global__ __launch_bounds__(512, 1) void k_gemm_w8(int64_t M, int N, int64_t K,
 // tile per workgroup of 512 threads = 8 wavefronts (4 along M x 2 along N, each
 // 64 x 80 = 4 x 5 MFMA 16x16x32 blocks). Each operand value is split into bf16
 // hi + lo once, while it is staged to LDS; the LDS tiles are double-buffered
-// with one barrier per k-step, the next step's global loads issued before the
-// current step's MFMAs. The 256-row tile halves the re-reads of the small
-// operand per row of the large one against a 128-row tile.
+// with one barrier per k-step. The 256-row tile halves the re-reads of the
+// small operand per row of the large one against a 128-row tile.
+//
+// The global loads run DEPTH k-steps ahead of the MFMAs (DEPTH = 2: two
+// register sets in flight). Every load is unconditional, so that none has to
+// complete before the next is issued: addresses are clamped into the operands
+// (rows past M or N load finite values of the last row and give output rows or
+// columns that are never stored, and row terms weighted by zero; k past K
+// loads finite values of the last k, and B's are replaced by zeros when
+// staged, so those k contribute nothing), the k-row offsets of A are
+// wave-uniform (scalar) and only the row offset within the tile is per lane.
+// VEC: A in 8-byte pairs of rows (M even, 8-byte aligned rows), B in 16-byte
+// groups along k (16-byte aligned rows); otherwise single floats.
 constexpr int kB3BM = 256, kB3BN = 160, kB3BK = 32, kB3PAD = kB3BK + 8;
 
 __device__ __forceinline__ void split_bf16x8(const float* x, bf16x8& h, bf16x8& l) {
@@ -516,28 +526,36 @@ __device__ __forceinline__ void split_bf16x8(const float* x, bf16x8& h, bf16x8& 
   }
 }
 
+template <int DEPTH, bool VEC>
 __global__ __launch_bounds__(512, 1) void k_gemm_b3w8(int64_t M, int N, int K, const float* __restrict__ A,
                                                      int64_t lda, const float* __restrict__ B, int64_t ldb,
-                                                     float* __restrict__ C, int vec_a, int vec_b, RowDot rd) {
+                                                     float* __restrict__ C, RowDot rd) {
   __shared__ __attribute__((aligned(16))) __bf16 sAh[2][kB3BM][kB3PAD];
   __shared__ __attribute__((aligned(16))) __bf16 sAl[2][kB3BM][kB3PAD];
   __shared__ __attribute__((aligned(16))) __bf16 sBh[2][kB3BN][kB3PAD];
   __shared__ __attribute__((aligned(16))) __bf16 sBl[2][kB3BN][kB3PAD];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t m0 = (int64_t)blockIdx.x * kB3BM;
   const int n0 = blockIdx.y * kB3BN;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 80;
   const int li = lane & 15, lk = lane >> 4;
-  // A slice: rows 4 (tid % 64) .. +3 (one 16-byte load per k, coalesced along
-  // m) at k 4 (tid / 64) .. +3
-  const int ar = (tid & 63) * 4, ak = (tid >> 6) * 4;
-  // B slice (threads < 320): row tid / 2, k 16 (tid % 2) .. +15
+  // A slice: rows ar, ar + 1 with ar = 2 (lane) + 128 (wave % 2) (8 bytes per
+  // lane, 512 contiguous bytes per wavefront and k) at k ak .. ak + 7 with ak =
+  // 8 (wave / 2), so that each row goes to LDS as one 16-byte chunk per plane:
+  // a wavefront's 64 rows of one chunk then fall on 16 of the 32 banks
+  const int ar = 2 * lane + 128 * (wave & 1), ak = 8 * (wave >> 1);
+  // row offsets within the tile, clamped to the last row(s) of A
+  const int64_t m_room = M - m0;   // rows of A from m0 on (> 0)
+  const int a_off = VEC ? (int)(ar < m_room - 2 ? ar : m_room - 2) : 0;
+  const int a_off0 = (int)(ar < m_room - 1 ? ar : m_room - 1);
+  const int a_off1 = (int)(ar + 1 < m_room - 1 ? ar + 1 : m_room - 1);
+  const float* __restrict__ Am = A + m0;
+  // B slice (wavefronts 0-4): row tid / 2, k 16 (tid % 2) .. +15
   const int br = tid >> 1, bk = (tid & 1) * 16;
-  const bool has_b = tid < 2 * kB3BN;
-  const bool a_in = m0 + ar < M;
-  const bool b_in = has_b && n0 + br < N;
-  const float* __restrict__ pa = A + (a_in ? m0 + ar : 0);
-  const float* __restrict__ pb = B + (int64_t)(b_in ? n0 + br : 0) * ldb;
+  const bool has_b = wave < 5;
+  const float* __restrict__ pb = B + (int64_t)(n0 + br < N ? n0 + br : N - 1) * ldb;
+  const int k_last4 = ((K - 1) / 4) * 4;   // start of the last 4-float group along k of B
   // gamma row terms: the tile's rows span at most two m (nq >= 256), so the coefficients
   // sg[i] cf[i, m] are 2 x 160 values, loaded here and stored to LDS after the main loop
   double c_sg = 0.0, c_cf = 0.0;
@@ -549,33 +567,31 @@ __global__ __launch_bounds__(512, 1) void k_gemm_b3w8(int64_t M, int N, int K, c
       c_cf = rd.cf[(int64_t)gn * rd.nao + mm];
     }
   }
-  // A and B are loaded one k-step ahead into registers and split into bf16
-  // hi + lo when stored to LDS. ra[4 q + c]: row ar + c, k ak + q
-  float ra[16], rb[16];
-  auto load_a = [&](int k0) {
+  // register sets: ra[2 q + c] row ar + c, k ak + q; rb[j] row br, k bk + j
+  float ra0[16], rb0[16], ra1[16], rb1[16];
+  auto load_a = [&](float (&ra)[16], int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 8; ++q) {
       const int k = k0 + ak + q;
-      const float* src = pa + (int64_t)k * lda;
-      if (vec_a && a_in && k < K) {
-        const float4 x = *reinterpret_cast<const float4*>(src);
-        ra[4 * q] = x.x;
-        ra[4 * q + 1] = x.y;
-        ra[4 * q + 2] = x.z;
-        ra[4 * q + 3] = x.w;
+      const float* __restrict__ row = Am + (int64_t)(k < K ? k : K - 1) * lda;
+      if (VEC) {
+        const float2 x = *reinterpret_cast<const float2*>(row + a_off);
+        ra[2 * q] = x.x;
+        ra[2 * q + 1] = x.y;
       } else {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) ra[4 * q + c] = (m0 + ar + c < M && k < K) ? src[c] : 0.f;
+        ra[2 * q] = row[a_off0];
+        ra[2 * q + 1] = row[a_off1];
       }
     }
   };
-  auto load_b = [&](int k0) {
+  auto load_b = [&](float (&rb)[16], int k0) {
     if (!has_b) return;
     const int kb = k0 + bk;
-    if (b_in && vec_b && kb + 16 <= K) {
+    if (VEC) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const float4 x = *reinterpret_cast<const float4*>(pb + kb + 4 * v);
+        const int kv = kb + 4 * v;
+        const float4 x = *reinterpret_cast<const float4*>(pb + (kv < k_last4 ? kv : k_last4));
         rb[4 * v] = x.x;
         rb[4 * v + 1] = x.y;
         rb[4 * v + 2] = x.z;
@@ -583,24 +599,33 @@ __global__ __launch_bounds__(512, 1) void k_gemm_b3w8(int64_t M, int N, int K, c
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) rb[j] = (b_in && kb + j < K) ? pb[kb + j] : 0.f;
+      for (int j = 0; j < 16; ++j) rb[j] = pb[kb + j < K ? kb + j : K - 1];
     }
   };
-  auto store = [&](int buf) {
+  auto load = [&](float (&ra)[16], float (&rb)[16], int k0) {
+    load_a(ra, k0);
+    load_b(rb, k0);
+  };
+  // kvalid = K - (first k of the step): B's values at k >= K become zeros (the last step only)
+  auto store = [&](const float (&ra)[16], float (&rb)[16], int buf, int kvalid) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      bf16x4 h, l;
+    for (int c = 0; c < 2; ++c) {
+      bf16x8 h, l;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float x = ra[4 * q + c];
+      for (int q = 0; q < 8; ++q) {
+        const float x = ra[2 * q + c];
         const __bf16 xh = (__bf16)x;
         h[q] = xh;
         l[q] = (__bf16)(x - (float)xh);
       }
-      *reinterpret_cast<bf16x4*>(&sAh[buf][ar + c][ak]) = h;
-      *reinterpret_cast<bf16x4*>(&sAl[buf][ar + c][ak]) = l;
+      *reinterpret_cast<bf16x8*>(&sAh[buf][ar + c][ak]) = h;
+      *reinterpret_cast<bf16x8*>(&sAl[buf][ar + c][ak]) = l;
     }
     if (has_b) {
+      if (kvalid < kB3BK) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) rb[j] = bk + j < kvalid ? rb[j] : 0.f;
+      }
 #pragma unroll
       for (int v = 0; v < 2; ++v) {
         bf16x8 h, l;
@@ -635,19 +660,37 @@ __global__ __launch_bounds__(512, 1) void k_gemm_b3w8(int64_t M, int N, int K, c
     }
   };
   const int nk = (K + kB3BK - 1) / kB3BK;
-  load_a(0);
-  load_b(0);
-  store(0);
-  __syncthreads();
-  for (int t = 0; t < nk; ++t) {
-    const int cur = t & 1;
-    if (t + 1 < nk) {
-      load_a((t + 1) * kB3BK);
-      load_b((t + 1) * kB3BK);
-    }
-    compute(cur);
-    if (t + 1 < nk) store(cur ^ 1);
+  if (DEPTH == 1) {
+    load(ra0, rb0, 0);
+    store(ra0, rb0, 0, K);
     __syncthreads();
+    for (int t = 0; t < nk; ++t) {
+      load(ra0, rb0, (t + 1) * kB3BK);   // past the last step: clamped loads, never stored
+      compute(t & 1);
+      if (t + 1 < nk) store(ra0, rb0, (t & 1) ^ 1, K - (t + 1) * kB3BK);
+      __syncthreads();
+    }
+  } else {
+    // at the top of an even step t, LDS buffer 0 holds step t, (ra1, rb1) step
+    // t + 1 and (ra0, rb0) step t + 2 (loads past the last step are clamped and
+    // never stored); the body covers steps t and t + 1 so that each register set
+    // keeps a fixed name
+    load(ra0, rb0, 0);
+    load(ra1, rb1, kB3BK);
+    store(ra0, rb0, 0, K);
+    __syncthreads();
+    load(ra0, rb0, 2 * kB3BK);
+    for (int t = 0; t < nk; t += 2) {
+      compute(0);
+      if (t + 1 < nk) store(ra1, rb1, 1, K - (t + 1) * kB3BK);
+      __syncthreads();
+      if (t + 1 >= nk) break;
+      load(ra1, rb1, (t + 3) * kB3BK);
+      compute(1);
+      if (t + 2 < nk) store(ra0, rb0, 0, K - (t + 2) * kB3BK);
+      __syncthreads();
+      load(ra0, rb0, (t + 4) * kB3BK);
+    }
   }
   // v_mfma_f32_16x16x32_bf16 result: lane l, register r holds row 4(l/16) + r, column l%16
 #pragma unroll
@@ -862,19 +905,22 @@ hipError_t launch_gemm_f32out_batched(const GemmArgs& g, float* C, int batch, in
 }
 
 // C = A B^T in FP32 by k_gemm_b3w8 (split-BF16, A M-contiguous, B K-contiguous, ldc = N);
-// rd.out (optional, N <= 160): the row terms of gamma.
-hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, const RowDot& rd, hipStream_t st) {
+// rd.out (optional, N <= 160): the row terms of gamma; depth: k-steps of operands in flight (1, 2).
+hipError_t launch_gemm_b3w8(const GemmArgs& g, float* C, const RowDot& rd, hipStream_t st, int depth) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
-  if (g.a_kcontig || !g.b_kcontig || g.K > 0x7fffffff) return hipErrorInvalidValue;
+  if (g.a_kcontig || !g.b_kcontig || g.K <= 0 || g.K > 0x7fffffff) return hipErrorInvalidValue;
   if (rd.out && (g.N > kB3BN || rd.nq < kB3BM)) return hipErrorInvalidValue;
   const float* A = static_cast<const float*>(g.A);
   const float* B = static_cast<const float*>(g.B);
-  // 16-byte loads of 4 consecutive rows of A; 4 floats at a time along k of B
-  const int vec_a = ((uintptr_t)A % 16 == 0) && (g.lda % 4 == 0) && (g.M % 4 == 0);
-  const int vec_b = ((uintptr_t)B % 16 == 0) && (g.ldb % 4 == 0);
+  // 8-byte loads of 2 consecutive rows of A (M even), 16-byte loads along k of B
+  const bool vec = ((uintptr_t)A % 8 == 0) && (g.lda % 2 == 0) && (g.M % 2 == 0) && ((uintptr_t)B % 16 == 0) &&
+                   (g.ldb % 4 == 0);
   const dim3 grid((unsigned)(((int64_t)g.M + kB3BM - 1) / kB3BM), (unsigned)((g.N + kB3BN - 1) / kB3BN));
-  return launch_k(k_gemm_b3w8, grid, dim3(512), st, (int64_t)g.M, g.N, (int)g.K, A, g.lda, B, g.ldb, C, vec_a,
-                  vec_b, rd);
+  const auto run = [&](auto kern) {
+    return launch_k(kern, grid, dim3(512), st, (int64_t)g.M, g.N, (int)g.K, A, g.lda, B, g.ldb, C, rd);
+  };
+  if (depth == 2) return vec ? run(k_gemm_b3w8<2, true>) : run(k_gemm_b3w8<2, false>);
+  return vec ? run(k_gemm_b3w8<1, true>) : run(k_gemm_b3w8<1, false>);
 }
 
 // Batched FP64 variant (no split, FP64 accumulate): for b < batch,

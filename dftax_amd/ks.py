@@ -39,8 +39,11 @@ class HipSpec:
     ``gemm`` picks the FP64 GEMM kernels: ``"auto"`` (library symmetric
     exchange kernel; for T = B^T C and the compact-grid XC products the
     fastest of rocBLAS and the library kernels, timed once per shape),
-    ``"rocblas"``, ``"mfma"`` (library kernels for all) or ``"mfma8"`` (as
-    ``"mfma"``, with the 8-wavefront 128 x 160 tile for T). Mixed steps use the library's
+    ``"rocblas"``, ``"mfma"`` (library kernels for all), ``"mfma8"`` (as
+    ``"mfma"``, with the 8-wavefront 128 x 160 tile for T) or ``"mfma8d"`` (as
+    ``"mfma8"``, with each product's further variant where it has one: the
+    split-BF16 T with two k-steps of operands in flight, the FP64 K split by
+    XCD). Mixed steps use the library's
     split-BF16 kernels for the exchange products and FP32 rocBLAS products on
     the compact XC grid. ``metric_cutoff`` is the relative eigenvalue cutoff of the
     metric pseudo-inverse (terms.py:190-192). ``density`` is how each SCF step
@@ -74,8 +77,8 @@ def hip(*, precision: str = "fp64", metric_cutoff: float = 1e-7, rho_thresh: flo
         ao_block: int | None = None, xc_grid: str = "blocks") -> HipSpec:
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
-    if gemm not in ("auto", "rocblas", "mfma", "mfma8"):
-        raise ValueError(f"gemm must be 'auto', 'rocblas', 'mfma' or 'mfma8', got {gemm!r}")
+    if gemm not in ("auto", "rocblas", "mfma", "mfma8", "mfma8d"):
+        raise ValueError(f"gemm must be 'auto', 'rocblas', 'mfma', 'mfma8' or 'mfma8d', got {gemm!r}")
     if density not in ("purify", "eigh"):
         raise ValueError(f"density must be 'purify' or 'eigh', got {density!r}")
     if ao_block is not None and (not isinstance(ao_block, int) or ao_block < 0):
@@ -201,7 +204,7 @@ class HipDFKS:
                                 spec.ao_block, int(0.85 * free))
         opts = _native.Options(1 if spec.precision == "mixed" else 0, float(spec.metric_cutoff),
                                float(spec.rho_thresh), int(spec.keep_int3c),
-                               {"auto": 0, "rocblas": 1, "mfma": 2, "mfma8": 3}[spec.gemm],
+                               {"auto": 0, "rocblas": 1, "mfma": 2, "mfma8": 3, "mfma8d": 4}[spec.gemm],
                                {"purify": 0, "eigh": 1}[spec.density], int(self.plan.aux_chunk),
                                float(screen or 0.0), int(self.plan.ao_block), int(spec.xc_grid == "blocks"))
         self.precision = spec.precision

@@ -82,7 +82,9 @@ typedef struct {
                               product T and the compact-grid XC products the fastest of rocBLAS and the
                               library kernels, timed once per shape and device (default); 1 rocBLAS
                               for all; 2 library kernels for all; 3 as 2 with the 8-wavefront tile for
-                              T. Mixed steps always use the library's split-BF16 kernels. */
+                              T; 4 as 3 with each product's further library variant where it has one
+                              (split-BF16 T with two k-steps of operands in flight, FP64 K split by
+                              XCD). Mixed steps always use the library's split-BF16 kernels. */
   int32_t density;         /* SCF density from the orthonormal-basis Fock matrix: 0 spectral projection
                               (SP2 purification + pivoted Cholesky factor; the eigensolver runs only
                               when purification does not converge, and once for dfks_scf_result's
