@@ -472,15 +472,27 @@ double hf_coeff(int xc) {
     case 1: return 0.0;
     case 2: return 0.25;
     case 3: return 0.20;
+    case 4: return 0.19;
     default: fail(1, "unknown xc_id " + std::to_string(xc));
   }
   return 0.0;
 }
 
+// Long-range exact-exchange fraction and range-separation parameter of a
+// range-separated hybrid, K_hf = hf K + hf_lr K_lr(omega) (dftax/energy/xc.py:
+// 49-55; CAM-B3LYP 711-750: 0.19 K + 0.46 K_lr(0.33)); zero for global hybrids.
+double hf_lr_coeff(int xc) { return xc == 4 ? 0.46 : 0.0; }
+double omega_of(int xc) { return xc == 4 ? 0.33 : 0.0; }
+
 }  // namespace
 
 struct dfks_ctx {
   int nao = 0, naux = 0, nfit = 0, nocc = 0, natom = 0, xc_id = 0, nspin = 1;
+  // range-separated hybrids: K_hf = hf K + hf_lr K_lr with K_lr from the fitted
+  // tensor of the erf(omega r)/r operator (nfit_lr columns of its own metric)
+  double omega = 0.0, hf_lr = 0.0;
+  int nfit_lr = 0;
+  double *d_Blr = nullptr, *d_Klr = nullptr;
   int64_t ng = 0;
   double hf = 0.0, e_nn = 0.0;
   dfks_options opt{};
@@ -609,7 +621,7 @@ struct dfks_ctx {
                     d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf, d_xc_pts, d_xc_w, d_xc_exc, d_xc_ao,
                     d_xc_zw, d_xc_q, d_xc_ao32, d_xc_cols, d_xc_vlen, d_xc_shl, d_xc_shoff, d_xc_vrow,
                     d_xc_vprow, d_xc_vpcol, d_xc_zw2, d_xc_q2, d_Vxc2, d_uP, d_uJ, d_uK, d_ug, d_xcf_tab,
-                    d_xcf_ws};
+                    d_xcf_ws, d_Blr, d_Klr};
     for (void* p : ptrs) dfree(p);
     void* ptrs32[] = {d_gws, d_gpart, d_B32, d_ao32, d_zw32, t32_alias ? nullptr : d_T32, d_Cf32, d_P32, d_ws32,
                       d_eps32, d_offd32, d_slab32, d_Lt32, d_Tslab32, d_aob32, d_zwb32};
@@ -1085,27 +1097,29 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   }
   h->build_ms[2] = now_ms() - t0;
   t0 = now_ms();
-  dfree(d_pairs);
-  dfree(d_auxl);
 
-  // metric pseudo-inverse factor L = U_keep w_keep^{-1/2}
+  // metric pseudo-inverse factor L = U_keep w_keep^{-1/2} (nfit columns kept by the relative cutoff)
   double* d_U = dalloc<double>((size_t)naux * naux);
   double* d_w = dalloc<double>(naux);
   double* d_e = dalloc<double>(naux);
-  HIPCHK(hipMemcpyAsync(d_U, h->d_int2c, sizeof(double) * naux * naux, hipMemcpyDeviceToDevice, st));
-  RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, naux, d_U, naux, d_w, d_e, h->d_info));
-  std::vector<double> w(naux);
-  HIPCHK(hipMemcpyAsync(w.data(), d_w, sizeof(double) * naux, hipMemcpyDeviceToHost, st));
-  int info = 0;
-  HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
-  sync(st);
-  if (info != 0) fail(5, "metric eigensolver failed, info=" + std::to_string(info));
-  const double wmax = w[naux - 1];
-  int k0 = 0;
-  while (k0 < naux && !(w[k0] > h->opt.metric_cutoff * wmax)) ++k0;
-  h->nfit = naux - k0;
-  double* d_Lt = dalloc<double>((size_t)h->nfit * naux);  // column-major nfit x naux
-  HIPCHK(launch_metric_factor(naux, k0, d_U, d_w, d_Lt, st));
+  const auto metric_factor = [&](const double* V, int& nkeep) {
+    HIPCHK(hipMemcpyAsync(d_U, V, sizeof(double) * naux * naux, hipMemcpyDeviceToDevice, st));
+    RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, naux, d_U, naux, d_w, d_e, h->d_info));
+    std::vector<double> w(naux);
+    HIPCHK(hipMemcpyAsync(w.data(), d_w, sizeof(double) * naux, hipMemcpyDeviceToHost, st));
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync(st);
+    if (info != 0) fail(5, "metric eigensolver failed, info=" + std::to_string(info));
+    const double wmax = w[naux - 1];
+    int k0 = 0;
+    while (k0 < naux && !(w[k0] > h->opt.metric_cutoff * wmax)) ++k0;
+    nkeep = naux - k0;
+    double* Lt = dalloc<double>((size_t)nkeep * naux);  // column-major nkeep x naux
+    HIPCHK(launch_metric_factor(naux, k0, d_U, d_w, Lt, st));
+    return Lt;
+  };
+  double* d_Lt = metric_factor(h->d_int2c, h->nfit);
   if (!h->recompute) {
     // B_cm (nfit x nao^2) = Lt (nfit x naux) * int3c_cm (naux x nao^2)
     h->d_B = dalloc<double>((size_t)nao * nao * h->nfit);
@@ -1120,6 +1134,45 @@ void build_all(dfks_ctx* h, hipStream_t st) {
     h->d_Lt = d_Lt;
     h->d_fitd = dalloc<double>(naux);
   }
+  if (h->hf_lr != 0.0) {
+    // long-range operator erf(omega r)/r of a range-separated hybrid (terms.py:578-612,
+    // energy.py:419-433): attenuated (P|Q) and (mn|P), the pseudo-inverse factor of the
+    // attenuated metric with the same cutoff, B_lr = (mn|P) L_lr
+    ShellSet Aw = A;
+    Aw.omega = h->omega;
+    double* d2lr = dalloc<double>((size_t)naux * naux);
+    for (int la = 0; la <= lx; ++la)
+      for (int lc = 0; lc <= lx; ++lc)
+        HIPCHK(launch_eri_wave(la, 0, lc, d_pairs + apair_off[la], (int)apairs[la].size(), d_auxl + aux_off[lc],
+                               (int)auxl[lc].size(), Aw, Aw, 1, h->d_boys, d2lr, naux, 0, 0, 0, st));
+    double* d3lr = dalloc<double>((size_t)nao * nao * naux);
+    if (h->slab_zero) HIPCHK(hipMemsetAsync(d3lr, 0, sizeof(double) * nao * nao * naux, st));
+    for (int la = 0; la <= lo; ++la)
+      for (int lb = 0; lb <= la; ++lb)
+        for (int lc = 0; lc <= lx; ++lc) {
+          const int cls = la * (lo + 1) + lb;
+          const int2* pp = h->d_pairs3 + h->pair3_off[cls];
+          const int npair = h->pair3_n[cls];
+          if (eri_thread_has(la, lb, lc) && A.spherical)
+            HIPCHK(launch_eri_thread(la, lb, lc, pp, npair, d_auxl + aux_off[lc], (int)auxl[lc].size(), O, Aw,
+                                     h->d_boys, d3lr, (int64_t)nao * naux, naux, 0, 1, st));
+          else
+            HIPCHK(launch_eri_cls(la, lb, lc, pp, npair, d_auxl + aux_off[lc], (int)auxl[lc].size(), O, Aw,
+                                  h->d_boys, d3lr, (int64_t)nao * naux, naux, 0, 1, st));
+        }
+    double* d_Ltlr = metric_factor(d2lr, h->nfit_lr);
+    h->d_Blr = dalloc<double>((size_t)nao * nao * h->nfit_lr);
+    const double one = 1.0, zero = 0.0;
+    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, h->nfit_lr, (int64_t)nao * nao,
+                           naux, &one, d_Ltlr, h->nfit_lr, d3lr, naux, &zero, h->d_Blr, h->nfit_lr));
+    sync(st);
+    dfree(d3lr);
+    dfree(d2lr);
+    dfree(d_Ltlr);
+    h->d_Klr = dalloc<double>((size_t)nao * nao);
+  }
+  dfree(d_pairs);
+  dfree(d_auxl);
   h->build_ms[3] = now_ms() - t0;
   t0 = now_ms();
   dfree(d_U); dfree(d_w); dfree(d_e);
@@ -1141,7 +1194,7 @@ void build_all(dfks_ctx* h, hipStream_t st) {
     }
   }
   // Fock-build scratch: exchange intermediate for up to nf_chunk factor columns
-  h->d_Tx = dalloc<double>((size_t)nao * h->nfit * h->nf_chunk);
+  h->d_Tx = dalloc<double>((size_t)nao * std::max(h->nfit, h->nfit_lr) * h->nf_chunk);
   h->d_gamma = dalloc<double>(h->nfit);
   h->d_J = dalloc<double>((size_t)nao * nao);
   h->d_K = dalloc<double>((size_t)nao * nao);
@@ -1378,13 +1431,65 @@ bool t_product(dfks_ctx* h, const double* C, int nc, const RowDot& rd, hipStream
   return fuse && v == 2;
 }
 
+// Kout = sg T^T T + beta Kout in FP64 for T = d_Tx (nao rows of kd contiguous
+// values): split-K partial tiles by the fastest of the library tiles and
+// layouts for this shape and the ordered reduction, or rocBLAS (gemm = 1).
+void k_product_f64(dfks_ctx* h, int64_t kd, double* Kout, double sg, double beta, hipStream_t st) {
+  const int nao = h->nao;
+  const int ntn = (nao + 127) / 128;
+  const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, kd);
+  const int nsx = 16;   // the split by XCD: each XCD's tiles on the same k-window of T
+  if (h->opt.gemm != 1) {
+    const GemmArgs g{nao, nao, kd, h->d_Tx, kd, true, h->d_Tx, kd, true};
+    const int v = tuned_product(h, 3, nao, kd, nsplit, 3, 1, [&](int c) {
+      HIPCHK(launch_gemm_partials(kF64, g, c == 2 ? nsx : nsplit, c == 2 ? 3 : 1, h->d_gws, c == 2 ? 1 : c, st));
+    }, st);
+    HIPCHK(launch_gemm_reduce(kF64, g, Kout, sg, beta, v == 2 ? nsx : nsplit, 1, h->d_gws, st));
+  } else {
+    RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_none, nao, nao, kd, &sg, h->d_Tx,
+                           kd, h->d_Tx, kd, &beta, Kout, nao));
+  }
+}
+
+// Long-range exchange of a range-separated hybrid from the factor of P alone:
+// K_lr = sum_i s_i T_i^T T_i with T = B_lr^T c over the fitted tensor of the
+// erf(omega r)/r operator (terms.py:604-612), into d_Klr. FP64.
+void k_lr_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, int nf, hipStream_t st) {
+  const int nao = h->nao;
+  // the products read the handle's fitted tensor and its width: point them at the long-range one
+  struct Swap {
+    dfks_ctx* h;
+    double* B;
+    int nfit;
+    Swap(dfks_ctx* hh) : h(hh), B(hh->d_B), nfit(hh->nfit) {
+      h->d_B = h->d_Blr;
+      h->nfit = h->nfit_lr;
+    }
+    ~Swap() {
+      h->d_B = B;
+      h->nfit = nfit;
+    }
+  } swap(h);
+  bool first = true;
+  int i0 = 0;
+  while (i0 < nf) {
+    int i1 = i0 + 1;
+    while (i1 < nf && i1 - i0 < h->nf_chunk && sgn[i1] == sgn[i0]) ++i1;
+    const int nc = i1 - i0;
+    (void)t_product(h, Cf + (int64_t)i0 * nao, nc, RowDot(), st);
+    k_product_f64(h, (int64_t)h->nfit_lr * nc, h->d_Klr, sgn[i0], first ? 0.0 : 1.0, st);
+    first = false;
+    i0 = i1;
+  }
+  if (first) HIPCHK(hipMemsetAsync(h->d_Klr, 0, sizeof(double) * nao * nao, st));
+}
+
 // gamma, J and K of P = sum_i s_i c_i c_i^T from its factor alone:
 // gamma = sum_i s_i T_i c_i, J = B gamma, K = sum_i s_i T_i^T T_i.
 void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, int nf, hipStream_t st) {
   const int nao = h->nao, nfit = h->nfit;
   const int64_t nn = (int64_t)nao * nao;
   const bool mixed = h->mixed_now;
-  const bool lib_k = h->opt.gemm != 1;
   // signs to the device (host copy kept in the handle until the next call)
   if (nf > (mixed ? nao : 2 * nao)) fail(1, "too many factor columns");
   h->h_sgn.assign(sgn.begin(), sgn.end());
@@ -1451,16 +1556,8 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
           HIPCHK(launch_gemm_partials(kBF16x3, g, c == 1 ? nsx : nsplit, c == 1 ? 3 : 1, h->d_gws, 0, st));
         }, st);
         HIPCHK(launch_gemm_reduce(kBF16x3, g, h->d_K, sg, beta, v == 1 ? nsx : nsplit, 1, h->d_gws, st));
-      } else if (lib_k) {
-        // split-K partial tiles by the fastest of the library tiles and layouts for this shape, then the reduction
-        const GemmArgs g{nao, nao, kd, h->d_Tx, kd, true, h->d_Tx, kd, true};
-        const int v = tuned_product(h, 3, nao, kd, nsplit, 3, 1, [&](int c) {
-          HIPCHK(launch_gemm_partials(kF64, g, c == 2 ? nsx : nsplit, c == 2 ? 3 : 1, h->d_gws, c == 2 ? 1 : c, st));
-        }, st);
-        HIPCHK(launch_gemm_reduce(kF64, g, h->d_K, sg, beta, v == 2 ? nsx : nsplit, 1, h->d_gws, st));
       } else {
-        RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_transpose, rocblas_operation_none, nao, nao, kd, &sg,
-                               h->d_Tx, kd, h->d_Tx, kd, &beta, h->d_K, nao));
+        k_product_f64(h, kd, h->d_K, sg, beta, st);
       }
     }
     h->mark(kStExchangeSyrk + 1, st);
@@ -1752,13 +1849,16 @@ void energy_fock_factored(dfks_ctx* h, const double* P, const double* Cf, const 
   // one-electron energy
   HIPCHK(launch_dot(nn, P, h->d_H, h->d_part, 1.0, E + DFKS_E_ONE, 0, st));
   jk_factored(h, Cf, sgn, nf, st);
+  if (h->hf_lr != 0.0) k_lr_factored(h, Cf, sgn, nf, st);
   // E_J = 1/2 |gamma|^2
   HIPCHK(launch_dot(nfit, h->d_gamma, h->d_gamma, h->d_part, 0.5, E + DFKS_E_J, 0, st));
-  // E_K = -1/4 a_x sum P o K
-  if (h->hf != 0.0) HIPCHK(launch_dot(nn, P, h->d_K, h->d_part, -0.25 * h->hf, E + DFKS_E_K, 0, st));
+  // E_K = -1/4 sum P o (a_x K + a_lr K_lr) (E was cleared above)
+  if (h->hf != 0.0) HIPCHK(launch_dot(nn, P, h->d_K, h->d_part, -0.25 * h->hf, E + DFKS_E_K, 1, st));
+  if (h->hf_lr != 0.0) HIPCHK(launch_dot(nn, P, h->d_Klr, h->d_part, -0.25 * h->hf_lr, E + DFKS_E_K, 1, st));
   xc_potential(h, P, E, st);
   HIPCHK(launch_fock(nao, h->d_H, h->d_J, h->hf != 0.0 ? h->d_K : nullptr, -0.5 * h->hf,
                      ng > 0 ? h->d_Vxc : nullptr, F, st));
+  if (h->hf_lr != 0.0) HIPCHK(launch_axpy(nn, -0.5 * h->hf_lr, h->d_Klr, F, 1, st));
   // totals: E_tot = e1 + eJ + eK + exc + enn
   HIPCHK(hipMemcpyAsync(E + DFKS_E_NN, &h->e_nn, sizeof(double), hipMemcpyHostToDevice, st));
   HIPCHK(launch_dot(5, E + 1, nullptr, h->d_part, 1.0, E + DFKS_E_TOT, 0, st));
@@ -2528,6 +2628,12 @@ __global__ __launch_bounds__(256) void k_becke_pairs(int natm, const double* __r
 
 // Entry points of the restricted closed-shell path: an nspin = 2 handle gets
 // status 7 (its SCF loop runs on the host over dfks_energy_fock).
+void no_range_sep(const dfks_ctx* h, const char* what) {
+  if (h && h->hf_lr != 0.0)
+    fail(7, std::string(what) + " is not implemented for range-separated functionals; their handles support "
+                                "dfks_energy_fock, the device SCF and the matrix and setup calls");
+}
+
 void closed_shell_only(const dfks_ctx* h, const char* what) {
   if (h && h->nspin == 2)
     fail(7, std::string(what) + " is implemented for nspin = 1; an nspin = 2 handle supports dfks_energy_fock "
@@ -2569,7 +2675,14 @@ int dfks_create(const dfks_system_desc* sys, const dfks_options* opt, dfks_handl
 
       h->xc_id = sys->xc_id;
       h->hf = hf_coeff(sys->xc_id);
+      h->hf_lr = hf_lr_coeff(sys->xc_id);
+      h->omega = omega_of(sys->xc_id);
       h->nspin = sys->nspin;
+      if (h->hf_lr != 0.0) {
+        if (sys->nspin != 1) fail(7, "range-separated functionals are implemented for nspin = 1");
+        if (h->opt.precision != 0) fail(7, "range-separated functionals run in FP64 (options.precision = 0)");
+        if (h->recompute) fail(7, "range-separated functionals need the materialised fitted tensors (aux_chunk = 0)");
+      }
       if (h->nspin == 1) {
         if (sys->nelec <= 0 || sys->nelec % 2) fail(1, "closed shell needs an even, positive electron count");
         h->nocc = sys->nelec / 2;
@@ -2775,6 +2888,7 @@ int dfks_xc_eval(int32_t xc_id, int64_t n, const double* rho, const double* sigm
 int dfks_xc_eval_pol(int32_t xc_id, int64_t n, const double* in5, double* out6, dfks_stream stream) {
   return guard([&] {
     hf_coeff(xc_id);
+    if (hf_lr_coeff(xc_id) != 0.0) fail(7, "spin-polarised range-separated functionals are not implemented");
     if (n < 0 || (n > 0 && (!in5 || !out6))) fail(1, "bad arguments");
     HIPCHK(launch_xc_pointwise_pol(n, xc_id, in5, out6, S_(stream)));
   });
@@ -2791,6 +2905,7 @@ int dfks_xc_eval2(int32_t xc_id, int64_t n, const double* rho, const double* sig
 int dfks_response_begin(dfks_handle h, const double* P, dfks_stream stream) {
   return guard([&] {
     closed_shell_only(h, "dfks_response_begin");
+    no_range_sep(h, "dfks_response_begin");
     if (!h || !P) fail(1, "null argument");
     StreamScope ss(h->st = S_(stream));
     response_begin(h, P, S_(stream));
@@ -2800,6 +2915,7 @@ int dfks_response_begin(dfks_handle h, const double* P, dfks_stream stream) {
 int dfks_response_fock(dfks_handle h, const double* U, const double* V, int32_t r, double* R, dfks_stream stream) {
   return guard([&] {
     closed_shell_only(h, "dfks_response_fock");
+    no_range_sep(h, "dfks_response_fock");
     if (!h || !U || !V || !R) fail(1, "null argument");
     StreamScope ss(h->st = S_(stream));
     response_fock(h, U, V, r, R, S_(stream));
@@ -2994,6 +3110,7 @@ int dfks_forces(dfks_handle h, const double* C_occ, int32_t nocc, int64_t ngrid,
                 dfks_stream stream) {
   return guard([&] {
     closed_shell_only(h, "dfks_forces");
+    no_range_sep(h, "dfks_forces");
     if (!h || !C_occ || !grad_parts || !bragg) fail(1, "null argument");
     if (ngrid < 0 || (ngrid > 0 && (!grid_xyz || !grid_raw_w || !grid_owner))) fail(1, "bad grid arguments");
     StreamScope ss(h->st = S_(stream));

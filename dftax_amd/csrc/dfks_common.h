@@ -120,6 +120,10 @@ struct ShellSet {
   const double* exps;
   const double* coefs;    // primitive norm (and the l<=1 contracted norm) folded in
   int32_t spherical;
+  // operator of the two- and three-centre integrals taken over this set (the
+  // auxiliary side): 0 the Coulomb operator 1/r, > 0 the long-range part
+  // erf(omega r)/r of a range-separated hybrid
+  double omega = 0.0;
 };
 
 // Final-span size of a shell.
@@ -193,6 +197,41 @@ __device__ __forceinline__ void boys_array_rt(int nmax, double T, double* F, con
     double it = 1.0 / T;
     F[0] = 0.88622692545275801365 * sqrt(it);
     for (int m = 0; m < nmax; ++m) F[m + 1] = F[m] * (2 * m + 1) * 0.5 * it;
+  }
+}
+
+// Boys values of the operator of an integral: omega = 0 the Coulomb operator,
+// F_m(T); omega > 0 the attenuated erf(omega r)/r, s^(m+1/2) F_m(s T) with
+// s = omega^2 / (omega^2 + rho) and rho the reduced exponent of the two charge
+// distributions (dftax/integrals/eri3c_bucketed.py:340-346).
+template <int NMAX>
+__device__ __forceinline__ void boys_op(double T, double rho, double omega, double* F, const double* __restrict__ tbl) {
+  if (omega > 0.0) {
+    const double s = omega * omega / (omega * omega + rho);
+    boys_array<NMAX>(s * T, F, tbl);
+    double f = sqrt(s);
+#pragma unroll
+    for (int m = 0; m <= NMAX; ++m) {
+      F[m] *= f;
+      f *= s;
+    }
+  } else {
+    boys_array<NMAX>(T, F, tbl);
+  }
+}
+
+__device__ __forceinline__ void boys_op_rt(int nmax, double T, double rho, double omega, double* F,
+                                           const double* __restrict__ tbl) {
+  if (omega > 0.0) {
+    const double s = omega * omega / (omega * omega + rho);
+    boys_array_rt(nmax, s * T, F, tbl);
+    double f = sqrt(s);
+    for (int m = 0; m <= nmax; ++m) {
+      F[m] *= f;
+      f *= s;
+    }
+  } else {
+    boys_array_rt(nmax, T, F, tbl);
   }
 }
 

@@ -181,7 +181,7 @@ __global__ __launch_bounds__(64) void k_eri_wave(
         double T = rho * (pcx * pcx + pcy * pcy + pcz * pcz);
         __syncthreads();
         if (lane == 0) {
-          boys_array_rt(L, T, s_F, boys);
+          boys_op_rt(L, T, rho, C.omega, s_F, boys);
           // single-centre E^k_tau for the aux Gaussian
           double i2g = 0.5 / ga;
           for (int q = 0; q < (lc + 1) * mt; ++q) s_Ec[q] = 0.0;

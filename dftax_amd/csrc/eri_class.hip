@@ -123,7 +123,7 @@ __global__ __launch_bounds__(64) void k_eri_cls(const int2* __restrict__ pairs, 
         const double T = rho * (pcx * pcx + pcy * pcy + pcz * pcz);
         __syncthreads();
         if (lane == 0) {
-          boys_array<L>(T, s_F, boys);
+          boys_op<L>(T, rho, C.omega, s_F, boys);
         } else if (lane == 1) {
           const double i2g = 0.5 / ga;
           for (int q = 0; q < (LC + 1) * MT; ++q) s_Ec[q] = 0.0;

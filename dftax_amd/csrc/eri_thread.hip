@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_eri_thread(const int2* __restrict__ pai
         if (LC & 1) pref = -pref;
         const double pcx = Px - Cx, pcy = Py - Cy, pcz = Pz - Cz;
         double F[L + 1];
-        boys_array<L>(rho * (pcx * pcx + pcy * pcy + pcz * pcz), F, boys);
+        boys_op<L>(rho * (pcx * pcx + pcy * pcy + pcz * pcz), rho, C.omega, F, boys);
         double Ra[nr_of(L)], Rb[nr_of(L)];
         const double* R = r_build<L>(-2.0 * rho, pcx, pcy, pcz, F, Ra, Rb);
         // single-centre expansion of the auxiliary Gaussian: Ec[k][tau], tau <= k

@@ -19,7 +19,7 @@
 
 namespace dfks {
 
-enum XcId { kXcPBE = 1, kXcPBE0 = 2, kXcB3LYP = 3 };
+enum XcId { kXcPBE = 1, kXcPBE0 = 2, kXcB3LYP = 3, kXcCAMB3LYP = 4 };
 
 // Forward-mode dual number over the two functional arguments: value, d/drho,
 // d/dsigma. Dual<double> gives f, f_rho, f_sigma; Dual<Dual<double>> (a dual
@@ -85,6 +85,7 @@ __device__ __forceinline__ double dexpm1(double x) { return expm1(x); }
 __device__ __forceinline__ double datan(double x) { return atan(x); }
 __device__ __forceinline__ double dasinh(double x) { return asinh(x); }
 __device__ __forceinline__ double dpow(double x, double p) { return pow(x, p); }
+__device__ __forceinline__ double derf(double x) { return erf(x); }
 
 template <class T>
 __device__ __forceinline__ Dual<T> dsqrt(const Dual<T>& a) { const T f = dsqrt(a.v); return chain(a, f, 0.5 / f); }
@@ -101,6 +102,10 @@ __device__ __forceinline__ Dual<T> datan(const Dual<T>& a) { return chain(a, dat
 template <class T>
 __device__ __forceinline__ Dual<T> dasinh(const Dual<T>& a) {
   return chain(a, dasinh(a.v), 1.0 / dsqrt(1.0 + a.v * a.v));
+}
+template <class T>
+__device__ __forceinline__ Dual<T> derf(const Dual<T>& a) {
+  return chain(a, derf(a.v), 1.1283791670955126 * dexp(-(a.v * a.v)));   // 2/sqrt(pi) e^{-x^2}
 }
 template <class T>
 __device__ __forceinline__ Dual<T> dpow(const Dual<T>& a, double p) {
@@ -169,9 +174,10 @@ __device__ N b88_x(N rho, N sig) {
   return -C * r13 - beta * r13 * x * x / dclip(den, 1e-20);
 }
 
+// Paramagnetic VWN correlation per electron (VWN-RPA and VWN5 differ in the
+// constants of this one branch function, xc.py:302-320 and 376-419)
 template <class N>
-__device__ N vwn_rpa_c(N rho) {
-  const double A = 0.0310907, b = 13.0720, c = 42.7198, x0 = -0.409286;
+__device__ N vwn_para(N rho, double A, double b, double c, double x0) {
   N r = dclip(rho, 1e-20);
   N rs = dpow(3.0 / (4.0 * kPi * r), 1.0 / 3.0);
   N x = dsqrt(rs);
@@ -187,6 +193,13 @@ __device__ N vwn_rpa_c(N rho) {
 }
 
 template <class N>
+__device__ N vwn_rpa_c(N rho) { return vwn_para(rho, 0.0310907, 13.0720, 42.7198, -0.409286); }
+
+// VWN5 (xc.py:323-352, paramagnetic parameters)
+template <class N>
+__device__ N vwn5_c(N rho) { return vwn_para(rho, 0.0310907, 3.72744, 12.9352, -0.10498); }
+
+template <class N>
 __device__ N lyp_c(N rho, N sig) {
   const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
   const double C_F = 0.3 * 9.5707800006273;  // (3 pi^2)^(2/3)
@@ -199,11 +212,43 @@ __device__ N lyp_c(N rho, N sig) {
   return -a * g - a * b * om * r * term_ff;
 }
 
+// Short-range fraction F(a) of an erfc-attenuated exchange hole (xc.py:609-631):
+// F(a) = 1 - (8/3) a [sqrt(pi) erf(1/2a) - 3a + 4a^3 + (2a - 4a^3) e^{-1/(4a^2)}],
+// and its asymptote 1/(36 a^2) from a = 8 on.
+template <class N>
+__device__ N erf_attenuation(N a) {
+  a = dclip(a, 1e-10);
+  if (val(a) < 8.0) {
+    N a2 = a * a;
+    N a3 = a2 * a;
+    return 1.0 - (8.0 / 3.0) * a *
+                     (1.7724538509055160 * derf(1.0 / (2.0 * a)) - 3.0 * a + 4.0 * a3 +
+                      (2.0 * a - 4.0 * a3) * dexp(-1.0 / (4.0 * a2)));
+  }
+  return 1.0 / (36.0 * a * a);
+}
+
+// Short-range B88 exchange by the ITYH scheme at omega = 0.33 (xc.py:664-709),
+// closed shell: the per-spin B88 energy eps at (rho/2, |grad rho|/2) times
+// F(a), a = omega sqrt(K) / (6 sqrt(pi) rho_s^{1/3}), K = -2 eps / rho_s^{1/3}.
+template <class N>
+__device__ N ityh_b88_x(N rho, N sig) {
+  const double omega = 0.33;
+  N eps = b88_x(rho, sig);
+  N rs = dclip(rho * 0.5, 1e-20);
+  N r13 = dpow(rs, 1.0 / 3.0);
+  N K = dclip(-2.0 * eps / r13, 1e-20);
+  N a = omega * dsqrt(K) / (6.0 * 1.7724538509055160 * r13);
+  return eps * erf_attenuation(a);
+}
+
 template <class N>
 __device__ N eps_xc(int id, N rho, N sig) {
   switch (id) {
     case kXcPBE: return pbe_x(rho, sig) + pbe_c(rho, sig);
     case kXcPBE0: return 0.75 * pbe_x(rho, sig) + pbe_c(rho, sig);
+    case kXcCAMB3LYP:   // xc.py:711-750: its exact exchange is 0.19 K + 0.46 K_lr(omega = 0.33)
+      return 0.35 * b88_x(rho, sig) + 0.46 * ityh_b88_x(rho, sig) + 0.81 * lyp_c(rho, sig) + 0.19 * vwn5_c(rho);
     default:
       return 0.08 * lda_x(rho) + 0.72 * b88_x(rho, sig) + 0.19 * vwn_rpa_c(rho) + 0.81 * lyp_c(rho, sig);
   }

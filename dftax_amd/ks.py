@@ -133,11 +133,13 @@ class MemoryPlan:
 
 
 def memory_plan(nao: int, naux: int, ngrid: int, nocc: int, precision: str, chunk, ao_block,
-                budget: int) -> MemoryPlan:
+                budget: int, tensors: int = 1) -> MemoryPlan:
     """Pick materialised vs recomputed DF and a resident vs streamed AO grid.
 
     Word counts (FP64, the mixed path adds FP32 copies): B is nao^2 naux and
-    the build holds (mn|P) and B together; the recompute path holds a slab
+    the build holds (mn|P) and B together (a range-separated hybrid,
+    ``tensors = 2``, keeps its long-range B beside it and is materialised
+    only); the recompute path holds a slab
     of nao^2 x chunk, the raw and fitted exchange intermediates (2 nao naux
     nf) and the metric factor; the AO grid is 4 ng nao values plus ng nao for
     Z/W. ``budget`` is the device bytes the handle may use.
@@ -146,8 +148,8 @@ def memory_plan(nao: int, naux: int, ngrid: int, nocc: int, precision: str, chun
     f32 = 1.5 if precision == "mixed" else 1.0
     nf = max(1, min(nao, 512, max(nocc, 1)))
     exch = nao * naux * nf * w * f32 + 64 * nao * nao * w
-    mat = max(2 * nao * nao * naux * w, nao * nao * naux * w * f32 + exch)
-    if chunk is None:
+    mat = max((1 + tensors) * nao * nao * naux * w, tensors * nao * nao * naux * w * f32 + exch)
+    if chunk is None or tensors > 1:
         aux_chunk = 0
     elif isinstance(chunk, int):
         aux_chunk = chunk
@@ -201,7 +203,7 @@ class HipDFKS:
         self.nspin = int(nspin)
         free = _native.device_free(self.device)
         self.plan = memory_plan(basis.nao, aux.nao, self._gw.shape[0], nelec // 2, spec.precision, chunk,
-                                spec.ao_block, int(0.85 * free))
+                                spec.ao_block, int(0.85 * free), 2 if xc.hf_coeff_lr else 1)
         opts = _native.Options(1 if spec.precision == "mixed" else 0, float(spec.metric_cutoff),
                                float(spec.rho_thresh), int(spec.keep_int3c),
                                {"auto": 0, "rocblas": 1, "mfma": 2, "mfma8": 3, "mfma8d": 4}[spec.gemm],

@@ -62,7 +62,12 @@ typedef struct {
   int64_t ngrid;
   const double* grid_xyz;  /* [3*ngrid] Bohr */
   const double* grid_w;    /* [ngrid] quadrature weights */
-  int32_t xc_id;           /* 1 PBE, 2 PBE0, 3 B3LYP */
+  int32_t xc_id;           /* 1 PBE, 2 PBE0, 3 B3LYP, 4 CAM-B3LYP (range-separated, dftax/energy/xc.py:
+                              711-750: exact exchange 0.19 K + 0.46 K_lr with K_lr from a second fitted
+                              tensor of the erf(0.33 r)/r operator, terms.py:578-612; nspin = 1, FP64
+                              and materialised tensors only, dfks_create returns status 7 otherwise;
+                              response and forces return status 7; E[DFKS_E_K] holds both exchange
+                              terms) */
   int32_t nelec;           /* electrons; nspin = 1: nocc = nelec / 2; nspin = 2: n_alpha, n_beta =
                               (nelec +- spin) / 2 */
   int32_t nspin;           /* spin blocks of P and F (the leading axis of dftax's P[nspin, nao, nao],

@@ -12,6 +12,8 @@
  *   - 3-centre (ab|c) with prefactor K_AB 2 pi^2.5 / (p q sqrt(p+q)) and the
  *     (-1)^tau sign on the auxiliary side (eri3c_bucketed.py:376-426);
  *   - 2-centre (P|Q) (eri3c_bucketed.py:697-734);
+ *   - the long-range operator erf(omega r)/r of range-separated hybrids by
+ *     attenuated Boys values (eri3c_bucketed.py:340-346), oracle_set_omega;
  *   - overlap, kinetic and nuclear attraction by the same E/R tables
  *     (eri3c_bucketed.py:534-569, 655-694).
  * Output is in the final AO span: cartesian components are contracted with
@@ -131,9 +133,23 @@ static void ec_table(int lc, double g, int mt, double *E /*[(lc+1)mt]*/) {
         }
 }
 
+/* Operator of the two-electron integrals: 0 the Coulomb operator 1/r, > 0 the
+ * long-range erf(omega r)/r of a range-separated hybrid, whose Boys values are
+ * s^(m+1/2) F_m(s T) with s = omega^2 / (omega^2 + rho) (eri3c_bucketed.py:340-346).
+ * Set by oracle_set_omega before an integral call (the oracle is driven from
+ * one thread). */
+static double g_omega = 0.0;
+
+void oracle_set_omega(double omega) { g_omega = omega; }
+
 /* R^0_{tuv}, t+u+v < mt, stored [t][u][v] in an mt^3 cube (eri3c_bucketed.py:326-369). */
 static void r_table(double rho, const double *pc, int mt, double *R) {
     double T = rho * (pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2]);
+    double s = 1.0;
+    if (g_omega > 0.0) {
+        s = g_omega * g_omega / (g_omega * g_omega + rho);
+        T *= s;
+    }
     static __thread double bufA[MTMAX * MTMAX * MTMAX], bufB[MTMAX * MTMAX * MTMAX];
     double *prev = bufA, *cur = bufB;
     memset(prev, 0, sizeof(double) * mt * mt * mt);
@@ -141,7 +157,7 @@ static void r_table(double rho, const double *pc, int mt, double *R) {
     double n2r = -2.0 * rho;
     for (int m = mt - 1; m >= 0; --m) {
         memset(cur, 0, sizeof(double) * mt * mt * mt);
-        cur[IX(0, 0, 0)] = pow(n2r, m) * oracle_boys(m, T);
+        cur[IX(0, 0, 0)] = pow(n2r, m) * oracle_boys(m, T) * (g_omega > 0.0 ? pow(s, m + 0.5) : 1.0);
         int lev = mt - 1 - m; /* highest t+u+v reachable at this level */
         for (int s = 1; s <= lev; ++s)
             for (int t = 0; t <= s; ++t)

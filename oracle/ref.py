@@ -103,20 +103,36 @@ def int1e(basis, atom_xyz, atom_z):
     return S, T, V
 
 
-def int2c(aux):
+class _Operator:
+    """The two-electron operator of the integral calls inside the block: omega =
+    0 the Coulomb operator, omega > 0 erf(omega r)/r (eri3c_bucketed.py:340-346)."""
+
+    def __init__(self, omega):
+        self.omega = float(omega)
+
+    def __enter__(self):
+        lib().oracle_set_omega(ctypes.c_double(self.omega))
+
+    def __exit__(self, *exc):
+        lib().oracle_set_omega(ctypes.c_double(0.0))
+
+
+def int2c(aux, omega=0.0):
     args, keep = _basis_args(aux)
     c2s, off = _c2s_pack()
     out = np.zeros((aux.nao, aux.nao))
-    lib().oracle_int2c(*args, _p(c2s), _p(off), _p(out))
+    with _Operator(omega):
+        lib().oracle_int2c(*args, _p(c2s), _p(off), _p(out))
     return out
 
 
-def int3c(basis, aux):
+def int3c(basis, aux, omega=0.0):
     a1, k1 = _basis_args(basis)
     a2, k2 = _basis_args(aux)
     c2s, off = _c2s_pack()
     out = np.zeros((basis.nao, basis.nao, aux.nao))
-    lib().oracle_int3c(*a1, *a2, _p(c2s), _p(off), _p(out))
+    with _Operator(omega):
+        lib().oracle_int3c(*a1, *a2, _p(c2s), _p(off), _p(out))
     return out
 
 
@@ -144,7 +160,7 @@ def int3c_triples(basis, aux, triples):
     return blocks
 
 
-def int4c_cart(basis):
+def int4c_cart(basis, omega=0.0):
     """Exact four-centre (ab|cd), cartesian span, orbital l <= 2 (n^4 FP64):
     the reference's exact-ERI path (dftax/integrals/eri4c.py), used only for
     the exact-Coulomb known-answer pin."""
@@ -155,7 +171,8 @@ def int4c_cart(basis):
     args, keep = _basis_args(basis)
     n = basis.nao
     out = np.zeros((n, n, n, n))
-    lib().oracle_int4c_cart(*args[:-2], ctypes.c_int(n), _p(out))
+    with _Operator(omega):
+        lib().oracle_int4c_cart(*args[:-2], ctypes.c_int(n), _p(out))
     return out
 
 
@@ -347,7 +364,9 @@ B88_BETA = 0.0042                                                          # xc.
 VWN_RPA_P = (0.0310907, 13.0720, 42.7198, -0.409286)                       # xc.py:388-391 (A, b, c, x0)
 VWN5_P = (0.0310907, 3.72744, 12.9352, -0.10498)                           # xc.py:330-333
 LYP = {"a": 0.04918, "b": 0.132, "c": 0.2533, "d": 0.349}                 # xc.py:427-519
-MIX = {"PBE0_x": 0.75, "B3LYP": (0.08, 0.72, 0.19, 0.81)}                  # xc.py:527-601
+MIX = {"PBE0_x": 0.75, "B3LYP": (0.08, 0.72, 0.19, 0.81),                  # xc.py:527-601
+       "CAM-B3LYP": (0.35, 0.46, 0.81, 0.19)}                                 # xc.py:711-750 (B88, SR-B88, LYP, VWN5)
+CAM_OMEGA = 0.33                                                             # xc.py:687, 725
 
 
 def lda_x(rho):
@@ -464,7 +483,37 @@ def lyp_c(rho, grad):
     return -a * g - a * b * omega * r * term_ff
 
 
+def erf_attenuation(a):
+    """Short-range fraction F(a) of the erfc-attenuated exchange hole
+    (xc.py:609-631, Gill/Adamson): 1 - 8a/3 [sqrt(pi) erf(1/2a) - 3a + 4a^3 +
+    (2a - 4a^3) exp(-1/4a^2)] below a = 8, the asymptote 1/(36 a^2) above."""
+    t = _torch()
+    a = t.clamp(a, min=1e-10)
+    a_small = t.clamp(a, max=8.0)
+    direct = 1.0 - (8.0 / 3.0) * a_small * (
+        math.sqrt(math.pi) * t.special.erf(1.0 / (2.0 * a_small)) - 3.0 * a_small + 4.0 * a_small ** 3
+        + (2.0 * a_small - 4.0 * a_small ** 3) * t.exp(-1.0 / (4.0 * a_small ** 2)))
+    asym = 1.0 / (36.0 * t.clamp(a, min=8.0) ** 2)
+    return t.where(a < 8.0, direct, asym)
+
+
+def ityh_b88_x(rho, grad, omega=CAM_OMEGA):
+    """Short-range B88 exchange, ITYH scheme (xc.py:664-709), closed shell: the
+    per-spin B88 energy at (rho/2, |grad rho|/2) times F(a) with
+    a = omega sqrt(K) / (6 sqrt(pi) rho_s^{1/3}), K = -2 eps / rho_s^{1/3}."""
+    t = _torch()
+    eps = b88_x(rho, grad)
+    rs = t.clamp(rho / 2, min=1e-20)
+    r13 = rs ** (1 / 3)
+    K = t.clamp(-2.0 * eps / r13, min=1e-20)
+    a = omega * t.sqrt(K) / (6.0 * math.sqrt(math.pi) * r13)
+    return eps * erf_attenuation(a)
+
+
 def eps_xc(name, rho, grad):
+    if name == "CAM-B3LYP":
+        b, sr, l, v = MIX["CAM-B3LYP"]
+        return b * b88_x(rho, grad) + sr * ityh_b88_x(rho, grad) + l * lyp_c(rho, grad) + v * vwn5_c(rho)
     if name == "PBE0":
         return MIX["PBE0_x"] * pbe_x(rho, grad) + pbe_c(rho, grad)
     if name == "PBE":
@@ -477,7 +526,10 @@ def eps_xc(name, rho, grad):
     raise ValueError(name)
 
 
-HF_COEFF = {"PBE0": 0.25, "B3LYP": 0.20, "PBE": 0.0, "LDA": 0.0}
+HF_COEFF = {"PBE0": 0.25, "B3LYP": 0.20, "PBE": 0.0, "LDA": 0.0, "CAM-B3LYP": 0.19}
+# range-separated hybrids: (long-range exact-exchange fraction, omega) of
+# K_hf = a_x K + a_lr K_lr(omega) (xc.py:711-750)
+HF_LR = {"CAM-B3LYP": (0.46, CAM_OMEGA)}
 # closed-shell exchange energy -1/4 a_x sum P o K(P) (terms.py:500-509)
 EXCHANGE_PREFACTOR = -0.25
 
@@ -671,6 +723,17 @@ class OracleKS:
         # exact Coulomb and exchange from four-centre integrals (the reference's
         # coulomb=exact() path, terms.py exact term) instead of density fitting
         self.eri = int4c_cart(self.basis) if exact else None
+        # range-separated hybrids: the long-range exchange K_lr from the
+        # erf(omega r)/r integrals, its own attenuated metric pseudo-inverse
+        # (energy.py:419-433, terms.py:578-612)
+        self.hf_lr, self.omega = HF_LR.get(xc, (0.0, 0.0))
+        self.int3c_lr = self.Vp_lr = self.L_lr = self.eri_lr = None
+        if self.hf_lr != 0.0:
+            if exact:
+                self.eri_lr = int4c_cart(self.basis, self.omega)
+            else:
+                self.Vp_lr, self.L_lr = metric_pinv(int2c(self.aux, self.omega))
+                self.int3c_lr = int3c(self.basis, self.aux, self.omega)
         spec = grid if grid is not None else becke_spec()
         self.grid_xyz, self.grid_w = becke_grid(mol.symbols, mol.coords, spec)
         self.ao, self.dao = eval_ao(self.basis, self.grid_xyz)
@@ -700,10 +763,21 @@ class OracleKS:
             if self.hf != 0.0:
                 Kmat = t.einsum("mlP,PQ,nsQ,ls->mn", I3, Vp, I3, P)
                 eK = EXCHANGE_PREFACTOR * self.hf * t.sum(P * Kmat)
+        if self.hf_lr != 0.0:
+            eK = eK + EXCHANGE_PREFACTOR * self.hf_lr * t.sum(P * self._exchange_lr_t(P))
         rho = t.einsum("gm,mn,gn->g", ao, P, ao)
         grad = 2.0 * t.einsum("gm,mn,gnx->gx", ao, P, dao)
         exc = xc_energy_t(self.xc, rho, grad, w, mask=getattr(self, "xc_mask", None))
         return e1 + eJ + eK + exc + self.e_nn, dict(e1=e1, eJ=eJ, eK=eK, exc=exc)
+
+    def _exchange_lr_t(self, P):
+        """K_lr(P)_mn of the erf(omega r)/r operator, exact or density fitted with
+        the attenuated metric, as a torch function."""
+        t = _torch()
+        if self.eri_lr is not None:
+            return t.einsum("mlns,ls->mn", t.from_numpy(self.eri_lr), P)
+        I3 = t.from_numpy(self.int3c_lr)
+        return t.einsum("mlP,PQ,nsQ,ls->mn", I3, t.from_numpy(self.Vp_lr), I3, P)
 
     def _exchange_matrix_t(self, P):
         """K(P)_mn = sum (ml|ns) P_ls, exact or density fitted, as a torch function."""
@@ -718,6 +792,8 @@ class OracleKS:
         Coulomb terms of the total density, exchange -1/2 a_x sum_s P_s . K(P_s)
         (terms.py:500-509), the spin-polarised XC (terms.py:838-874)."""
         t = _torch()
+        if self.hf_lr != 0.0:
+            raise NotImplementedError("the spin-polarised range-separated hybrids are not restated")
         Pa, Pb = Ps[0], Ps[1]
         P = Pa + Pb
         e1 = t.sum(P * t.from_numpy(self.hcore))
@@ -790,6 +866,15 @@ class OracleKS:
                     Bq = B[:, :, q]
                     K += Bq @ P @ Bq
                 eK = EXCHANGE_PREFACTOR * self.hf * float(np.sum(P * K))
+        K_lr = np.zeros_like(P)
+        if self.hf_lr != 0.0:
+            if self.eri_lr is not None:
+                K_lr = np.einsum("mlns,ls->mn", self.eri_lr, P)
+            else:
+                Bl = np.einsum("mnP,PQ->mnQ", self.int3c_lr, self.L_lr, optimize=True)
+                for q in range(Bl.shape[2]):
+                    K_lr += Bl[:, :, q] @ P @ Bl[:, :, q]
+            eK += EXCHANGE_PREFACTOR * self.hf_lr * float(np.sum(P * K_lr))
         # XC via pointwise autograd of the functional
         rho = np.einsum("gm,mn,gn->g", self.ao, P, self.ao, optimize=True)
         grad = 2.0 * np.einsum("gm,mn,gnx->gx", self.ao, P, self.dao, optimize=True)
@@ -805,8 +890,8 @@ class OracleKS:
         Vxc = Vxc + Vxc.T
         e1 = float(np.sum(P * self.hcore))
         e = e1 + eJ + eK + float(exc_t) + self.e_nn
-        F = self.hcore + J + 2.0 * EXCHANGE_PREFACTOR * self.hf * K + Vxc
-        return e, F, dict(e1=e1, eJ=eJ, eK=eK, exc=float(exc_t), J=J, K=K, Vxc=Vxc)
+        F = self.hcore + J + 2.0 * EXCHANGE_PREFACTOR * (self.hf * K + self.hf_lr * K_lr) + Vxc
+        return e, F, dict(e1=e1, eJ=eJ, eK=eK, exc=float(exc_t), J=J, K=K, K_lr=K_lr, Vxc=Vxc)
 
 
 # ---------------------------------------------------------------------------
