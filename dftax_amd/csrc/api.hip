@@ -1071,6 +1071,95 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   sync(st);
   h->build_ms[1] = now_ms() - t0;
   t0 = now_ms();
+  // The metric's eigendecomposition (rocSOLVER, a long chain of small kernels
+  // on an n = naux matrix) needs only (P|Q): it runs on a side stream with its
+  // own rocBLAS handle while the three-centre build and the AO grid plan run on
+  // st, and is collected before the metric factor. d_U, d_w, d_e, the side
+  // stream and handle stay with this build until then.
+  double* d_U = dalloc<double>((size_t)naux * naux);
+  double* d_w = dalloc<double>(naux);
+  double* d_e = dalloc<double>(naux);
+  hipStream_t s_eig = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s_eig, hipStreamNonBlocking));
+  rocblas_handle rb_eig = rb_acquire();
+  struct SideStream {
+    hipStream_t s;
+    rocblas_handle rb;
+    ~SideStream() {
+      if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
+      if (rb) (void)rocblas_set_stream(rb, nullptr);
+      rb_release(rb);
+    }
+  } side{s_eig, rb_eig};
+  RBCHK(rocblas_set_stream(rb_eig, s_eig));
+  const auto eig_launch = [&](const double* V, rocblas_handle rb, hipStream_t s) {
+    HIPCHK(hipMemcpyAsync(d_U, V, sizeof(double) * naux * naux, hipMemcpyDeviceToDevice, s));
+    RBCHK(rocsolver_dsyevd(rb, rocblas_evect_original, rocblas_fill_upper, naux, d_U, naux, d_w, d_e, h->d_info));
+  };
+  // metric pseudo-inverse factor L = U_keep w_keep^{-1/2} (nkeep columns kept by the relative cutoff) from
+  // the eigenpairs in d_U, d_w once stream s has finished them
+  // the side stream's eigensolve runs on its own host thread: rocSOLVER's divide and conquer
+  // holds its calling thread for much of its run, which would otherwise delay the three-centre
+  // launches queued behind it
+  struct EigThread {
+    std::thread t;
+    int code = 0;
+    std::string msg;
+    void finish() {
+      if (t.joinable()) t.join();
+      if (code) fail(code, msg);
+    }
+    ~EigThread() {
+      if (t.joinable()) t.join();
+    }
+  } eig;
+  const auto eig_factor = [&](int& nkeep, hipStream_t s) {
+    if (s == s_eig) eig.finish();
+    std::vector<double> w(naux);
+    HIPCHK(hipMemcpyAsync(w.data(), d_w, sizeof(double) * naux, hipMemcpyDeviceToHost, s));
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (info != 0) fail(5, "metric eigensolver failed, info=" + std::to_string(info));
+    const double wmax = w[naux - 1];
+    int k0 = 0;
+    while (k0 < naux && !(w[k0] > h->opt.metric_cutoff * wmax)) ++k0;
+    nkeep = naux - k0;
+    double* Lt = dalloc<double>((size_t)nkeep * naux);  // column-major nkeep x naux
+    HIPCHK(launch_metric_factor(naux, k0, d_U, d_w, Lt, st));
+    return Lt;
+  };
+  {
+    // the side stream starts after st's (P|Q) and the allocations above (stream-ordered on st, and a
+    // reused buffer's wait for its previous user is queued on st)
+    hipEvent_t ready = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    const hipError_t e1 = hipEventRecord(ready, st);
+    const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(s_eig, ready, 0) : e1;
+    (void)hipEventDestroy(ready);
+    HIPCHK(e2);
+  }
+  {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    eig.t = std::thread([&eig, &eig_launch, h, dev, rb_eig, s_eig] {
+      try {
+        HIPCHK(hipSetDevice(dev));
+        StreamScope ss(s_eig);
+        eig_launch(h->d_int2c, rb_eig, s_eig);
+        HIPCHK(hipStreamSynchronize(s_eig));
+      } catch (const Fail& f) {
+        eig.code = f.code;
+        eig.msg = g_err;
+      } catch (const std::exception& e) {
+        eig.code = 9;
+        eig.msg = e.what();
+      }
+    });
+  }
   // bra pairs (Schwarz-screened when screen > 0)
   setup_pairs(h, pairs, d_pairs, pair_off, st);
   // three-centre (materialised mode only)
@@ -1098,28 +1187,28 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   h->build_ms[2] = now_ms() - t0;
   t0 = now_ms();
 
-  // metric pseudo-inverse factor L = U_keep w_keep^{-1/2} (nfit columns kept by the relative cutoff)
-  double* d_U = dalloc<double>((size_t)naux * naux);
-  double* d_w = dalloc<double>(naux);
-  double* d_e = dalloc<double>(naux);
-  const auto metric_factor = [&](const double* V, int& nkeep) {
-    HIPCHK(hipMemcpyAsync(d_U, V, sizeof(double) * naux * naux, hipMemcpyDeviceToDevice, st));
-    RBCHK(rocsolver_dsyevd(h->rb, rocblas_evect_original, rocblas_fill_upper, naux, d_U, naux, d_w, d_e, h->d_info));
-    std::vector<double> w(naux);
-    HIPCHK(hipMemcpyAsync(w.data(), d_w, sizeof(double) * naux, hipMemcpyDeviceToHost, st));
-    int info = 0;
-    HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
-    sync(st);
-    if (info != 0) fail(5, "metric eigensolver failed, info=" + std::to_string(info));
-    const double wmax = w[naux - 1];
-    int k0 = 0;
-    while (k0 < naux && !(w[k0] > h->opt.metric_cutoff * wmax)) ++k0;
-    nkeep = naux - k0;
-    double* Lt = dalloc<double>((size_t)nkeep * naux);  // column-major nkeep x naux
-    HIPCHK(launch_metric_factor(naux, k0, d_U, d_w, Lt, st));
-    return Lt;
-  };
-  double* d_Lt = metric_factor(h->d_int2c, h->nfit);
+  // AO values and gradients on the grid: compact blocks, resident, or per block in every Fock build
+  if (h->ng > 0 && h->opt.xc_blocks) {
+    plan_xc_blocks(h, st);
+    h->ao_block = 0;
+  } else if (h->ng > 0) {
+    h->d_exc = dalloc<double>(h->ng);
+    if (h->ao_block == 0) {
+      h->d_ao = dalloc<double>((size_t)4 * h->ng * nao);
+      h->d_zw = dalloc<double>((size_t)h->ng * nao);
+      HIPCHK(launch_ao(h->ng, h->d_gxyz, O, nao, h->d_ao, st));
+    } else {
+      const int64_t gb = std::min<int64_t>(h->ao_block, h->ng);
+      h->d_aob = dalloc<double>((size_t)4 * gb * nao);
+      h->d_zwb = dalloc<double>((size_t)gb * nao);
+    }
+  }
+  sync(st);
+  h->build_ms[4] = now_ms() - t0;
+  t0 = now_ms();
+
+  // the metric factor from the side stream's eigenpairs, then B
+  double* d_Lt = eig_factor(h->nfit, s_eig);
   if (!h->recompute) {
     // B_cm (nfit x nao^2) = Lt (nfit x naux) * int3c_cm (naux x nao^2)
     h->d_B = dalloc<double>((size_t)nao * nao * h->nfit);
@@ -1160,7 +1249,9 @@ void build_all(dfks_ctx* h, hipStream_t st) {
             HIPCHK(launch_eri_cls(la, lb, lc, pp, npair, d_auxl + aux_off[lc], (int)auxl[lc].size(), O, Aw,
                                   h->d_boys, d3lr, (int64_t)nao * naux, naux, 0, 1, st));
         }
-    double* d_Ltlr = metric_factor(d2lr, h->nfit_lr);
+    sync(st);
+    eig_launch(d2lr, h->rb, st);
+    double* d_Ltlr = eig_factor(h->nfit_lr, st);
     h->d_Blr = dalloc<double>((size_t)nao * nao * h->nfit_lr);
     const double one = 1.0, zero = 0.0;
     RBCHK(rocblas_dgemm_64(h->rb, rocblas_operation_none, rocblas_operation_none, h->nfit_lr, (int64_t)nao * nao,
@@ -1173,26 +1264,11 @@ void build_all(dfks_ctx* h, hipStream_t st) {
   }
   dfree(d_pairs);
   dfree(d_auxl);
+  sync(st);
   h->build_ms[3] = now_ms() - t0;
   t0 = now_ms();
   dfree(d_U); dfree(d_w); dfree(d_e);
 
-  // AO values and gradients on the grid: compact blocks, resident, or per block in every Fock build
-  if (h->ng > 0 && h->opt.xc_blocks) {
-    plan_xc_blocks(h, st);
-    h->ao_block = 0;
-  } else if (h->ng > 0) {
-    h->d_exc = dalloc<double>(h->ng);
-    if (h->ao_block == 0) {
-      h->d_ao = dalloc<double>((size_t)4 * h->ng * nao);
-      h->d_zw = dalloc<double>((size_t)h->ng * nao);
-      HIPCHK(launch_ao(h->ng, h->d_gxyz, O, nao, h->d_ao, st));
-    } else {
-      const int64_t gb = std::min<int64_t>(h->ao_block, h->ng);
-      h->d_aob = dalloc<double>((size_t)4 * gb * nao);
-      h->d_zwb = dalloc<double>((size_t)gb * nao);
-    }
-  }
   // Fock-build scratch: exchange intermediate for up to nf_chunk factor columns
   h->d_Tx = dalloc<double>((size_t)nao * std::max(h->nfit, h->nfit_lr) * h->nf_chunk);
   h->d_gamma = dalloc<double>(h->nfit);
@@ -1245,7 +1321,7 @@ void build_all(dfks_ctx* h, hipStream_t st) {
     h->d_offd32 = dalloc<float>(nao);
   }
   sync(st);
-  h->build_ms[4] = now_ms() - t0;
+  h->build_ms[4] += now_ms() - t0;   // the AO grid plan above plus the Fock-build scratch
   h->built = true;
 }
 
