@@ -185,7 +185,7 @@ def stage_times(handle) -> dict:
 
 
 STATS = ["pairs_kept", "pairs_total", "aux_slabs", "max_slab", "ao_block", "eigh_steps", "sp2_iterations",
-         "xc_blocks", "xc_bins", "xc_width_sum"]
+         "xc_blocks", "xc_bins", "xc_width_sum", "pchol_steps"]
 
 
 def stats(handle) -> dict:

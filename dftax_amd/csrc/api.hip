@@ -543,6 +543,10 @@ struct dfks_ctx {
   double *d_coef = nullptr, *d_dots = nullptr, *d_Fx = nullptr, *d_C = nullptr, *d_eps = nullptr, *d_Ws = nullptr;
   // purification: state, Gershgorin bounds, residual diagonal, occupied factor, pivots
   double *d_sp2 = nullptr, *d_lohi = nullptr, *d_dvec = nullptr, *d_L = nullptr;
+  // the occupied factor's projection from the previous step: W = X L_prev (nmo x nocc),
+  // S = W^T W (nocc x nocc); l_valid while d_L holds the previous step's factor
+  double *d_Lw = nullptr, *d_Ls = nullptr;
+  bool l_valid = false;
   double* d_lz = nullptr;   // Lanczos vectors and coefficients of the SP2 spectral bounds
   hipGraphExec_t lz_exec = nullptr;  // the Lanczos bounds as one graph (per nmo)
   int lz_exec_n = -1;
@@ -588,6 +592,7 @@ struct dfks_ctx {
   int sp2_variant = 0;      // tile of the purification products (autotuned when the graph is built)
   bool c_stale = false;     // d_C / d_eps not yet computed for the last step (purified density)
   int n_eigh_steps = 0;     // steps that fell back to the eigensolver
+  int n_pchol_steps = 0;    // purified steps whose occupied factor came from the pivoted Cholesky
   // recompute mode (options.aux_chunk > 0): screened bra pairs per (la, lb)
   // class, auxiliary slabs with per-lc shell lists, slab and exchange buffers
   bool recompute = false;
@@ -617,7 +622,7 @@ struct dfks_ctx {
     void* ptrs[] = {d_boys, d_atm, d_S, d_T, d_V, d_H, d_int2c, d_int3c, d_B, d_gxyz, d_gw, d_ao, d_zw, d_exc,
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
                     d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws,
-                    d_sp2, d_lz, d_lohi, d_dvec, d_L, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
+                    d_sp2, d_lz, d_lohi, d_dvec, d_L, d_Lw, d_Ls, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
                     d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf, d_xc_pts, d_xc_w, d_xc_exc, d_xc_ao,
                     d_xc_zw, d_xc_q, d_xc_ao32, d_xc_cols, d_xc_vlen, d_xc_shl, d_xc_shoff, d_xc_vrow,
                     d_xc_vprow, d_xc_vpcol, d_xc_zw2, d_xc_q2, d_Vxc2, d_uP, d_uJ, d_uK, d_ug, d_xcf_tab,
@@ -2419,11 +2424,22 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   realloc(h->d_lohi, 2 * (size_t)nmo);
   realloc(h->d_dvec, nmo);
   realloc(h->d_L, (size_t)nmo * h->nocc);
+  realloc(h->d_Lw, (size_t)nmo * h->nocc);
+  realloc(h->d_Ls, (size_t)h->nocc * h->nocc);
+  h->l_valid = false;
+  {
+    // load rocSOLVER's Cholesky here rather than in a timed step: a 1 x 1 factor of 1
+    const double one = 1.0;
+    HIPCHK(hipMemcpyAsync(h->d_Ls, &one, sizeof(double), hipMemcpyHostToDevice, st));
+    RBCHK(rocsolver_dpotrf(h->rb, rocblas_fill_lower, 1, h->d_Ls, 1, h->d_info));
+    sync(st);
+  }
   dfree(h->d_piv);
   h->d_piv = dalloc<int32_t>(h->nocc);
   h->sp2_guess = 24;
   h->c_stale = false;
   h->n_eigh_steps = 0;
+  h->n_pchol_steps = 0;
   HIPCHK(hipMemsetAsync(h->d_gram, 0, sizeof(double) * m * m, st));
   HIPCHK(hipMemsetAsync(h->d_Fs, 0, sizeof(double) * m * nn, st));
   HIPCHK(hipMemsetAsync(h->d_Es, 0, sizeof(double) * m * nmo * nmo, st));
@@ -2520,6 +2536,47 @@ void sp2_bounds(dfks_ctx* h, hipStream_t st) {
   else HIPCHK(launch_sp2_lanczos(n, h->d_Ws, h->d_lz, h->d_sp2, st));
 }
 
+// The occupied factor of the purified X from the previous step's factor L_p
+// (orthonormal columns: L_p L_p^T was a rank-nocc projector): W = X L_p,
+// S = W^T W = R R^T (Cholesky), L = W R^{-T}. Then L L^T = W S^{-1} W^T is the
+// orthogonal projector onto range(X L_p), which is range(X) while X L_p has
+// full rank, i.e. L L^T = X, and L^T L = I. Rounding in L grows with the
+// condition of S (whose eigenvalues are the squared cosines of the angles
+// between the two occupied spaces), and so does the deviation of the computed
+// L^T L from I: the factor is taken when max |L^T L - I| <= 1e-10 (S
+// conditioned to ~1e6 or better), the pivoted Cholesky runs otherwise. Two
+// GEMMs of nmo^2 nocc and nmo nocc^2 and a Cholesky of nocc^2 instead of the
+// pivoted Cholesky's nocc sequential pivots.
+bool project_occupied_factor(dfks_ctx* h, const double* X, hipStream_t st) {
+  const int n = h->nmo, nocc = h->nocc;
+  gemm(h, false, false, n, nocc, n, 1.0, X, n, h->d_L, n, 0.0, h->d_Lw, n);
+  gemm(h, true, false, nocc, nocc, n, 1.0, h->d_Lw, n, h->d_Lw, n, 0.0, h->d_Ls, nocc);
+  RBCHK(rocsolver_dpotrf(h->rb, rocblas_fill_lower, nocc, h->d_Ls, nocc, h->d_info));
+  int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+  sync(st);
+  if (info != 0) return false;
+  const double one = 1.0;
+  RBCHK(rocblas_dtrsm(h->rb, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                      rocblas_diagonal_non_unit, n, nocc, &one, h->d_Ls, nocc, h->d_Lw, n));
+  // L^T L into d_Ls (the Cholesky factor is no longer needed)
+  gemm(h, true, false, nocc, nocc, n, 1.0, h->d_Lw, n, h->d_Lw, n, 0.0, h->d_Ls, nocc);
+  std::vector<double> G((size_t)nocc * nocc);
+  HIPCHK(hipMemcpyAsync(G.data(), h->d_Ls, sizeof(double) * G.size(), hipMemcpyDeviceToHost, st));
+  sync(st);
+  double dev = 0.0;
+  for (int j = 0; j < nocc; ++j)
+    for (int i = 0; i < nocc; ++i) dev = std::max(dev, std::fabs(G[(size_t)j * nocc + i] - (i == j ? 1.0 : 0.0)));
+  if (!(dev <= 1e-10)) return false;
+  HIPCHK(hipMemcpyAsync(h->d_L, h->d_Lw, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice, st));
+  // the step's factor check reads the residual trace and smallest pivot of the pivoted
+  // Cholesky: this factor reproduces X as a converged pivoted one would
+  const double chk[2] = {0.0, 1.0};
+  HIPCHK(hipMemcpyAsync(h->d_sp2 + kSpResid, chk, sizeof(chk), hipMemcpyHostToDevice, st));
+  sync(st);
+  return true;
+}
+
 bool purify_density(dfks_ctx* h, hipStream_t st) {
   const int n = h->nmo, nocc = h->nocc, nao = h->nao;
   double* X = h->d_W1;
@@ -2548,17 +2605,22 @@ bool purify_density(dfks_ctx* h, hipStream_t st) {
   h->sp2_guess = std::max(8, (int)stv[kSpIter]);
   if (!(std::fabs(stv[kSpErr]) < 1e-10 * std::max(1, nocc) && std::fabs(stv[kSpTrace] - nocc) < 1e-8 * std::max(1, nocc)))
     return false;
-  // X = L L^T by pivoted Cholesky, panel by panel (X is overwritten by the residual)
-  HIPCHK(launch_diag(n, X, h->d_dvec, st));
-  for (int k0 = 0; k0 < nocc; k0 += kPanel) {
-    const int b = std::min(kPanel, nocc - k0);
-    const int last = k0 + b >= nocc;
-    HIPCHK(launch_pchol_panel(n, k0, b, X, h->d_dvec, h->d_L, h->d_piv, h->d_sp2, last, st));
-    if (!last) {
-      const GemmArgs u{n, n, b, h->d_L + (int64_t)k0 * n, n, false, h->d_L + (int64_t)k0 * n, n, false};
-      HIPCHK(launch_gemm(kF64, u, X, -1.0, 1.0, 1, 1, h->d_gws, st));
+  // X = L L^T: projected from the previous step's factor, or by pivoted Cholesky, panel by
+  // panel (X is overwritten by the residual)
+  if (!(h->l_valid && project_occupied_factor(h, X, st))) {
+    HIPCHK(launch_diag(n, X, h->d_dvec, st));
+    for (int k0 = 0; k0 < nocc; k0 += kPanel) {
+      const int b = std::min(kPanel, nocc - k0);
+      const int last = k0 + b >= nocc;
+      HIPCHK(launch_pchol_panel(n, k0, b, X, h->d_dvec, h->d_L, h->d_piv, h->d_sp2, last, st));
+      if (!last) {
+        const GemmArgs u{n, n, b, h->d_L + (int64_t)k0 * n, n, false, h->d_L + (int64_t)k0 * n, n, false};
+        HIPCHK(launch_gemm(kF64, u, X, -1.0, 1.0, 1, 1, h->d_gws, st));
+      }
     }
+    h->n_pchol_steps += 1;
   }
+  h->l_valid = true;
   gemm(h, false, false, nao, nocc, n, std::sqrt(2.0), h->d_X, nao, h->d_L, n, 0.0, h->d_Cf, nao);
   return true;
 }
@@ -2610,6 +2672,7 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
     HIPCHK(launch_axpy((int64_t)nao * nocc, std::sqrt(2.0), h->d_C, h->d_Cf, 0, st));
     h->c_stale = false;
     h->n_eigh_steps += 1;
+    h->l_valid = false;
   } else {
     h->c_stale = true;
   }
@@ -2887,9 +2950,10 @@ int dfks_stats(dfks_handle h, int64_t* out, int32_t n) {
     if (!h || !out) fail(1, "null argument");
     int64_t width = 0;
     for (const auto& bn : h->xbins) width += bn.nblk * bn.n;
-    const int64_t v[10] = {h->pairs_kept, h->pairs_total, (int64_t)h->slab_np.size(), h->max_np, h->ao_block,
-                           h->n_eigh_steps, h->sp2_guess, h->xc_nblk, (int64_t)h->xbins.size(), width};
-    for (int i = 0; i < n; ++i) out[i] = i < 10 ? v[i] : 0;
+    const int64_t v[11] = {h->pairs_kept, h->pairs_total, (int64_t)h->slab_np.size(), h->max_np, h->ao_block,
+                           h->n_eigh_steps, h->sp2_guess, h->xc_nblk, (int64_t)h->xbins.size(), width,
+                           h->n_pchol_steps};
+    for (int i = 0; i < n; ++i) out[i] = i < 11 ? v[i] : 0;
   });
 }
 

@@ -141,8 +141,9 @@ int dfks_sizes(dfks_handle h, int64_t* out5);
  * [2] auxiliary slabs (recompute mode, else 0), [3] largest slab in functions,
  * [4] AO block (0: resident grid), [5] SCF steps that fell back to the eigensolver,
  * [6] SP2 iterations of the last purified step, [7] compact XC blocks (0: dense
- * rows), [8] their bins of equal width, [9] the sum of their padded widths.
- * Fills n values (zeros past 10). */
+ * rows), [8] their bins of equal width, [9] the sum of their padded widths,
+ * [10] purified SCF steps whose occupied factor came from the pivoted Cholesky
+ * (the others project the previous step's factor). Fills n values (zeros past 11). */
 int dfks_stats(dfks_handle h, int64_t* out, int32_t n);
 
 int dfks_get_matrix(dfks_handle h, int32_t which, double* dev_out, dfks_stream stream);

@@ -158,6 +158,9 @@ def test_h2o32_fitted_tensor_and_metric(h2o32, torch_cuda):
     rng = np.random.default_rng(32)
     m = rng.integers(0, nao, 40)
     n = rng.integers(0, nao, 40)
+    # the copy of B (51 GB) comes from torch's allocator: hand back the buffers the library keeps
+    # for reuse (earlier handles of this process) first
+    N.release_memory(ks.coulomb.device)
     B = ks.coulomb.matrix(N.DFKS_BFIT, (nao * nao, nfit))
     Bs = B[torch_cuda.tensor(m * nao + n, device=B.device)].cpu().numpy()
     del B

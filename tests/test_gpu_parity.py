@@ -264,7 +264,8 @@ def test_purification_graph_survives_scf_restart(dimer_backends):
 
 
 def test_purified_density_matches_eigensolver(dimer_backends, torch_cuda):
-    """SP2 purification + pivoted-Cholesky factor against an FP64 eigensolve in every step.
+    """SP2 purification + occupied factor (pivoted Cholesky, then projections of the
+    previous step's factor) against an FP64 eigensolve in every step.
 
     Both runs start from the same guess; with a HOMO-LUMO gap the purified
     projector is the aufbau density, so the two SCF trajectories agree step by
@@ -273,10 +274,14 @@ def test_purified_density_matches_eigensolver(dimer_backends, torch_cuda):
     from dftax_amd import scf
     a, e = dimer_backends["rocblas"], dimer_backends["eigh"]
     ra = scf(a, e_tol=1e-10, d_tol=1e-7)
+    npc = a.coulomb.stats()["pchol_steps"]
     re = scf(e, e_tol=1e-10, d_tol=1e-7)
-    _say(f"purified {ra.e_tot:.12f} ({ra.n_iter} it); eigensolver {re.e_tot:.12f} ({re.n_iter} it)")
+    _say(f"purified {ra.e_tot:.12f} ({ra.n_iter} it, {npc} pivoted-Cholesky factors, the rest projected); "
+         f"eigensolver {re.e_tot:.12f} ({re.n_iter} it)")
     assert ra.converged and re.converged
     assert ra.n_iter == re.n_iter
+    # the first purified step factors X by pivoted Cholesky; later steps project the previous factor
+    assert 1 <= npc < ra.n_iter - 2
     dEs = [abs(x[0] - y[0]) for x, y in zip(ra.history, re.history)]
     assert max(dEs) < 1e-9, dEs
     assert abs(ra.e_tot - re.e_tot) < 1e-10
