@@ -1669,9 +1669,11 @@ void xcf_prepare(dfks_ctx* h, hipStream_t st) {
   h->xcf_tab.clear();
   h->xcf_fused.assign(h->xbins.size(), 0);
   int64_t part = 0, coef = 0, wz = 0, wp = 0, wv = 0;
+  // the fused kernels address P with 32-bit offsets: larger systems take the per-bin path
+  const bool small_p = (int64_t)h->nao * h->nao <= 0x7fffffff;
   for (size_t i = 0; i < h->xbins.size(); ++i) {
     const auto& bn = h->xbins[i];
-    if (bn.n > kXfMaxN || bn.n % 32 != 0) continue;
+    if (!small_p || bn.n > kXfMaxN || bn.n % 32 != 0) continue;
     int nt = 0, nks = 0;
     int64_t pd = 0, cf = 0;
     xcf_bin_geometry(bn.n, (int)bn.nblk, nt, nks, pd, cf);

@@ -940,25 +940,25 @@ __device__ __forceinline__ void xf_split8(const float* x, __bf16* hi, __bf16* lo
 __device__ __forceinline__ void xf_mfma_step(const __bf16 (*sAh)[kXfPad], const __bf16 (*sAl)[kXfPad],
                                              const __bf16 (*sBh)[kXfPad], const __bf16 (*sBl)[kXfPad], int wm,
                                              int wn, int li, int lk, xflt4 (&acc)[4][4]) {
-  xbf16x8 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    ah[i] = *reinterpret_cast<const xbf16x8*>(&sAh[wm + 16 * i + li][8 * lk]);
-    al[i] = *reinterpret_cast<const xbf16x8*>(&sAl[wm + 16 * i + li][8 * lk]);
-  }
+  // the B fragments stay in registers for the step; the A fragments are read one row block
+  // at a time (40 operand registers live instead of 64)
+  xbf16x8 bh[4], bl[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     bh[j] = *reinterpret_cast<const xbf16x8*>(&sBh[wn + 16 * j + li][8 * lk]);
     bl[j] = *reinterpret_cast<const xbf16x8*>(&sBl[wn + 16 * j + li][8 * lk]);
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    const xbf16x8 ah = *reinterpret_cast<const xbf16x8*>(&sAh[wm + 16 * i + li][8 * lk]);
+    const xbf16x8 al = *reinterpret_cast<const xbf16x8*>(&sAl[wm + 16 * i + li][8 * lk]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
     }
+  }
 }
 
 // Z tile rows staged through LDS for the rho reduction: 64 rows of 128 floats
@@ -970,7 +970,7 @@ static_assert(64 * kXfZPad * 4 <= 4 * 128 * kXfPad * 2, "Z half tile fits the st
 // and Z = Phi Pg are linear in both, so the tiles and K chunks of a block run as
 // separate workgroups and their sums add up (k_xcf_pt, in a fixed order).
 // part[((b nt + t) nks + ks)][4][128]: sum_c phi Z, sum_c d_x phi Z, d_y, d_z.
-__global__ __launch_bounds__(256, 2) void k_xcf_z(const XcfBin* __restrict__ tab, int nbins, int ztot, int kchunk,
+__global__ __launch_bounds__(256, 3) void k_xcf_z(const XcfBin* __restrict__ tab, int nbins, int ztot, int kchunk,
                                                   const float* __restrict__ ao_base,
                                                   const int32_t* __restrict__ cols_base, const float* __restrict__ P,
                                                   int64_t nao, double* __restrict__ part_base) {
@@ -1024,6 +1024,7 @@ __global__ __launch_bounds__(256, 2) void k_xcf_z(const XcfBin* __restrict__ tab
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = xflt4{0.f, 0.f, 0.f, 0.f};
   const int32_t pc = c0 + br < n ? s_col[c0 + br] : -1;
+  const int nao32 = (int)nao;
   float a[16], b[16];
   auto load = [&](int k0) {
     const float4* src = reinterpret_cast<const float4*>(arow + k0 + sk);
@@ -1032,10 +1033,14 @@ __global__ __launch_bounds__(256, 2) void k_xcf_z(const XcfBin* __restrict__ tab
       const float4 x = src[v];
       a[4 * v] = x.x; a[4 * v + 1] = x.y; a[4 * v + 2] = x.z; a[4 * v + 3] = x.w;
     }
+    // 32-bit offsets from P (nao^2 < 2^31, checked at launch), clamped to element 0 for
+    // padding so that every lane loads
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int32_t kc = s_col[k0 + bk + j];
-      b[j] = (pc >= 0 && kc >= 0) ? P[(int64_t)kc * nao + pc] : 0.f;
+      const bool ok = pc >= 0 && kc >= 0;
+      const float v = P[ok ? (uint32_t)(kc * nao32 + pc) : 0u];
+      b[j] = ok ? v : 0.f;
     }
   };
   load(kb);
@@ -1253,6 +1258,7 @@ hipError_t launch_xcf_all(const XcfBin* tab, int nbins, int ztot, int ptot, int 
                           double* exc, double* part, float* coef, float* q, hipStream_t st) {
   if (nbins == 0) return hipSuccess;
   if ((uintptr_t)ao % 16 != 0 || (uintptr_t)coef % 16 != 0) return hipErrorInvalidValue;
+  if (nao * nao > 0x7fffffff) return hipErrorInvalidValue;   // k_xcf_z addresses P with 32-bit offsets
   // 8 ceil(total / 8) workgroups: the XCD mappings produce every id < total only then
   hipLaunchKernelGGL(k_xcf_z, dim3((unsigned)(8 * (((int64_t)ztot + 7) / 8))), dim3(256), 0, st, tab, nbins, ztot,
                      kXfKChunk, ao, cols, P, nao, part);
