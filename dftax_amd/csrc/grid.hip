@@ -217,14 +217,21 @@ template hipError_t launch_gather_sym<float, float>(int64_t, int, const int32_t*
 
 // Vh[m][:] = sum over the (block, local column i) entries of row m of the
 // block products prod_b[i][:], scattered to the blocks' columns. Workgroup
-// (m, s) takes slice s of row m's entries, one entry per wavefront at a time,
-// and accumulates the row in LDS (FP64 LDS atomics: wavefronts may hit the
-// same column); slice s writes out + s * nao^2 (the slices are summed after).
+// (m, s) takes slice s of row m's entries, two entries per wavefront at a
+// time, and accumulates the row in LDS (FP64 LDS atomics: wavefronts may hit
+// the same column); slice s writes out + s * nao^2 (the slices are summed
+// after). Each lane reads 16 bytes of a product row (4 floats or 2 doubles)
+// and the matching column indices per load; the block widths are multiples
+// of 32, so every row and column list starts 16-byte aligned. Loads past a
+// row's end are clamped to its start and masked, so every lane loads.
 template <typename T>
 __global__ __launch_bounds__(256) void k_vassemble(int nao, int nslice, const int64_t* __restrict__ rowptr,
                                                     const int64_t* __restrict__ prow, const int64_t* __restrict__ pcol,
                                                     const int32_t* __restrict__ plen, const T* __restrict__ q,
                                                     const int32_t* __restrict__ cols, double* __restrict__ out) {
+  constexpr int V = 16 / sizeof(T);   // elements per 16-byte load
+  struct alignas(16) Tv { T x[V]; };
+  struct alignas(4 * V) Cv { int32_t c[V]; };
   extern __shared__ double acc[];
   const int m = blockIdx.x, s = blockIdx.y;
   if (m >= nao || s >= nslice) return;
@@ -233,32 +240,32 @@ __global__ __launch_bounds__(256) void k_vassemble(int nao, int nslice, const in
   __syncthreads();
   const int64_t r0 = rowptr[m], cnt = rowptr[m + 1] - r0;
   const int64_t e0 = r0 + cnt * s / nslice, e1 = r0 + cnt * (s + 1) / nslice;
-  // two entries per wavefront in flight
+  auto add = [&](const Tv& x, const Cv& c, bool ok) {
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (ok && c.c[v] >= 0) atomicAdd(&acc[c.c[v]], (double)x.x[v]);
+  };
   int64_t e = e0 + wave;
   for (; e + 4 < e1; e += 8) {
-    const T* s0 = q + prow[e];
-    const T* s1 = q + prow[e + 4];
-    const int32_t* d0 = cols + pcol[e];
-    const int32_t* d1 = cols + pcol[e + 4];
-    const int n0 = plen[e], n1 = plen[e + 4];
+    const Tv* s0 = reinterpret_cast<const Tv*>(q + prow[e]);
+    const Tv* s1 = reinterpret_cast<const Tv*>(q + prow[e + 4]);
+    const Cv* d0 = reinterpret_cast<const Cv*>(cols + pcol[e]);
+    const Cv* d1 = reinterpret_cast<const Cv*>(cols + pcol[e + 4]);
+    const int n0 = plen[e] / V, n1 = plen[e + 4] / V;
     const int nmax = n0 > n1 ? n0 : n1;
     for (int j = lane; j < nmax; j += 64) {
-      const int32_t c0 = j < n0 ? d0[j] : -1;
-      const int32_t c1 = j < n1 ? d1[j] : -1;
-      const double v0 = c0 >= 0 ? (double)s0[j] : 0.0;
-      const double v1 = c1 >= 0 ? (double)s1[j] : 0.0;
-      if (c0 >= 0) atomicAdd(&acc[c0], v0);
-      if (c1 >= 0) atomicAdd(&acc[c1], v1);
+      const bool ok0 = j < n0, ok1 = j < n1;
+      const Tv x0 = s0[ok0 ? j : 0], x1 = s1[ok1 ? j : 0];
+      const Cv c0 = d0[ok0 ? j : 0], c1 = d1[ok1 ? j : 0];
+      add(x0, c0, ok0);
+      add(x1, c1, ok1);
     }
   }
   for (; e < e1; e += 4) {
-    const T* src = q + prow[e];
-    const int32_t* dst = cols + pcol[e];
-    const int n = plen[e];
-    for (int j = lane; j < n; j += 64) {
-      const int32_t c = dst[j];
-      if (c >= 0) atomicAdd(&acc[c], (double)src[j]);
-    }
+    const Tv* src = reinterpret_cast<const Tv*>(q + prow[e]);
+    const Cv* dst = reinterpret_cast<const Cv*>(cols + pcol[e]);
+    const int n = plen[e] / V;
+    for (int j = lane; j < n; j += 64) add(src[j], dst[j], true);
   }
   __syncthreads();
   double* o = out + (int64_t)s * nao * nao + (int64_t)m * nao;
@@ -282,6 +289,8 @@ hipError_t launch_vassemble(int nao, const int64_t* rowptr, const int64_t* prow,
                             hipStream_t st) {
   if (nao <= 0) return hipSuccess;
   if ((size_t)nao * sizeof(double) > 64 * 1024 || nslice < 1 || (nslice > 1 && !ws)) return hipErrorInvalidValue;
+  // 16-byte loads: the product rows and column lists start at multiples of 32 elements
+  if ((uintptr_t)q % 16 != 0 || (uintptr_t)cols % 16 != 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_vassemble<T>, dim3((unsigned)nao, (unsigned)nslice), dim3(256), nao * sizeof(double), st, nao,
                      nslice, rowptr, prow, pcol, plen, q, cols, nslice > 1 ? ws : V);
   hipError_t e = hipGetLastError();
