@@ -2458,11 +2458,37 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
 // factor L (pivoted Cholesky) and Cf = sqrt(2) X L. Returns false when the
 // purification does not reach an idempotent projector of trace nocc (no gap
 // at the Fermi level); the caller then takes the eigensolver path.
+// Split-K factor of the purification product X X (n x n, upper 128 x 128 tiles, one
+// 512-thread workgroup per CU): the s in 1..8 (K chunks of at least 64) that minimises
+// rounds x (chunk + 128), rounds = ceil(tiles s / ncu) and 128 the fixed cost of a
+// workgroup (pipeline fill, partial-tile epilogue) in units of K. n = 1376 (66 tiles,
+// 256 CUs): 3, one round of 198 workgroups (2 used half the CUs, 4 needs a second
+// round for 8 of them).
+int sp2_split(int n) {
+  static const int ncu = [] {
+    int dev = 0, c = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c < 1)
+      c = 256;
+    return c;
+  }();
+  const int64_t ntn = (n + 127) / 128, tiles = ntn * (ntn + 1) / 2;
+  int best = 1;
+  double best_t = 1e300;
+  for (int sp = 1; sp <= 8 && (sp == 1 || n / sp >= 64); ++sp) {
+    const double t = (double)((tiles * sp + ncu - 1) / ncu) * ((n + sp - 1) / sp + 128);
+    if (t < best_t - 1e-12) {
+      best_t = t;
+      best = sp;
+    }
+  }
+  return best;
+}
+
 // kSp2Block purification iterations (product, trace, update), launched eagerly
 hipError_t sp2_block(dfks_ctx* h, hipStream_t st) {
   const int n = h->nmo;
-  const int ntn = (n + 127) / 128;
-  const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, n);
+  const int nsplit = sp2_split(n);
   const GemmArgs g{n, n, n, h->d_W1, n, true, h->d_W1, n, true};
   for (int c = 0; c < kSp2Block; ++c) {
     hipError_t e = launch_gemm_partials(kF64, g, nsplit, 1, h->d_gws, h->sp2_variant, st);
@@ -2485,8 +2511,7 @@ void sp2_run_blocks(dfks_ctx* h, int nblocks, hipStream_t st) {
     h->sp2_exec_n = h->nmo;
     // the products' tile, timed eagerly on the current X (the partials are recomputed by the graph)
     const int n = h->nmo;
-    const int ntn = (n + 127) / 128;
-    const int nsplit = split_for((int64_t)ntn * (ntn + 1) / 2, n);
+    const int nsplit = sp2_split(n);
     const GemmArgs g{n, n, n, h->d_W1, n, true, h->d_W1, n, true};
     h->sp2_variant = tuned_product(h, 5, n, n, nsplit, 2, 1, [&](int c) {
       HIPCHK(launch_gemm_partials(kF64, g, nsplit, 1, h->d_gws, c, st));
