@@ -146,6 +146,8 @@ hipError_t launch_sp2_lanczos(int n, const double* F, double* lz, double* st, hi
 hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, hipStream_t s);
 hipError_t launch_sp2_begin(int n, const double* X, int nocc, double* st, hipStream_t s);
 hipError_t launch_sp2_update(int n, int nsplit, const double* ws, double* X, int nocc, double* st, hipStream_t s);
+int sp2_step_tiles(int n);
+hipError_t launch_sp2_step(int n, const double* X, double* Xn, int nocc, double* st, hipStream_t s);
 hipError_t launch_diag(int n, const double* R, double* d, hipStream_t s);
 hipError_t launch_pchol_panel(int n, int k0, int b, const double* R, double* d, double* L, int32_t* piv, double* st,
                               int last, hipStream_t s);
@@ -2485,9 +2487,23 @@ int sp2_split(int n) {
   return best;
 }
 
-// kSp2Block purification iterations (product, trace, update), launched eagerly
+// kSp2Block purification iterations, launched eagerly. Up to kSp2Groups upper 64 x 64
+// tiles (n <= 2015): one fused kernel per iteration (product, update, trace sums) that
+// alternates X between d_W1 and the split-K workspace, so the block (an even number of
+// iterations) leaves X in d_W1. Larger n: the split-K product's partial tiles, then the
+// update kernel.
 hipError_t sp2_block(dfks_ctx* h, hipStream_t st) {
   const int n = h->nmo;
+  static_assert(kSp2Block % 2 == 0, "the fused iterations end in d_W1");
+  if (sp2_step_tiles(n) <= kSp2Groups) {
+    for (int c = 0; c < kSp2Block; ++c) {
+      const double* X = c % 2 == 0 ? h->d_W1 : h->d_gws;
+      double* Xn = c % 2 == 0 ? h->d_gws : h->d_W1;
+      const hipError_t e = launch_sp2_step(n, X, Xn, h->nocc, h->d_sp2, st);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   const int nsplit = sp2_split(n);
   const GemmArgs g{n, n, n, h->d_W1, n, true, h->d_W1, n, true};
   for (int c = 0; c < kSp2Block; ++c) {
@@ -2511,11 +2527,13 @@ void sp2_run_blocks(dfks_ctx* h, int nblocks, hipStream_t st) {
     h->sp2_exec_n = h->nmo;
     // the products' tile, timed eagerly on the current X (the partials are recomputed by the graph)
     const int n = h->nmo;
-    const int nsplit = sp2_split(n);
-    const GemmArgs g{n, n, n, h->d_W1, n, true, h->d_W1, n, true};
-    h->sp2_variant = tuned_product(h, 5, n, n, nsplit, 2, 1, [&](int c) {
-      HIPCHK(launch_gemm_partials(kF64, g, nsplit, 1, h->d_gws, c, st));
-    }, st);
+    if (sp2_step_tiles(n) > kSp2Groups) {
+      const int nsplit = sp2_split(n);
+      const GemmArgs g{n, n, n, h->d_W1, n, true, h->d_W1, n, true};
+      h->sp2_variant = tuned_product(h, 5, n, n, nsplit, 2, 1, [&](int c) {
+        HIPCHK(launch_gemm_partials(kF64, g, nsplit, 1, h->d_gws, c, st));
+      }, st);
+    }
     hipGraph_t gr = nullptr;
     if (hipGraphCreate(&gr, 0) == hipSuccess && gr) {
       GraphRec rec;

@@ -208,6 +208,127 @@ __global__ __launch_bounds__(256) void k_sp2_reduce_update(int n, int nsplit, co
   sp2_partials(a, b, st);
 }
 
+// One purification iteration in one kernel, for n with at most kSp2Groups upper
+// 64 x 64 tiles (n <= 2015). Workgroup b < ntiles takes upper tile (ti, tj),
+// ti <= tj, of S = X X (FP64 MFMA over the full K; both operands are rows of
+// the symmetric X), applies the decided branch, Xn = S or 2X - S, writes the
+// tile and its mirror (through LDS, so both stores are row-contiguous), and
+// its share of tr Xn and sum_ij Xn_ij^2 (off-diagonal tiles count twice) to
+// the partial slot b; workgroups ntiles..kSp2Groups-1 write zero partials.
+// Once frozen, Xn = X. X and Xn alternate between iterations.
+typedef double sq4 __attribute__((ext_vector_type(4)));
+constexpr int kSqT = 64;   // tile edge
+constexpr int kSqK = 64;   // k per LDS stage (a few global round trips per tile: one per stage)
+
+__global__ __launch_bounds__(256) void k_sp2_step(int n, int ntiles, const double* __restrict__ X,
+                                                  double* __restrict__ Xn, double* __restrict__ st) {
+  // operand stages, rows padded to 66 doubles (the MFMA operand reads of a half-wavefront,
+  // 16 rows x 2 k, fall on distinct bank pairs); after the k loop the first one holds the
+  // updated tile for its mirror
+  constexpr int kLd = kSqK + 2;
+  __shared__ double sA[kSqT][kLd];
+  __shared__ double sB[kSqT][kLd];
+  static_assert(kSqK == kSqT, "the mirror tile reuses sA");
+  double (*sT)[kLd] = sA;
+  const int b = blockIdx.x;
+  if (b >= ntiles) {
+    sp2_partials(0.0, 0.0, st);
+    return;
+  }
+  const int ntn = (n + kSqT - 1) / kSqT;
+  int ti = 0, t = b;
+  while (t >= ntn - ti) {
+    t -= ntn - ti;
+    ++ti;
+  }
+  const int tj = ti + t;
+  const int m0 = ti * kSqT, n0 = tj * kSqT;
+  const double flag = st[kSpFlag];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  sq4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = sq4{0, 0, 0, 0};
+  if (flag >= 0.0) {
+    // staging: a wavefront loads 64 consecutive k of one row per instruction (512
+    // contiguous bytes); thread t takes k = t % 64 of rows t / 64 + 4 j, j < 16, clamped
+    // to element 0 outside X and masked
+    const int lq = tid & 63, lr = tid >> 6;
+    double xa[16], xb[16];
+    auto load = [&](int k0) {
+      const int k = k0 + lq;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int ra = m0 + lr + 4 * j, rb = n0 + lr + 4 * j;
+        const bool oka = ra < n && k < n, okb = rb < n && k < n;
+        const double va = X[oka ? (int64_t)ra * n + k : 0];
+        const double vb = X[okb ? (int64_t)rb * n + k : 0];
+        xa[j] = oka ? va : 0.0;
+        xb[j] = okb ? vb : 0.0;
+      }
+    };
+    load(0);
+    for (int k0 = 0; k0 < n; k0 += kSqK) {
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        sA[lr + 4 * j][lq] = xa[j];
+        sB[lr + 4 * j][lq] = xb[j];
+      }
+      __syncthreads();
+      if (k0 + kSqK < n) load(k0 + kSqK);
+#pragma unroll
+      for (int kk = 0; kk < kSqK; kk += 4) {
+        double a[2], bb[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = sA[wm + 16 * i + li][kk + lk];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bb[j] = sB[wn + 16 * j + li][kk + lk];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  // v_mfma_f64_16x16x4f64 result: lane l, register r holds row (l/16) + 4r, column l%16
+  double ta = 0.0, tb = 0.0;
+  const double wsq = ti == tj ? 1.0 : 2.0;
+  __syncthreads();   // every wavefront is done reading sA before it takes the tile
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int lm = wm + 16 * i + lk + 4 * r, ln = wn + 16 * j + li;
+        const int gm = m0 + lm, gn = n0 + ln;
+        double x = 0.0;
+        if (gm < n && gn < n) {
+          const double x0 = X[(int64_t)gm * n + gn];
+          const double sq = acc[i][j][r];
+          x = flag < 0.0 ? x0 : flag > 0.5 ? sq : 2.0 * x0 - sq;
+          Xn[(int64_t)gm * n + gn] = x;
+          if (gm == gn) ta += x;
+          tb += wsq * x * x;
+        }
+        sT[lm][ln] = x;
+      }
+  if (ti != tj) {
+    __syncthreads();
+    // mirror: Xn[n0 + c][m0 + r] = tile[r][c], 64 consecutive columns per wavefront row
+    for (int e = tid; e < kSqT * kSqT; e += 256) {
+      const int c = e >> 6, r = e & 63;
+      const int gm = n0 + c, gn = m0 + r;
+      if (gm < n && gn < n) Xn[(int64_t)gm * n + gn] = sT[r][c];
+    }
+  }
+  sp2_partials(ta, tb, st);
+}
+
 // d[i] = R_ii
 __global__ void k_diag(int n, const double* __restrict__ R, double* __restrict__ d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -485,6 +606,22 @@ hipError_t launch_sp2_begin(int n, const double* X, int nocc, double* st, hipStr
 hipError_t launch_sp2_update(int n, int nsplit, const double* ws, double* X, int nocc, double* st, hipStream_t s) {
   const hipError_t e = launch_k(k_sp2_reduce_update, dim3(kSp2Groups), dim3(256), s, n, nsplit, ws,
                                 (int64_t)n * n, X, st);
+  if (e != hipSuccess) return e;
+  return launch_k(k_sp2_decide, dim3(1), dim3(kSp2Groups), s, nocc, st);
+}
+
+// Upper 64 x 64 tiles of an n x n purification step; k_sp2_step takes n when this is
+// at most kSp2Groups.
+int sp2_step_tiles(int n) {
+  const int ntn = (n + kSqT - 1) / kSqT;
+  return ntn * (ntn + 1) / 2;
+}
+
+// One fused iteration, X -> Xn, then the next decision.
+hipError_t launch_sp2_step(int n, const double* X, double* Xn, int nocc, double* st, hipStream_t s) {
+  const int ntiles = sp2_step_tiles(n);
+  if (ntiles > kSp2Groups) return hipErrorInvalidValue;
+  const hipError_t e = launch_k(k_sp2_step, dim3(kSp2Groups), dim3(256), s, n, ntiles, X, Xn, st);
   if (e != hipSuccess) return e;
   return launch_k(k_sp2_decide, dim3(1), dim3(kSp2Groups), s, nocc, st);
 }
