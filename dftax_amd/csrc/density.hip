@@ -137,14 +137,15 @@ __device__ void sp2_decide(double t1, double t2, int nocc, double* __restrict__ 
 }
 
 // Per-workgroup partial sums of tr X and of sum_ij X_ij^2 (= tr X^2 for the
-// symmetric X) into st[kSpPart + 2 b], st[kSpPart + 2 b + 1]; kSp2Groups
-// workgroups of 256 threads stride over the n^2 elements.
+// symmetric X) into st[kSpPart + 2 b], st[kSpPart + 2 b + 1], from the NT
+// threads of workgroup b (kSp2Groups workgroups).
+template <int NT = 256>
 __device__ void sp2_partials(double a, double b, double* __restrict__ st) {
-  __shared__ double s_a[256], s_b[256];
+  __shared__ double s_a[NT], s_b[NT];
   s_a[threadIdx.x] = a;
   s_b[threadIdx.x] = b;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = NT / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       s_a[threadIdx.x] += s_a[threadIdx.x + w];
       s_b[threadIdx.x] += s_b[threadIdx.x + w];
@@ -219,8 +220,9 @@ __global__ __launch_bounds__(256) void k_sp2_reduce_update(int n, int nsplit, co
 typedef double sq4 __attribute__((ext_vector_type(4)));
 constexpr int kSqT = 64;   // tile edge
 constexpr int kSqK = 64;   // k per LDS stage (a few global round trips per tile: one per stage)
+constexpr int kSqN = 512;  // threads: 8 wavefronts (two per SIMD), each a 32 x 16 sub-tile
 
-__global__ __launch_bounds__(256) void k_sp2_step(int n, int ntiles, const double* __restrict__ X,
+__global__ __launch_bounds__(kSqN) void k_sp2_step(int n, int ntiles, const double* __restrict__ X,
                                                   double* __restrict__ Xn, double* __restrict__ st) {
   // operand stages, rows padded to 66 doubles (the MFMA operand reads of a half-wavefront,
   // 16 rows x 2 k, fall on distinct bank pairs); after the k loop the first one holds the
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(256) void k_sp2_step(int n, int ntiles, const doubl
   double (*sT)[kLd] = sA;
   const int b = blockIdx.x;
   if (b >= ntiles) {
-    sp2_partials(0.0, 0.0, st);
+    sp2_partials<kSqN>(0.0, 0.0, st);
     return;
   }
   const int ntn = (n + kSqT - 1) / kSqT;
@@ -245,24 +247,22 @@ __global__ __launch_bounds__(256) void k_sp2_step(int n, int ntiles, const doubl
   const int m0 = ti * kSqT, n0 = tj * kSqT;
   const double flag = st[kSpFlag];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave >> 2) * 32, wn = (wave & 3) * 16;
   const int li = lane & 15, lk = lane >> 4;
-  sq4 acc[2][2];
+  sq4 acc[2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = sq4{0, 0, 0, 0};
+  for (int i = 0; i < 2; ++i) acc[i] = sq4{0, 0, 0, 0};
   if (flag >= 0.0) {
     // staging: a wavefront loads 64 consecutive k of one row per instruction (512
-    // contiguous bytes); thread t takes k = t % 64 of rows t / 64 + 4 j, j < 16, clamped
+    // contiguous bytes); thread t takes k = t % 64 of rows t / 64 + 8 j, j < 8, clamped
     // to element 0 outside X and masked
     const int lq = tid & 63, lr = tid >> 6;
-    double xa[16], xb[16];
+    double xa[8], xb[8];
     auto load = [&](int k0) {
       const int k = k0 + lq;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int ra = m0 + lr + 4 * j, rb = n0 + lr + 4 * j;
+      for (int j = 0; j < 8; ++j) {
+        const int ra = m0 + lr + 8 * j, rb = n0 + lr + 8 * j;
         const bool oka = ra < n && k < n, okb = rb < n && k < n;
         const double va = X[oka ? (int64_t)ra * n + k : 0];
         const double vb = X[okb ? (int64_t)rb * n + k : 0];
@@ -274,23 +274,18 @@ __global__ __launch_bounds__(256) void k_sp2_step(int n, int ntiles, const doubl
     for (int k0 = 0; k0 < n; k0 += kSqK) {
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        sA[lr + 4 * j][lq] = xa[j];
-        sB[lr + 4 * j][lq] = xb[j];
+      for (int j = 0; j < 8; ++j) {
+        sA[lr + 8 * j][lq] = xa[j];
+        sB[lr + 8 * j][lq] = xb[j];
       }
       __syncthreads();
       if (k0 + kSqK < n) load(k0 + kSqK);
 #pragma unroll
       for (int kk = 0; kk < kSqK; kk += 4) {
-        double a[2], bb[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = sA[wm + 16 * i + li][kk + lk];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) bb[j] = sB[wn + 16 * j + li][kk + lk];
+        const double bb = sB[wn + li][kk + lk];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[wm + 16 * i + li][kk + lk], bb, acc[i], 0, 0, 0);
       }
     }
   }
@@ -301,32 +296,30 @@ __global__ __launch_bounds__(256) void k_sp2_step(int n, int ntiles, const doubl
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int lm = wm + 16 * i + lk + 4 * r, ln = wn + 16 * j + li;
-        const int gm = m0 + lm, gn = n0 + ln;
-        double x = 0.0;
-        if (gm < n && gn < n) {
-          const double x0 = X[(int64_t)gm * n + gn];
-          const double sq = acc[i][j][r];
-          x = flag < 0.0 ? x0 : flag > 0.5 ? sq : 2.0 * x0 - sq;
-          Xn[(int64_t)gm * n + gn] = x;
-          if (gm == gn) ta += x;
-          tb += wsq * x * x;
-        }
-        sT[lm][ln] = x;
+    for (int r = 0; r < 4; ++r) {
+      const int lm = wm + 16 * i + lk + 4 * r, ln = wn + li;
+      const int gm = m0 + lm, gn = n0 + ln;
+      double x = 0.0;
+      if (gm < n && gn < n) {
+        const double x0 = X[(int64_t)gm * n + gn];
+        const double sq = acc[i][r];
+        x = flag < 0.0 ? x0 : flag > 0.5 ? sq : 2.0 * x0 - sq;
+        Xn[(int64_t)gm * n + gn] = x;
+        if (gm == gn) ta += x;
+        tb += wsq * x * x;
       }
+      sT[lm][ln] = x;
+    }
   if (ti != tj) {
     __syncthreads();
     // mirror: Xn[n0 + c][m0 + r] = tile[r][c], 64 consecutive columns per wavefront row
-    for (int e = tid; e < kSqT * kSqT; e += 256) {
+    for (int e = tid; e < kSqT * kSqT; e += kSqN) {
       const int c = e >> 6, r = e & 63;
       const int gm = n0 + c, gn = m0 + r;
       if (gm < n && gn < n) Xn[(int64_t)gm * n + gn] = sT[r][c];
     }
   }
-  sp2_partials(ta, tb, st);
+  sp2_partials<kSqN>(ta, tb, st);
 }
 
 // d[i] = R_ii
@@ -621,7 +614,7 @@ int sp2_step_tiles(int n) {
 hipError_t launch_sp2_step(int n, const double* X, double* Xn, int nocc, double* st, hipStream_t s) {
   const int ntiles = sp2_step_tiles(n);
   if (ntiles > kSp2Groups) return hipErrorInvalidValue;
-  const hipError_t e = launch_k(k_sp2_step, dim3(kSp2Groups), dim3(256), s, n, ntiles, X, Xn, st);
+  const hipError_t e = launch_k(k_sp2_step, dim3(kSp2Groups), dim3(kSqN), s, n, ntiles, X, Xn, st);
   if (e != hipSuccess) return e;
   return launch_k(k_sp2_decide, dim3(1), dim3(kSp2Groups), s, nocc, st);
 }
