@@ -461,12 +461,14 @@ __global__ __launch_bounds__(1024) void k_lz_update(int n, int nparts, int k, do
   double* w = lz + 2 * (int64_t)n;
   const double* part = lz + 3 * (int64_t)n;
   double* ab = lz + 3 * (int64_t)n + kLzParts;
-  if (threadIdx.x == 0) {
-    double a = 0.0;
-    for (int i = 0; i < nparts; ++i) a += part[i];
-    s_alpha = a;
+  {
+    // alpha from the matrix-vector kernel's partials: strided per thread, then the tree
+    double pa = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) pa += part[i];
+    const double a = block_sum_1024(pa, s_red);
+    if (threadIdx.x == 0) s_alpha = a;
+    __syncthreads();
   }
-  __syncthreads();
   const double alpha = s_alpha, bprev = k > 0 ? ab[2 * (k - 1) + 1] : 0.0;
   double q = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -523,11 +525,19 @@ __device__ double lz_residual(const double* ab, int m, double theta) {
   return fabs(ab[2 * (m - 1) + 1] * last) / sqrt(nrm);
 }
 
-// One thread: the extreme Ritz values by bisection, their residuals, and the
-// bounds in st (intersected with the Gershgorin bounds already there).
-__global__ void k_lz_bounds(int n, const double* __restrict__ lz, double* __restrict__ st) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double* ab = lz + 3 * (int64_t)n + kLzParts;
+// One wavefront: the extreme Ritz values by multisection (lanes 0-31 close in on
+// the lowest, lanes 32-63 on the highest: each round tests 32 points of the
+// current interval by Sturm counts and keeps the sub-interval where the count
+// steps), their residuals, and the bounds in st (intersected with the
+// Gershgorin bounds already there).
+__global__ __launch_bounds__(64) void k_lz_bounds(int n, const double* __restrict__ lz, double* __restrict__ st) {
+  __shared__ double s_ab[2 * kLzSteps];
+  if (blockIdx.x != 0) return;
+  const int lane = threadIdx.x;
+  const double* ab_g = lz + 3 * (int64_t)n + kLzParts;
+  for (int i = lane; i < 2 * kLzSteps; i += 64) s_ab[i] = ab_g[i];
+  __syncthreads();
+  const double* ab = s_ab;
   int m = 0;
   while (m < kLzSteps) {
     ++m;
@@ -539,19 +549,29 @@ __global__ void k_lz_bounds(int n, const double* __restrict__ lz, double* __rest
     lo = fmin(lo, ab[2 * i] - r);
     hi = fmax(hi, ab[2 * i] + r);
   }
-  // smallest and largest eigenvalue of T by bisection
+  // half 0 looks for the point where the count of eigenvalues below reaches 1,
+  // half 1 where it reaches m
+  const int half = lane >> 5, j = lane & 31;
+  const int target = half == 0 ? 1 : m;
   double a = lo, b = hi;
-  for (int it = 0; it < 200 && b - a > 1e-13 * fmax(1.0, fabs(a) + fabs(b)); ++it) {
-    const double mid = 0.5 * (a + b);
-    (lz_count_below(ab, m, mid) >= 1 ? b : a) = mid;
+  for (int round = 0; round < 16; ++round) {
+    if (!(b - a > 1e-13 * fmax(1.0, fabs(a) + fabs(b)))) break;   // uniform within the half
+    const double x = a + (b - a) * (double)(j + 1) / 33.0;
+    const bool reached = lz_count_below(ab, m, x) >= target;
+    const unsigned long long bal = __ballot(reached);
+    const unsigned mask = (unsigned)(half == 0 ? bal : bal >> 32);
+    // the count is monotone in x: the first point that reaches the target closes the interval
+    const int first = mask ? __ffs(mask) - 1 : 32;
+    const double na = first == 0 ? a : a + (b - a) * (double)first / 33.0;
+    const double nb = first == 32 ? b : a + (b - a) * (double)(first + 1) / 33.0;
+    a = na;
+    b = nb;
   }
-  const double tmin = 0.5 * (a + b);
-  a = lo; b = hi;
-  for (int it = 0; it < 200 && b - a > 1e-13 * fmax(1.0, fabs(a) + fabs(b)); ++it) {
-    const double mid = 0.5 * (a + b);
-    (lz_count_below(ab, m, mid) >= m ? b : a) = mid;
-  }
-  const double tmax = 0.5 * (a + b);
+  __shared__ double s_t[2];
+  if (j == 0) s_t[half] = 0.5 * (a + b);
+  __syncthreads();
+  if (lane != 0) return;
+  const double tmin = s_t[0], tmax = s_t[1];
   const bool exact = ab[2 * (m - 1) + 1] == 0.0;   // invariant subspace: the Ritz values are eigenvalues
   const double rmin = exact ? 0.0 : lz_residual(ab, m, tmin);
   const double rmax = exact ? 0.0 : lz_residual(ab, m, tmax);
