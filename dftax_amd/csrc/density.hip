@@ -217,21 +217,25 @@ __global__ __launch_bounds__(256) void k_sp2_reduce_update(int n, int nsplit, co
 // its share of tr Xn and sum_ij Xn_ij^2 (off-diagonal tiles count twice) to
 // the partial slot b; workgroups ntiles..kSp2Groups-1 write zero partials.
 // Once frozen, Xn = X. X and Xn alternate between iterations.
+//
+// 16 wavefronts: two groups of 8 (two per SIMD each) take the two halves of K
+// in stages of 64, each wavefront a 32 x 16 sub-tile; the second group's sums
+// join the first's through LDS at the end. Half the stages of one group over
+// the whole K, so half the barriers and global round trips per tile.
 typedef double sq4 __attribute__((ext_vector_type(4)));
 constexpr int kSqT = 64;   // tile edge
-constexpr int kSqK = 64;   // k per LDS stage (a few global round trips per tile: one per stage)
-constexpr int kSqN = 512;  // threads: 8 wavefronts (two per SIMD), each a 32 x 16 sub-tile
+constexpr int kSqK = 64;   // k per LDS stage
+constexpr int kSqN = 1024; // threads
 
 __global__ __launch_bounds__(kSqN) void k_sp2_step(int n, int ntiles, const double* __restrict__ X,
-                                                  double* __restrict__ Xn, double* __restrict__ st) {
-  // operand stages, rows padded to 66 doubles (the MFMA operand reads of a half-wavefront,
-  // 16 rows x 2 k, fall on distinct bank pairs); after the k loop the first one holds the
-  // updated tile for its mirror
+                                                   double* __restrict__ Xn, double* __restrict__ st) {
+  // operand stages per K half, rows padded to 66 doubles (the MFMA operand reads of a
+  // half-wavefront, 16 rows x 2 k, fall on distinct bank pairs); after the k loop the
+  // first one holds the second half's sums, then the updated tile for its mirror
   constexpr int kLd = kSqK + 2;
-  __shared__ double sA[kSqT][kLd];
-  __shared__ double sB[kSqT][kLd];
+  __shared__ double sA[2][kSqT][kLd];
+  __shared__ double sB[2][kSqT][kLd];
   static_assert(kSqK == kSqT, "the mirror tile reuses sA");
-  double (*sT)[kLd] = sA;
   const int b = blockIdx.x;
   if (b >= ntiles) {
     sp2_partials<kSqN>(0.0, 0.0, st);
@@ -247,16 +251,23 @@ __global__ __launch_bounds__(kSqN) void k_sp2_step(int n, int ntiles, const doub
   const int m0 = ti * kSqT, n0 = tj * kSqT;
   const double flag = st[kSpFlag];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 2) * 32, wn = (wave & 3) * 16;
+  const int kh = wave >> 3, wg = wave & 7;                 // K half, wavefront in its group
+  const int wm = (wg >> 2) * 32, wn = (wg & 3) * 16;
   const int li = lane & 15, lk = lane >> 4;
+  double (*hA)[kLd] = sA[kh];
+  double (*hB)[kLd] = sB[kh];
   sq4 acc[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) acc[i] = sq4{0, 0, 0, 0};
   if (flag >= 0.0) {
+    // stages of each half: both groups run the same count (barriers are shared); stages
+    // past n load zeros
+    const int nst = ((n + kSqK - 1) / kSqK + 1) / 2;
+    const int kbeg = kh * nst * kSqK;
     // staging: a wavefront loads 64 consecutive k of one row per instruction (512
-    // contiguous bytes); thread t takes k = t % 64 of rows t / 64 + 8 j, j < 8, clamped
-    // to element 0 outside X and masked
-    const int lq = tid & 63, lr = tid >> 6;
+    // contiguous bytes); thread t of a group takes k = t % 64 of rows t / 64 + 8 j,
+    // j < 8, clamped to element 0 outside X and masked
+    const int gt = tid & 511, lq = gt & 63, lr = gt >> 6;
     double xa[8], xb[8];
     auto load = [&](int k0) {
       const int k = k0 + lq;
@@ -270,46 +281,56 @@ __global__ __launch_bounds__(kSqN) void k_sp2_step(int n, int ntiles, const doub
         xb[j] = okb ? vb : 0.0;
       }
     };
-    load(0);
-    for (int k0 = 0; k0 < n; k0 += kSqK) {
+    load(kbeg);
+    for (int s2 = 0; s2 < nst; ++s2) {
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        sA[lr + 8 * j][lq] = xa[j];
-        sB[lr + 8 * j][lq] = xb[j];
+        hA[lr + 8 * j][lq] = xa[j];
+        hB[lr + 8 * j][lq] = xb[j];
       }
       __syncthreads();
-      if (k0 + kSqK < n) load(k0 + kSqK);
+      if (s2 + 1 < nst) load(kbeg + (s2 + 1) * kSqK);
 #pragma unroll
       for (int kk = 0; kk < kSqK; kk += 4) {
-        const double bb = sB[wn + li][kk + lk];
+        const double bb = hB[wn + li][kk + lk];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-          acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[wm + 16 * i + li][kk + lk], bb, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(hA[wm + 16 * i + li][kk + lk], bb, acc[i], 0, 0, 0);
       }
     }
   }
   // v_mfma_f64_16x16x4f64 result: lane l, register r holds row (l/16) + 4r, column l%16
+  double (*sT)[kLd] = sA[0];
+  __syncthreads();   // every wavefront is done reading the stages
+  if (kh == 1) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sT[wm + 16 * i + lk + 4 * r][wn + li] = acc[i][r];
+  }
+  __syncthreads();
   double ta = 0.0, tb = 0.0;
-  const double wsq = ti == tj ? 1.0 : 2.0;
-  __syncthreads();   // every wavefront is done reading sA before it takes the tile
+  if (kh == 0) {
+    const double wsq = ti == tj ? 1.0 : 2.0;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int lm = wm + 16 * i + lk + 4 * r, ln = wn + li;
-      const int gm = m0 + lm, gn = n0 + ln;
-      double x = 0.0;
-      if (gm < n && gn < n) {
-        const double x0 = X[(int64_t)gm * n + gn];
-        const double sq = acc[i][r];
-        x = flag < 0.0 ? x0 : flag > 0.5 ? sq : 2.0 * x0 - sq;
-        Xn[(int64_t)gm * n + gn] = x;
-        if (gm == gn) ta += x;
-        tb += wsq * x * x;
+      for (int r = 0; r < 4; ++r) {
+        const int lm = wm + 16 * i + lk + 4 * r, ln = wn + li;
+        const int gm = m0 + lm, gn = n0 + ln;
+        double x = 0.0;
+        if (gm < n && gn < n) {
+          const double x0 = X[(int64_t)gm * n + gn];
+          const double sq = acc[i][r] + sT[lm][ln];
+          x = flag < 0.0 ? x0 : flag > 0.5 ? sq : 2.0 * x0 - sq;
+          Xn[(int64_t)gm * n + gn] = x;
+          if (gm == gn) ta += x;
+          tb += wsq * x * x;
+        }
+        sT[lm][ln] = x;   // each element is read above by the same thread only
       }
-      sT[lm][ln] = x;
-    }
+  }
   if (ti != tj) {
     __syncthreads();
     // mirror: Xn[n0 + c][m0 + r] = tile[r][c], 64 consecutive columns per wavefront row
