@@ -541,6 +541,10 @@ struct dfks_ctx {
   // SCF state
   int nmo = 0, m = 8, it = 0;
   double level_shift = 0.0, e_prev = 0.0;
+  // the orthonormal nmo x nocc factor the SCF density was formed from, P = 2 X D D^T X^T
+  // (d_L after a purified step, the occupied eigenvectors in d_Ws after an eigensolve);
+  // null while P is the caller's guess
+  const double* p_fac = nullptr;
   double *d_X = nullptr, *d_P = nullptr, *d_F = nullptr, *d_Fs = nullptr, *d_Es = nullptr, *d_gram = nullptr;
   double *d_coef = nullptr, *d_dots = nullptr, *d_Fx = nullptr, *d_C = nullptr, *d_eps = nullptr, *d_Ws = nullptr;
   // purification: state, Gershgorin bounds, residual diagonal, occupied factor, pivots
@@ -2431,6 +2435,7 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   realloc(h->d_Lw, (size_t)nmo * h->nocc);
   realloc(h->d_Ls, (size_t)h->nocc * h->nocc);
   h->l_valid = false;
+  h->p_fac = nullptr;
   {
     // load rocSOLVER's Cholesky here rather than in a timed step: a 1 x 1 factor of 1
     const double one = 1.0;
@@ -2683,30 +2688,42 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
     if (!h->scf_ev[0]) for (auto& e : h->scf_ev) (void)hipEventCreate(&e);
     (void)hipEventRecord(h->scf_ev[0], st);
   }
-  // err = X^T (F P S - S P F) X ; S P F = (F P S)^T for symmetric F, P, S
-  gemm(h, false, false, nao, nao, nao, 1.0, h->d_F, nao, h->d_P, nao, 0.0, h->d_W1, nao);
-  gemm(h, false, false, nao, nao, nao, 1.0, h->d_W1, nao, h->d_S, nao, 0.0, h->d_W2, nao);
-  HIPCHK(launch_antisym(nao, h->d_W2, h->d_W1, st));
-  gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_W1, nao, 0.0, h->d_W2, nmo);
-  gemm(h, false, false, nmo, nmo, nao, 1.0, h->d_W2, nmo, h->d_X, nao, 0.0, err, nmo);
+  const int nocc = h->nocc;
+  // F' = X^T F X, kept as this step's DIIS history entry: the extrapolation runs in the
+  // orthogonal basis, Fx' = sum_i c_i F'_i = X^T (sum_i c_i F_i) X
+  double* Fo = h->d_Fs + slot * mm;
+  gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_F, nao, 0.0, h->d_W2, nmo);
+  gemm(h, false, false, nmo, nmo, nao, 1.0, h->d_W2, nmo, h->d_X, nao, 0.0, Fo, nmo);
+  if (h->p_fac) {
+    // err = X^T (F P S - S P F) X = 2 (F' D D^T - D D^T F') when P = 2 X D D^T X^T and
+    // X^T S X = I: two thin products instead of four of nao^3
+    const double* D = h->p_fac;
+    gemm(h, false, false, nmo, nocc, nmo, 1.0, Fo, nmo, D, nmo, 0.0, h->d_Lw, nmo);
+    gemm(h, false, true, nmo, nmo, nocc, 2.0, h->d_Lw, nmo, D, nmo, 0.0, err, nmo);
+    gemm(h, false, true, nmo, nmo, nocc, -2.0, D, nmo, h->d_Lw, nmo, 1.0, err, nmo);
+  } else {
+    // err = X^T (F P S - S P F) X ; S P F = (F P S)^T for symmetric F, P, S
+    gemm(h, false, false, nao, nao, nao, 1.0, h->d_F, nao, h->d_P, nao, 0.0, h->d_W1, nao);
+    gemm(h, false, false, nao, nao, nao, 1.0, h->d_W1, nao, h->d_S, nao, 0.0, h->d_W2, nao);
+    HIPCHK(launch_antisym(nao, h->d_W2, h->d_W1, st));
+    gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_W1, nao, 0.0, h->d_W2, nmo);
+    gemm(h, false, false, nmo, nmo, nao, 1.0, h->d_W2, nmo, h->d_X, nao, 0.0, err, nmo);
+  }
   // DIIS history
-  HIPCHK(hipMemcpyAsync(h->d_Fs + slot * nn, h->d_F, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
   for (int i = 0; i < count; ++i)
     HIPCHK(launch_dot(mm, err, h->d_Es + i * mm, h->d_part, 1.0, h->d_dots + i, 0, st));
   HIPCHK(launch_gram_update(m, slot, count, h->d_dots, h->d_gram, st));
   HIPCHK(launch_diis_solve(m, count, h->d_gram, h->d_coef, st));
-  HIPCHK(launch_diis_extrap(m, nn, h->d_coef, h->d_Fs, h->d_Fx, st));
+  HIPCHK(launch_diis_extrap(m, mm, h->d_coef, h->d_Fs, h->d_Ws, st));
   if (h->level_shift != 0.0) {
-    // F += b (S - 1/2 S P S)
+    // F' += b X^T (S - 1/2 S P S) X (the AO term in d_Fx)
     gemm(h, false, false, nao, nao, nao, 1.0, h->d_S, nao, h->d_P, nao, 0.0, h->d_W1, nao);
-    gemm(h, false, false, nao, nao, nao, -0.5 * h->level_shift, h->d_W1, nao, h->d_S, nao, 1.0, h->d_Fx, nao);
-    HIPCHK(launch_axpy(nn, h->level_shift, h->d_S, h->d_Fx, 1, st));
+    HIPCHK(hipMemcpyAsync(h->d_Fx, h->d_S, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+    gemm(h, false, false, nao, nao, nao, -0.5, h->d_W1, nao, h->d_S, nao, 1.0, h->d_Fx, nao);
+    gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_Fx, nao, 0.0, h->d_W2, nmo);
+    gemm(h, false, false, nmo, nmo, nao, h->level_shift, h->d_W2, nmo, h->d_X, nao, 1.0, h->d_Ws, nmo);
   }
   if (h->timing) (void)hipEventRecord(h->scf_ev[1], st);
-  // F' = X^T F X; occupied factor Cf (P = Cf Cf^T = 2 C_occ C_occ^T)
-  gemm(h, true, false, nmo, nao, nao, 1.0, h->d_X, nao, h->d_Fx, nao, 0.0, h->d_W2, nmo);
-  gemm(h, false, false, nmo, nmo, nao, 1.0, h->d_W2, nmo, h->d_X, nao, 0.0, h->d_Ws, nmo);
-  const int nocc = h->nocc;
   const bool purified = h->opt.density == 0 && purify_density(h, st);
   if (!purified) {
     // FP64 eigensolve (also in mixed steps: an FP32 solve's eigenvector error,
@@ -2722,6 +2739,7 @@ void scf_step(dfks_ctx* h, double* out3, hipStream_t st) {
     h->c_stale = true;
   }
   gemm(h, false, true, nao, nao, nocc, 1.0, h->d_Cf, nao, h->d_Cf, nao, 0.0, h->d_P, nao);
+  h->p_fac = purified ? h->d_L : h->d_Ws;
   if (h->timing) (void)hipEventRecord(h->scf_ev[2], st);
   std::vector<double> sg(nocc, 1.0);
   energy_fock_factored(h, h->d_P, h->d_Cf, sg, nocc, h->d_Edev, h->d_F, st);

@@ -345,45 +345,99 @@ hipError_t launch_antisym(int n, const double* A, double* E, hipStream_t st) {
 // Builds A, solves A c = (0,..,0,-1) by Gaussian elimination with partial pivoting
 // on one lane and writes c[0..m). Mirrors dftax/ks/scf.py:252-276 (identity on
 // empty slots, 1e-10 ridge on filled ones).
-__global__ void k_diis_solve(int m, int count, const double* __restrict__ gram, double* __restrict__ coef) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double A[17][18];
+// DIIS coefficients from the bordered system [[G + 1e-10 I, -1], [-1^T, 0]] c = [0, -1]
+// (unused slots become identity rows with c = 0), by Gaussian elimination with
+// partial pivoting. One wavefront: lane i holds row i (m + 1 <= 17 rows, 18
+// columns with the right-hand side) in registers; pivot search, row swap and the
+// pivot row go through shuffles. The pivot (first largest |a_ik|), the updates
+// and the back substitution run in the same order as a serial elimination.
+__global__ __launch_bounds__(64) void k_diis_solve(int m, int count, const double* __restrict__ gram,
+                                                   double* __restrict__ coef) {
+  constexpr int kR = 17, kC = 18;
+  const int lane = threadIdx.x;
   const int n = m + 1;
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j <= n; ++j) A[i][j] = 0.0;
-  for (int i = 0; i < m; ++i)
-    for (int j = 0; j < m; ++j) {
-      bool vi = i < count, vj = j < count;
-      A[i][j] = (vi && vj) ? gram[i * m + j] : (i == j ? 1.0 : 0.0);
-      if (i == j && vi) A[i][j] += 1e-10;
+  double r[kC];
+#pragma unroll
+  for (int j = 0; j < kC; ++j) {
+    double v = 0.0;
+    if (lane < m) {
+      if (j < m) {
+        const bool vi = lane < count, vj = j < count;
+        v = (vi && vj) ? gram[lane * m + j] : (lane == j ? 1.0 : 0.0);
+        if (lane == j && vi) v += 1e-10;
+      } else if (j == m) {
+        v = lane < count ? -1.0 : 0.0;
+      }
+    } else if (lane == m) {
+      if (j < m) v = j < count ? -1.0 : 0.0;
+      else if (j == n) v = -1.0;   // right-hand side
     }
-  for (int i = 0; i < m; ++i) {
-    double b = i < count ? -1.0 : 0.0;
-    A[i][m] = b;
-    A[m][i] = b;
+    r[j] = v;
   }
-  A[m][n] = -1.0;  // rhs column
+  // k stays a runtime value: every register array is indexed by unrolled j only
   for (int k = 0; k < n; ++k) {
-    int piv = k;
-    double best = fabs(A[k][k]);
-    for (int i = k + 1; i < n; ++i)
-      if (fabs(A[i][k]) > best) { best = fabs(A[i][k]); piv = i; }
-    if (piv != k)
-      for (int j = 0; j <= n; ++j) { double t = A[k][j]; A[k][j] = A[piv][j]; A[piv][j] = t; }
-    double d = A[k][k];
-    for (int i = k + 1; i < n; ++i) {
-      double f = A[i][k] / d;
-      if (f == 0.0) continue;
-      for (int j = k; j <= n; ++j) A[i][j] -= f * A[k][j];
+    double rk = 0.0;
+#pragma unroll
+    for (int j = 0; j < kC; ++j)
+      if (j == k) rk = r[j];
+    double best = (lane >= k && lane < n) ? fabs(rk) : -1.0;
+    int bi = lane;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    const int piv = bi;
+    if (piv != k) {
+#pragma unroll
+      for (int j = 0; j < kC; ++j) {
+        const double vk = __shfl(r[j], k), vp = __shfl(r[j], piv);
+        if (lane == k) r[j] = vp;
+        else if (lane == piv) r[j] = vk;
+      }
+    }
+    double p[kC], pk = 0.0;
+    rk = 0.0;
+#pragma unroll
+    for (int j = 0; j < kC; ++j) {
+      p[j] = __shfl(r[j], k);
+      if (j == k) {
+        pk = p[j];
+        rk = r[j];
+      }
+    }
+    if (lane > k && lane < n) {
+      const double f = rk / pk;
+      if (f != 0.0) {
+#pragma unroll
+        for (int j = 0; j < kC; ++j)
+          if (j >= k) r[j] -= f * p[j];
+      }
     }
   }
-  double x[17];
-  for (int i = n - 1; i >= 0; --i) {
-    double s = A[i][n];
-    for (int j = i + 1; j < n; ++j) s -= A[i][j] * x[j];
-    x[i] = s / A[i][i];
+  double rhs = 0.0;
+#pragma unroll
+  for (int j = 0; j < kC; ++j)
+    if (j == n) rhs = r[j];
+  double x[kR];
+#pragma unroll
+  for (int i = kR - 1; i >= 0; --i) {
+    x[i] = 0.0;
+    if (i >= n) continue;
+    double sum = rhs;
+#pragma unroll
+    for (int j = i + 1; j < kR; ++j)
+      if (j < n) sum -= r[j] * x[j];
+    x[i] = __shfl(sum / r[i], i);
   }
-  for (int i = 0; i < m; ++i) coef[i] = x[i];
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < kR; ++i)
+      if (i < m) coef[i] = x[i];
+  }
 }
 
 hipError_t launch_diis_solve(int m, int count, const double* gram, double* coef, hipStream_t st) {
