@@ -83,6 +83,7 @@ hipError_t launch_factor_cols(int n, int nf, const double* V, const double* lam,
                               double* sgn, hipStream_t st);
 hipError_t launch_axpy(int64_t n, double a, const double* x, double* y, int acc, hipStream_t st);
 hipError_t launch_antisym(int n, const double* A, double* E, hipStream_t st);
+hipError_t launch_eye_dev(int n, const double* G, double* out, hipStream_t st);
 hipError_t launch_diis_solve(int m, int count, const double* gram, double* coef, hipStream_t st);
 hipError_t launch_diis_extrap(int m, int64_t n, const double* coef, const double* Fs, double* out, hipStream_t st);
 hipError_t launch_gram_update(int m, int slot, int count, const double* dots, double* gram, hipStream_t st);
@@ -551,7 +552,7 @@ struct dfks_ctx {
   double *d_sp2 = nullptr, *d_lohi = nullptr, *d_dvec = nullptr, *d_L = nullptr;
   // the occupied factor's projection from the previous step: W = X L_prev (nmo x nocc),
   // S = W^T W (nocc x nocc); l_valid while d_L holds the previous step's factor
-  double *d_Lw = nullptr, *d_Ls = nullptr;
+  double *d_Lw = nullptr, *d_Ls = nullptr, *d_Lq = nullptr;
   bool l_valid = false;
   double* d_lz = nullptr;   // Lanczos vectors and coefficients of the SP2 spectral bounds
   hipGraphExec_t lz_exec = nullptr;  // the Lanczos bounds as one graph (per nmo)
@@ -628,7 +629,7 @@ struct dfks_ctx {
     void* ptrs[] = {d_boys, d_atm, d_S, d_T, d_V, d_H, d_int2c, d_int3c, d_B, d_gxyz, d_gw, d_ao, d_zw, d_exc,
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
                     d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws,
-                    d_sp2, d_lz, d_lohi, d_dvec, d_L, d_Lw, d_Ls, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
+                    d_sp2, d_lz, d_lohi, d_dvec, d_L, d_Lw, d_Ls, d_Lq, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
                     d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf, d_xc_pts, d_xc_w, d_xc_exc, d_xc_ao,
                     d_xc_zw, d_xc_q, d_xc_ao32, d_xc_cols, d_xc_vlen, d_xc_shl, d_xc_shoff, d_xc_vrow,
                     d_xc_vprow, d_xc_vpcol, d_xc_zw2, d_xc_q2, d_Vxc2, d_uP, d_uJ, d_uK, d_ug, d_xcf_tab,
@@ -2433,6 +2434,7 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   realloc(h->d_dvec, nmo);
   realloc(h->d_L, (size_t)nmo * h->nocc);
   realloc(h->d_Lw, (size_t)nmo * h->nocc);
+  realloc(h->d_Lq, (size_t)nmo * h->nocc);
   realloc(h->d_Ls, (size_t)h->nocc * h->nocc);
   h->l_valid = false;
   h->p_fac = nullptr;
@@ -2597,7 +2599,7 @@ void sp2_bounds(dfks_ctx* h, hipStream_t st) {
 // conditioned to ~1e6 or better), the pivoted Cholesky runs otherwise. Two
 // GEMMs of nmo^2 nocc and nmo nocc^2 and a Cholesky of nocc^2 instead of the
 // pivoted Cholesky's nocc sequential pivots.
-bool project_occupied_factor(dfks_ctx* h, const double* X, hipStream_t st) {
+bool project_occupied_factor_chol(dfks_ctx* h, const double* X, hipStream_t st) {
   const int n = h->nmo, nocc = h->nocc;
   gemm(h, false, false, n, nocc, n, 1.0, X, n, h->d_L, n, 0.0, h->d_Lw, n);
   gemm(h, true, false, nocc, nocc, n, 1.0, h->d_Lw, n, h->d_Lw, n, 0.0, h->d_Ls, nocc);
@@ -2619,6 +2621,49 @@ bool project_occupied_factor(dfks_ctx* h, const double* X, hipStream_t st) {
     for (int i = 0; i < nocc; ++i) dev = std::max(dev, std::fabs(G[(size_t)j * nocc + i] - (i == j ? 1.0 : 0.0)));
   if (!(dev <= 1e-10)) return false;
   HIPCHK(hipMemcpyAsync(h->d_L, h->d_Lw, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice, st));
+  // the step's factor check reads the residual trace and smallest pivot of the pivoted
+  // Cholesky: this factor reproduces X as a converged pivoted one would
+  const double chk[2] = {0.0, 1.0};
+  HIPCHK(hipMemcpyAsync(h->d_sp2 + kSpResid, chk, sizeof(chk), hipMemcpyHostToDevice, st));
+  sync(st);
+  return true;
+}
+
+// The same factor as the orthogonal polar factor of W = X L_p, L = W (W^T W)^{-1/2},
+// by the Newton-Schulz iteration W <- W (3 I - W^T W) / 2, two thin products per
+// step. With the singular values of W at sqrt(1 - e), one step takes e to
+// 3/4 e^2 (1 + e/3), so the step count comes from ||W^T W - I||_F, an upper bound
+// on every |e| (one read): two steps when the occupied space barely turned, at most
+// six, and none of it when the bound is 0.5 or more; the Gram matrix of the last
+// iterate is the orthonormality check (max |L^T L - I| <= 1e-10). L = W Z with Z a
+// polynomial in W^T W, so L L^T is the projector onto range(W) = range(X). No
+// Cholesky or triangular solve (rocSOLVER's dpotrf and rocBLAS's dtrsm at nocc = 160
+// took ~0.5 ms together). Returns false, with L_p untouched, when the occupied
+// space turned too far or the check fails.
+bool project_occupied_factor(dfks_ctx* h, const double* X, hipStream_t st) {
+  const int n = h->nmo, nocc = h->nocc;
+  double* W = h->d_Lw;
+  double* Wn = h->d_Lq;
+  gemm(h, false, false, n, nocc, n, 1.0, X, n, h->d_L, n, 0.0, W, n);
+  gemm(h, true, false, nocc, nocc, n, 1.0, W, n, W, n, 0.0, h->d_Ls, nocc);
+  double dev[2] = {0.0, 0.0};
+  HIPCHK(launch_eye_dev(nocc, h->d_Ls, h->d_part, st));
+  HIPCHK(hipMemcpyAsync(dev, h->d_part, sizeof(dev), hipMemcpyDeviceToHost, st));
+  sync(st);
+  if (!(dev[1] < 0.5)) return project_occupied_factor_chol(h, X, st);
+  int steps = 0;
+  for (double e = dev[1]; e > 1e-14 && steps < 6; e = 0.75 * e * e * (1.0 + e)) ++steps;
+  for (int k = 0; k < steps; ++k) {
+    HIPCHK(hipMemcpyAsync(Wn, W, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice, st));
+    gemm(h, false, false, n, nocc, nocc, -0.5, W, n, h->d_Ls, nocc, 1.5, Wn, n);
+    std::swap(W, Wn);
+    gemm(h, true, false, nocc, nocc, n, 1.0, W, n, W, n, 0.0, h->d_Ls, nocc);
+  }
+  HIPCHK(launch_eye_dev(nocc, h->d_Ls, h->d_part, st));
+  HIPCHK(hipMemcpyAsync(dev, h->d_part, sizeof(dev), hipMemcpyDeviceToHost, st));
+  sync(st);
+  if (!(dev[0] <= 1e-10)) return project_occupied_factor_chol(h, X, st);
+  HIPCHK(hipMemcpyAsync(h->d_L, W, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice, st));
   // the step's factor check reads the residual trace and smallest pivot of the pivoted
   // Cholesky: this factor reproduces X as a converged pivoted one would
   const double chk[2] = {0.0, 1.0};

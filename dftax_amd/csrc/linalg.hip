@@ -335,6 +335,38 @@ __global__ void k_antisym(int n, const double* __restrict__ A, double* __restric
   E[idx] = A[idx] - A[(int64_t)c * n + r];
 }
 
+// out[0] = max_ij |G_ij - delta_ij| and out[1] = ||G - I||_F of the n x n matrix G
+// (one workgroup, fixed-order tree; NaN counts as infinite).
+__global__ __launch_bounds__(256) void k_eye_dev(int n, const double* __restrict__ G, double* __restrict__ out) {
+  __shared__ double s[256], q[256];
+  double m = 0.0, f = 0.0;
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = threadIdx.x; idx < nn; idx += 256) {
+    const double d = G[idx] - (idx / n == idx % n ? 1.0 : 0.0);
+    m = d == d ? fmax(m, fabs(d)) : HUGE_VAL;
+    f += d * d;
+  }
+  s[threadIdx.x] = m;
+  q[threadIdx.x] = f;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+      q[threadIdx.x] += q[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s[0];
+    out[1] = q[0] == q[0] ? sqrt(q[0]) : HUGE_VAL;
+  }
+}
+
+hipError_t launch_eye_dev(int n, const double* G, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_eye_dev, dim3(1), dim3(256), 0, st, n, G, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_antisym(int n, const double* A, double* E, hipStream_t st) {
   int64_t tot = (int64_t)n * n;
   hipLaunchKernelGGL(k_antisym, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, n, A, E);
