@@ -84,6 +84,7 @@ hipError_t launch_factor_cols(int n, int nf, const double* V, const double* lam,
 hipError_t launch_axpy(int64_t n, double a, const double* x, double* y, int acc, hipStream_t st);
 hipError_t launch_antisym(int n, const double* A, double* E, hipStream_t st);
 hipError_t launch_eye_dev(int n, const double* G, double* out, hipStream_t st);
+hipError_t launch_store2(double* dst, double a, double b, hipStream_t st);
 hipError_t launch_diis_solve(int m, int count, const double* gram, double* coef, hipStream_t st);
 hipError_t launch_diis_extrap(int m, int64_t n, const double* coef, const double* Fs, double* out, hipStream_t st);
 hipError_t launch_gram_update(int m, int slot, int count, const double* dots, double* gram, hipStream_t st);
@@ -2632,43 +2633,49 @@ bool project_occupied_factor_chol(dfks_ctx* h, const double* X, hipStream_t st) 
 // The same factor as the orthogonal polar factor of W = X L_p, L = W (W^T W)^{-1/2},
 // by the Newton-Schulz iteration W <- W (3 I - W^T W) / 2, two thin products per
 // step. With the singular values of W at sqrt(1 - e), one step takes e to
-// 3/4 e^2 (1 + e/3), so the step count comes from ||W^T W - I||_F, an upper bound
-// on every |e| (one read): two steps when the occupied space barely turned, at most
-// six, and none of it when the bound is 0.5 or more; the Gram matrix of the last
-// iterate is the orthonormality check (max |L^T L - I| <= 1e-10). L = W Z with Z a
-// polynomial in W^T W, so L L^T is the projector onto range(W) = range(X). No
-// Cholesky or triangular solve (rocSOLVER's dpotrf and rocBLAS's dtrsm at nocc = 160
-// took ~0.5 ms together). Returns false, with L_p untouched, when the occupied
-// space turned too far or the check fails.
+// 3/4 e^2 (1 + e/3): two steps cover e up to ~3e-4 (the occupied space of a settling
+// SCF turns less), then one read of the last Gram matrix gives the check
+// (max |L^T L - I| <= 1e-10) and ||W^T W - I||_F, an upper bound on every |e|, from
+// which further steps are counted (at most six in all; none when the bound is 0.5
+// or more). L = W Z with Z a polynomial in W^T W, so L L^T is the projector onto
+// range(W) = range(X). No Cholesky or triangular solve (rocSOLVER's dpotrf and
+// rocBLAS's dtrsm at nocc = 160 took ~0.5 ms together). Returns false, with L_p
+// untouched, when the occupied space turned too far or the check fails.
 bool project_occupied_factor(dfks_ctx* h, const double* X, hipStream_t st) {
   const int n = h->nmo, nocc = h->nocc;
   double* W = h->d_Lw;
   double* Wn = h->d_Lq;
   gemm(h, false, false, n, nocc, n, 1.0, X, n, h->d_L, n, 0.0, W, n);
   gemm(h, true, false, nocc, nocc, n, 1.0, W, n, W, n, 0.0, h->d_Ls, nocc);
+  auto polar_steps = [&](int k) {
+    for (int i = 0; i < k; ++i) {
+      HIPCHK(hipMemcpyAsync(Wn, W, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice, st));
+      gemm(h, false, false, n, nocc, nocc, -0.5, W, n, h->d_Ls, nocc, 1.5, Wn, n);
+      std::swap(W, Wn);
+      gemm(h, true, false, nocc, nocc, n, 1.0, W, n, W, n, 0.0, h->d_Ls, nocc);
+    }
+  };
+  auto read_dev = [&](double* dev) {
+    HIPCHK(launch_eye_dev(nocc, h->d_Ls, h->d_part, st));
+    HIPCHK(hipMemcpyAsync(dev, h->d_part, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync(st);
+  };
+  int done = 2;
+  polar_steps(done);
   double dev[2] = {0.0, 0.0};
-  HIPCHK(launch_eye_dev(nocc, h->d_Ls, h->d_part, st));
-  HIPCHK(hipMemcpyAsync(dev, h->d_part, sizeof(dev), hipMemcpyDeviceToHost, st));
-  sync(st);
-  if (!(dev[1] < 0.5)) return project_occupied_factor_chol(h, X, st);
-  int steps = 0;
-  for (double e = dev[1]; e > 1e-14 && steps < 6; e = 0.75 * e * e * (1.0 + e)) ++steps;
-  for (int k = 0; k < steps; ++k) {
-    HIPCHK(hipMemcpyAsync(Wn, W, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice, st));
-    gemm(h, false, false, n, nocc, nocc, -0.5, W, n, h->d_Ls, nocc, 1.5, Wn, n);
-    std::swap(W, Wn);
-    gemm(h, true, false, nocc, nocc, n, 1.0, W, n, W, n, 0.0, h->d_Ls, nocc);
+  read_dev(dev);
+  if (!(dev[0] <= 1e-10)) {
+    int more = 0;
+    for (double e = dev[1]; e > 1e-14 && done + more < 6; e = 0.75 * e * e * (1.0 + e)) ++more;
+    if (!(dev[1] < 0.5) || more == 0) return project_occupied_factor_chol(h, X, st);
+    polar_steps(more);
+    read_dev(dev);
+    if (!(dev[0] <= 1e-10)) return project_occupied_factor_chol(h, X, st);
   }
-  HIPCHK(launch_eye_dev(nocc, h->d_Ls, h->d_part, st));
-  HIPCHK(hipMemcpyAsync(dev, h->d_part, sizeof(dev), hipMemcpyDeviceToHost, st));
-  sync(st);
-  if (!(dev[0] <= 1e-10)) return project_occupied_factor_chol(h, X, st);
   HIPCHK(hipMemcpyAsync(h->d_L, W, sizeof(double) * n * nocc, hipMemcpyDeviceToDevice, st));
   // the step's factor check reads the residual trace and smallest pivot of the pivoted
   // Cholesky: this factor reproduces X as a converged pivoted one would
-  const double chk[2] = {0.0, 1.0};
-  HIPCHK(hipMemcpyAsync(h->d_sp2 + kSpResid, chk, sizeof(chk), hipMemcpyHostToDevice, st));
-  sync(st);
+  HIPCHK(launch_store2(h->d_sp2 + kSpResid, 0.0, 1.0, st));
   return true;
 }
 

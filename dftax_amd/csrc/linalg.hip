@@ -362,6 +362,19 @@ __global__ __launch_bounds__(256) void k_eye_dev(int n, const double* __restrict
   }
 }
 
+// dst[0] = a, dst[1] = b (a stream-ordered store of two values, no host copy)
+__global__ void k_store2(double* __restrict__ dst, double a, double b) {
+  if (threadIdx.x == 0) {
+    dst[0] = a;
+    dst[1] = b;
+  }
+}
+
+hipError_t launch_store2(double* dst, double a, double b, hipStream_t st) {
+  hipLaunchKernelGGL(k_store2, dim3(1), dim3(64), 0, st, dst, a, b);
+  return hipGetLastError();
+}
+
 hipError_t launch_eye_dev(int n, const double* G, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_eye_dev, dim3(1), dim3(256), 0, st, n, G, out);
   return hipGetLastError();
