@@ -224,7 +224,8 @@ def main() -> int:
         flops = {"exchange_syrk": flops["exchange_syrk"]}
 
     def stage_table(avg, elem_bytes, peak_tf):
-        bytes_ = {} if recompute else {"coulomb_j": elem_bytes * nao * nao * nfit}
+        # J = B gamma reads the rows m >= l of B (symmetric in its two orbital indices)
+        bytes_ = {} if recompute else {"coulomb_j": elem_bytes * nao * (nao + 1) / 2 * nfit}
         if not recompute and not compact:
             bytes_["xc_kernel"] = elem_bytes * ng * nao * 6
         tab = {}

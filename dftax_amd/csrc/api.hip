@@ -74,7 +74,7 @@ hipError_t launch_dot(int64_t n, const double* x, const double* y, double* part,
                       int accumulate, hipStream_t st);
 hipError_t launch_gamma(int nao, int nq, int nf, const double* T, const double* Cf, const double* sgn, double* gamma,
                         double* part, int accumulate, hipStream_t st);
-hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, double* J, hipStream_t st);
+hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, double* J, int sym_n, hipStream_t st);
 hipError_t launch_fock(int n, const double* H, const double* J, const double* Ku, double cK, const double* Vu,
                        double* F, hipStream_t st);
 hipError_t launch_symfill(int n, double* A, hipStream_t st);
@@ -97,7 +97,8 @@ hipError_t launch_xc_f32(int64_t ng, int64_t nao, const float* ao, const double*
                          double thresh, double* exc, hipStream_t st);
 hipError_t launch_gamma_f32(int nao, int nq, int nf, const float* T, const double* Cf, const double* sgn,
                             double* gamma, double* part, int accumulate, hipStream_t st);
-hipError_t launch_jvec_f32(int64_t nrows, int nq, const float* B, const double* g, double* J, hipStream_t st);
+hipError_t launch_jvec_f32(int64_t nrows, int nq, const float* B, const double* g, double* J, int sym_n,
+                           hipStream_t st);
 hipError_t launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st);
 hipError_t launch_to_f64(int64_t n, const float* x, double* y, hipStream_t st);
 hipError_t launch_eri_grad(const int2* pairs, int npair, int nket, const ShellSet& A, const ShellSet& C,
@@ -1421,7 +1422,7 @@ void coulomb_recompute(dfks_ctx* h, hipStream_t st) {
   for (size_t s = 0; s < h->slab_np.size(); ++s) {
     const int np = h->slab_np[s];
     eri_slab(h, (int)s, st);
-    if (s == 0) HIPCHK(launch_jvec((int64_t)nao * nao, np, h->d_slab, h->d_fitd + h->slab_p0[s], h->d_J, st));
+    if (s == 0) HIPCHK(launch_jvec((int64_t)nao * nao, np, h->d_slab, h->d_fitd + h->slab_p0[s], h->d_J, 0, st));
     else HIPCHK(launch_jvec_acc((int64_t)nao * nao, np, h->d_slab, h->d_fitd + h->slab_p0[s], h->d_J, st));
   }
 }
@@ -1655,8 +1656,10 @@ void jk_factored(dfks_ctx* h, const double* Cf, const std::vector<double>& sgn, 
   }
   // J = B gamma
   if (h->recompute) coulomb_recompute(h, st);
-  else if (mixed) HIPCHK(launch_jvec_f32(nn, nfit, h->d_B32, h->d_gamma, h->d_J, st));
-  else HIPCHK(launch_jvec(nn, nfit, h->d_B, h->d_gamma, h->d_J, st));
+  // B = (mn|P) L is symmetric in (m, n) (the three-centre kernels write both orders):
+  // the upper rows only, J mirrored
+  else if (mixed) HIPCHK(launch_jvec_f32(nn, nfit, h->d_B32, h->d_gamma, h->d_J, nao, st));
+  else HIPCHK(launch_jvec(nn, nfit, h->d_B, h->d_gamma, h->d_J, nao, st));
   h->mark(kStCoulomb + 1, st);
 }
 

@@ -752,6 +752,46 @@ __global__ void k_gemm_reduce(int M, int N, int nsplit, const W* __restrict__ ws
   out[(int64_t)m * ldo + n] = (Out)v;
 }
 
+// Symmetric mode of the reduce for N x N: one workgroup per upper 64 x 64 tile
+// (ti <= tj) sums the partials of the tile's upper elements (coalesced, fixed
+// order), writes them, and writes the mirror tile through LDS, so both stores
+// are row-contiguous (the elementwise kernel read the lower half's partials
+// down columns). out = alpha * sum (+ beta * out) for every element, the
+// lower one from its upper partner's sum, as in the elementwise kernel.
+template <typename W, typename Out>
+__global__ __launch_bounds__(256) void k_gemm_reduce_sym(int N, int nsplit, const W* __restrict__ ws,
+                                                         int64_t split_stride, int64_t ldw, Out* __restrict__ out,
+                                                         int64_t ldo, double alpha, double beta) {
+  __shared__ double t[64][65];
+  const int ntn = (N + 63) / 64;
+  int ti = 0, b = (int)blockIdx.x;
+  while (b >= ntn - ti) {
+    b -= ntn - ti;
+    ++ti;
+  }
+  const int tj = ti + b;
+  if (ti >= ntn) return;
+  const int m0 = 64 * ti, n0 = 64 * tj, c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+  for (int j = 0; j < 16; ++j) {
+    const int r = r0 + 4 * j, m = m0 + r, n = n0 + c;
+    if (m < N && n < N && (ti != tj || r <= c)) {
+      double s = 0.0;
+      for (int z = 0; z < nsplit; ++z) s += (double)ws[z * split_stride + (int64_t)m * ldw + n];
+      const double v = alpha * s;
+      t[r][c] = v;
+      out[(int64_t)m * ldo + n] = (Out)(beta != 0.0 ? v + beta * (double)out[(int64_t)m * ldo + n] : v);
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < 16; ++j) {
+    const int r = r0 + 4 * j, m = n0 + r, n = m0 + c;   // mirror element (n0 + r, m0 + c) of tile (c, r)
+    if (m < N && n < N && (ti != tj || c < r)) {
+      const double v = t[c][r];
+      out[(int64_t)m * ldo + n] = (Out)(beta != 0.0 ? v + beta * (double)out[(int64_t)m * ldo + n] : v);
+    }
+  }
+}
+
 struct GemmArgs {
   int M, N;
   int64_t K;
@@ -854,6 +894,15 @@ hipError_t launch_gemm_reduce(int prec, const GemmArgs& g, double* C, double alp
                               const void* ws, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   const int64_t stride = (int64_t)g.M * g.N;
+  if ((sym & 1) && g.M == g.N) {
+    const int ntn = (g.N + 63) / 64;
+    const unsigned nt = (unsigned)((int64_t)ntn * (ntn + 1) / 2);
+    if (prec == kF64)
+      return launch_k(k_gemm_reduce_sym<double, double>, dim3(nt), dim3(256), st, g.N, nsplit,
+                      static_cast<const double*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta);
+    return launch_k(k_gemm_reduce_sym<float, double>, dim3(nt), dim3(256), st, g.N, nsplit,
+                    static_cast<const float*>(ws), stride, (int64_t)g.N, C, (int64_t)g.N, alpha, beta);
+  }
   const unsigned nb = (unsigned)((stride + 255) / 256);
   if (prec == kF64)
     return launch_k(k_gemm_reduce<double, double>, dim3(nb), dim3(256), st, g.M, g.N, nsplit,

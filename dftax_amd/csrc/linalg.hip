@@ -134,9 +134,12 @@ hipError_t launch_gamma_f32(int nao, int nq, int nf, const float* T, const doubl
 }
 
 // J[r] = sum_q B[r*nq + q] g[q]; one wavefront per row, 4 rows per block per step.
+// sym_n > 0: the rows are the pairs r = l sym_n + m of a B symmetric in (l, m); only
+// the rows m >= l are read (half of B), and each sum goes to J[l][m] and J[m][l].
 template <typename TB, bool VEC>
 __global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const TB* __restrict__ B,
-                                              const double* __restrict__ g, double* __restrict__ J, int accumulate) {
+                                              const double* __restrict__ g, double* __restrict__ J, int accumulate,
+                                              int sym_n) {
   extern __shared__ double s_g[];
   for (int i = threadIdx.x; i < nq; i += blockDim.x) s_g[i] = g[i];
   __syncthreads();
@@ -144,6 +147,12 @@ __global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const TB* _
   constexpr int PER = 16 / sizeof(TB);
   typedef TB vecT __attribute__((ext_vector_type(PER)));
   for (int64_t r = blockIdx.x * 4 + (threadIdx.x >> 6); r < nrows; r += (int64_t)gridDim.x * 4) {
+    int64_t rt = r;   // the mirrored row
+    if (sym_n > 0) {
+      const int64_t l = r / sym_n, m = r % sym_n;
+      if (m < l) continue;
+      rt = m * sym_n + l;
+    }
     const TB* row = B + r * nq;
     double acc = 0.0;
     if (VEC) {
@@ -173,20 +182,26 @@ __global__ __launch_bounds__(256) void k_jvec(int64_t nrows, int nq, const TB* _
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) J[r] = accumulate ? J[r] + acc : acc;
+    if (lane == 0) {
+      J[r] = accumulate ? J[r] + acc : acc;
+      if (rt != r) J[rt] = accumulate ? J[rt] + acc : acc;
+    }
   }
 }
 
-hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, double* J, hipStream_t st) {
+// sym_n > 0: B (nrows = sym_n^2 rows) is symmetric in its two orbital indices, and only
+// the rows of the upper triangle are read (see k_jvec).
+hipError_t launch_jvec(int64_t nrows, int nq, const double* B, const double* g, double* J, int sym_n, hipStream_t st) {
+  if (sym_n > 0 && (int64_t)sym_n * sym_n != nrows) return hipErrorInvalidValue;
   int64_t nb = (nrows + 3) / 4;
   if (nb > 65536) nb = 65536;
   const bool vec = (nq % 2 == 0) && ((uintptr_t)B % 16 == 0);
   if (vec)
     hipLaunchKernelGGL((k_jvec<double, true>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g,
-                       J, 0);
+                       J, 0, sym_n);
   else
     hipLaunchKernelGGL((k_jvec<double, false>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B,
-                       g, J, 0);
+                       g, J, 0, sym_n);
   return hipGetLastError();
 }
 
@@ -197,23 +212,25 @@ hipError_t launch_jvec_acc(int64_t nrows, int nq, const double* B, const double*
   const bool vec = (nq % 2 == 0) && ((uintptr_t)B % 16 == 0);
   if (vec)
     hipLaunchKernelGGL((k_jvec<double, true>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g,
-                       J, 1);
+                       J, 1, 0);
   else
     hipLaunchKernelGGL((k_jvec<double, false>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B,
-                       g, J, 1);
+                       g, J, 1, 0);
   return hipGetLastError();
 }
 
-hipError_t launch_jvec_f32(int64_t nrows, int nq, const float* B, const double* g, double* J, hipStream_t st) {
+hipError_t launch_jvec_f32(int64_t nrows, int nq, const float* B, const double* g, double* J, int sym_n,
+                           hipStream_t st) {
+  if (sym_n > 0 && (int64_t)sym_n * sym_n != nrows) return hipErrorInvalidValue;
   int64_t nb = (nrows + 3) / 4;
   if (nb > 65536) nb = 65536;
   const bool vec = (nq % 4 == 0) && ((uintptr_t)B % 16 == 0);
   if (vec)
     hipLaunchKernelGGL((k_jvec<float, true>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B, g,
-                       J, 0);
+                       J, 0, sym_n);
   else
     hipLaunchKernelGGL((k_jvec<float, false>), dim3((unsigned)nb), dim3(256), nq * sizeof(double), st, nrows, nq, B,
-                       g, J, 0);
+                       g, J, 0, sym_n);
   return hipGetLastError();
 }
 
