@@ -2467,10 +2467,6 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   h->e_prev = e[0];
 }
 
-// Occupied projector of F' = d_Ws (nmo x nmo) by SP2 purification, then its
-// factor L (pivoted Cholesky) and Cf = sqrt(2) X L. Returns false when the
-// purification does not reach an idempotent projector of trace nocc (no gap
-// at the Fermi level); the caller then takes the eigensolver path.
 // Split-K factor of the purification product X X (n x n, upper 128 x 128 tiles, one
 // 512-thread workgroup per CU): the s in 1..8 (K chunks of at least 64) that minimises
 // rounds x (chunk + 128), rounds = ceil(tiles s / ncu) and 128 the fixed cost of a
@@ -2682,6 +2678,11 @@ bool project_occupied_factor(dfks_ctx* h, const double* X, hipStream_t st) {
   return true;
 }
 
+// Occupied projector of F' = d_Ws (nmo x nmo) by SP2 purification, then its
+// factor L (projected from the previous step's factor, else by pivoted Cholesky)
+// and Cf = sqrt(2) X L. Returns false when the purification does not reach an
+// idempotent projector of trace nocc (no gap at the Fermi level); the caller then
+// takes the eigensolver path.
 bool purify_density(dfks_ctx* h, hipStream_t st) {
   const int n = h->nmo, nocc = h->nocc, nao = h->nao;
   double* X = h->d_W1;

@@ -15,16 +15,20 @@
 // so e_k = tr X_k - tr X_k^2 = sum lambda (1 - lambda) >= 0 bounds every
 // eigenvalue's distance from {0, 1}. Once e_k < 1e-9 two forced steps follow
 // (X^2, then 2X - X^2), which square the remaining deviations on both sides,
-// and the iteration freezes. The products X_k^2 are FP64 MFMA GEMMs (split-K
-// partial tiles); one kernel here sums the partials and applies the branch
-// while it accumulates tr X_{k+1} and sum_ij (X_{k+1})_ij^2 = tr X_{k+1}^2,
-// which decide the next branch before its product. The stop rule runs on the
-// device, so the host only reads the state between chunks of iterations.
+// and the iteration freezes. For n <= 2015 one kernel per iteration
+// (k_sp2_step) forms the upper 64 x 64 tiles of X_k^2 on FP64 MFMA, applies
+// the branch, writes X_{k+1} (tile and mirror) into the other of two buffers
+// and accumulates tr X_{k+1} and sum_ij (X_{k+1})_ij^2 = tr X_{k+1}^2, which
+// decide the next branch before its product; larger n take split-K partial
+// tiles of the product and a kernel that sums them and does the same update.
+// The stop rule runs on the device, so the host only reads the state between
+// chunks of iterations.
 //
 // The occupied factor L (X = L L^T, n x nocc) comes from a pivoted Cholesky of
 // the converged projector in panels of kPanel columns: one workgroup picks the
 // panel's pivots (largest residual diagonal) and forms its columns, then a
-// GEMM removes the panel from the trailing residual.
+// GEMM removes the panel from the trailing residual. (The SCF driver uses it
+// on the first step; later steps project the previous factor.)
 #include <cstdint>
 
 #include "dfks_common.h"
