@@ -147,6 +147,8 @@ hipError_t launch_gemm_reduce(int prec, const GemmArgs& g, double* C, double alp
 hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s);
 hipError_t launch_sp2_lanczos(int n, const double* F, double* lz, double* st, hipStream_t s);
 hipError_t launch_sp2_init(int n, const double* F, const double* st, double* X, hipStream_t s);
+hipError_t launch_sp2_weyl(int n, const double* F, double* Fp, const double* pb, double* st, hipStream_t s);
+hipError_t launch_save_bounds(const double* st, double* pb, hipStream_t s);
 hipError_t launch_sp2_begin(int n, const double* X, int nocc, double* st, hipStream_t s);
 hipError_t launch_sp2_update(int n, int nsplit, const double* ws, double* X, int nocc, double* st, hipStream_t s);
 int sp2_step_tiles(int n);
@@ -557,6 +559,10 @@ struct dfks_ctx {
   double *d_Lw = nullptr, *d_Ls = nullptr, *d_Lq = nullptr;
   bool l_valid = false;
   double* d_lz = nullptr;   // Lanczos vectors and coefficients of the SP2 spectral bounds
+  // the previous purified step's F' and spectral bounds (Weyl bounds for the next step)
+  double *d_Fp = nullptr, *d_pb = nullptr;
+  int pb_valid_n = -1;      // nmo of the step d_Fp, d_pb belong to (-1: none)
+  double lz_range = 0.0;    // width of the bounds of the last Lanczos run
   hipGraphExec_t lz_exec = nullptr;  // the Lanczos bounds as one graph (per nmo)
   int lz_exec_n = -1;
   int32_t* d_piv = nullptr;
@@ -631,7 +637,7 @@ struct dfks_ctx {
     void* ptrs[] = {d_boys, d_atm, d_S, d_T, d_V, d_H, d_int2c, d_int3c, d_B, d_gxyz, d_gw, d_ao, d_zw, d_exc,
                     d_Tx, d_gamma, d_J, d_K, d_Vxc, d_part, d_Edev, d_Cf, d_sgn, d_W1, d_W2, d_evals, d_offd,
                     d_info, d_sel, d_X, d_P, d_F, d_Fs, d_Es, d_gram, d_coef, d_dots, d_Fx, d_C, d_eps, d_Ws,
-                    d_sp2, d_lz, d_lohi, d_dvec, d_L, d_Lw, d_Ls, d_Lq, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
+                    d_sp2, d_lz, d_Fp, d_pb, d_lohi, d_dvec, d_L, d_Lw, d_Ls, d_Lq, d_piv, d_pairs3, d_slab_aux, d_slab, d_Tslab, d_Ttil, d_Lt,
                     d_fitd, d_aob, d_zwb, d_fxc, d_rho0, d_Rf, d_xc_pts, d_xc_w, d_xc_exc, d_xc_ao,
                     d_xc_zw, d_xc_q, d_xc_ao32, d_xc_cols, d_xc_vlen, d_xc_shl, d_xc_shoff, d_xc_vrow,
                     d_xc_vprow, d_xc_vpcol, d_xc_zw2, d_xc_q2, d_Vxc2, d_uP, d_uJ, d_uK, d_ug, d_xcf_tab,
@@ -2435,6 +2441,9 @@ void scf_begin(dfks_ctx* h, const double* P0, double lindep, int m, double level
   realloc(h->d_eps, nmo);
   realloc(h->d_Ws, (size_t)nao * nao);
   realloc(h->d_lohi, 2 * (size_t)nmo);
+  realloc(h->d_Fp, (size_t)nmo * nmo);
+  realloc(h->d_pb, 2);
+  h->pb_valid_n = -1;
   realloc(h->d_dvec, nmo);
   realloc(h->d_L, (size_t)nmo * h->nocc);
   realloc(h->d_Lw, (size_t)nmo * h->nocc);
@@ -2687,7 +2696,17 @@ bool purify_density(dfks_ctx* h, hipStream_t st) {
   const int n = h->nmo, nocc = h->nocc, nao = h->nao;
   double* X = h->d_W1;
   HIPCHK(launch_gersh(n, h->d_Ws, h->d_lohi, h->d_sp2, st));
-  sp2_bounds(h, st);
+  // spectral bounds: Lanczos on an SCF's first purification (or after one that failed),
+  // afterwards the previous bounds widened by ||F' - F'_prev||_F (Weyl)
+  const bool weyl = h->pb_valid_n == n;
+  if (weyl) {
+    HIPCHK(launch_sp2_weyl(n, h->d_Ws, h->d_Fp, h->d_pb, h->d_sp2, st));
+  } else {
+    sp2_bounds(h, st);
+    HIPCHK(hipMemcpyAsync(h->d_Fp, h->d_Ws, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, st));
+  }
+  HIPCHK(launch_save_bounds(h->d_sp2, h->d_pb, st));
+  h->pb_valid_n = -1;   // valid again once this purification succeeds
   HIPCHK(launch_sp2_init(n, h->d_Ws, h->d_sp2, X, st));
   HIPCHK(launch_sp2_begin(n, X, nocc, h->d_sp2, st));
   int done = 0;
@@ -2711,6 +2730,11 @@ bool purify_density(dfks_ctx* h, hipStream_t st) {
   h->sp2_guess = std::max(8, (int)stv[kSpIter]);
   if (!(std::fabs(stv[kSpErr]) < 1e-10 * std::max(1, nocc) && std::fabs(stv[kSpTrace] - nocc) < 1e-8 * std::max(1, nocc)))
     return false;
+  // Weyl bounds only widen from step to step: once they are 2 % wider than the last
+  // Lanczos bounds (about 0.03 more SP2 iterations), the next step runs Lanczos again
+  const double range = stv[kSpEmax] - stv[kSpEmin];
+  if (!weyl) h->lz_range = range;
+  h->pb_valid_n = range <= 1.02 * h->lz_range ? n : -1;
   // X = L L^T: projected from the previous step's factor, or by pivoted Cholesky, panel by
   // panel (X is overwritten by the residual)
   if (!(h->l_valid && project_occupied_factor(h, X, st))) {

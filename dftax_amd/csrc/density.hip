@@ -621,6 +621,62 @@ hipError_t launch_sp2_lanczos(int n, const double* F, double* lz, double* st, hi
   return e;
 }
 
+// Spectral bounds of F from those of the previous step's F' (Fp) without a
+// Lanczos run: by Weyl's inequality every eigenvalue moves by at most
+// ||F - Fp||_2 <= ||F - Fp||_F, so [lo_p - d, hi_p + d] with d = ||F - Fp||_F
+// contains the spectrum of F whenever [lo_p, hi_p] contained that of Fp.
+// k_weyl_part: per-workgroup partial sums of the squared differences (kSp2Groups
+// workgroups of 256 threads), and Fp <- F in the same pass.
+__global__ __launch_bounds__(256) void k_weyl_part(int n, const double* __restrict__ F, double* __restrict__ Fp,
+                                                   double* __restrict__ st) {
+  const int64_t nn = (int64_t)n * n;
+  double a = 0.0;
+  for (int64_t idx = blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)kSp2Groups * 256) {
+    const double f = F[idx], d = f - Fp[idx];
+    a += d * d;
+    Fp[idx] = f;
+  }
+  sp2_partials(a, 0.0, st);
+}
+
+// One block: d from the partials (fixed order), then the widened previous bounds
+// pb intersected with the Gershgorin bounds already in st.
+__global__ __launch_bounds__(kSp2Groups) void k_weyl_bounds(const double* __restrict__ pb, double* __restrict__ st) {
+  __shared__ double s_a[kSp2Groups];
+  s_a[threadIdx.x] = st[kSpPart + 2 * threadIdx.x];
+  __syncthreads();
+  for (int w = kSp2Groups / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s_a[threadIdx.x] += s_a[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double d = sqrt(s_a[0]);
+    const double lo = pb[0] - d, hi = pb[1] + d;
+    if (isfinite(lo) && isfinite(hi) && lo < hi) {
+      st[kSpEmin] = fmax(st[kSpEmin], lo);
+      st[kSpEmax] = fmin(st[kSpEmax], hi);
+    }
+  }
+}
+
+// pb <- the bounds the purification used
+__global__ void k_save_bounds(const double* __restrict__ st, double* __restrict__ pb) {
+  if (threadIdx.x == 0) {
+    pb[0] = st[kSpEmin];
+    pb[1] = st[kSpEmax];
+  }
+}
+
+hipError_t launch_sp2_weyl(int n, const double* F, double* Fp, const double* pb, double* st, hipStream_t s) {
+  hipError_t e = launch_k(k_weyl_part, dim3(kSp2Groups), dim3(256), s, n, F, Fp, st);
+  if (e == hipSuccess) e = launch_k(k_weyl_bounds, dim3(1), dim3(kSp2Groups), s, pb, st);
+  return e;
+}
+
+hipError_t launch_save_bounds(const double* st, double* pb, hipStream_t s) {
+  return launch_k(k_save_bounds, dim3(1), dim3(64), s, st, pb);
+}
+
 hipError_t launch_gersh(int n, const double* F, double* lohi, double* st, hipStream_t s) {
   hipLaunchKernelGGL(k_gersh, dim3(n), dim3(256), 0, s, n, F, lohi);
   hipLaunchKernelGGL(k_sp2_start, dim3(1), dim3(256), 0, s, n, lohi, st);
